@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""bench.py -- R-tree build throughput (and the join / overlay phases) on one
+MI355X, printed as one JSON line.
+
+Workload (the step): top-down R-tree build, fanout 16, of both layers of
+1,048,576 fp32 rects (centres U[0,1]^2, sides U[1e-4, 1e-3]), inputs
+resident in HBM; value = rects built per second (Mrect/s).  The same line
+carries, measured separately: the bottom-up build of 1,048,576 rects, the
+65,536-window join, the layer-layer join, and the plain FP64 overlay of the
+first 100,000 sorted layer pairs (with its area error against the FP64 CPU
+oracle).  roofline: the dominant kernel of the step, its average duration
+from HIP events recorded on its launch stream.  cpu_baseline: the oracle's
+serial top-down build on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "R-tree build Mrect/s; overlay pairs/s batched vs plain at ≤1e-12 area err, 1 GPU"
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+# algorithmic HBM bytes per processed item, per kernel (minimal traffic:
+# every input read once, every output written once)
+BYTES_PER_ITEM = {
+    "k_scatter": 16,        # 8-B item read + 8-B item written
+    "k_hist": 8,            # 8-B item read
+    "k_td_items": 24,       # 16-B rect read, 8-B item written
+    "k_td_gather": 48,      # 8-B item, 16-B rect + 4-B id read; 16 + 4 written
+    "k_td_lds_sort": 40,    # 16-B rect + 4-B id read, 16 + 4 written
+    "k_td_leaves": 20,      # 16-B rect read, 4-B child written (+16-B box per slot)
+    "k_morton_items": 24,   # 16-B rect read, 8-B item written
+    "k_bu_leaves": 52,      # 8-B item, 16-B rect read; 4+4 key/id, 16+4 entry written
+}
+
+
+def timed(fn, steps: int, warmup: int) -> float:
+    """Average ms per call of fn, bracketed by device syncs and HIP events on
+    the current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-phases", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import rtree, workloads
+    from parlllelrtree_amd.profiling import KernelTimes
+
+    dev = torch.device("cuda:0")
+    log(f"device {torch.cuda.get_device_name(0)}")
+    n = 1 << 20
+    ra, rb = workloads.w1(n)
+    da, db = torch.from_numpy(ra).to(dev), torch.from_numpy(rb).to(dev)
+    ta = rtree.build_top_down(da, 16)
+    tb = rtree.build_top_down(db, 16)
+    torch.cuda.synchronize()
+
+    def step():
+        rtree.rebuild(ta, da)
+        rtree.rebuild(tb, db)
+
+    log(f"timing top-down build of 2 x {n} rects: {args.warmup} warmup + {args.steps} steps")
+    ms = timed(step, args.steps, args.warmup)
+    value = 2 * n / (ms * 1e-3) / 1e6
+    log(f"step {ms:.3f} ms -> {value:.1f} Mrect/s")
+
+    # dominant kernel of the step, durations from HIP events on its stream
+    with KernelTimes() as kt:
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+    dom = max((k for k in kt.stats if k in BYTES_PER_ITEM), key=lambda k: kt.stats[k][1])
+    cnt, tot = kt.stats[dom]
+    avg_ms = tot / cnt
+    alg_bytes = BYTES_PER_ITEM[dom] * n
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "avg_us": round(avg_ms * 1e3, 2),
+                "bytes_per_launch": alg_bytes,
+                "kernels_ms_per_step": {k: round(v[1] / 3, 4) for k, v in
+                                        sorted(kt.stats.items(), key=lambda kv: -kv[1][1])}}
+    log(f"dominant kernel {dom}: {avg_ms * 1e3:.1f} us avg, {achieved:.0f} GB/s algorithmic")
+
+    phases = {}
+    if not args.no_phases:
+        import oracle as O
+        # bottom-up build of 1,048,576 rects + 65,536 windows
+        r0, q0 = workloads.w0()
+        d0, dq0 = torch.from_numpy(r0).to(dev), torch.from_numpy(q0).to(dev)
+        t0 = rtree.build_bottom_up(d0, 16)
+        ms_bu = timed(lambda: rtree.rebuild(t0, d0), args.steps, args.warmup)
+        phases["bu_build_1M"] = {"ms": round(ms_bu, 4), "Mrect_s": round(n / ms_bu / 1e3, 1)}
+        cap0 = rtree.join(t0, dq0).shape[0]
+        j0 = rtree.Joiner(t0, q0.shape[0], cap0, dev)
+        ms_j0 = timed(lambda: j0.run(dq0), max(args.steps // 2, 3), 1)
+        phases["join_windows_65536"] = {"ms": round(ms_j0, 4), "pairs": cap0,
+                                        "Mpairs_s": round(cap0 / ms_j0 / 1e3, 1)}
+        log(f"bu build {ms_bu:.3f} ms, window join {ms_j0:.3f} ms ({cap0} pairs)")
+        # layer-layer join (top-down tree of A, queries = B)
+        cap1 = rtree.join(ta, db).shape[0]
+        j1 = rtree.Joiner(ta, n, cap1, dev)
+        ms_j1 = timed(lambda: j1.run(db), max(args.steps // 2, 3), 1)
+        pairs_qr = j1.result().cpu().numpy().view(np.uint32)
+        phases["join_layers_1M_x_1M"] = {"ms": round(ms_j1, 4), "pairs": cap1,
+                                         "Mpairs_s": round(cap1 / ms_j1 / 1e3, 1)}
+        log(f"layer join {ms_j1:.3f} ms ({cap1} pairs)")
+        # plain FP64 overlay of the first 100,000 sorted pairs
+        cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
+        pd = torch.from_numpy(cp.view(np.int32)).to(dev)
+        Ad, Bd = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
+        area = torch.empty(len(cp), dtype=torch.float64, device=dev)
+        from parlllelrtree_amd._lib import lib
+        ws = torch.empty(lib().prt_overlay_plain_workspace_size(len(cp), Ad.max_verts +
+                                                                 Bd.max_verts),
+                         dtype=torch.uint8, device=dev)
+        ms_cp = timed(lambda: rtree.overlay_plain(pd, Ad, Bd, area, ws), 5, 1)
+        got = area.cpu().numpy()
+        ref, flops = O.overlay(cp, A, B, nthreads=min(os.cpu_count() or 1, 16), with_flops=True)
+        pa = np.abs([O.poly_area(A.poly(i)) for i in cp[:, 0]])
+        pb = np.abs([O.poly_area(B.poly(i)) for i in cp[:, 1]])
+        den = np.maximum(np.abs(ref), 1e-12 * np.minimum(pa, pb))
+        rel = np.abs(got - ref) / np.where(den == 0, 1.0, den)
+        phases["overlay_plain_100k"] = {
+            "ms": round(ms_cp, 4), "pairs_s": round(len(cp) / ms_cp * 1e3, 1),
+            "max_rel_err": float(rel.max()), "bit_identical": bool(np.array_equal(got, ref)),
+            "nonempty": int((ref > 0).sum()), "oracle_fp64_flops": int(flops),
+            "fp64_tflops": round(flops / (ms_cp * 1e-3) / 1e12, 4)}
+        log(f"plain overlay {ms_cp:.3f} ms, max rel err {rel.max():.2e}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.td_build(ra, 16)
+            t.append(time.perf_counter() - t0)
+        t_med = sorted(t)[1]
+        cpu = {"value": round(n / t_med / 1e6, 3), "unit": "Mrect/s", "cores": 1, "kind": "port",
+               "sample": f"serial top-down build of one 1,048,576-rect layer (oracle, C), "
+                         f"median of 3 = {t_med:.3f} s; host {cpu_model()}"}
+        log(f"cpu baseline {cpu['value']} Mrect/s")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mrect/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 boxes, u64 sort keys", "data": "synthetic",
+        "config": {"workload": "top-down R-tree build, fanout 16, two layers of 1,048,576 "
+                               "uniform fp32 rects (sides U[1e-4,1e-3])",
+                   "rects_per_step": 2 * n, "fanout": 16, "parallelism": "1 GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "phases": phases,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

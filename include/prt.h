@@ -1,0 +1,151 @@
+/*
+ * prt.h -- C-ABI of the MI355X (gfx950) R-tree build / spatial join /
+ * polygon overlay library (libprt.so).
+ *
+ * Every compute entry point takes DEVICE pointers (suffix _h marks host
+ * pointers), a caller-owned workspace and a hipStream_t passed as void*.
+ * Calls are stream-ordered and never synchronise the device, except the
+ * reference-shaped shim prt_run(), which synchronises like the reference.
+ * No entry point allocates device memory except prt_run().
+ * Return value: 0 (PRT_OK) or a negative PRT_E_* code; prt_last_error()
+ * returns a message for the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (BU = BotomUp-rtree.cu, TD =
+ * TopDown-rtree.cu, OV = vector_map_overlay_gpc_kernel.cu):
+ *   prt_morton_center30 / prt_morton_ref4d  <- CalcuateZOrderValue BU:73-94,
+ *                                              Interleave BU:98-123
+ *   prt_bu_build                            <- LocalSort/scan1/scan2/Reorder
+ *                                              BU:125-266 + Construction_init/
+ *                                              Construction1 BU:373-426, driven
+ *                                              by main BU:836-869
+ *   prt_td_build                            <- changeX/changeY TD:465-481, the
+ *                                              radix kernels TD:162-261, fun/
+ *                                              fun1/fun2 TD:309-353, driven by
+ *                                              main TD:629-679
+ *   prt_join_bu / prt_join_td               <- init/query1 BU:592-630,
+ *                                              TD:355-377, 450-463 (with the
+ *                                              hit semantics of RtreeSearch
+ *                                              BU:439-483, TD:56-76)
+ *   prt_overlay_plain                       <- Execute OV:81-113 +
+ *                                              gpc_polygon_clip OV:115-750
+ *   prt_run                                 <- run OV:61-79
+ */
+#ifndef PRT_H
+#define PRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRT_OK 0
+#define PRT_E_INVAL (-1)
+#define PRT_E_NOMEM (-2)
+#define PRT_E_HIP (-3)
+#define PRT_E_OVERFLOW (-4)
+#define PRT_E_UNSUPPORTED (-5)
+
+#define PRT_MAX_LEVELS 40
+#define PRT_MAX_FANOUT 64
+#define PRT_MAX_POLY_VERTS 4096
+
+/* Bottom-up tree, SoA.  Node ids run level by level, leaves first, root
+ * last (BU:788-830).  Entry slot j of node i lives at i*fanout + j. */
+typedef struct prt_bu_tree {
+    int32_t fanout;
+    int32_t num_levels;
+    int64_t num_rects;
+    int64_t num_nodes;
+    int64_t level_start[PRT_MAX_LEVELS];
+    int64_t level_count[PRT_MAX_LEVELS];
+    /* device arrays, caller allocated */
+    uint32_t *keys;       /* [num_rects] Morton keys in sorted order          */
+    uint32_t *perm;       /* [num_rects] leaf order: original rect ids        */
+    int32_t *node_count;  /* [num_nodes] entries used                        */
+    int32_t *node_start;  /* [num_nodes] first sorted position covered       */
+    int32_t *node_end;    /* [num_nodes] last sorted position covered        */
+    float *entry_mbr;     /* [num_nodes*fanout*4] xmin,ymin,xmax,ymax;
+                             unused slots (+inf,+inf,-inf,-inf)              */
+    int32_t *entry_ref;   /* [num_nodes*fanout] leaf: rect id; else child
+                             node id; unused: -1                             */
+} prt_bu_tree;
+
+/* Top-down tree: implicit slot layout of TD:504-524. */
+typedef struct prt_td_tree {
+    int32_t fanout;
+    int32_t num_levels;
+    int64_t num_rects;
+    int64_t num_slots;
+    int64_t level_offset[PRT_MAX_LEVELS + 1];
+    int32_t *slot_child;  /* [num_slots] >0 child block, <0 leaf -pos-1, 0 empty */
+    float *slot_mbr;      /* [num_slots*4]; empty slots (+inf,+inf,-inf,-inf)   */
+    uint32_t *perm;       /* [num_rects] original id at final sorted position   */
+} prt_td_tree;
+
+/* Autotuned launch shape of the batched overlay, per vertex bucket. */
+#define PRT_TUNE_BUCKETS 6
+typedef struct prt_tune {
+    int32_t bucket_max_verts[PRT_TUNE_BUCKETS]; /* upper bound of n1+n2 per bucket */
+    int32_t lanes_per_pair[PRT_TUNE_BUCKETS];   /* 8, 16, 32 or 64                  */
+    int32_t waves_per_block[PRT_TUNE_BUCKETS];  /* 1..8                             */
+} prt_tune;
+
+const char *prt_version(void);
+const char *prt_last_error(void);
+
+/* ---- optional per-kernel timing (HIP events around every launch) ----
+ * No reference counterpart (the reference times whole phases with
+ * cudaEvent, BU:836-874).  prt_profile_read aggregates per kernel name:
+ * names = nul-separated, counts[k] launches, total_ms[k] summed duration;
+ * returns the number of names and clears the record. */
+void prt_profile_enable(int on);
+int prt_profile_read(char *names, int names_bytes, int64_t *counts, double *total_ms,
+                     int max_kernels);
+
+/* ---- keys ---------------------------------------------------------- */
+int prt_morton_center30(const float *rects, int64_t n, uint32_t *keys, void *stream);
+int prt_morton_ref4d(const int64_t *rects, int64_t n, int64_t scale, uint64_t *keys,
+                     void *stream);
+
+/* ---- stable radix sort of (key, index) ------------------------------ */
+size_t prt_sort_workspace_size(int64_t n);
+int prt_sort_pairs_u32(const uint32_t *keys_in, int64_t n, int key_bits,
+                       uint32_t *keys_out, uint32_t *idx_out, void *ws, size_t ws_bytes,
+                       void *stream);
+
+/* ---- bottom-up build ------------------------------------------------ */
+int prt_bu_plan(int64_t n, int fanout, prt_bu_tree *tree);
+size_t prt_bu_workspace_size(int64_t n, int fanout);
+int prt_bu_build(const float *rects, int64_t n, int fanout, prt_bu_tree *tree,
+                 void *ws, size_t ws_bytes, void *stream);
+
+/* ---- top-down build ------------------------------------------------- */
+int prt_td_plan(int64_t n, int fanout, prt_td_tree *tree);
+size_t prt_td_workspace_size(int64_t n, int fanout);
+int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tree *tree,
+                 void *ws, size_t ws_bytes, void *stream);
+
+/* ---- join: pairs (q, r) sorted by (q, r); *count_d gets the total,
+ *      only the first cap pairs are written --------------------------- */
+size_t prt_join_workspace_size(int64_t nq);
+int prt_join_bu(const prt_bu_tree *tree, const float *queries, int64_t nq,
+                uint32_t *pairs, int64_t cap, int64_t *count_d, void *ws,
+                size_t ws_bytes, void *stream);
+int prt_join_td(const prt_td_tree *tree, const float *queries, int64_t nq,
+                uint32_t *pairs, int64_t cap, int64_t *count_d, void *ws,
+                size_t ws_bytes, void *stream);
+
+/* ---- overlay: area of A[pairs[2t]] intersect B[pairs[2t+1]], FP64 ----
+ * Polygons are CSR: xy = interleaved (x, y) doubles, off[i]..off[i+1]. */
+size_t prt_overlay_plain_workspace_size(int64_t npairs, int32_t max_verts);
+int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                      const int64_t *offA, const double *xyB, const int64_t *offB,
+                      int32_t max_verts, double *area, void *ws, size_t ws_bytes,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,988 @@
+/*
+ * prt_oracle.c -- CPU restatement (C99, FP64 where the reference is FP64)
+ * of the ParlllelRtree hot path.  TEST INFRASTRUCTURE ONLY: the checker for
+ * the HIP library, never linked into it.
+ *
+ * Reference files (BU = BotomUp-rtree.cu, TD = TopDown-rtree.cu,
+ * OV = vector_map_overlay_gpc_kernel.cu) cannot be compiled here (no nvcc,
+ * a compile error at BU:391, absent gpc.h/common.h/XMalloc), so this file
+ * restates their host semantics.  How each part is pinned (tests/ hold the
+ * checks):
+ *   orc_interleave_ref4 / orc_morton_ref4d : closed-form bit table KATs.
+ *   orc_morton_center30                    : per-bit textbook Morton loop.
+ *   orc_stable_order_*                     : numpy argsort(kind="stable").
+ *   orc_bu_build                           : hand-derived tiny trees + tree
+ *                                            invariants (tight unions, level
+ *                                            chain, root last).
+ *   orc_td_build                           : hand-derived tiny layout + numpy
+ *                                            lexsort restatement + invariants.
+ *   orc_join_*                             : brute force (tree-independent
+ *                                            hit set).
+ *   orc_clip_area                          : closed forms (rect/rect, P/P,
+ *                                            disjoint, nested), Sutherland-
+ *                                            Hodgman on convex inputs, an
+ *                                            independent Green's-theorem area
+ *                                            on random stars, symmetry and
+ *                                            power-of-two translation.
+ *   orc_area_grad                          : central finite differences of
+ *                                            orc_clip_area and the closed form
+ *                                            dA/dx_i = (y_{i+1}-y_{i-1})/2 for
+ *                                            vertices deep inside the other
+ *                                            polygon.
+ * Build with -ffp-contract=off: the HIP kernels evaluate the same FP64
+ * expressions in the same order, and contraction would change roundings.
+ */
+#include "prt_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ====================================================================== */
+/* Morton keys                                                            */
+/* ====================================================================== */
+
+/* Build-defined CENTER30 quantisation (the configs key rect centres, 15 bits
+ * per axis).  c = 0.5f*(lo+hi) in fp32, q = floor(c*2^15) clamped to
+ * [0, 32767]; NaN maps to 0.  Replaces the integer scale rule of
+ * BU:704-721 + BU:80-83 for fp32 inputs. */
+uint32_t orc_quant15(float lo, float hi)
+{
+    volatile float sum = lo + hi;      /* keep fp32 rounding of the sum */
+    float c = 0.5f * sum;
+    float s = c * 32768.0f;
+    if (!(s >= 0.0f)) return 0u;
+    if (s >= 32767.0f) return 32767u;
+    return (uint32_t)s;                /* truncation == floor for s >= 0 */
+}
+
+/* Spread the low 15 bits of v onto the even bit positions (bit k -> 2k). */
+uint32_t orc_spread15(uint32_t v)
+{
+    uint32_t out = 0;
+    for (int k = 0; k < 15; ++k) out |= ((v >> k) & 1u) << (2 * k);
+    return out;
+}
+
+/* CENTER30 key: x on even bits, y on odd bits. */
+void orc_morton_center30(const float *r4, int64_t n, uint32_t *keys)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t qx = orc_quant15(r4[4 * i + 0], r4[4 * i + 2]);
+        uint32_t qy = orc_quant15(r4[4 * i + 1], r4[4 * i + 3]);
+        keys[i] = orc_spread15(qx) | (orc_spread15(qy) << 1);
+    }
+}
+
+/* Reference 4-D interleave, BU:98-123: bit k (k < 16) of a1 lands on key
+ * bit 8*(k/2) + 6 + (k&1), a2 on +4, a3 on +2, a4 on +0; bits >= 16 of each
+ * input are dropped by the masks.  Written as an explicit per-bit map. */
+uint64_t orc_interleave_ref4(uint64_t a1, uint64_t a2, uint64_t a3, uint64_t a4)
+{
+    uint64_t in[4] = {a1, a2, a3, a4};
+    uint64_t out = 0;
+    for (int w = 0; w < 4; ++w) {
+        int base = 6 - 2 * w;
+        for (int k = 0; k < 16; ++k) {
+            uint64_t bit = (in[w] >> k) & 1u;
+            out |= bit << (8 * (k / 2) + base + (k & 1));
+        }
+    }
+    return out;
+}
+
+/* Reference key on int64 rects, BU:73-94: divide each coordinate by the
+ * scale (C truncating division), subtract min (0 after BU:720-721), then
+ * interleave (x1, y1, x2, y2). */
+void orc_morton_ref4d(const int64_t *r4, int64_t n, int64_t scale, uint64_t *keys)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t a[4];
+        for (int j = 0; j < 4; ++j) a[j] = (uint64_t)(r4[4 * i + j] / scale);
+        keys[i] = orc_interleave_ref4(a[0], a[1], a[2], a[3]);
+    }
+}
+
+/* Scale rule BU:711-717: the smallest power of ten 10^k with range/10^k
+ * <= 1e4 under repeated integer division by ten. */
+int64_t orc_ref_scale(int64_t range)
+{
+    int64_t s = 1;
+    while (range > 10000) {
+        s *= 10;
+        range /= 10;
+    }
+    return s;
+}
+
+/* ====================================================================== */
+/* Stable ordering (the sort both builds rely on)                          */
+/* ====================================================================== */
+
+/* perm = positions ordered by (key, position): a stable ascending sort, the
+ * order the reference's LSD radix sort produces (BU:125-266, TD:162-261) and
+ * glibc's merge-sort qsort produces for the 0/1 comparator (BU:638-640,
+ * TD:22-24). */
+void orc_stable_order_u64(const uint64_t *keys, int64_t n, uint32_t *perm)
+{
+    if (n <= 0) return;
+    uint32_t *tmp = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    for (int64_t i = 0; i < n; ++i) perm[i] = (uint32_t)i;
+    uint32_t *src = perm, *dst = tmp;
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            int64_t mid = lo + w < n ? lo + w : n;
+            int64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, o = lo;
+            while (i < mid && j < hi) {
+                if (keys[src[j]] < keys[src[i]]) dst[o++] = src[j++];
+                else dst[o++] = src[i++];
+            }
+            while (i < mid) dst[o++] = src[i++];
+            while (j < hi) dst[o++] = src[j++];
+        }
+        uint32_t *t = src; src = dst; dst = t;
+    }
+    if (src != perm) memcpy(perm, src, (size_t)n * sizeof(uint32_t));
+    free(tmp);
+}
+
+void orc_stable_order_u32(const uint32_t *keys, int64_t n, uint32_t *perm)
+{
+    if (n <= 0) return;
+    uint64_t *k = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    for (int64_t i = 0; i < n; ++i) k[i] = keys[i];
+    orc_stable_order_u64(k, n, perm);
+    free(k);
+}
+
+/* ====================================================================== */
+/* Bottom-up build                                                         */
+/* ====================================================================== */
+
+static void set_empty(float *m)
+{
+    m[0] = INFINITY; m[1] = INFINITY; m[2] = -INFINITY; m[3] = -INFINITY;
+}
+
+static void merge_into(float *dst, const float *src)
+{
+    if (src[0] < dst[0]) dst[0] = src[0];
+    if (src[1] < dst[1]) dst[1] = src[1];
+    if (src[2] > dst[2]) dst[2] = src[2];
+    if (src[3] > dst[3]) dst[3] = src[3];
+}
+
+/* Level chain of BU:788-830: ceil(n/f) leaves, then ceil(count/f) per level
+ * until one node remains.  Node ids run level by level, leaves first, root
+ * last.  Returns the number of levels. */
+int orc_bu_levels(int64_t n, int f, int64_t *level_start, int64_t *level_count)
+{
+    if (n <= 0 || f < 2) return 0;
+    int L = 0;
+    int64_t cnt = (n + f - 1) / f, start = 0;
+    for (;;) {
+        level_start[L] = start;
+        level_count[L] = cnt;
+        start += cnt;
+        ++L;
+        if (cnt <= 1) break;
+        cnt = (cnt + f - 1) / f;
+    }
+    return L;
+}
+
+/* Host build of BU:776-833 with the GPU twin's start/end fields
+ * (BU:373-426): stable sort by key, leaves = consecutive chunks of f sorted
+ * rects (entry ref = original id, BU:794-801), each upper entry = union of
+ * every entry of its child (BU:816-826, ref = child id).  Unused entry slots
+ * hold the empty box (+inf,+inf,-inf,-inf) and ref -1.  Returns #nodes. */
+int orc_bu_build(const float *r4, int64_t n, int f, const uint32_t *keys,
+                 uint32_t *perm, int32_t *node_count, int32_t *node_start,
+                 int32_t *node_end, float *entry_mbr, int32_t *entry_ref)
+{
+    int64_t ls[64], lc[64];
+    int L = orc_bu_levels(n, f, ls, lc);
+    if (L == 0) return -1;
+    orc_stable_order_u32(keys, n, perm);
+    for (int64_t i = 0; i < lc[0]; ++i) {
+        int64_t b = i * f, e = b + f - 1 < n - 1 ? b + f - 1 : n - 1;
+        node_count[i] = (int32_t)(e - b + 1);
+        node_start[i] = (int32_t)b;
+        node_end[i] = (int32_t)e;
+        for (int j = 0; j < f; ++j) {
+            float *m = entry_mbr + 4 * (i * f + j);
+            if (b + j <= e) {
+                uint32_t id = perm[b + j];
+                memcpy(m, r4 + 4 * (int64_t)id, 4 * sizeof(float));
+                entry_ref[i * f + j] = (int32_t)id;
+            } else {
+                set_empty(m);
+                entry_ref[i * f + j] = -1;
+            }
+        }
+    }
+    for (int l = 1; l < L; ++l) {
+        for (int64_t i = 0; i < lc[l]; ++i) {
+            int64_t node = ls[l] + i;
+            int64_t b = ls[l - 1] + i * f;
+            int64_t last = ls[l - 1] + lc[l - 1] - 1;
+            int64_t e = b + f - 1 < last ? b + f - 1 : last;
+            node_count[node] = (int32_t)(e - b + 1);
+            node_start[node] = node_start[b];
+            node_end[node] = node_end[e];
+            for (int j = 0; j < f; ++j) {
+                float *m = entry_mbr + 4 * (node * f + j);
+                set_empty(m);
+                if (b + j <= e) {
+                    int64_t c = b + j;
+                    for (int k = 0; k < node_count[c]; ++k)
+                        merge_into(m, entry_mbr + 4 * (c * f + k));
+                    entry_ref[node * f + j] = (int32_t)c;
+                } else {
+                    entry_ref[node * f + j] = -1;
+                }
+            }
+        }
+    }
+    return (int)(ls[L - 1] + lc[L - 1]);
+}
+
+/* ====================================================================== */
+/* Top-down build                                                          */
+/* ====================================================================== */
+
+/* Slot offsets of TD:504-524: repeat num = ceil(num/f)*f; push; num /= f
+ * while num > 1, then push 0, reverse and prefix-sum.  level_offset gets
+ * L+1 entries; returns L.  (n == 1 is undefined in the reference; here it
+ * takes the n <= f layout: one root block.) */
+int orc_td_levels(int64_t n, int f, int64_t *level_offset)
+{
+    if (n <= 0 || f < 2) return 0;
+    int64_t pushed[64];
+    int L = 0;
+    int64_t num = n;
+    do {
+        num = ((num + f - 1) / f) * f;
+        pushed[L++] = num;
+        num /= f;
+    } while (num > 1);
+    level_offset[0] = 0;
+    for (int p = 0; p < L; ++p) level_offset[p + 1] = level_offset[p] + pushed[L - 1 - p];
+    return L;
+}
+
+/* Order-preserving fp32 -> u32 map; -0 is folded onto +0 so that equal
+ * values tie (the reference compares integers, where -0 does not exist). */
+static uint32_t ord32(float v)
+{
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    if (u == 0x80000000u) u = 0;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+/* Host build of TD:572-627: for level p, key = (pos / a) << 32 | x1 (p even)
+ * or y1 (p odd) -- the exact host form TD:577-586, with ord32 for fp32 --
+ * stable sort, a /= f, then entry i of level p covers sorted [i*a,
+ * min(n, i*a+a)) with child = offset[p+1] + i*f (0 if empty, TD:591-613).
+ * Last level: leaf slot -pos-1, MBR of the rect at pos (TD:616-624); slots
+ * past n are 0.  Empty slots carry the empty box.  perm[pos] = original id
+ * of the rect at final sorted position pos.  Returns #slots. */
+int orc_td_build(const float *r4, int64_t n, int f, int32_t *slot_child,
+                 float *slot_mbr, uint32_t *perm)
+{
+    int64_t off[65];
+    int L = orc_td_levels(n, f, off);
+    if (L == 0) return -1;
+    int64_t a = 1;
+    for (int p = 0; p < L; ++p) a *= f;
+    uint32_t *cur = perm;
+    uint32_t *nxt = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    uint32_t *ord = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+    uint64_t *key = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    for (int64_t i = 0; i < n; ++i) cur[i] = (uint32_t)i;
+    for (int p = 0; p < L; ++p) {
+        int comp = (p & 1) ? 1 : 0;   /* x1 on even levels, y1 on odd */
+        for (int64_t i = 0; i < n; ++i)
+            key[i] = ((uint64_t)(i / a) << 32) | ord32(r4[4 * (int64_t)cur[i] + comp]);
+        orc_stable_order_u64(key, n, ord);
+        for (int64_t i = 0; i < n; ++i) nxt[i] = cur[ord[i]];
+        memcpy(cur, nxt, (size_t)n * sizeof(uint32_t));
+        a /= f;
+        for (int64_t i = 0; i < off[p + 1] - off[p]; ++i) {
+            int64_t s = off[p] + i;
+            int64_t b = i * a, e = b + a < n ? b + a : n;
+            float *m = slot_mbr + 4 * s;
+            set_empty(m);
+            if (b < n) {
+                slot_child[s] = (int32_t)(off[p + 1] + i * f);
+                for (int64_t k = b; k < e; ++k) merge_into(m, r4 + 4 * (int64_t)cur[k]);
+            } else {
+                slot_child[s] = 0;
+            }
+        }
+    }
+    for (int64_t i = 0; i < off[L] - off[L - 1]; ++i) {
+        int64_t s = off[L - 1] + i;
+        float *m = slot_mbr + 4 * s;
+        if (i < n) {
+            slot_child[s] = (int32_t)(-i - 1);
+            memcpy(m, r4 + 4 * (int64_t)cur[i], 4 * sizeof(float));
+        } else {
+            slot_child[s] = 0;
+            set_empty(m);
+        }
+    }
+    free(nxt);
+    free(ord);
+    free(key);
+    return (int)off[L];
+}
+
+/* ====================================================================== */
+/* Spatial join                                                            */
+/* ====================================================================== */
+
+/* Closed-interval overlap.  For the reference's integer rects the test
+ * (w_a + w_b) + 1e-8 < max(x2) - min(x1) => disjoint (BU:496-506,
+ * TD:26-39) is exactly a.x1 <= b.x2 && b.x1 <= a.x2 per axis: touching
+ * counts.  On fp32 the closed test is exact. */
+int orc_overlap(const float *a, const float *b)
+{
+    return a[0] <= b[2] && b[0] <= a[2] && a[1] <= b[3] && b[1] <= a[3];
+}
+
+static int cmp_u32(const void *x, const void *y)
+{
+    uint32_t a = *(const uint32_t *)x, b = *(const uint32_t *)y;
+    return (a > b) - (a < b);
+}
+
+typedef struct {
+    uint32_t *buf;
+    int64_t len, cap;
+} hitvec;
+
+static void hv_push(hitvec *h, uint32_t r)
+{
+    if (h->len == h->cap) {
+        h->cap = h->cap ? 2 * h->cap : 256;
+        h->buf = (uint32_t *)realloc(h->buf, (size_t)h->cap * sizeof(uint32_t));
+    }
+    h->buf[h->len++] = r;
+}
+
+/* Recursive search of BU:455-483: test every entry of an internal node and
+ * descend into overlapping children; at a leaf every overlapping entry is a
+ * hit.  `tested` counts entry tests (the reference's `num`). */
+static void bu_dfs(int64_t node, int f, int64_t num_leaves, const int32_t *cnt,
+                   const float *mbr, const int32_t *ref, const float *q,
+                   hitvec *h, int64_t *tested)
+{
+    for (int i = 0; i < cnt[node]; ++i) {
+        ++*tested;
+        const float *m = mbr + 4 * (node * f + i);
+        if (!orc_overlap(q, m)) continue;
+        if (node < num_leaves) hv_push(h, (uint32_t)ref[node * f + i]);
+        else bu_dfs(ref[node * f + i], f, num_leaves, cnt, mbr, ref, q, h, tested);
+    }
+}
+
+static int64_t emit_sorted(hitvec *h, uint32_t q, uint32_t *pairs, int64_t total, int64_t cap)
+{
+    qsort(h->buf, (size_t)h->len, sizeof(uint32_t), cmp_u32);
+    for (int64_t k = 0; k < h->len; ++k, ++total) {
+        if (total < cap) {
+            pairs[2 * total] = q;
+            pairs[2 * total + 1] = h->buf[k];
+        }
+    }
+    h->len = 0;
+    return total;
+}
+
+/* Query every window against the BU tree from the root (the last node,
+ * BU:867-869); pairs (q, r) sorted by (q, r).  Returns the pair count;
+ * writes at most cap pairs. */
+int64_t orc_join_bu(int64_t num_nodes, int f, int64_t num_leaves,
+                    const int32_t *node_count, const float *entry_mbr,
+                    const int32_t *entry_ref, const float *q4, int64_t nq,
+                    uint32_t *pairs, int64_t cap, int64_t *tested)
+{
+    hitvec h = {0, 0, 0};
+    int64_t total = 0, t = 0;
+    for (int64_t q = 0; q < nq; ++q) {
+        bu_dfs(num_nodes - 1, f, num_leaves, node_count, entry_mbr, entry_ref,
+               q4 + 4 * q, &h, &t);
+        total = emit_sorted(&h, (uint32_t)q, pairs, total, cap);
+    }
+    free(h.buf);
+    if (tested) *tested = t;
+    return total;
+}
+
+/* Recursive search of TD:56-76 over the implicit slot layout: a block of f
+ * slots; child > 0 is the next block, child < 0 a leaf at sorted position
+ * -child-1, 0 an empty slot. */
+static void td_dfs(int64_t index, int f, const int32_t *child, const float *mbr,
+                   const uint32_t *perm, const float *q, hitvec *h, int64_t *tested)
+{
+    for (int i = 0; i < f; ++i) {
+        int64_t s = index + i;
+        ++*tested;
+        if (child[s] == 0) continue;
+        if (!orc_overlap(mbr + 4 * s, q)) continue;
+        if (child[s] > 0) td_dfs(child[s], f, child, mbr, perm, q, h, tested);
+        else hv_push(h, perm[-(int64_t)child[s] - 1]);
+    }
+}
+
+int64_t orc_join_td(int f, const int32_t *slot_child, const float *slot_mbr,
+                    const uint32_t *perm, const float *q4, int64_t nq,
+                    uint32_t *pairs, int64_t cap, int64_t *tested)
+{
+    hitvec h = {0, 0, 0};
+    int64_t total = 0, t = 0;
+    for (int64_t q = 0; q < nq; ++q) {
+        td_dfs(0, f, slot_child, slot_mbr, perm, q4 + 4 * q, &h, &t);
+        total = emit_sorted(&h, (uint32_t)q, pairs, total, cap);
+    }
+    free(h.buf);
+    if (tested) *tested = t;
+    return total;
+}
+
+/* The hit set is tree independent: {(q, r) : overlap(q, r)}. */
+int64_t orc_join_brute(const float *r4, int64_t n, const float *q4, int64_t nq,
+                       uint32_t *pairs, int64_t cap)
+{
+    int64_t total = 0;
+    for (int64_t q = 0; q < nq; ++q)
+        for (int64_t r = 0; r < n; ++r)
+            if (orc_overlap(q4 + 4 * q, r4 + 4 * r)) {
+                if (total < cap) {
+                    pairs[2 * total] = (uint32_t)q;
+                    pairs[2 * total + 1] = (uint32_t)r;
+                }
+                ++total;
+            }
+    return total;
+}
+
+/* ====================================================================== */
+/* Polygon intersection area: GPC-style scanbeam sweep                     */
+/* ====================================================================== */
+/*
+ * The reference intersects each candidate pair with GPC (gpc_polygon_clip,
+ * OV:115-750, GPC_INT) and keeps the result contours.  The area of those
+ * contours is the area swept by the scanbeam loop, which is what this
+ * function accumulates:
+ *   - minimax pre-test (OV:1656-1709): disjoint closed bounding boxes give
+ *     an empty result;
+ *   - scanbeam table: the sorted distinct vertex y's (OV:1019-1085,
+ *     add_to_sbtree/build_sbt), restricted to the common y range;
+ *   - edges bottom->top with dx = (top.x-bot.x)/(top.y-bot.y)
+ *     (OV:1241-1242, 1287-1288); horizontal edges span no beam;
+ *   - edge x at a beam boundary: top.x when y == top.y, else
+ *     bot.x + dx*(y - bot.y) (OV:454-457);
+ *   - crossings inside a beam (OV:1395-1445): den = (xt_L-xb_L)-(xt_R-xb_R),
+ *     r = (xb_R-xb_L)/den, iy = r*dy + yb (OV:1431-1433, 480); the beam is
+ *     cut at every crossing height so that each sub-beam is crossing free;
+ *   - even-odd parity per operand (GPC's parity[], OV:140, 313-330): a gap
+ *     between two neighbouring edges belongs to the intersection when both
+ *     the subject and the clip parity are odd.
+ * Each such gap is a trapezoid; its doubled area
+ *   ((xa_R - xa_L) + (xc_R - xc_L)) * (yc - ya)
+ * is added, in order of beam, sub-beam, then left to right.  Area = half
+ * the sum.  Edges in a sub-beam are ordered by (xa+xc, xa, edge id), a
+ * strict total order, so the result does not depend on the input order of
+ * the active table.  Edge id = vertex index for P, nP + index for Q.
+ */
+
+typedef struct {
+    double bx, by, tx, ty, dx;
+    int id, poly;
+} cedge;
+
+static inline double edge_x(const cedge *e, double y)
+{
+    if (y == e->ty) return e->tx;
+    return e->bx + e->dx * (y - e->by);
+}
+
+static int cmp_double(const void *x, const void *y)
+{
+    double a = *(const double *)x, b = *(const double *)y;
+    return (a > b) - (a < b);
+}
+
+static int cmp_edge_by(const void *x, const void *y)
+{
+    const cedge *a = (const cedge *)x, *b = (const cedge *)y;
+    if (a->by != b->by) return a->by < b->by ? -1 : 1;
+    return a->id - b->id;
+}
+
+static void bbox(const double *P, int n, double *bb)
+{
+    bb[0] = bb[2] = P[0];
+    bb[1] = bb[3] = P[1];
+    for (int i = 1; i < n; ++i) {
+        double x = P[2 * i], y = P[2 * i + 1];
+        if (x < bb[0]) bb[0] = x;
+        if (x > bb[2]) bb[2] = x;
+        if (y < bb[1]) bb[1] = y;
+        if (y > bb[3]) bb[3] = y;
+    }
+}
+
+/* before(i, j): strict order (xa+xc, xa, id) used inside a sub-beam */
+static inline int sb_before(const double *km, const double *xa, const cedge *const *act,
+                            int i, int j)
+{
+    if (km[i] != km[j]) return km[i] < km[j];
+    if (xa[i] != xa[j]) return xa[i] < xa[j];
+    return act[i]->id < act[j]->id;
+}
+
+typedef struct {
+    cedge *edges;
+    double *ys;
+    const cedge **act;
+    double *xb, *xt, *xa, *xc, *km, *cy;
+    int *ord;
+    int cap_e, cap_y;
+} clip_ws;
+
+static void ws_reserve(clip_ws *w, int ne, int ny)
+{
+    if (ne > w->cap_e) {
+        w->cap_e = ne;
+        w->edges = (cedge *)realloc(w->edges, (size_t)ne * sizeof(cedge));
+        w->act = (const cedge **)realloc(w->act, (size_t)ne * sizeof(cedge *));
+        w->xb = (double *)realloc(w->xb, (size_t)ne * sizeof(double));
+        w->xt = (double *)realloc(w->xt, (size_t)ne * sizeof(double));
+        w->xa = (double *)realloc(w->xa, (size_t)ne * sizeof(double));
+        w->xc = (double *)realloc(w->xc, (size_t)ne * sizeof(double));
+        w->km = (double *)realloc(w->km, (size_t)ne * sizeof(double));
+        w->ord = (int *)realloc(w->ord, (size_t)ne * sizeof(int));
+        /* crossings per beam are bounded by m*(m-1)/2 <= ne*ne/2 */
+        w->cy = (double *)realloc(w->cy, ((size_t)ne * ne / 2 + 2) * sizeof(double));
+    }
+    if (ny > w->cap_y) {
+        w->cap_y = ny;
+        w->ys = (double *)realloc(w->ys, (size_t)ny * sizeof(double));
+    }
+}
+
+static void ws_free(clip_ws *w)
+{
+    free(w->edges); free(w->act); free(w->xb); free(w->xt); free(w->xa);
+    free(w->xc); free(w->km); free(w->ord); free(w->cy); free(w->ys);
+}
+
+/* One crossing-free sub-beam [ya, yc] with edge x's xa[], xc[] (indexed like
+ * act[]).  Returns the doubled-area contribution, summed left to right. */
+static double sub_beam(clip_ws *w, int m, double ya, double yc, double acc, uint64_t *fl)
+{
+    double h = yc - ya;
+    for (int i = 0; i < m; ++i) w->km[i] = w->xa[i] + w->xc[i];
+    for (int i = 0; i < m; ++i) w->ord[i] = i;
+    for (int i = 1; i < m; ++i) {           /* insertion sort, m is small */
+        int v = w->ord[i], j = i - 1;
+        while (j >= 0 && sb_before(w->km, w->xa, w->act, v, w->ord[j])) {
+            w->ord[j + 1] = w->ord[j];
+            --j;
+        }
+        w->ord[j + 1] = v;
+    }
+    int pp = 0, pq = 0;
+    uint64_t ops = 1 + (uint64_t)m;
+    for (int t = 0; t + 1 < m; ++t) {
+        int l = w->ord[t], r = w->ord[t + 1];
+        if (w->act[l]->poly == 0) pp ^= 1; else pq ^= 1;
+        if (pp && pq) {
+            acc += ((w->xa[r] - w->xa[l]) + (w->xc[r] - w->xc[l])) * h;
+            ops += 5;
+        }
+    }
+    if (fl) *fl += ops;
+    return acc;
+}
+
+static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q, int nQ,
+                           uint64_t *fl)
+{
+    if (nP < 3 || nQ < 3) return 0.0;
+    double bp[4], bq[4];
+    bbox(P, nP, bp);
+    bbox(Q, nQ, bq);
+    if (bp[2] < bq[0] || bq[2] < bp[0] || bp[3] < bq[1] || bq[3] < bp[1]) return 0.0;
+    double ylo = bp[1] > bq[1] ? bp[1] : bq[1];
+    double yhi = bp[3] < bq[3] ? bp[3] : bq[3];
+    if (!(ylo < yhi)) return 0.0;
+    ws_reserve(w, nP + nQ, nP + nQ);
+    uint64_t ops = 0;
+
+    /* edge table */
+    int ne = 0;
+    for (int poly = 0; poly < 2; ++poly) {
+        const double *V = poly ? Q : P;
+        int n = poly ? nQ : nP;
+        for (int i = 0; i < n; ++i) {
+            int j = i + 1 == n ? 0 : i + 1;
+            double x0 = V[2 * i], y0 = V[2 * i + 1], x1 = V[2 * j], y1 = V[2 * j + 1];
+            if (y0 == y1) continue;
+            cedge e;
+            if (y0 < y1) { e.bx = x0; e.by = y0; e.tx = x1; e.ty = y1; }
+            else { e.bx = x1; e.by = y1; e.tx = x0; e.ty = y0; }
+            if (e.ty <= ylo || e.by >= yhi) continue;
+            e.dx = (e.tx - e.bx) / (e.ty - e.by);
+            ops += 3;
+            e.id = poly ? nP + i : i;
+            e.poly = poly;
+            w->edges[ne++] = e;
+        }
+    }
+    qsort(w->edges, (size_t)ne, sizeof(cedge), cmp_edge_by);
+
+    /* scanbeam table: distinct vertex y's within [ylo, yhi] */
+    int ny = 0;
+    for (int poly = 0; poly < 2; ++poly) {
+        const double *V = poly ? Q : P;
+        int n = poly ? nQ : nP;
+        for (int i = 0; i < n; ++i) {
+            double y = V[2 * i + 1];
+            if (y >= ylo && y <= yhi) w->ys[ny++] = y;
+        }
+    }
+    qsort(w->ys, (size_t)ny, sizeof(double), cmp_double);
+    int u = 0;
+    for (int i = 0; i < ny; ++i)
+        if (u == 0 || w->ys[i] != w->ys[u - 1]) w->ys[u++] = w->ys[i];
+    ny = u;
+
+    double acc = 0.0;
+    int m = 0, next = 0;
+    for (int k = 0; k + 1 < ny; ++k) {
+        double yb = w->ys[k], yt = w->ys[k + 1];
+        while (next < ne && w->edges[next].by <= yb) w->act[m++] = &w->edges[next++];
+        int keep = 0;
+        for (int i = 0; i < m; ++i)
+            if (w->act[i]->ty > yb) w->act[keep++] = w->act[i];
+        m = keep;
+        if (m < 2) continue;
+        for (int i = 0; i < m; ++i) {
+            w->xb[i] = edge_x(w->act[i], yb);
+            w->xt[i] = edge_x(w->act[i], yt);
+        }
+        ops += 6 * (uint64_t)m;
+        /* crossing heights inside (yb, yt) */
+        int nc = 0;
+        double dy = yt - yb;
+        for (int i = 0; i < m; ++i) {
+            for (int j = i + 1; j < m; ++j) {
+                int L, R;
+                if (w->xb[i] < w->xb[j] && w->xt[i] > w->xt[j]) { L = i; R = j; }
+                else if (w->xb[j] < w->xb[i] && w->xt[j] > w->xt[i]) { L = j; R = i; }
+                else continue;
+                double den = (w->xt[L] - w->xb[L]) - (w->xt[R] - w->xb[R]);
+                ops += 3;
+                if (!(den > 0.0)) continue;
+                double r = (w->xb[R] - w->xb[L]) / den;
+                double iy = r * dy + yb;
+                ops += 4;
+                if (iy > yb && iy < yt) w->cy[nc++] = iy;
+            }
+        }
+        if (nc == 0) {
+            for (int i = 0; i < m; ++i) { w->xa[i] = w->xb[i]; w->xc[i] = w->xt[i]; }
+            acc = sub_beam(w, m, yb, yt, acc, &ops);
+            continue;
+        }
+        qsort(w->cy, (size_t)nc, sizeof(double), cmp_double);
+        u = 0;
+        for (int i = 0; i < nc; ++i)
+            if (u == 0 || w->cy[i] != w->cy[u - 1]) w->cy[u++] = w->cy[i];
+        nc = u;
+        double ya = yb;
+        for (int i = 0; i < m; ++i) w->xa[i] = w->xb[i];
+        for (int c = 0; c <= nc; ++c) {
+            double yc = c < nc ? w->cy[c] : yt;
+            if (c < nc) {
+                for (int i = 0; i < m; ++i) w->xc[i] = edge_x(w->act[i], yc);
+                ops += 3 * (uint64_t)m;
+            } else {
+                for (int i = 0; i < m; ++i) w->xc[i] = w->xt[i];
+            }
+            acc = sub_beam(w, m, ya, yc, acc, &ops);
+            for (int i = 0; i < m; ++i) w->xa[i] = w->xc[i];
+            ya = yc;
+        }
+    }
+    if (fl) *fl += ops + 1;
+    return 0.5 * acc;
+}
+
+double orc_clip_area(const double *P, int nP, const double *Q, int nQ, uint64_t *flops)
+{
+    clip_ws w;
+    memset(&w, 0, sizeof w);
+    double a = clip_area_ws(&w, P, nP, Q, nQ, flops);
+    ws_free(&w);
+    return a;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Independent checks of the area (different algorithms)                  */
+/* ---------------------------------------------------------------------- */
+
+/* Signed shoelace area, vertices taken relative to the first vertex. */
+double orc_poly_area(const double *P, int n)
+{
+    if (n < 3) return 0.0;
+    double ox = P[0], oy = P[1], s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        int j = i + 1 == n ? 0 : i + 1;
+        double xi = P[2 * i] - ox, yi = P[2 * i + 1] - oy;
+        double xj = P[2 * j] - ox, yj = P[2 * j + 1] - oy;
+        s += xi * yj - xj * yi;
+    }
+    return 0.5 * s;
+}
+
+/* Even-odd point in polygon (crossing number). */
+static int point_in(const double *B, int nB, double px, double py, double ox, double oy)
+{
+    int in = 0;
+    for (int i = 0; i < nB; ++i) {
+        int j = i + 1 == nB ? 0 : i + 1;
+        double xi = B[2 * i] - ox, yi = B[2 * i + 1] - oy;
+        double xj = B[2 * j] - ox, yj = B[2 * j + 1] - oy;
+        if ((yi > py) != (yj > py)) {
+            double xc = (xj - xi) * (py - yi) / (yj - yi) + xi;
+            if (px < xc) in ^= 1;
+        }
+    }
+    return in;
+}
+
+/* Parameters t in (0,1) where edge a->b of A crosses the boundary of B,
+ * sorted and bracketed by 0 and 1.  Coordinates relative to (ox, oy). */
+static int edge_cuts(double ax, double ay, double bx, double by, const double *B, int nB,
+                     double ox, double oy, double *ts)
+{
+    int nt = 0;
+    ts[nt++] = 0.0;
+    double rx = bx - ax, ry = by - ay;
+    for (int i = 0; i < nB; ++i) {
+        int j = i + 1 == nB ? 0 : i + 1;
+        double cx = B[2 * i] - ox, cy = B[2 * i + 1] - oy;
+        double sx = B[2 * j] - ox - cx, sy = B[2 * j + 1] - oy - cy;
+        double den = rx * sy - ry * sx;
+        if (den == 0.0) continue;
+        double qx = cx - ax, qy = cy - ay;
+        double t = (qx * sy - qy * sx) / den;
+        double u = (qx * ry - qy * rx) / den;
+        /* generous on u: a crossing at a vertex of B may be found twice (a
+         * zero-length part, harmless) but is never lost to rounding */
+        if (t > 0.0 && t < 1.0 && u >= -1e-9 && u <= 1.0 + 1e-9) ts[nt++] = t;
+    }
+    ts[nt++] = 1.0;
+    qsort(ts + 1, (size_t)(nt - 2), sizeof(double), cmp_double);
+    return nt;
+}
+
+/* Green's theorem: area(A cap B) = sum over the parts of dA inside B and the
+ * parts of dB inside A of  integral x dy  (each polygon taken
+ * counter-clockwise).  Inside-ness of each part is decided by a point test
+ * at its midpoint. */
+static double green_part(const double *A, int nA, const double *B, int nB,
+                         double ox, double oy, double *ts)
+{
+    double s = 0.0;
+    for (int i = 0; i < nA; ++i) {
+        int j = i + 1 == nA ? 0 : i + 1;
+        double ax = A[2 * i] - ox, ay = A[2 * i + 1] - oy;
+        double bx = A[2 * j] - ox, by = A[2 * j + 1] - oy;
+        int nt = edge_cuts(ax, ay, bx, by, B, nB, ox, oy, ts);
+        for (int k = 0; k + 1 < nt; ++k) {
+            double t0 = ts[k], t1 = ts[k + 1];
+            if (!(t1 > t0)) continue;
+            double tm = 0.5 * (t0 + t1);
+            if (!point_in(B, nB, ax + tm * (bx - ax), ay + tm * (by - ay), ox, oy)) continue;
+            double x0 = ax + t0 * (bx - ax), y0 = ay + t0 * (by - ay);
+            double x1 = ax + t1 * (bx - ax), y1 = ay + t1 * (by - ay);
+            s += 0.5 * (x0 + x1) * (y1 - y0);
+        }
+    }
+    return s;
+}
+
+double orc_green_area(const double *P, int nP, const double *Q, int nQ)
+{
+    if (nP < 3 || nQ < 3) return 0.0;
+    double bp[4];
+    bbox(P, nP, bp);
+    double ox = 0.5 * (bp[0] + bp[2]), oy = 0.5 * (bp[1] + bp[3]);
+    double *ts = (double *)malloc((size_t)(nP + nQ + 2) * sizeof(double));
+    double sp = orc_poly_area(P, nP) < 0 ? -1.0 : 1.0;
+    double sq = orc_poly_area(Q, nQ) < 0 ? -1.0 : 1.0;
+    double a = sp * green_part(P, nP, Q, nQ, ox, oy, ts) + sq * green_part(Q, nQ, P, nP, ox, oy, ts);
+    free(ts);
+    return a;
+}
+
+/* Sutherland-Hodgman: clip P by each edge half-plane of the convex polygon
+ * Q (orientation detected), shoelace area of the result. */
+double orc_sh_convex_area(const double *P, int nP, const double *Q, int nQ)
+{
+    if (nP < 3 || nQ < 3) return 0.0;
+    int cap = 2 * (nP + nQ) + 8;
+    double *a = (double *)malloc((size_t)cap * 2 * sizeof(double));
+    double *b = (double *)malloc((size_t)cap * 2 * sizeof(double));
+    double ox = Q[0], oy = Q[1];
+    int na = nP;
+    for (int i = 0; i < nP; ++i) { a[2 * i] = P[2 * i] - ox; a[2 * i + 1] = P[2 * i + 1] - oy; }
+    double orient = orc_poly_area(Q, nQ) < 0 ? -1.0 : 1.0;
+    for (int e = 0; e < nQ && na > 0; ++e) {
+        int f = e + 1 == nQ ? 0 : e + 1;
+        double cx = Q[2 * e] - ox, cy = Q[2 * e + 1] - oy;
+        double dx = Q[2 * f] - ox - cx, dy = Q[2 * f + 1] - oy - cy;
+        int nb = 0;
+        for (int i = 0; i < na; ++i) {
+            int j = i + 1 == na ? 0 : i + 1;
+            double px = a[2 * i], py = a[2 * i + 1], qx = a[2 * j], qy = a[2 * j + 1];
+            double sp = orient * (dx * (py - cy) - dy * (px - cx));
+            double sq = orient * (dx * (qy - cy) - dy * (qx - cx));
+            if (sp >= 0) { b[2 * nb] = px; b[2 * nb + 1] = py; ++nb; }
+            if ((sp >= 0) != (sq >= 0)) {
+                double t = sp / (sp - sq);
+                b[2 * nb] = px + t * (qx - px);
+                b[2 * nb + 1] = py + t * (qy - py);
+                ++nb;
+            }
+        }
+        double *t = a; a = b; b = t;
+        na = nb;
+    }
+    double area = na >= 3 ? fabs(orc_poly_area(a, na)) : 0.0;
+    free(a);
+    free(b);
+    return area;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Gradient of the intersection area with the clipping topology fixed     */
+/* ---------------------------------------------------------------------- */
+/*
+ * dArea = integral over (dA inside B) of V.n ds, plus the same for B.  For
+ * edge i of A (v_i -> v_{i+1}, direction d), the parts inside B form a set
+ * I of t in [0,1]; with the outward normal N = s*(d.y, -d.x) (s = +1 for a
+ * counter-clockwise polygon, -1 otherwise)
+ *   dArea/dv_i     += N * integral_I (1 - t) dt
+ *   dArea/dv_{i+1} += N * integral_I t dt
+ * The reference has no gradient; this is the build's definition.
+ */
+static void grad_part(const double *A, int nA, const double *B, int nB, double *gA,
+                      double ox, double oy, double *ts)
+{
+    double s = orc_poly_area(A, nA) < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < 2 * nA; ++i) gA[i] = 0.0;
+    for (int i = 0; i < nA; ++i) {
+        int j = i + 1 == nA ? 0 : i + 1;
+        double ax = A[2 * i] - ox, ay = A[2 * i + 1] - oy;
+        double bx = A[2 * j] - ox, by = A[2 * j + 1] - oy;
+        int nt = edge_cuts(ax, ay, bx, by, B, nB, ox, oy, ts);
+        double j0 = 0.0, j1 = 0.0;
+        for (int k = 0; k + 1 < nt; ++k) {
+            double t0 = ts[k], t1 = ts[k + 1];
+            if (!(t1 > t0)) continue;
+            double tm = 0.5 * (t0 + t1);
+            if (!point_in(B, nB, ax + tm * (bx - ax), ay + tm * (by - ay), ox, oy)) continue;
+            double sq = 0.5 * (t1 * t1 - t0 * t0);
+            j0 += (t1 - t0) - sq;
+            j1 += sq;
+        }
+        double nx = s * (by - ay), ny = -s * (bx - ax);
+        gA[2 * i] += j0 * nx;
+        gA[2 * i + 1] += j0 * ny;
+        gA[2 * j] += j1 * nx;
+        gA[2 * j + 1] += j1 * ny;
+    }
+}
+
+void orc_area_grad(const double *P, int nP, const double *Q, int nQ, double *gP, double *gQ)
+{
+    if (nP < 3 || nQ < 3) {
+        for (int i = 0; i < 2 * nP; ++i) gP[i] = 0.0;
+        for (int i = 0; i < 2 * nQ; ++i) gQ[i] = 0.0;
+        return;
+    }
+    double bp[4];
+    bbox(P, nP, bp);
+    double ox = 0.5 * (bp[0] + bp[2]), oy = 0.5 * (bp[1] + bp[3]);
+    double *ts = (double *)malloc((size_t)(nP + nQ + 2) * sizeof(double));
+    grad_part(P, nP, Q, nQ, gP, ox, oy, ts);
+    grad_part(Q, nQ, P, nP, gQ, ox, oy, ts);
+    free(ts);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Batch driver (the Execute loop of OV:81-113 on the CPU)                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    const uint32_t *pairs;
+    const double *xyA, *xyB;
+    const int64_t *offA, *offB;
+    double *area;
+    int64_t lo, hi;
+    uint64_t flops;
+} ov_job;
+
+static void *ov_run(void *arg)
+{
+    ov_job *j = (ov_job *)arg;
+    clip_ws w;
+    memset(&w, 0, sizeof w);
+    uint64_t fl = 0;
+    for (int64_t t = j->lo; t < j->hi; ++t) {
+        uint32_t a = j->pairs[2 * t], b = j->pairs[2 * t + 1];
+        const double *P = j->xyA + 2 * j->offA[a];
+        const double *Q = j->xyB + 2 * j->offB[b];
+        int nP = (int)(j->offA[a + 1] - j->offA[a]);
+        int nQ = (int)(j->offB[b + 1] - j->offB[b]);
+        j->area[t] = clip_area_ws(&w, P, nP, Q, nQ, &fl);
+    }
+    j->flops = fl;
+    ws_free(&w);
+    return 0;
+}
+
+/* pairs[2t] indexes layer A (base), pairs[2t+1] layer B (overlay). */
+void orc_overlay(const uint32_t *pairs, int64_t np, const double *xyA, const int64_t *offA,
+                 const double *xyB, const int64_t *offB, double *area, uint64_t *flops,
+                 int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    ov_job jobs[256];
+    pthread_t th[256];
+    for (int k = 0; k < nthreads; ++k) {
+        jobs[k].pairs = pairs; jobs[k].xyA = xyA; jobs[k].xyB = xyB;
+        jobs[k].offA = offA; jobs[k].offB = offB; jobs[k].area = area;
+        jobs[k].lo = np * k / nthreads; jobs[k].hi = np * (k + 1) / nthreads;
+        jobs[k].flops = 0;
+    }
+    if (nthreads == 1) ov_run(&jobs[0]);
+    else {
+        for (int k = 0; k < nthreads; ++k) pthread_create(&th[k], 0, ov_run, &jobs[k]);
+        for (int k = 0; k < nthreads; ++k) pthread_join(th[k], 0);
+    }
+    uint64_t fl = 0;
+    for (int k = 0; k < nthreads; ++k) fl += jobs[k].flops;
+    if (flops) *flops = fl;
+}

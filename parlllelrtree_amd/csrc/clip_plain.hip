@@ -1,0 +1,308 @@
+// clip_plain.hip -- plain per-pair FP64 polygon intersection area, one lane
+// per pair (the launch shape of the reference's Execute kernel,
+// vector_map_overlay_gpc_kernel.cu:81-113, <<<ceil(P/128),128>>>).
+//
+// The reference runs GPC's scanbeam clipper (gpc_polygon_clip, :115-750)
+// per thread with the device heap allocator.  This kernel runs the same
+// scanbeam sweep but accumulates the area of the intersection instead of
+// building contours, with no allocation: each lane owns a slice of a
+// caller-provided arena (arrays interleaved by lane so that lanes touching
+// the same index hit neighbouring addresses).
+//
+// Arithmetic (compiled with -ffp-contract=off; each expression is evaluated
+// exactly as written, so results are bit-identical to the FP64 CPU oracle):
+//   dx    = (top.x - bot.x) / (top.y - bot.y)                 (:1241-1242)
+//   x(y)  = y == top.y ? top.x : bot.x + dx * (y - bot.y)     (:454-457)
+//   den   = (xt_L - xb_L) - (xt_R - xb_R), r = (xb_R - xb_L) / den,
+//   iy    = r * (yt - yb) + yb                                (:1413-1433, 480)
+//   term  = ((xa_R - xa_L) + (xc_R - xc_L)) * (yc - ya), area = 0.5 * sum
+// Sub-beam boundaries are the distinct crossing heights in (yb, yt), taken
+// in increasing order; inside a sub-beam, edges are ordered by
+// (xa + xc, xa, edge id); a gap is inside when both even-odd parities
+// (subject = A, clip = B) are odd.
+#include "prt_common.hpp"
+#include "prt_profile.hpp"
+
+namespace prt {
+namespace {
+
+constexpr int CP_THREADS = 128;
+constexpr int64_t CP_MAX_LANES = 65536;
+constexpr int CP_BYTES_PER_SLOT = 11 * 8 + 4 * 4;  // 11 double + 4 int arrays
+
+struct PlainWs {
+    double *bx, *by, *tx, *ty, *dx, *ys, *xb, *xt, *xa, *xc, *spare;
+    int32_t *id, *eord, *act, *ord;
+    int64_t T;
+    int cap;
+};
+
+struct Lane {
+    const PlainWs &w;
+    int64_t t;
+    __device__ Lane(const PlainWs &ws, int64_t lane) : w(ws), t(lane) {}
+    template <class T> __device__ T &at(T *base, int64_t k) const { return base[k * w.T + t]; }
+};
+
+__device__ inline double edge_x(const Lane &L, int e, double y) {
+    if (y == L.at(L.w.ty, e)) return L.at(L.w.tx, e);
+    return L.at(L.w.bx, e) + L.at(L.w.dx, e) * (y - L.at(L.w.by, e));
+}
+
+// heap sort of eord[0..n) by (by, id)
+__device__ inline bool edge_less(const Lane &L, int a, int b) {
+    double ya = L.at(L.w.by, a), yb = L.at(L.w.by, b);
+    if (ya != yb) return ya < yb;
+    return L.at(L.w.id, a) < L.at(L.w.id, b);
+}
+
+__device__ void sift_edges(const Lane &L, int root, int n) {
+    for (;;) {
+        int c = 2 * root + 1;
+        if (c >= n) return;
+        if (c + 1 < n && edge_less(L, L.at(L.w.eord, c), L.at(L.w.eord, c + 1))) ++c;
+        int rv = L.at(L.w.eord, root), cv = L.at(L.w.eord, c);
+        if (!edge_less(L, rv, cv)) return;
+        L.at(L.w.eord, root) = cv;
+        L.at(L.w.eord, c) = rv;
+        root = c;
+    }
+}
+
+__device__ void sort_edges(const Lane &L, int n) {
+    for (int i = n / 2 - 1; i >= 0; --i) sift_edges(L, i, n);
+    for (int end = n - 1; end > 0; --end) {
+        int v = L.at(L.w.eord, 0);
+        L.at(L.w.eord, 0) = L.at(L.w.eord, end);
+        L.at(L.w.eord, end) = v;
+        sift_edges(L, 0, end);
+    }
+}
+
+__device__ void sift_ys(const Lane &L, int root, int n) {
+    for (;;) {
+        int c = 2 * root + 1;
+        if (c >= n) return;
+        if (c + 1 < n && L.at(L.w.ys, c) < L.at(L.w.ys, c + 1)) ++c;
+        double rv = L.at(L.w.ys, root), cv = L.at(L.w.ys, c);
+        if (!(rv < cv)) return;
+        L.at(L.w.ys, root) = cv;
+        L.at(L.w.ys, c) = rv;
+        root = c;
+    }
+}
+
+__device__ void sort_ys(const Lane &L, int n) {
+    for (int i = n / 2 - 1; i >= 0; --i) sift_ys(L, i, n);
+    for (int end = n - 1; end > 0; --end) {
+        double v = L.at(L.w.ys, 0);
+        L.at(L.w.ys, 0) = L.at(L.w.ys, end);
+        L.at(L.w.ys, end) = v;
+        sift_ys(L, 0, end);
+    }
+}
+
+// strict order (xa + xc, xa, id) of active slots i, j
+__device__ inline bool sb_before(const Lane &L, int i, int j) {
+    double ki = L.at(L.w.xa, i) + L.at(L.w.xc, i);
+    double kj = L.at(L.w.xa, j) + L.at(L.w.xc, j);
+    if (ki != kj) return ki < kj;
+    double ai = L.at(L.w.xa, i), aj = L.at(L.w.xa, j);
+    if (ai != aj) return ai < aj;
+    return L.at(L.w.id, L.at(L.w.act, i)) < L.at(L.w.id, L.at(L.w.act, j));
+}
+
+__device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, double acc) {
+    const double h = yc - ya;
+    for (int i = 0; i < m; ++i) L.at(L.w.ord, i) = i;
+    for (int i = 1; i < m; ++i) {
+        int v = L.at(L.w.ord, i), j = i - 1;
+        while (j >= 0 && sb_before(L, v, L.at(L.w.ord, j))) {
+            L.at(L.w.ord, j + 1) = L.at(L.w.ord, j);
+            --j;
+        }
+        L.at(L.w.ord, j + 1) = v;
+    }
+    int pp = 0, pq = 0;
+    for (int t = 0; t + 1 < m; ++t) {
+        int l = L.at(L.w.ord, t), r = L.at(L.w.ord, t + 1);
+        if (L.at(L.w.id, L.at(L.w.act, l)) < nP) pp ^= 1;
+        else pq ^= 1;
+        if (pp && pq)
+            acc += ((L.at(L.w.xa, r) - L.at(L.w.xa, l)) + (L.at(L.w.xc, r) - L.at(L.w.xc, l))) * h;
+    }
+    return acc;
+}
+
+__device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
+                           const double *__restrict__ Q, int nQ) {
+    if (nP < 3 || nQ < 3) return 0.0;
+    if (nP + nQ > L.w.cap) return __builtin_nan("");
+    double bp[4] = {P[0], P[1], P[0], P[1]}, bq[4] = {Q[0], Q[1], Q[0], Q[1]};
+    for (int i = 1; i < nP; ++i) {
+        double x = P[2 * i], y = P[2 * i + 1];
+        bp[0] = x < bp[0] ? x : bp[0];
+        bp[2] = x > bp[2] ? x : bp[2];
+        bp[1] = y < bp[1] ? y : bp[1];
+        bp[3] = y > bp[3] ? y : bp[3];
+    }
+    for (int i = 1; i < nQ; ++i) {
+        double x = Q[2 * i], y = Q[2 * i + 1];
+        bq[0] = x < bq[0] ? x : bq[0];
+        bq[2] = x > bq[2] ? x : bq[2];
+        bq[1] = y < bq[1] ? y : bq[1];
+        bq[3] = y > bq[3] ? y : bq[3];
+    }
+    if (bp[2] < bq[0] || bq[2] < bp[0] || bp[3] < bq[1] || bq[3] < bp[1]) return 0.0;
+    const double ylo = bp[1] > bq[1] ? bp[1] : bq[1];
+    const double yhi = bp[3] < bq[3] ? bp[3] : bq[3];
+    if (!(ylo < yhi)) return 0.0;
+
+    int ne = 0, ny = 0;
+    for (int poly = 0; poly < 2; ++poly) {
+        const double *V = poly ? Q : P;
+        const int n = poly ? nQ : nP;
+        for (int i = 0; i < n; ++i) {
+            const int j = i + 1 == n ? 0 : i + 1;
+            const double x0 = V[2 * i], y0 = V[2 * i + 1], x1 = V[2 * j], y1 = V[2 * j + 1];
+            if (y0 >= ylo && y0 <= yhi) L.at(L.w.ys, ny++) = y0;
+            if (y0 == y1) continue;
+            double bx, by, tx, ty;
+            if (y0 < y1) { bx = x0; by = y0; tx = x1; ty = y1; }
+            else { bx = x1; by = y1; tx = x0; ty = y0; }
+            if (ty <= ylo || by >= yhi) continue;
+            L.at(L.w.bx, ne) = bx;
+            L.at(L.w.by, ne) = by;
+            L.at(L.w.tx, ne) = tx;
+            L.at(L.w.ty, ne) = ty;
+            L.at(L.w.dx, ne) = (tx - bx) / (ty - by);
+            L.at(L.w.id, ne) = poly ? nP + i : i;
+            L.at(L.w.eord, ne) = ne;
+            ++ne;
+        }
+    }
+    sort_edges(L, ne);
+    sort_ys(L, ny);
+    int u = 0;
+    for (int i = 0; i < ny; ++i) {
+        double y = L.at(L.w.ys, i);
+        if (u == 0 || y != L.at(L.w.ys, u - 1)) L.at(L.w.ys, u++) = y;
+    }
+    ny = u;
+
+    double acc = 0.0;
+    int m = 0, next = 0;
+    for (int k = 0; k + 1 < ny; ++k) {
+        const double yb = L.at(L.w.ys, k), yt = L.at(L.w.ys, k + 1);
+        while (next < ne && L.at(L.w.by, L.at(L.w.eord, next)) <= yb)
+            L.at(L.w.act, m++) = L.at(L.w.eord, next++);
+        int keep = 0;
+        for (int i = 0; i < m; ++i) {
+            int e = L.at(L.w.act, i);
+            if (L.at(L.w.ty, e) > yb) L.at(L.w.act, keep++) = e;
+        }
+        m = keep;
+        if (m < 2) continue;
+        for (int i = 0; i < m; ++i) {
+            int e = L.at(L.w.act, i);
+            double b = edge_x(L, e, yb), t = edge_x(L, e, yt);
+            L.at(L.w.xb, i) = b;
+            L.at(L.w.xt, i) = t;
+            L.at(L.w.xa, i) = b;
+        }
+        const double dy = yt - yb;
+        double ya = yb;
+        for (;;) {
+            // next distinct crossing height above ya (or yt)
+            double yc = yt;
+            for (int i = 0; i < m; ++i) {
+                const double xbi = L.at(L.w.xb, i), xti = L.at(L.w.xt, i);
+                for (int j = i + 1; j < m; ++j) {
+                    const double xbj = L.at(L.w.xb, j), xtj = L.at(L.w.xt, j);
+                    double xbL, xtL, xbR, xtR;
+                    if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+                    else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+                    else continue;
+                    const double den = (xtL - xbL) - (xtR - xbR);
+                    if (!(den > 0.0)) continue;
+                    const double r = (xbR - xbL) / den;
+                    const double iy = r * dy + yb;
+                    if (iy > ya && iy < yc) yc = iy;
+                }
+            }
+            if (yc == yt) {
+                for (int i = 0; i < m; ++i) L.at(L.w.xc, i) = L.at(L.w.xt, i);
+            } else {
+                for (int i = 0; i < m; ++i) L.at(L.w.xc, i) = edge_x(L, L.at(L.w.act, i), yc);
+            }
+            acc = sub_beam(L, m, nP, ya, yc, acc);
+            if (yc == yt) break;
+            for (int i = 0; i < m; ++i) L.at(L.w.xa, i) = L.at(L.w.xc, i);
+            ya = yc;
+        }
+    }
+    return 0.5 * acc;
+}
+
+__global__ __launch_bounds__(CP_THREADS) void k_clip_plain(
+    const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
+    const int64_t *__restrict__ offA, const double *__restrict__ xyB,
+    const int64_t *__restrict__ offB, double *__restrict__ area, PlainWs w) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= w.T) return;
+    Lane L(w, t);
+    for (int64_t p = t; p < np; p += w.T) {
+        const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int64_t oa = offA[a], ob = offB[b];
+        const int nA = static_cast<int>(offA[a + 1] - oa);
+        const int nB = static_cast<int>(offB[b + 1] - ob);
+        area[p] = clip_one(L, xyA + 2 * oa, nA, xyB + 2 * ob, nB);
+    }
+}
+
+int64_t plain_lanes(int64_t np) {
+    int64_t T = np < CP_MAX_LANES ? np : CP_MAX_LANES;
+    return T < 1 ? 1 : T;
+}
+
+}  // namespace
+}  // namespace prt
+
+using namespace prt;
+
+extern "C" size_t prt_overlay_plain_workspace_size(int64_t npairs, int32_t max_verts) {
+    if (npairs < 0 || max_verts < 0) return 0;
+    const int64_t T = plain_lanes(npairs);
+    const int64_t cap = max_verts < 3 ? 3 : max_verts;
+    return static_cast<size_t>(T) * static_cast<size_t>(cap) * CP_BYTES_PER_SLOT + 16 * 256;
+}
+
+extern "C" int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                 const int64_t *offA, const double *xyB, const int64_t *offB,
+                                 int32_t max_verts, double *area, void *ws, size_t ws_bytes,
+                                 void *stream) {
+    PRT_REQUIRE(npairs >= 0, PRT_E_INVAL, "overlay_plain: npairs < 0");
+    if (npairs == 0) return PRT_OK;
+    PRT_REQUIRE(pairs && xyA && offA && xyB && offB && area, PRT_E_INVAL,
+                "overlay_plain: null device pointer");
+    PRT_REQUIRE(max_verts >= 0 && max_verts <= 2 * PRT_MAX_POLY_VERTS, PRT_E_INVAL,
+                "overlay_plain: max_verts %d out of range", max_verts);
+    const size_t need = prt_overlay_plain_workspace_size(npairs, max_verts);
+    PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "overlay_plain: workspace %zu < %zu bytes", ws_bytes,
+                need);
+    PlainWs w;
+    w.T = plain_lanes(npairs);
+    w.cap = max_verts < 3 ? 3 : max_verts;
+    Carver c(ws, ws_bytes);
+    const size_t cnt = static_cast<size_t>(w.T) * w.cap;
+    double **dptr[] = {&w.bx, &w.by, &w.tx, &w.ty, &w.dx, &w.ys, &w.xb, &w.xt, &w.xa, &w.xc, &w.spare};
+    for (double **p : dptr) *p = c.take<double>(cnt);
+    int32_t **iptr[] = {&w.id, &w.eord, &w.act, &w.ord};
+    for (int32_t **p : iptr) *p = c.take<int32_t>(cnt);
+    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "overlay_plain: workspace too small");
+    const unsigned grid = static_cast<unsigned>(ceil_div(w.T, CP_THREADS));
+    PRT_LAUNCH(k_clip_plain, grid, CP_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA, xyB,
+               offB, area, w);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}

@@ -1,0 +1,128 @@
+// prt_common.hpp -- shared host/device helpers of libprt (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/prt.h"
+
+namespace prt {
+
+// ---- error reporting (per host thread) ---------------------------------
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define PRT_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t _e = (call);                                                \
+        if (_e != hipSuccess) return ::prt::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define PRT_LAUNCHED()                                                         \
+    do {                                                                       \
+        hipError_t _e = hipGetLastError();                                     \
+        if (_e != hipSuccess) return ::prt::hip_fail(_e, "kernel launch", __FILE__, __LINE__); \
+    } while (0)
+
+#define PRT_REQUIRE(cond, code, ...)                                           \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::prt::set_error(__VA_ARGS__);                                     \
+            return code;                                                       \
+        }                                                                      \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---- workspace carving: 256-byte aligned sub-buffers --------------------
+struct Carver {
+    char *base;
+    size_t used = 0;
+    size_t cap;
+    Carver(void *p, size_t bytes) : base(static_cast<char *>(p)), cap(bytes) {}
+    template <class T> T *take(size_t count) {
+        size_t off = (used + 255) & ~size_t(255);
+        size_t need = off + count * sizeof(T);
+        used = need;
+        if (base == nullptr || need > cap) return nullptr;
+        return reinterpret_cast<T *>(base + off);
+    }
+    bool ok() const { return base != nullptr && used <= cap; }
+};
+// size-only twin of Carver, used by the *_workspace_size functions
+struct Sizer {
+    size_t used = 0;
+    template <class T> void take(size_t count) {
+        used = ((used + 255) & ~size_t(255)) + count * sizeof(T);
+    }
+    size_t bytes() const { return used + 256; }
+};
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- key helpers (host and device) --------------------------------------
+
+// Order-preserving fp32 -> u32; -0 folds onto +0.
+__host__ __device__ inline uint32_t ord32(float v) {
+    uint32_t u = __builtin_bit_cast(uint32_t, v);
+    if (u == 0x80000000u) u = 0u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Centre quantisation of one axis: floor(0.5f*(lo+hi) * 2^15) in [0, 32767].
+__host__ __device__ inline uint32_t quant15(float lo, float hi) {
+    float c = 0.5f * (lo + hi);
+    float s = c * 32768.0f;
+    if (!(s >= 0.0f)) return 0u;
+    if (s >= 32767.0f) return 32767u;
+    return static_cast<uint32_t>(s);
+}
+
+__host__ __device__ inline uint32_t spread15(uint32_t x) {
+    x &= 0x7FFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+__host__ __device__ inline uint32_t morton_center30(float4 r) {
+    return spread15(quant15(r.x, r.z)) | (spread15(quant15(r.y, r.w)) << 1);
+}
+
+// Closed-interval MBR overlap (touching counts).
+__host__ __device__ inline bool overlap(float4 a, float4 b) {
+    return a.x <= b.z && b.x <= a.z && a.y <= b.w && b.y <= a.w;
+}
+
+__host__ __device__ inline float4 empty_box() {
+    return make_float4(__builtin_huge_valf(), __builtin_huge_valf(), -__builtin_huge_valf(),
+                       -__builtin_huge_valf());
+}
+
+__host__ __device__ inline float4 box_union(float4 a, float4 b) {
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+}
+
+// ---- internal launchers shared between translation units ---------------
+// Stable LSD radix sort of 64-bit items on bits [bit_lo, bit_hi), segmented:
+// items never leave their segment [s*seg, (s+1)*seg).  Returns the buffer
+// holding the result (one of buf0/buf1; `in` may alias buf0).
+struct RadixWs {
+    uint32_t *hist;      // [256 * tiles]
+    uint32_t *offs;      // [256 * tiles]
+    uint32_t *partials;  // scan partial sums
+};
+size_t radix_ws_bytes(int64_t n, int64_t seg);
+int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w);
+int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n, int64_t seg,
+                   int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s, uint64_t **result);
+
+// Exclusive scan of int64 counts; out[i] = sum_{j<i} in[j]; *total = sum.
+size_t scan_i64_ws_bytes(int64_t n);
+int scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total, void *ws,
+             hipStream_t s);
+
+}  // namespace prt

@@ -1,0 +1,224 @@
+"""Host-side mirror of the reference's R-tree / overlay operators on torch
+device tensors, calling libprt through its C-ABI.
+
+Reference operators (BU = BotomUp-rtree.cu, TD = TopDown-rtree.cu,
+OV = vector_map_overlay_gpc_kernel.cu):
+  build_bottom_up   <- the GPU build of BU main (:836-869)
+  build_top_down    <- the GPU build of TD main (:629-679)
+  join              <- the query stage (BU:914-987, TD:694-758) with the hit
+                       semantics of RtreeSearch (BU:439-483, TD:56-76)
+  overlay_plain     <- run/Execute (OV:61-113)
+Errors raise PrtError (the reference prints and exits, OV:25-29).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import torch
+
+from ._lib import BUTreeStruct, TDTreeStruct, check, lib
+
+__all__ = ["BottomUpTree", "TopDownTree", "build_bottom_up", "build_top_down", "join",
+           "overlay_plain", "PolyDev", "to_device_polys", "morton_center30", "sort_pairs_u32"]
+
+
+def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ws(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def _cuda_f32_rects(rects: torch.Tensor) -> torch.Tensor:
+    if not (rects.is_cuda and rects.dtype == torch.float32 and rects.dim() == 2
+            and rects.shape[1] == 4):
+        raise ValueError("rects must be a (n, 4) float32 CUDA tensor (xmin, ymin, xmax, ymax)")
+    return rects.contiguous()
+
+
+@dataclass
+class BottomUpTree:
+    st: BUTreeStruct
+    keys: torch.Tensor        # int32 view of u32 keys, sorted
+    perm: torch.Tensor        # int32 (u32 rect ids) in leaf order
+    node_count: torch.Tensor
+    node_start: torch.Tensor
+    node_end: torch.Tensor
+    entry_mbr: torch.Tensor   # (num_nodes * fanout, 4) float32
+    entry_ref: torch.Tensor
+    workspace: torch.Tensor | None = field(default=None, repr=False)
+
+    @property
+    def num_nodes(self) -> int:
+        return self.st.num_nodes
+
+    @property
+    def root(self) -> int:
+        return self.st.num_nodes - 1
+
+    @property
+    def levels(self) -> list[tuple[int, int]]:
+        return [(self.st.level_start[i], self.st.level_count[i]) for i in range(self.st.num_levels)]
+
+
+@dataclass
+class TopDownTree:
+    st: TDTreeStruct
+    slot_child: torch.Tensor
+    slot_mbr: torch.Tensor    # (num_slots, 4) float32
+    perm: torch.Tensor
+    workspace: torch.Tensor | None = field(default=None, repr=False)
+
+    @property
+    def level_offset(self) -> list[int]:
+        return [self.st.level_offset[i] for i in range(self.st.num_levels + 1)]
+
+
+def morton_center30(rects: torch.Tensor) -> torch.Tensor:
+    rects = _cuda_f32_rects(rects)
+    keys = torch.empty(rects.shape[0], dtype=torch.int32, device=rects.device)
+    check(lib().prt_morton_center30(_ptr(rects), rects.shape[0], _ptr(keys), _stream()))
+    return keys
+
+
+def sort_pairs_u32(keys: torch.Tensor, key_bits: int = 32) -> tuple[torch.Tensor, torch.Tensor]:
+    n = keys.shape[0]
+    out_k = torch.empty_like(keys)
+    out_i = torch.empty(n, dtype=torch.int32, device=keys.device)
+    ws = _ws(lib().prt_sort_workspace_size(n), keys.device)
+    check(lib().prt_sort_pairs_u32(_ptr(keys), n, key_bits, _ptr(out_k), _ptr(out_i), _ptr(ws),
+                                   ws.numel(), _stream()))
+    return out_k, out_i
+
+
+def build_bottom_up(rects: torch.Tensor, fanout: int = 16, workspace: torch.Tensor | None = None
+                    ) -> BottomUpTree:
+    rects = _cuda_f32_rects(rects)
+    n, dev = rects.shape[0], rects.device
+    st = BUTreeStruct()
+    check(lib().prt_bu_plan(n, fanout, ctypes.byref(st)))
+    nn = st.num_nodes
+    i32 = dict(dtype=torch.int32, device=dev)
+    t = BottomUpTree(st, torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(nn, **i32),
+                     torch.empty(nn, **i32), torch.empty(nn, **i32),
+                     torch.empty((nn * fanout, 4), dtype=torch.float32, device=dev),
+                     torch.empty(nn * fanout, **i32))
+    for name in ("keys", "perm", "node_count", "node_start", "node_end", "entry_mbr", "entry_ref"):
+        setattr(st, name, getattr(t, name).data_ptr())
+    need = lib().prt_bu_workspace_size(n, fanout)
+    ws = workspace if workspace is not None and workspace.numel() >= need else _ws(need, dev)
+    check(lib().prt_bu_build(_ptr(rects), n, fanout, ctypes.byref(st), _ptr(ws), ws.numel(),
+                             _stream()))
+    t.workspace = ws
+    return t
+
+
+def rebuild(tree: BottomUpTree | TopDownTree, rects: torch.Tensor) -> None:
+    """Rebuild `tree` in place from rects of the same count (reuses every
+    output array and the workspace; launches only)."""
+    if isinstance(tree, BottomUpTree):
+        fn = lib().prt_bu_build
+    else:
+        fn = lib().prt_td_build
+    check(fn(_ptr(rects), tree.st.num_rects, tree.st.fanout, ctypes.byref(tree.st),
+             _ptr(tree.workspace), tree.workspace.numel(), _stream()))
+
+
+def build_top_down(rects: torch.Tensor, fanout: int = 16, workspace: torch.Tensor | None = None
+                   ) -> TopDownTree:
+    rects = _cuda_f32_rects(rects)
+    n, dev = rects.shape[0], rects.device
+    st = TDTreeStruct()
+    check(lib().prt_td_plan(n, fanout, ctypes.byref(st)))
+    ns = st.num_slots
+    t = TopDownTree(st, torch.empty(ns, dtype=torch.int32, device=dev),
+                    torch.empty((ns, 4), dtype=torch.float32, device=dev),
+                    torch.empty(n, dtype=torch.int32, device=dev))
+    for name in ("slot_child", "slot_mbr", "perm"):
+        setattr(st, name, getattr(t, name).data_ptr())
+    need = lib().prt_td_workspace_size(n, fanout)
+    ws = workspace if workspace is not None and workspace.numel() >= need else _ws(need, dev)
+    check(lib().prt_td_build(_ptr(rects), n, fanout, ctypes.byref(st), _ptr(ws), ws.numel(),
+                             _stream()))
+    t.workspace = ws
+    return t
+
+
+class Joiner:
+    """Preallocated join of a fixed query count: run() only launches."""
+
+    def __init__(self, tree: BottomUpTree | TopDownTree, nq: int, cap: int, device):
+        self.tree, self.nq, self.cap = tree, nq, cap
+        self.fn = lib().prt_join_bu if isinstance(tree, BottomUpTree) else lib().prt_join_td
+        self.count = torch.zeros(2, dtype=torch.int64, device=device)
+        self.ws = _ws(lib().prt_join_workspace_size(nq), device)
+        self.pairs = torch.empty((max(cap, 1), 2), dtype=torch.int32, device=device)
+
+    def run(self, queries: torch.Tensor) -> None:
+        check(self.fn(ctypes.byref(self.tree.st), _ptr(queries), self.nq, _ptr(self.pairs),
+                      self.cap, _ptr(self.count), _ptr(self.ws), self.ws.numel(), _stream()))
+
+    def result(self) -> torch.Tensor:
+        total, status = (int(v) for v in self.count.tolist())
+        if total > self.cap or status:
+            raise RuntimeError(f"join: total {total} cap {self.cap} status {status}")
+        return self.pairs[:total]
+
+
+def join(tree: BottomUpTree | TopDownTree, queries: torch.Tensor, cap: int | None = None
+         ) -> torch.Tensor:
+    """All (q, r) with overlap(queries[q], rect r), sorted by (q, r): an
+    (np, 2) int32 tensor holding u32 ids.  With cap=None the pair count is
+    read back first (one device sync) and the buffer sized exactly."""
+    queries = _cuda_f32_rects(queries)
+    nq, dev = queries.shape[0], queries.device
+    fn = lib().prt_join_bu if isinstance(tree, BottomUpTree) else lib().prt_join_td
+    count = torch.zeros(2, dtype=torch.int64, device=dev)
+    ws = _ws(lib().prt_join_workspace_size(nq), dev)
+    if cap is None:
+        check(fn(ctypes.byref(tree.st), _ptr(queries), nq, None, 0, _ptr(count), _ptr(ws),
+                 ws.numel(), _stream()))
+        cap = int(count[0].item())
+    pairs = torch.empty((max(cap, 1), 2), dtype=torch.int32, device=dev)
+    check(fn(ctypes.byref(tree.st), _ptr(queries), nq, _ptr(pairs), cap, _ptr(count), _ptr(ws),
+             ws.numel(), _stream()))
+    total, status = (int(v) for v in count.tolist())
+    if total > cap:
+        raise RuntimeError(f"join produced {total} pairs > cap {cap}")
+    if status:
+        raise RuntimeError("join: a query has more hits than the per-query sort supports")
+    return pairs[:total]
+
+
+@dataclass
+class PolyDev:
+    xy: torch.Tensor    # (V, 2) float64 CUDA
+    off: torch.Tensor   # (n + 1,) int64 CUDA
+    max_verts: int
+
+
+def to_device_polys(p, device="cuda") -> PolyDev:
+    return PolyDev(torch.from_numpy(p.xy).to(device), torch.from_numpy(p.off).to(device),
+                   p.max_verts())
+
+
+def overlay_plain(pairs: torch.Tensor, A: PolyDev, B: PolyDev, area: torch.Tensor | None = None,
+                  workspace: torch.Tensor | None = None) -> torch.Tensor:
+    """FP64 area of A[pairs[t,0]] intersect B[pairs[t,1]], one lane per pair."""
+    pairs = pairs.contiguous()
+    npairs = pairs.shape[0]
+    if area is None:
+        area = torch.empty(npairs, dtype=torch.float64, device=pairs.device)
+    mv = A.max_verts + B.max_verts
+    need = lib().prt_overlay_plain_workspace_size(npairs, mv)
+    ws = workspace if workspace is not None and workspace.numel() >= need else \
+        _ws(need, pairs.device)
+    check(lib().prt_overlay_plain(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
+                                  _ptr(B.off), mv, _ptr(area), _ptr(ws), ws.numel(), _stream()))
+    return area

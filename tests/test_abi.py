@@ -1,0 +1,90 @@
+"""C-ABI boundary on the CPU: libprt.so loads, exports every symbol that
+include/prt.h declares, and its host-only planning functions agree with the
+oracle.  No kernel is launched here."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parlllelrtree_amd import _lib, gen
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "prt.h"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    decl = r"^(?:const\s+)?[a-z_0-9]+\s*\*?\s*(prt_[a-z0-9_]+)\s*\("
+    return sorted(set(re.findall(decl, text, flags=re.M)))
+
+
+def test_header_symbols_exported():
+    lib = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), f"{name} missing from libprt.so"
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_version_and_error():
+    lib = _lib.lib()
+    assert lib.prt_version().startswith(b"prt ")
+    st = _lib.BUTreeStruct()
+    assert lib.prt_bu_plan(0, 16, ctypes.byref(st)) == -1
+    assert b"out of range" in lib.prt_last_error()
+    assert lib.prt_bu_plan(10, 1, ctypes.byref(st)) == -1
+    assert lib.prt_bu_plan(10, 65, ctypes.byref(st)) == -1
+
+
+@pytest.mark.parametrize("n,f", [(1, 16), (2, 2), (17, 16), (1 << 20, 16), (1 << 24, 16),
+                                 (123457, 10), (999, 64)])
+def test_plans_match_oracle(n, f):
+    lib = _lib.lib()
+    st = _lib.BUTreeStruct()
+    assert lib.prt_bu_plan(n, f, ctypes.byref(st)) == 0
+    lv = O.bu_levels(n, f)
+    assert st.num_levels == len(lv)
+    assert [(st.level_start[i], st.level_count[i]) for i in range(st.num_levels)] == lv
+    td = _lib.TDTreeStruct()
+    assert lib.prt_td_plan(n, f, ctypes.byref(td)) == 0
+    off = O.td_levels(n, f)
+    assert [td.level_offset[i] for i in range(td.num_levels + 1)] == off
+    assert lib.prt_bu_workspace_size(n, f) > 0
+    assert lib.prt_td_workspace_size(n, f) > 0
+
+
+def test_known_layout_sizes():
+    lib = _lib.lib()
+    st = _lib.BUTreeStruct()
+    lib.prt_bu_plan(1 << 20, 16, ctypes.byref(st))
+    assert st.num_nodes == 69905
+    td = _lib.TDTreeStruct()
+    lib.prt_td_plan(1 << 20, 16, ctypes.byref(td))
+    assert td.num_slots == 1118480 and td.num_levels == 5
+
+
+def test_generators_deterministic_and_in_range():
+    a = gen.rects_uniform(gen.seed(0), 1000)
+    b = gen.rects_uniform(gen.seed(0), 1000)
+    np.testing.assert_array_equal(a, b)
+    w = a[:, 2] - a[:, 0]
+    assert (w > 0.99e-4).all() and (w < 1.01e-3).all()
+    g = gen.stars_grid(gen.seed(2), n=2000)
+    k = np.diff(g.off)
+    assert k.min() >= 4 and k.max() <= 512
+    assert 15 < np.mean(k) < 45
+    m = gen.poly_mbr(g)
+    for i in range(0, 2000, 97):
+        p = g.poly(i)
+        assert m[i, 0] <= p[:, 0].min() and m[i, 2] >= p[:, 0].max()
+        assert m[i, 1] <= p[:, 1].min() and m[i, 3] >= p[:, 1].max()
+
+
+def test_generator_thread_independence():
+    # 70000 > the single-thread threshold: the split must not change values
+    big = gen.rects_uniform(gen.seed(4), 70000)
+    small = gen.rects_uniform(gen.seed(4), 1000)
+    np.testing.assert_array_equal(big[:1000], small)

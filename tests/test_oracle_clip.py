@@ -1,0 +1,144 @@
+"""Oracle: scanbeam intersection area and its gradient, pinned by closed forms,
+an independent Green's-theorem area, Sutherland-Hodgman on convex inputs,
+symmetry/translation invariants and central finite differences."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parlllelrtree_amd import gen
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "golden.json").read_text())
+
+
+def star(rng, k, cx, cy, R):
+    """simple star: one random angle per sector of width 2pi/k"""
+    a = 2 * np.pi * (np.arange(k) + rng.uniform(0, 1, k)) / k
+    r = rng.uniform(0.5, 1.0, k) * R
+    return np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], 1)
+
+
+def convex(rng, k, cx, cy, R):
+    a = np.sort(rng.uniform(0, 2 * np.pi, k))
+    return np.stack([cx + R * np.cos(a), cy + R * np.sin(a)], 1)
+
+
+@pytest.mark.parametrize("case", GOLD["clip"], ids=lambda c: c["name"])
+def test_clip_golden(case):
+    P, Q = np.array(case["P"], float), np.array(case["Q"], float)
+    assert O.clip_area(P, Q) == pytest.approx(case["area"], rel=1e-15, abs=1e-15)
+    assert O.clip_area(Q, P) == pytest.approx(case["area"], rel=1e-15, abs=1e-15)
+
+
+def test_rect_rect_closed_form():
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        a = np.sort(rng.uniform(0, 1, (2, 2)), 0)
+        b = np.sort(rng.uniform(0, 1, (2, 2)), 0)
+        P = np.array([[a[0, 0], a[0, 1]], [a[1, 0], a[0, 1]], [a[1, 0], a[1, 1]], [a[0, 0], a[1, 1]]])
+        Q = np.array([[b[0, 0], b[0, 1]], [b[1, 0], b[0, 1]], [b[1, 0], b[1, 1]], [b[0, 0], b[1, 1]]])
+        w = max(0.0, min(a[1, 0], b[1, 0]) - max(a[0, 0], b[0, 0]))
+        h = max(0.0, min(a[1, 1], b[1, 1]) - max(a[0, 1], b[0, 1]))
+        assert O.clip_area(P, Q) == pytest.approx(w * h, rel=1e-13, abs=1e-16)
+
+
+def test_convex_matches_sutherland_hodgman():
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        P = convex(rng, int(rng.integers(3, 40)), 0.5, 0.5, 0.1)
+        Q = convex(rng, int(rng.integers(3, 40)), 0.5 + rng.uniform(-0.1, 0.1),
+                   0.5 + rng.uniform(-0.1, 0.1), 0.1)
+        sh = O.sh_convex_area(P, Q)
+        assert O.clip_area(P, Q) == pytest.approx(sh, rel=1e-11, abs=1e-17)
+
+
+def test_stars_match_green_theorem():
+    rng = np.random.default_rng(6)
+    rel = []
+    for _ in range(400):
+        P = star(rng, int(rng.integers(4, 120)), 0.3, 0.7, 2e-3)
+        Q = star(rng, int(rng.integers(4, 120)), 0.3 + rng.uniform(-2e-3, 2e-3),
+                 0.7 + rng.uniform(-2e-3, 2e-3), 2e-3)
+        a, g = O.clip_area(P, Q), O.green_area(P, Q)
+        den = max(abs(g), 1e-12 * min(abs(O.poly_area(P)), abs(O.poly_area(Q))))
+        rel.append(abs(a - g) / den)
+    assert max(rel) < 1e-9
+    assert np.median(rel) < 1e-13
+
+
+def test_self_nested_disjoint_symmetry_translation():
+    rng = np.random.default_rng(8)
+    for _ in range(100):
+        P = star(rng, int(rng.integers(4, 80)), 0.5, 0.5, 0.01)
+        assert O.clip_area(P, P) == pytest.approx(abs(O.poly_area(P)), rel=1e-13)
+        inner = 0.5 + 0.3 * (P - 0.5)       # scaled copy strictly inside (star-shaped)
+        assert O.clip_area(P, inner) == pytest.approx(abs(O.poly_area(inner)), rel=1e-12)
+        far = P + 1.0
+        assert O.clip_area(P, far) == 0.0
+        Q = star(rng, int(rng.integers(4, 80)), 0.505, 0.497, 0.01)
+        a = O.clip_area(P, Q)
+        assert O.clip_area(Q, P) == pytest.approx(a, rel=1e-13, abs=1e-20)
+        # power-of-two translation keeps every coordinate difference exact
+        assert O.clip_area(P + 0.25, Q + 0.25) == pytest.approx(a, rel=1e-12, abs=1e-20)
+        assert a <= min(abs(O.poly_area(P)), abs(O.poly_area(Q))) * (1 + 1e-12)
+
+
+def test_degenerate_sizes():
+    sq = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    assert O.clip_area(sq[:2], sq) == 0.0
+    assert O.clip_area(sq, sq[:0]) == 0.0
+
+
+def test_flop_counter_positive():
+    rng = np.random.default_rng(9)
+    P = star(rng, 30, 0, 0, 1)
+    Q = star(rng, 30, 0.2, 0.1, 1)
+    a, fl = O.clip_area(P, Q, with_flops=True)
+    assert a > 0 and fl > 100
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(10)
+    errs = []
+    for _ in range(20):
+        P = star(rng, int(rng.integers(8, 40)), 0, 0, 1.0)
+        Q = star(rng, int(rng.integers(8, 40)), rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), 1.0)
+        gP, gQ = O.area_grad(P, Q)
+        h = 1e-6
+        for X, G in ((P, gP), (Q, gQ)):
+            fd = np.zeros_like(X)
+            for i in range(len(X)):
+                for c in range(2):
+                    Xp, Xm = X.copy(), X.copy()
+                    Xp[i, c] += h
+                    Xm[i, c] -= h
+                    if X is P:
+                        fd[i, c] = (O.clip_area(Xp, Q) - O.clip_area(Xm, Q)) / (2 * h)
+                    else:
+                        fd[i, c] = (O.clip_area(P, Xp) - O.clip_area(P, Xm)) / (2 * h)
+            errs.append(np.abs(fd - G).max() / max(np.abs(G).max(), 1e-30))
+    assert max(errs) < 1e-6
+
+
+def test_gradient_closed_form_inside_vertices():
+    rng = np.random.default_rng(12)
+    P = star(rng, 25, 0, 0, 0.2)          # entirely inside the big square
+    Q = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], float)
+    gP, gQ = O.area_grad(P, Q)
+    nxt, prv = np.roll(P, -1, 0), np.roll(P, 1, 0)
+    np.testing.assert_allclose(gP[:, 0], 0.5 * (nxt[:, 1] - prv[:, 1]), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(gP[:, 1], -0.5 * (nxt[:, 0] - prv[:, 0]), rtol=1e-12, atol=1e-15)
+    assert np.abs(gQ).max() == 0.0
+
+
+def test_generated_stars_are_ccw_and_inscribed():
+    r = gen.rects_uniform(gen.seed(1), 50)
+    ids = np.arange(50, dtype=np.uint32)
+    S = gen.stars_in_rects(gen.seed(1) + gen.STAR_STREAM, r, ids)
+    for i in range(S.n):
+        p = S.poly(i)
+        assert 8 <= len(p) <= 64
+        assert O.poly_area(p) > 0
+        assert (p[:, 0] >= r[i, 0] - 1e-7).all() and (p[:, 0] <= r[i, 2] + 1e-7).all()
