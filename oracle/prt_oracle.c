@@ -495,10 +495,13 @@ int64_t orc_join_brute(const float *r4, int64_t n, const float *q4, int64_t nq,
  *     the subject and the clip parity are odd.
  * Each such gap is a trapezoid; its doubled area
  *   ((xa_R - xa_L) + (xc_R - xc_L)) * (yc - ya)
- * is added, in order of beam, sub-beam, then left to right.  Area = half
- * the sum.  Edges in a sub-beam are ordered by (xa+xc, xa, edge id), a
- * strict total order, so the result does not depend on the input order of
- * the active table.  Edge id = vertex index for P, nP + index for Q.
+ * is added to the beam's sum (starting from +0), in order of sub-beam, then
+ * left to right.  The beam sums s[0..nb) are then added by a fixed pairwise
+ * tree: for w = 1, 2, 4, ...: s[i] += s[i + w] for i = 0, 2w, 4w, ... with
+ * i + w < nb; area = 0.5 * s[0].  Edges in a sub-beam are ordered by
+ * (xa+xc, xa, edge id), a strict total order, so nothing depends on the
+ * input order of the active table, and beams may be processed in any order
+ * or in parallel.  Edge id = vertex index for P, nP + index for Q.
  */
 
 typedef struct {
@@ -551,7 +554,7 @@ typedef struct {
     cedge *edges;
     double *ys;
     const cedge **act;
-    double *xb, *xt, *xa, *xc, *km, *cy;
+    double *xb, *xt, *xa, *xc, *km, *cy, *bsum;
     int *ord;
     int cap_e, cap_y;
 } clip_ws;
@@ -574,13 +577,14 @@ static void ws_reserve(clip_ws *w, int ne, int ny)
     if (ny > w->cap_y) {
         w->cap_y = ny;
         w->ys = (double *)realloc(w->ys, (size_t)ny * sizeof(double));
+        w->bsum = (double *)realloc(w->bsum, (size_t)ny * sizeof(double));
     }
 }
 
 static void ws_free(clip_ws *w)
 {
     free(w->edges); free(w->act); free(w->xb); free(w->xt); free(w->xa);
-    free(w->xc); free(w->km); free(w->ord); free(w->cy); free(w->ys);
+    free(w->xc); free(w->km); free(w->ord); free(w->cy); free(w->ys); free(w->bsum);
 }
 
 /* One crossing-free sub-beam [ya, yc] with edge x's xa[], xc[] (indexed like
@@ -664,10 +668,12 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
         if (u == 0 || w->ys[i] != w->ys[u - 1]) w->ys[u++] = w->ys[i];
     ny = u;
 
-    double acc = 0.0;
+    if (ny < 2) return 0.0;
+    double *bsum = w->bsum;   /* beam sums; the scanbeam table is still needed */
     int m = 0, next = 0;
     for (int k = 0; k + 1 < ny; ++k) {
         double yb = w->ys[k], yt = w->ys[k + 1];
+        bsum[k] = 0.0;
         while (next < ne && w->edges[next].by <= yb) w->act[m++] = &w->edges[next++];
         int keep = 0;
         for (int i = 0; i < m; ++i)
@@ -699,7 +705,7 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
         }
         if (nc == 0) {
             for (int i = 0; i < m; ++i) { w->xa[i] = w->xb[i]; w->xc[i] = w->xt[i]; }
-            acc = sub_beam(w, m, yb, yt, acc, &ops);
+            bsum[k] = sub_beam(w, m, yb, yt, 0.0, &ops);
             continue;
         }
         qsort(w->cy, (size_t)nc, sizeof(double), cmp_double);
@@ -707,7 +713,7 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
         for (int i = 0; i < nc; ++i)
             if (u == 0 || w->cy[i] != w->cy[u - 1]) w->cy[u++] = w->cy[i];
         nc = u;
-        double ya = yb;
+        double ya = yb, bs = 0.0;
         for (int i = 0; i < m; ++i) w->xa[i] = w->xb[i];
         for (int c = 0; c <= nc; ++c) {
             double yc = c < nc ? w->cy[c] : yt;
@@ -717,13 +723,18 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
             } else {
                 for (int i = 0; i < m; ++i) w->xc[i] = w->xt[i];
             }
-            acc = sub_beam(w, m, ya, yc, acc, &ops);
+            bs = sub_beam(w, m, ya, yc, bs, &ops);
             for (int i = 0; i < m; ++i) w->xa[i] = w->xc[i];
             ya = yc;
         }
+        bsum[k] = bs;
     }
-    if (fl) *fl += ops + 1;
-    return 0.5 * acc;
+    /* fixed pairwise tree over the beam sums */
+    const int nb = ny - 1;
+    for (int wdt = 1; wdt < nb; wdt *= 2)
+        for (int i = 0; i + wdt < nb; i += 2 * wdt) bsum[i] += bsum[i + wdt];
+    if (fl) *fl += ops + (uint64_t)nb;
+    return 0.5 * bsum[0];
 }
 
 double orc_clip_area(const double *P, int nP, const double *Q, int nQ, uint64_t *flops)

@@ -15,7 +15,10 @@
 //   x(y)  = y == top.y ? top.x : bot.x + dx * (y - bot.y)     (:454-457)
 //   den   = (xt_L - xb_L) - (xt_R - xb_R), r = (xb_R - xb_L) / den,
 //   iy    = r * (yt - yb) + yb                                (:1413-1433, 480)
-//   term  = ((xa_R - xa_L) + (xc_R - xc_L)) * (yc - ya), area = 0.5 * sum
+//   term  = ((xa_R - xa_L) + (xc_R - xc_L)) * (yc - ya), summed per beam
+//   from +0 in sub-beam then left-to-right order; the beam sums s[k] are
+//   added by the fixed tree s[i] += s[i + w] (w = 1, 2, 4, ...; i = 0, 2w,
+//   ...), area = 0.5 * s[0].
 // Sub-beam boundaries are the distinct crossing heights in (yb, yt), taken
 // in increasing order; inside a sub-beam, edges are ordered by
 // (xa + xc, xa, edge id); a gap is inside when both even-odd parities
@@ -31,7 +34,7 @@ constexpr int64_t CP_MAX_LANES = 65536;
 constexpr int CP_BYTES_PER_SLOT = 11 * 8 + 4 * 4;  // 11 double + 4 int arrays
 
 struct PlainWs {
-    double *bx, *by, *tx, *ty, *dx, *ys, *xb, *xt, *xa, *xc, *spare;
+    double *bx, *by, *tx, *ty, *dx, *ys, *xb, *xt, *xa, *xc, *bsum;
     int32_t *id, *eord, *act, *ord;
     int64_t T;
     int cap;
@@ -190,10 +193,11 @@ __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
     }
     ny = u;
 
-    double acc = 0.0;
+    if (ny < 2) return 0.0;
     int m = 0, next = 0;
     for (int k = 0; k + 1 < ny; ++k) {
         const double yb = L.at(L.w.ys, k), yt = L.at(L.w.ys, k + 1);
+        L.at(L.w.bsum, k) = 0.0;
         while (next < ne && L.at(L.w.by, L.at(L.w.eord, next)) <= yb)
             L.at(L.w.act, m++) = L.at(L.w.eord, next++);
         int keep = 0;
@@ -211,7 +215,7 @@ __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
             L.at(L.w.xa, i) = b;
         }
         const double dy = yt - yb;
-        double ya = yb;
+        double ya = yb, bs = 0.0;
         for (;;) {
             // next distinct crossing height above ya (or yt)
             double yc = yt;
@@ -235,13 +239,18 @@ __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
             } else {
                 for (int i = 0; i < m; ++i) L.at(L.w.xc, i) = edge_x(L, L.at(L.w.act, i), yc);
             }
-            acc = sub_beam(L, m, nP, ya, yc, acc);
+            bs = sub_beam(L, m, nP, ya, yc, bs);
             if (yc == yt) break;
             for (int i = 0; i < m; ++i) L.at(L.w.xa, i) = L.at(L.w.xc, i);
             ya = yc;
         }
+        L.at(L.w.bsum, k) = bs;
     }
-    return 0.5 * acc;
+    // fixed pairwise tree over the beam sums (same shape as the oracle)
+    const int nb = ny - 1;
+    for (int wdt = 1; wdt < nb; wdt *= 2)
+        for (int i = 0; i + wdt < nb; i += 2 * wdt) L.at(L.w.bsum, i) += L.at(L.w.bsum, i + wdt);
+    return 0.5 * L.at(L.w.bsum, 0);
 }
 
 __global__ __launch_bounds__(CP_THREADS) void k_clip_plain(
@@ -295,7 +304,7 @@ extern "C" int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const do
     w.cap = max_verts < 3 ? 3 : max_verts;
     Carver c(ws, ws_bytes);
     const size_t cnt = static_cast<size_t>(w.T) * w.cap;
-    double **dptr[] = {&w.bx, &w.by, &w.tx, &w.ty, &w.dx, &w.ys, &w.xb, &w.xt, &w.xa, &w.xc, &w.spare};
+    double **dptr[] = {&w.bx, &w.by, &w.tx, &w.ty, &w.dx, &w.ys, &w.xb, &w.xt, &w.xa, &w.xc, &w.bsum};
     for (double **p : dptr) *p = c.take<double>(cnt);
     int32_t **iptr[] = {&w.id, &w.eord, &w.act, &w.ord};
     for (int32_t **p : iptr) *p = c.take<int32_t>(cnt);
