@@ -145,6 +145,17 @@ int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const double *xyA,
                       int32_t max_verts, double *area, void *ws, size_t ws_bytes,
                       void *stream);
 
+/* Batched overlay: same areas as prt_overlay_plain (bit-identical), pairs
+ * bucketed by n1+n2 and run by persistent work-stealing launches, one per
+ * bucket, with the team shape of `tune` (NULL = prt_tune_default).  Also
+ * replaces Execute OV:81-113 / gpc_polygon_clip OV:115-750. */
+int prt_tune_default(prt_tune *tune);
+size_t prt_overlay_batched_workspace_size(int64_t npairs, int32_t max_verts);
+int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                        const int64_t *offA, const double *xyB, const int64_t *offB,
+                        int32_t max_verts, const prt_tune *tune, double *area, void *ws,
+                        size_t ws_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

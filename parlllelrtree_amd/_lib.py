@@ -80,6 +80,11 @@ SIGNATURES = {
     "prt_overlay_plain_workspace_size": (c_size_t, [c_int64, c_int32]),
     "prt_overlay_plain": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "prt_tune_default": (c_int, [POINTER(TuneStruct)]),
+    "prt_overlay_batched_workspace_size": (c_size_t, [c_int64, c_int32]),
+    "prt_overlay_batched": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_int32, POINTER(TuneStruct), c_void_p, c_void_p, c_size_t,
+                                    c_void_p]),
     # host-only generators (csrc/gen.cpp)
     "prt_gen_u01": (c_double, [c_uint64, c_uint64]),
     "prt_gen_rects_uniform": (None, [c_uint64, c_int64, c_double, c_double, c_void_p]),

@@ -1,0 +1,647 @@
+// clip_batched.hip -- batched FP64 polygon intersection area: pairs are
+// bucketed by vertex count, each bucket runs one persistent, work-stealing
+// launch whose team shape (lanes per pair, waves per block) comes from the
+// autotuner (prt_tune).
+//
+// Same scanbeam sweep and the same FP64 expressions as the plain kernel
+// (clip_plain.hip) and the CPU oracle, so areas are bit-identical, but:
+//   * a TEAM of TL lanes (8..64, a slice of one wave64) owns a pair; the
+//     pair's edges, its scanbeam table and its per-beam active-edge lists
+//     live in LDS;
+//   * the scanbeam table is sorted with a bitonic network across the team;
+//   * every edge registers itself in the lists of the beams it spans (LDS
+//     atomics + a team scan), so a beam finds its active edges without
+//     scanning the polygon;
+//   * lane k sweeps beams k, k+TL, ...; beams are independent, their sums
+//     are combined with the fixed pairwise tree the oracle uses;
+//   * teams take the next pair from a per-bucket ticket (work stealing), so
+//     polygons of very uneven size do not leave lanes idle.
+// Pairs whose active-edge lists do not fit the team's LDS are handed to the
+// plain kernel through an overflow list (same results, slower).
+#include "prt_common.hpp"
+#include "prt_profile.hpp"
+
+namespace prt {
+
+int plain_list_launch(const uint32_t *pairs, const uint32_t *list, const int64_t *count,
+                      int64_t max_count, const double *xyA, const int64_t *offA, const double *xyB,
+                      const int64_t *offB, int32_t max_verts, double *area, void *ws,
+                      size_t ws_bytes, hipStream_t s);
+size_t plain_list_ws_bytes(int64_t max_count, int32_t max_verts);
+
+namespace {
+
+constexpr int BK_THREADS_MAX = 512;
+constexpr int NBUCKET = PRT_TUNE_BUCKETS;
+
+__device__ inline void team_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+struct TeamLds {
+    double *bx, *by, *tx, *ty, *dx;  // [NV] edge slot = vertex index (P: i, Q: nP + j)
+    double *ys;                      // [P2] scanbeam table (sorted, then unique)
+    double *bsum;                    // [NV] beam sums (also the unique-compaction buffer)
+    int32_t *cnt;                    // [NV + 1]
+    int32_t *boff;                   // [NV + 1]
+    uint16_t *list;                  // [SCAP]
+};
+
+__host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+
+__host__ __device__ inline int pow2_ge(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+__host__ __device__ inline size_t team_bytes(int NV, int SCAP) {
+    const int P2 = pow2_ge(NV);
+    return align16(5 * 8 * size_t(NV)) + align16(8 * size_t(P2)) + align16(8 * size_t(NV)) +
+           2 * align16(4 * size_t(NV + 1)) + align16(2 * size_t(SCAP));
+}
+
+__device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
+    TeamLds t;
+    const int P2 = pow2_ge(NV);
+    t.bx = reinterpret_cast<double *>(p);
+    t.by = t.bx + NV;
+    t.tx = t.by + NV;
+    t.ty = t.tx + NV;
+    t.dx = t.ty + NV;
+    p += align16(5 * 8 * size_t(NV));
+    t.ys = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(P2));
+    t.bsum = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(NV));
+    t.cnt = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV + 1));
+    t.boff = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV + 1));
+    t.list = reinterpret_cast<uint16_t *>(p);
+    return t;
+}
+
+__device__ inline double ex(const TeamLds &t, int e, double y) {
+    if (y == t.ty[e]) return t.tx[e];
+    return t.bx[e] + t.dx[e] * (y - t.by[e]);
+}
+
+// first index k in [0, n) with ys[k] >= v (n if none)
+__device__ inline int lower_bound(const double *ys, int n, double v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (ys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// team-wide min/max/sum over TL lanes (butterfly inside the team's lanes)
+template <int TL> __device__ inline double team_min(double v) {
+#pragma unroll
+    for (int o = 1; o < TL; o <<= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <int TL> __device__ inline double team_max(double v) {
+#pragma unroll
+    for (int o = 1; o < TL; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// inclusive scan inside the team
+template <int TL> __device__ inline int team_incl_scan(int v, int tl) {
+#pragma unroll
+    for (int o = 1; o < TL; o <<= 1) {
+        int y = __shfl_up(v, o, TL);
+        if (tl >= o) v += y;
+    }
+    return v;
+}
+template <int TL> __device__ inline int team_bcast(int v, int src_lane_in_team, int team_base) {
+    return __shfl(v, team_base + src_lane_in_team, 64);
+}
+
+// Strict order (xa + xc, xa, id) inside a sub-beam.
+__device__ inline bool key_before(double ka, double xa_a, int ea, double kb, double xa_b, int eb) {
+    if (ka != kb) return ka < kb;
+    if (xa_a != xa_b) return xa_a < xa_b;
+    return ea < eb;
+}
+
+// Doubled area of beam k (sum over sub-beams, left to right), recomputing
+// every edge x from the LDS edge table.
+__device__ double beam_sum(const TeamLds &t, int k, int nP) {
+    const int lo = t.boff[k], hi = t.boff[k + 1];
+    const int m = hi - lo;
+    if (m < 2) return 0.0;
+    const double yb = t.ys[k], yt = t.ys[k + 1];
+    const double dy = yt - yb;
+    double ya = yb, bs = 0.0;
+    for (;;) {
+        double yc = yt;
+        for (int i = lo; i < hi; ++i) {
+            const int ei = t.list[i];
+            const double xbi = ex(t, ei, yb), xti = ex(t, ei, yt);
+            for (int j = i + 1; j < hi; ++j) {
+                const int ej = t.list[j];
+                const double xbj = ex(t, ej, yb), xtj = ex(t, ej, yt);
+                double xbL, xtL, xbR, xtR;
+                if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+                else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+                else continue;
+                const double den = (xtL - xbL) - (xtR - xbR);
+                if (!(den > 0.0)) continue;
+                const double r = (xbR - xbL) / den;
+                const double iy = r * dy + yb;
+                if (iy > ya && iy < yc) yc = iy;
+            }
+        }
+        const double h = yc - ya;
+        // selection walk in key order
+        int cur = -1;
+        double kc = 0.0, xac = 0.0, xcc = 0.0;
+        for (int i = lo; i < hi; ++i) {
+            const int e = t.list[i];
+            const double xa = ex(t, e, ya), xc = ex(t, e, yc), kk = xa + xc;
+            if (cur < 0 || key_before(kk, xa, e, kc, xac, cur)) {
+                cur = e; kc = kk; xac = xa; xcc = xc;
+            }
+        }
+        int pp = 0, pq = 0;
+        for (int step = 0; step + 1 < m; ++step) {
+            if (cur < nP) pp ^= 1;
+            else pq ^= 1;
+            int nxt = -1;
+            double kn = 0.0, xan = 0.0, xcn = 0.0;
+            for (int i = lo; i < hi; ++i) {
+                const int e = t.list[i];
+                const double xa = ex(t, e, ya), xc = ex(t, e, yc), kk = xa + xc;
+                if (!key_before(kc, xac, cur, kk, xa, e)) continue;  // not after cur
+                if (nxt < 0 || key_before(kk, xa, e, kn, xan, nxt)) {
+                    nxt = e; kn = kk; xan = xa; xcn = xc;
+                }
+            }
+            if (pp && pq) bs += ((xan - xac) + (xcn - xcc)) * h;
+            cur = nxt; kc = kn; xac = xan; xcc = xcn;
+        }
+        if (yc == yt) break;
+        ya = yc;
+    }
+    return bs;
+}
+
+struct BatchArgs {
+    const uint32_t *pairs;
+    const uint32_t *order;   // pair indices grouped by bucket
+    const int64_t *start;    // first index of this bucket in order (device)
+    const int64_t *count;    // number of pairs in the bucket (device)
+    const double *xyA, *xyB;
+    const int64_t *offA, *offB;
+    double *area;
+    unsigned long long *ticket;
+    uint32_t *ovf_list;
+    unsigned long long *ovf_count;
+    int NV, SCAP;
+};
+
+// Returns 1 when the pair was handled, 0 when it must go to the overflow list.
+template <int TL>
+__device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int tl, int base,
+                         double &result) {
+    const uint32_t a = A.pairs[2 * p], b = A.pairs[2 * p + 1];
+    const int64_t oa = A.offA[a], ob = A.offB[b];
+    const int nP = static_cast<int>(A.offA[a + 1] - oa);
+    const int nQ = static_cast<int>(A.offB[b + 1] - ob);
+    const double *P = A.xyA + 2 * oa, *Q = A.xyB + 2 * ob;
+    result = 0.0;
+    if (nP < 3 || nQ < 3) return 1;
+    const int n = nP + nQ;
+    if (n > A.NV) return 0;
+
+    // bounding boxes
+    double bp0 = __builtin_huge_val(), bp1 = bp0, bp2 = -bp0, bp3 = -bp0;
+    double bq0 = bp0, bq1 = bp0, bq2 = -bp0, bq3 = -bp0;
+    for (int i = tl; i < nP; i += TL) {
+        const double x = P[2 * i], y = P[2 * i + 1];
+        bp0 = fmin(bp0, x); bp2 = fmax(bp2, x); bp1 = fmin(bp1, y); bp3 = fmax(bp3, y);
+    }
+    for (int i = tl; i < nQ; i += TL) {
+        const double x = Q[2 * i], y = Q[2 * i + 1];
+        bq0 = fmin(bq0, x); bq2 = fmax(bq2, x); bq1 = fmin(bq1, y); bq3 = fmax(bq3, y);
+    }
+    bp0 = team_min<TL>(bp0); bp1 = team_min<TL>(bp1); bp2 = team_max<TL>(bp2); bp3 = team_max<TL>(bp3);
+    bq0 = team_min<TL>(bq0); bq1 = team_min<TL>(bq1); bq2 = team_max<TL>(bq2); bq3 = team_max<TL>(bq3);
+    if (bp2 < bq0 || bq2 < bp0 || bp3 < bq1 || bq3 < bp1) return 1;
+    const double ylo = bp1 > bq1 ? bp1 : bq1;
+    const double yhi = bp3 < bq3 ? bp3 : bq3;
+    if (!(ylo < yhi)) return 1;
+
+    // edge table and raw scanbeam table
+    const int P2 = pow2_ge(n);
+    const double INF = __builtin_huge_val();
+    for (int s = tl; s < P2; s += TL) {
+        if (s >= n) {
+            t.ys[s] = INF;
+            continue;
+        }
+        const bool inP = s < nP;
+        const double *V = inP ? P : Q;
+        const int nn = inP ? nP : nQ;
+        const int i = inP ? s : s - nP;
+        const int j = i + 1 == nn ? 0 : i + 1;
+        const double x0 = V[2 * i], y0 = V[2 * i + 1], x1 = V[2 * j], y1 = V[2 * j + 1];
+        t.ys[s] = (y0 >= ylo && y0 <= yhi) ? y0 : INF;
+        double bx, by, tx, ty;
+        if (y0 < y1) { bx = x0; by = y0; tx = x1; ty = y1; }
+        else { bx = x1; by = y1; tx = x0; ty = y0; }
+        const bool valid = y0 != y1 && !(ty <= ylo || by >= yhi);
+        t.bx[s] = bx;
+        t.by[s] = by;
+        t.tx[s] = tx;
+        t.ty[s] = valid ? ty : -INF;  // -inf top: never active
+        t.dx[s] = valid ? (tx - bx) / (ty - by) : 0.0;
+    }
+    team_sync();
+    // bitonic sort of ys[0, P2)
+    for (int k = 2; k <= P2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tl; q < P2 / 2; q += TL) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i | j;
+                const bool up = (i & k) == 0;
+                const double x = t.ys[i], y = t.ys[l];
+                if ((x > y) == up) {
+                    t.ys[i] = y;
+                    t.ys[l] = x;
+                }
+            }
+            team_sync();
+        }
+    }
+    // unique (finite values) -> compacted into bsum, then back into ys
+    const int per = (P2 + TL - 1) / TL;
+    const int c0 = tl * per, c1 = c0 + per < P2 ? c0 + per : P2;
+    int mine = 0;
+    for (int s = c0; s < c1; ++s) {
+        const double y = t.ys[s];
+        if (y != INF && (s == 0 || y != t.ys[s - 1])) ++mine;
+    }
+    const int incl = team_incl_scan<TL>(mine, tl);
+    const int NY = team_bcast<TL>(incl, TL - 1, base);
+    int w = incl - mine;
+    for (int s = c0; s < c1; ++s) {
+        const double y = t.ys[s];
+        if (y != INF && (s == 0 || y != t.ys[s - 1])) t.bsum[w++] = y;
+    }
+    team_sync();
+    for (int s = tl; s < NY; s += TL) t.ys[s] = t.bsum[s];
+    if (NY < 2) return 1;
+    const int nb = NY - 1;
+    for (int k = tl; k <= nb; k += TL) t.cnt[k] = 0;
+    team_sync();
+    // count list entries per beam
+    for (int e = tl; e < n; e += TL) {
+        const double ty = t.ty[e];
+        if (ty == -INF) continue;
+        const int kb = lower_bound(t.ys, NY, t.by[e]);
+        int kt = lower_bound(t.ys, NY, ty);
+        kt = kt < nb ? kt : nb;
+        for (int k = kb; k < kt; ++k) atomicAdd(&t.cnt[k], 1);
+    }
+    team_sync();
+    // exclusive scan of cnt[0, nb) -> boff, boff[nb] = S
+    const int pb = (nb + TL - 1) / TL;
+    const int b0 = tl * pb, b1 = b0 + pb < nb ? b0 + pb : nb;
+    int sum = 0;
+    for (int k = b0; k < b1; ++k) sum += t.cnt[k];
+    const int sincl = team_incl_scan<TL>(sum, tl);
+    const int S = team_bcast<TL>(sincl, TL - 1, base);
+    int run = sincl - sum;
+    for (int k = b0; k < b1; ++k) {
+        t.boff[k] = run;
+        run += t.cnt[k];
+    }
+    if (tl == 0) t.boff[nb] = S;
+    if (S > A.SCAP) return 0;
+    team_sync();
+    for (int k = tl; k < nb; k += TL) t.cnt[k] = 0;
+    team_sync();
+    for (int e = tl; e < n; e += TL) {
+        const double ty = t.ty[e];
+        if (ty == -INF) continue;
+        const int kb = lower_bound(t.ys, NY, t.by[e]);
+        int kt = lower_bound(t.ys, NY, ty);
+        kt = kt < nb ? kt : nb;
+        for (int k = kb; k < kt; ++k) t.list[t.boff[k] + atomicAdd(&t.cnt[k], 1)] =
+            static_cast<uint16_t>(e);
+    }
+    team_sync();
+    for (int k = tl; k < nb; k += TL) t.bsum[k] = beam_sum(t, k, nP);
+    team_sync();
+    for (int wdt = 1; wdt < nb; wdt *= 2) {
+        for (int i = tl * 2 * wdt; i + wdt < nb; i += TL * 2 * wdt) t.bsum[i] += t.bsum[i + wdt];
+        team_sync();
+    }
+    result = 0.5 * t.bsum[0];
+    return 1;
+}
+
+template <int TL>
+__global__ __launch_bounds__(BK_THREADS_MAX) void k_clip_batched(BatchArgs A) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiw = lane / TL, tl = lane % TL;
+    const int base = tiw * TL;
+    const int team = wave * (64 / TL) + tiw;
+    const TeamLds t = carve_team(lds + team * team_bytes(A.NV, A.SCAP), A.NV, A.SCAP);
+    const int64_t count = *A.count;
+    const uint32_t *order = A.order + *A.start;
+    for (;;) {
+        unsigned long long idx = 0;
+        if (tl == 0) idx = atomicAdd(A.ticket, 1ull);
+        idx = __shfl(idx, base, 64);
+        if (static_cast<int64_t>(idx) >= count) break;
+        const uint32_t p = order[idx];
+        double result;
+        const int ok = team_pair<TL>(t, A, p, tl, base, result);
+        if (tl == 0) {
+            if (ok) A.area[p] = result;
+            else A.ovf_list[atomicAdd(A.ovf_count, 1ull)] = p;
+        }
+        team_sync();
+    }
+}
+
+// Appends every pair of a bucket to the overflow list (bucket too large for
+// the team's LDS).
+__global__ void k_route_bucket(BatchArgs A) {
+    const int64_t count = *A.count;
+    const uint32_t *order = A.order + *A.start;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        A.ovf_list[atomicAdd(A.ovf_count, 1ull)] = order[i];
+}
+
+// ---- bucketing -----------------------------------------------------------
+struct Bounds {
+    int32_t v[NBUCKET];
+};
+
+__device__ inline int bucket_of(int n, const Bounds &bd) {
+    for (int b = 0; b < NBUCKET; ++b)
+        if (n <= bd.v[b]) return b;
+    return NBUCKET - 1;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// per-bucket counts (block-aggregated atomics)
+__global__ __launch_bounds__(256) void k_bucket_count(const uint32_t *__restrict__ pairs, int64_t np,
+                                                      const int64_t *__restrict__ offA,
+                                                      const int64_t *__restrict__ offB, Bounds bd,
+                                                      unsigned long long *__restrict__ counts) {
+    __shared__ unsigned int c[NBUCKET];
+    if (threadIdx.x < NBUCKET) c[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < np;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int n = static_cast<int>((offA[a + 1] - offA[a]) + (offB[b + 1] - offB[b]));
+        atomicAdd(&c[bucket_of(n, bd)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBUCKET && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
+}
+
+// bucket b's pairs go to order[start[b] + ...]; start = exclusive scan
+__global__ void k_bucket_starts(const unsigned long long *counts, int64_t *starts,
+                                int64_t *counts_i64, unsigned long long *fill) {
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int b = 0; b < NBUCKET; ++b) {
+            starts[b] = run;
+            counts_i64[b] = static_cast<int64_t>(counts[b]);
+            run += static_cast<int64_t>(counts[b]);
+            fill[b] = 0ull;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t *__restrict__ pairs,
+                                                        int64_t np, const int64_t *__restrict__ offA,
+                                                        const int64_t *__restrict__ offB, Bounds bd,
+                                                        const int64_t *__restrict__ starts,
+                                                        unsigned long long *__restrict__ fill,
+                                                        uint32_t *__restrict__ order) {
+    __shared__ unsigned int c[NBUCKET];
+    __shared__ unsigned long long base[NBUCKET];
+    for (int64_t p0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; p0 < np;
+         p0 += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (threadIdx.x < NBUCKET) c[threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t p = p0 + threadIdx.x;
+        int bk = -1;
+        unsigned int r = 0;
+        if (p < np) {
+            const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+            const int n = static_cast<int>((offA[a + 1] - offA[a]) + (offB[b + 1] - offB[b]));
+            bk = bucket_of(n, bd);
+            r = atomicAdd(&c[bk], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < NBUCKET && c[threadIdx.x])
+            base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], static_cast<unsigned long long>(c[threadIdx.x]));
+        __syncthreads();
+        if (bk >= 0) order[starts[bk] + static_cast<int64_t>(base[bk]) + r] = static_cast<uint32_t>(p);
+        __syncthreads();
+    }
+}
+
+int scap_for(int NV) { return 6 * NV; }
+
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+template <int TL>
+int launch_bucket(const BatchArgs &A, int waves, size_t lds, hipStream_t s) {
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_clip_batched<TL>, waves * 64, lds) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const unsigned grid = static_cast<unsigned>(device_cus() * per_cu);
+    PRT_LAUNCH(k_clip_batched<TL>, grid, waves * 64, lds, s, A);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}
+
+}  // namespace
+}  // namespace prt
+
+using namespace prt;
+
+extern "C" int prt_tune_default(prt_tune *t) {
+    PRT_REQUIRE(t != nullptr, PRT_E_INVAL, "tune_default: null");
+    const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
+    const int lanes[NBUCKET] = {8, 16, 32, 64, 64, 64};
+    const int waves[NBUCKET] = {4, 4, 4, 2, 1, 1};
+    for (int b = 0; b < NBUCKET; ++b) {
+        t->bucket_max_verts[b] = bounds[b];
+        t->lanes_per_pair[b] = lanes[b];
+        t->waves_per_block[b] = waves[b];
+    }
+    return PRT_OK;
+}
+
+namespace {
+int check_tune(const prt_tune *t) {
+    for (int b = 0; b < NBUCKET; ++b) {
+        const int L = t->lanes_per_pair[b], W = t->waves_per_block[b];
+        PRT_REQUIRE(L == 8 || L == 16 || L == 32 || L == 64, PRT_E_INVAL,
+                    "tune: lanes_per_pair[%d] = %d not in {8,16,32,64}", b, L);
+        PRT_REQUIRE(W >= 1 && W <= 8, PRT_E_INVAL, "tune: waves_per_block[%d] = %d", b, W);
+        PRT_REQUIRE(t->bucket_max_verts[b] >= 3 && (b == 0 || t->bucket_max_verts[b] >
+                                                                  t->bucket_max_verts[b - 1]),
+                    PRT_E_INVAL, "tune: bucket bounds must increase");
+    }
+    return PRT_OK;
+}
+
+struct BatchWs {
+    unsigned long long *counts, *fill, *tickets, *ovf_count;
+    int64_t *starts, *counts_i64;
+    uint32_t *order, *ovf_list;
+    void *plain_ws;
+    size_t plain_bytes;
+};
+
+template <class C> void batch_layout(C &c, int64_t np, int32_t max_verts, BatchWs *w) {
+    const size_t pb = plain_list_ws_bytes(np, max_verts);
+    auto u64 = c.template take<unsigned long long>(4 * NBUCKET + 1);
+    auto i64 = c.template take<int64_t>(2 * NBUCKET);
+    auto ord = c.template take<uint32_t>(np > 0 ? np : 1);
+    auto ovf = c.template take<uint32_t>(np > 0 ? np : 1);
+    auto pw = c.template take<char>(pb);
+    if (w) {
+        w->counts = u64;
+        w->fill = u64 + NBUCKET;
+        w->tickets = u64 + 2 * NBUCKET;
+        w->ovf_count = u64 + 4 * NBUCKET;
+        w->starts = i64;
+        w->counts_i64 = i64 + NBUCKET;
+        w->order = ord;
+        w->ovf_list = ovf;
+        w->plain_ws = pw;
+        w->plain_bytes = pb;
+    }
+}
+struct SizerT : Sizer {
+    template <class T> T *take(size_t n) {
+        Sizer::take<T>(n);
+        return nullptr;
+    }
+};
+}  // namespace
+
+extern "C" size_t prt_overlay_batched_workspace_size(int64_t npairs, int32_t max_verts) {
+    if (npairs < 0 || max_verts < 0) return 0;
+    SizerT s;
+    batch_layout(s, npairs, max_verts, nullptr);
+    return s.bytes();
+}
+
+extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                   const int64_t *offA, const double *xyB, const int64_t *offB,
+                                   int32_t max_verts, const prt_tune *tune, double *area, void *ws,
+                                   size_t ws_bytes, void *stream) {
+    PRT_REQUIRE(npairs >= 0 && npairs < (int64_t(1) << 32), PRT_E_INVAL,
+                "overlay_batched: npairs out of range");
+    if (npairs == 0) return PRT_OK;
+    PRT_REQUIRE(pairs && xyA && offA && xyB && offB && area, PRT_E_INVAL,
+                "overlay_batched: null device pointer");
+    PRT_REQUIRE(max_verts >= 0 && max_verts <= 2 * PRT_MAX_POLY_VERTS, PRT_E_INVAL,
+                "overlay_batched: max_verts %d out of range", max_verts);
+    prt_tune def;
+    if (!tune) {
+        prt_tune_default(&def);
+        tune = &def;
+    }
+    int rc = check_tune(tune);
+    if (rc) return rc;
+    const size_t need = prt_overlay_batched_workspace_size(npairs, max_verts);
+    PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "overlay_batched: workspace %zu < %zu bytes",
+                ws_bytes, need);
+    hipStream_t s = as_stream(stream);
+    Carver c(ws, ws_bytes);
+    BatchWs w;
+    batch_layout(c, npairs, max_verts, &w);
+    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "overlay_batched: workspace too small");
+
+    Bounds bd;
+    for (int b = 0; b < NBUCKET; ++b) bd.v[b] = tune->bucket_max_verts[b];
+    PRT_LAUNCH(k_zero_u64, 1, 64, 0, s, w.counts, 4 * NBUCKET + 1);
+    PRT_LAUNCHED();
+    const unsigned g = static_cast<unsigned>(ceil_div(npairs, 256) < 2048 ? ceil_div(npairs, 256) : 2048);
+    PRT_LAUNCH(k_bucket_count, g, 256, 0, s, pairs, npairs, offA, offB, bd, w.counts);
+    PRT_LAUNCHED();
+    PRT_LAUNCH(k_bucket_starts, 1, 64, 0, s, w.counts, w.starts, w.counts_i64, w.fill);
+    PRT_LAUNCHED();
+    PRT_LAUNCH(k_bucket_scatter, g, 256, 0, s, pairs, npairs, offA, offB, bd, w.starts, w.fill,
+               w.order);
+    PRT_LAUNCHED();
+
+    // largest buckets first: the long pairs start early
+    int max_lds = 0, dev = 0;
+    PRT_HIP(hipGetDevice(&dev));
+    PRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    for (int b = NBUCKET - 1; b >= 0; --b) {
+        const int lo = b == 0 ? 0 : tune->bucket_max_verts[b - 1];
+        if (max_verts <= lo) continue;  // no pair can fall in this bucket
+        BatchArgs A;
+        A.pairs = pairs;
+        A.order = w.order;
+        A.start = w.starts + b;
+        A.count = w.counts_i64 + b;
+        A.xyA = xyA;
+        A.xyB = xyB;
+        A.offA = offA;
+        A.offB = offB;
+        A.area = area;
+        A.ticket = w.tickets + b;
+        A.ovf_list = w.ovf_list;
+        A.ovf_count = w.ovf_count;
+        A.NV = tune->bucket_max_verts[b] < max_verts ? tune->bucket_max_verts[b] : max_verts;
+        A.SCAP = scap_for(A.NV);
+        const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
+        const size_t lds = static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
+        if (lds > static_cast<size_t>(max_lds)) {
+            // too large for LDS: the whole bucket takes the plain path
+            PRT_LAUNCH(k_route_bucket, 256, 256, 0, s, A);
+            PRT_LAUNCHED();
+            continue;
+        }
+        switch (L) {
+        case 8: rc = launch_bucket<8>(A, W, lds, s); break;
+        case 16: rc = launch_bucket<16>(A, W, lds, s); break;
+        case 32: rc = launch_bucket<32>(A, W, lds, s); break;
+        default: rc = launch_bucket<64>(A, W, lds, s); break;
+        }
+        if (rc) return rc;
+    }
+    return plain_list_launch(pairs, w.ovf_list, reinterpret_cast<const int64_t *>(w.ovf_count),
+                             npairs, xyA, offA, xyB, offB, max_verts, area, w.plain_ws,
+                             w.plain_bytes, s);
+}

@@ -269,21 +269,83 @@ __global__ __launch_bounds__(CP_THREADS) void k_clip_plain(
     }
 }
 
+// Same as k_clip_plain over the pairs list[0 .. *count) (overflow pairs of
+// the batched kernel); *count is read on the device.
+__global__ __launch_bounds__(CP_THREADS) void k_clip_plain_list(
+    const uint32_t *__restrict__ pairs, const uint32_t *__restrict__ list,
+    const int64_t *__restrict__ count, const double *__restrict__ xyA,
+    const int64_t *__restrict__ offA, const double *__restrict__ xyB,
+    const int64_t *__restrict__ offB, double *__restrict__ area, PlainWs w) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= w.T) return;
+    Lane L(w, t);
+    const int64_t np = *count;
+    for (int64_t i = t; i < np; i += w.T) {
+        const uint32_t p = list[i];
+        const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int64_t oa = offA[a], ob = offB[b];
+        const int nA = static_cast<int>(offA[a + 1] - oa);
+        const int nB = static_cast<int>(offB[b + 1] - ob);
+        area[p] = clip_one(L, xyA + 2 * oa, nA, xyB + 2 * ob, nB);
+    }
+}
+
 int64_t plain_lanes(int64_t np) {
     int64_t T = np < CP_MAX_LANES ? np : CP_MAX_LANES;
     return T < 1 ? 1 : T;
 }
 
+size_t plain_ws_bytes(int64_t T, int32_t max_verts) {
+    const int64_t cap = max_verts < 3 ? 3 : max_verts;
+    return static_cast<size_t>(T) * static_cast<size_t>(cap) * CP_BYTES_PER_SLOT + 16 * 256;
+}
+
+int carve_plain(PlainWs &w, int64_t T, int32_t max_verts, void *ws, size_t ws_bytes) {
+    w.T = T;
+    w.cap = max_verts < 3 ? 3 : max_verts;
+    Carver c(ws, ws_bytes);
+    const size_t cnt = static_cast<size_t>(w.T) * w.cap;
+    double **dptr[] = {&w.bx, &w.by, &w.tx, &w.ty, &w.dx, &w.ys, &w.xb, &w.xt, &w.xa, &w.xc, &w.bsum};
+    for (double **p : dptr) *p = c.take<double>(cnt);
+    int32_t **iptr[] = {&w.id, &w.eord, &w.act, &w.ord};
+    for (int32_t **p : iptr) *p = c.take<int32_t>(cnt);
+    return c.ok() ? PRT_OK : PRT_E_NOMEM;
+}
+
+// lanes of the overflow (list) launch: the list is short, a few thousand
+// lanes keep the arena small
+constexpr int64_t CP_LIST_LANES = 4096;
+
 }  // namespace
+
+size_t plain_list_ws_bytes(int64_t max_count, int32_t max_verts) {
+    int64_t T = max_count < CP_LIST_LANES ? max_count : CP_LIST_LANES;
+    return plain_ws_bytes(T < 1 ? 1 : T, max_verts);
+}
+
+int plain_list_launch(const uint32_t *pairs, const uint32_t *list, const int64_t *count,
+                      int64_t max_count, const double *xyA, const int64_t *offA, const double *xyB,
+                      const int64_t *offB, int32_t max_verts, double *area, void *ws,
+                      size_t ws_bytes, hipStream_t s) {
+    int64_t T = max_count < CP_LIST_LANES ? max_count : CP_LIST_LANES;
+    T = T < 1 ? 1 : T;
+    PlainWs w;
+    PRT_REQUIRE(carve_plain(w, T, max_verts, ws, ws_bytes) == PRT_OK, PRT_E_NOMEM,
+                "overlay: overflow workspace too small");
+    const unsigned grid = static_cast<unsigned>(ceil_div(w.T, CP_THREADS));
+    PRT_LAUNCH(k_clip_plain_list, grid, CP_THREADS, 0, s, pairs, list, count, xyA, offA, xyB, offB,
+               area, w);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}
+
 }  // namespace prt
 
 using namespace prt;
 
 extern "C" size_t prt_overlay_plain_workspace_size(int64_t npairs, int32_t max_verts) {
     if (npairs < 0 || max_verts < 0) return 0;
-    const int64_t T = plain_lanes(npairs);
-    const int64_t cap = max_verts < 3 ? 3 : max_verts;
-    return static_cast<size_t>(T) * static_cast<size_t>(cap) * CP_BYTES_PER_SLOT + 16 * 256;
+    return plain_ws_bytes(plain_lanes(npairs), max_verts);
 }
 
 extern "C" int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const double *xyA,
@@ -300,15 +362,8 @@ extern "C" int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const do
     PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "overlay_plain: workspace %zu < %zu bytes", ws_bytes,
                 need);
     PlainWs w;
-    w.T = plain_lanes(npairs);
-    w.cap = max_verts < 3 ? 3 : max_verts;
-    Carver c(ws, ws_bytes);
-    const size_t cnt = static_cast<size_t>(w.T) * w.cap;
-    double **dptr[] = {&w.bx, &w.by, &w.tx, &w.ty, &w.dx, &w.ys, &w.xb, &w.xt, &w.xa, &w.xc, &w.bsum};
-    for (double **p : dptr) *p = c.take<double>(cnt);
-    int32_t **iptr[] = {&w.id, &w.eord, &w.act, &w.ord};
-    for (int32_t **p : iptr) *p = c.take<int32_t>(cnt);
-    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "overlay_plain: workspace too small");
+    PRT_REQUIRE(carve_plain(w, plain_lanes(npairs), max_verts, ws, ws_bytes) == PRT_OK, PRT_E_NOMEM,
+                "overlay_plain: workspace too small");
     const unsigned grid = static_cast<unsigned>(ceil_div(w.T, CP_THREADS));
     PRT_LAUNCH(k_clip_plain, grid, CP_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA, xyB,
                offB, area, w);

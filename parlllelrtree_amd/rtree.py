@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import torch
 
-from ._lib import BUTreeStruct, TDTreeStruct, check, lib
+from ._lib import BUTreeStruct, TDTreeStruct, TuneStruct, check, lib
 
 __all__ = ["BottomUpTree", "TopDownTree", "build_bottom_up", "build_top_down", "join",
            "overlay_plain", "PolyDev", "to_device_polys", "morton_center30", "sort_pairs_u32"]
@@ -206,6 +206,32 @@ class PolyDev:
 def to_device_polys(p, device="cuda") -> PolyDev:
     return PolyDev(torch.from_numpy(p.xy).to(device), torch.from_numpy(p.off).to(device),
                    p.max_verts())
+
+
+def default_tune() -> TuneStruct:
+    t = TuneStruct()
+    check(lib().prt_tune_default(ctypes.byref(t)))
+    return t
+
+
+def overlay_batched(pairs: torch.Tensor, A: PolyDev, B: PolyDev, tune: TuneStruct | None = None,
+                    area: torch.Tensor | None = None, workspace: torch.Tensor | None = None
+                    ) -> torch.Tensor:
+    """Same areas as overlay_plain (bit-identical): bucketed, persistent,
+    work-stealing launches with the team shapes of `tune`."""
+    pairs = pairs.contiguous()
+    npairs = pairs.shape[0]
+    if area is None:
+        area = torch.empty(npairs, dtype=torch.float64, device=pairs.device)
+    mv = A.max_verts + B.max_verts
+    need = lib().prt_overlay_batched_workspace_size(npairs, mv)
+    ws = workspace if workspace is not None and workspace.numel() >= need else \
+        _ws(need, pairs.device)
+    tp = ctypes.byref(tune) if tune is not None else None
+    check(lib().prt_overlay_batched(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
+                                    _ptr(B.off), mv, tp, _ptr(area), _ptr(ws), ws.numel(),
+                                    _stream()))
+    return area
 
 
 def overlay_plain(pairs: torch.Tensor, A: PolyDev, B: PolyDev, area: torch.Tensor | None = None,

@@ -93,3 +93,80 @@ def test_workload1_first_100k_pairs(dev):
     assert len(cp) == 100_000
     assert err.max() <= REL_TOL
     assert np.array_equal(got, ref)
+
+
+def _grid_pairs():
+    S = gen.stars_grid(gen.seed(9), n=4000, grid=64, jitter=0.25, rfac=1.2, kmax=300)
+    T = gen.stars_grid(gen.seed(9, True), n=4000, grid=64, jitter=0.25, rfac=1.2, kmax=300)
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 4000, 20000)
+    b = np.clip(a + rng.integers(-70, 70, 20000), 0, 3999)
+    return np.stack([a, b], 1).astype(np.uint32), S, T
+
+
+def run_batched(pairs, A, B, tune=None):
+    area = rtree.overlay_batched(torch.from_numpy(pairs.view(np.int32)).cuda(),
+                                 rtree.to_device_polys(A), rtree.to_device_polys(B), tune)
+    return area.cpu().numpy()
+
+
+@pytest.mark.parametrize("lanes,waves", [(8, 4), (16, 4), (32, 2), (64, 1), (64, 4)])
+def test_batched_bit_exact_all_team_shapes(dev, lanes, waves):
+    pairs, S, T = _grid_pairs()
+    tune = rtree.default_tune()
+    for b in range(6):
+        tune.lanes_per_pair[b] = lanes
+        tune.waves_per_block[b] = waves
+    got = run_batched(pairs, S, T, tune)
+    ref = O.overlay(pairs, S, T, nthreads=8)
+    assert np.array_equal(got, ref), f"{(got != ref).sum()} areas differ"
+
+
+def test_batched_routes_large_buckets_to_plain(dev):
+    pairs, S, T = _grid_pairs()
+    tune = rtree.default_tune()
+    bounds = [8, 12, 16, 20, 24, 8192]   # almost every pair in the last (too large) bucket
+    for b in range(6):
+        tune.bucket_max_verts[b] = bounds[b]
+        tune.lanes_per_pair[b] = 8
+        tune.waves_per_block[b] = 8
+    got = run_batched(pairs, S, T, tune)
+    ref = O.overlay(pairs, S, T, nthreads=8)
+    assert np.array_equal(got, ref)
+
+
+def test_batched_golden_and_edges(dev):
+    cases = GOLD["clip"]
+    A = csr([c["P"] if len(c["P"]) else [[0, 0]] for c in cases])
+    B = csr([c["Q"] for c in cases])
+    idx = np.arange(len(cases), dtype=np.uint32)
+    got = run_batched(np.stack([idx, idx], 1), A, B)
+    np.testing.assert_allclose(got, [c["area"] for c in cases], rtol=1e-15, atol=1e-15)
+    empty = rtree.overlay_batched(torch.zeros((0, 2), dtype=torch.int32, device=dev),
+                                  rtree.to_device_polys(A), rtree.to_device_polys(B))
+    assert empty.numel() == 0
+
+
+def test_batched_workload1_first_100k_pairs(dev):
+    ra, rb = workloads.w1()
+    t = rtree.build_top_down(torch.from_numpy(ra).cuda(), 16)
+    pairs_qr = rtree.join(t, torch.from_numpy(rb).cuda()).cpu().numpy().view(np.uint32)
+    cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
+    got = run_batched(cp, A, B)
+    ref = O.overlay(cp, A, B, nthreads=8)
+    assert np.array_equal(got, ref)
+
+
+def test_batched_lognormal_grid_sample(dev):
+    """Stars of the 708x708 grid layers (lognormal 4..512 vertices): a
+    neighbourhood sample of candidate pairs, batched vs plain vs oracle."""
+    A, B = workloads.w2(60_000)
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 60_000, 30_000)
+    b = np.clip(a + rng.choice([-709, -708, -707, -1, 0, 1, 707, 708, 709], 30_000), 0, 59_999)
+    pairs = np.stack([a, b], 1).astype(np.uint32)
+    got = run_batched(pairs, A, B)
+    ref = O.overlay(pairs, A, B, nthreads=8)
+    assert (ref > 0).sum() > 10_000
+    assert np.array_equal(got, ref)
+    assert np.array_equal(run_plain(pairs, A, B), ref)
