@@ -78,6 +78,82 @@ def timed(fn, steps: int, warmup: int) -> float:
     return a.elapsed_time(b) / steps
 
 
+def overlay_phase(cp, A, B, dev, args, O, check_sample=None, plain_reps=(2, 1)):
+    """Plain vs batched FP64 overlay of pairs cp over CSR sets A, B; the
+    areas are compared with the oracle (on `check_sample` pairs if given)."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import rtree
+    from parlllelrtree_amd._lib import lib
+    from parlllelrtree_amd.autotune import autotune, tune_to_json
+
+    pd = torch.from_numpy(np.ascontiguousarray(cp).view(np.int32)).to(dev)
+    Ad, Bd = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
+    mv = Ad.max_verts + Bd.max_verts
+    npairs = len(cp)
+    out = {"pairs": npairs, "max_verts_pair": mv}
+    area_p = torch.empty(npairs, dtype=torch.float64, device=dev)
+    ws_p = torch.empty(lib().prt_overlay_plain_workspace_size(npairs, mv), dtype=torch.uint8,
+                       device=dev)
+    ms_p = timed(lambda: rtree.overlay_plain(pd, Ad, Bd, area_p, ws_p), *plain_reps)
+    del ws_p
+    tune, tstats = autotune(cp, A, B, Ad, Bd)
+    area_b = torch.empty(npairs, dtype=torch.float64, device=dev)
+    ws_b = torch.empty(lib().prt_overlay_batched_workspace_size(npairs, mv), dtype=torch.uint8,
+                       device=dev)
+    ms_b = timed(lambda: rtree.overlay_batched(pd, Ad, Bd, tune, area_b, ws_b), 5, 1)
+    got_p, got_b = area_p.cpu().numpy(), area_b.cpu().numpy()
+    sel = np.arange(npairs) if check_sample is None or check_sample >= npairs else \
+        np.sort(np.random.default_rng(0).choice(npairs, check_sample, replace=False))
+    ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16), with_flops=True)
+    pa = np.abs([O.poly_area(A.poly(i)) for i in cp[sel, 0]])
+    pb = np.abs([O.poly_area(B.poly(i)) for i in cp[sel, 1]])
+    den = np.maximum(np.abs(ref), 1e-12 * np.minimum(pa, pb))
+    den = np.where(den == 0, 1.0, den)
+    rel_p = np.abs(got_p[sel] - ref) / den
+    rel_b = np.abs(got_b[sel] - ref) / den
+    flops_all = flops * npairs / len(sel)
+    out.update({
+        "plain_ms": round(ms_p, 3), "plain_pairs_s": round(npairs / ms_p * 1e3, 1),
+        "batched_ms": round(ms_b, 3), "batched_pairs_s": round(npairs / ms_b * 1e3, 1),
+        "batched_vs_plain": round(ms_p / ms_b, 2),
+        "checked_pairs": int(len(sel)), "nonempty": int((ref > 0).sum()),
+        "max_rel_err_plain": float(rel_p.max()), "max_rel_err_batched": float(rel_b.max()),
+        "bit_identical": bool(np.array_equal(got_p[sel], ref) and np.array_equal(got_b[sel], ref)),
+        "plain_equals_batched": bool(np.array_equal(got_p, got_b)),
+        "total_area": float(got_b.sum()),
+        "oracle_fp64_flops_per_pair": round(flops / len(sel), 1),
+        "batched_fp64_tflops": round(flops_all / (ms_b * 1e-3) / 1e12, 4),
+        "tune": tune_to_json(tune, tstats)["buckets"]})
+    log(f"overlay {npairs} pairs: plain {ms_p:.2f} ms, batched {ms_b:.2f} ms "
+        f"({ms_p / ms_b:.2f}x), max rel err {max(rel_p.max(), rel_b.max()):.1e}")
+    return out
+
+
+def grid_phase(dev, args, O):
+    """2 x 500,000 stars on the jittered 708x708 grid: bottom-up tree of
+    layer A's MBRs, join with layer B's MBRs, overlay of every pair."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import gen, rtree, workloads
+
+    A, B = workloads.w2()
+    ma, mb = gen.poly_mbr(A), gen.poly_mbr(B)
+    da, db = torch.from_numpy(ma).to(dev), torch.from_numpy(mb).to(dev)
+    t = rtree.build_bottom_up(da, 16)
+    cap = rtree.join(t, db).shape[0]
+    j = rtree.Joiner(t, mb.shape[0], cap, dev)
+    ms_j = timed(lambda: j.run(db), 3, 1)
+    pqr = j.result().cpu().numpy().view(np.uint32)
+    cp = np.ascontiguousarray(pqr[:, ::-1])      # (A index, B index)
+    log(f"grid join {ms_j:.3f} ms ({cap} pairs)")
+    out = overlay_phase(cp, A, B, dev, args, O, check_sample=300_000, plain_reps=(1, 0))
+    out["join_ms"] = round(ms_j, 3)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,28 +232,11 @@ def main() -> int:
         phases["join_layers_1M_x_1M"] = {"ms": round(ms_j1, 4), "pairs": cap1,
                                          "Mpairs_s": round(cap1 / ms_j1 / 1e3, 1)}
         log(f"layer join {ms_j1:.3f} ms ({cap1} pairs)")
-        # plain FP64 overlay of the first 100,000 sorted pairs
+        # plain and batched FP64 overlay of the first 100,000 sorted pairs
         cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
-        pd = torch.from_numpy(cp.view(np.int32)).to(dev)
-        Ad, Bd = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
-        area = torch.empty(len(cp), dtype=torch.float64, device=dev)
-        from parlllelrtree_amd._lib import lib
-        ws = torch.empty(lib().prt_overlay_plain_workspace_size(len(cp), Ad.max_verts +
-                                                                 Bd.max_verts),
-                         dtype=torch.uint8, device=dev)
-        ms_cp = timed(lambda: rtree.overlay_plain(pd, Ad, Bd, area, ws), 5, 1)
-        got = area.cpu().numpy()
-        ref, flops = O.overlay(cp, A, B, nthreads=min(os.cpu_count() or 1, 16), with_flops=True)
-        pa = np.abs([O.poly_area(A.poly(i)) for i in cp[:, 0]])
-        pb = np.abs([O.poly_area(B.poly(i)) for i in cp[:, 1]])
-        den = np.maximum(np.abs(ref), 1e-12 * np.minimum(pa, pb))
-        rel = np.abs(got - ref) / np.where(den == 0, 1.0, den)
-        phases["overlay_plain_100k"] = {
-            "ms": round(ms_cp, 4), "pairs_s": round(len(cp) / ms_cp * 1e3, 1),
-            "max_rel_err": float(rel.max()), "bit_identical": bool(np.array_equal(got, ref)),
-            "nonempty": int((ref > 0).sum()), "oracle_fp64_flops": int(flops),
-            "fp64_tflops": round(flops / (ms_cp * 1e-3) / 1e12, 4)}
-        log(f"plain overlay {ms_cp:.3f} ms, max rel err {rel.max():.2e}")
+        phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, O)
+        # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate pair
+        phases["overlay_grid_2x500k"] = grid_phase(dev, args, O)
 
     cpu = None
     if not args.no_cpu_baseline:

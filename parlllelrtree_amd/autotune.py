@@ -1,0 +1,107 @@
+"""Autotuner of the batched overlay's launch shape per vertex bucket.
+
+For every bucket of n1 + n2 (the bounds of prt_tune_default), the pairs of
+a workload that fall into it are timed under each feasible team shape
+(lanes per pair x waves per block) and the fastest is kept.  The result is
+a TuneStruct for prt_overlay_batched, and optionally a JSON cache:
+
+    {"device": "...", "buckets": [{"max_verts": 32, "lanes": 8, "waves": 4,
+                                   "pairs": 1234, "ms": 0.12}, ...]}
+"""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import rtree
+from ._lib import PRT_TUNE_BUCKETS, TuneStruct
+
+LANES = (8, 16, 32, 64)
+WAVES = (1, 2, 4, 8)
+LDS_BYTES = 160 * 1024
+
+
+def _team_bytes(nv: int) -> int:
+    """LDS bytes of one team (mirrors team_bytes() of clip_batched.hip)."""
+    a16 = lambda v: (v + 15) & ~15  # noqa: E731
+    p2 = 1
+    while p2 < nv:
+        p2 <<= 1
+    return a16(40 * nv) + a16(8 * p2) + a16(8 * nv) + 2 * a16(4 * (nv + 1)) + a16(2 * 6 * nv)
+
+
+def _time(fn, reps: int = 3) -> float:
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune_to_json(t: TuneStruct, stats=None) -> dict:
+    out = {"device": torch.cuda.get_device_name(0), "buckets": []}
+    for b in range(PRT_TUNE_BUCKETS):
+        e = {"max_verts": t.bucket_max_verts[b], "lanes": t.lanes_per_pair[b],
+             "waves": t.waves_per_block[b]}
+        if stats and b in stats:
+            e.update(stats[b])
+        out["buckets"].append(e)
+    return out
+
+
+def tune_from_json(d: dict) -> TuneStruct:
+    t = TuneStruct()
+    for b, e in enumerate(d["buckets"]):
+        t.bucket_max_verts[b] = e["max_verts"]
+        t.lanes_per_pair[b] = e["lanes"]
+        t.waves_per_block[b] = e["waves"]
+    return t
+
+
+def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000,
+             cache: str | Path | None = None, reps: int = 3) -> tuple[TuneStruct, dict]:
+    """pairs: (np, 2) u32 host array; A, B: host PolyCSR; Ad, Bd: device
+    PolyDev of the same polygons.  Returns (tune, per-bucket stats)."""
+    tune = rtree.default_tune()
+    n = np.diff(A.off)[pairs[:, 0]] + np.diff(B.off)[pairs[:, 1]]
+    mv = Ad.max_verts + Bd.max_verts
+    lo = 0
+    stats = {}
+    rng = np.random.default_rng(0)
+    for b in range(PRT_TUNE_BUCKETS):
+        hi = tune.bucket_max_verts[b]
+        idx = np.nonzero((n > lo) & (n <= hi))[0]
+        lo = hi
+        if len(idx) == 0:
+            continue
+        if len(idx) > max_pairs_per_bucket:
+            idx = np.sort(rng.choice(idx, max_pairs_per_bucket, replace=False))
+        sub = torch.from_numpy(np.ascontiguousarray(pairs[idx]).view(np.int32)).cuda()
+        area = torch.empty(len(idx), dtype=torch.float64, device="cuda")
+        nv = min(hi, mv)
+        best = None
+        for lanes in LANES:
+            for waves in WAVES:
+                if waves * (64 // lanes) * _team_bytes(nv) > LDS_BYTES:
+                    continue
+                cand = rtree.default_tune()
+                for k in range(PRT_TUNE_BUCKETS):
+                    cand.lanes_per_pair[k] = lanes
+                    cand.waves_per_block[k] = waves
+                ms = _time(lambda: rtree.overlay_batched(sub, Ad, Bd, cand, area), reps)
+                if best is None or ms < best[0]:
+                    best = (ms, lanes, waves)
+        if best is not None:
+            tune.lanes_per_pair[b] = best[1]
+            tune.waves_per_block[b] = best[2]
+            stats[b] = {"pairs": int(len(idx)), "ms": round(best[0], 4)}
+    if cache is not None:
+        Path(cache).write_text(json.dumps(tune_to_json(tune, stats), indent=1) + "\n")
+    return tune, stats

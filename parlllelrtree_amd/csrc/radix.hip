@@ -21,9 +21,9 @@
 namespace prt {
 namespace {
 
-constexpr int RS_THREADS = 256;
+constexpr int RS_THREADS = 1024;
 constexpr int RS_WAVES = RS_THREADS / 64;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 items per tile
 constexpr int RS_BINS = 256;
 constexpr int SCAN_THREADS = 1024;
@@ -68,9 +68,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist(const uint64_t *__restrict_
         atomicAdd(&cnt[wave][d], 1u);
     }
     __syncthreads();
-    uint32_t s = 0;
-    for (int w = 0; w < RS_WAVES; ++w) s += cnt[w][tid];
-    hist[static_cast<int64_t>(tid) * t.num_tiles + blockIdx.x] = s;
+    if (tid < RS_BINS) {
+        uint32_t s = 0;
+        for (int w = 0; w < RS_WAVES; ++w) s += cnt[w][tid];
+        hist[static_cast<int64_t>(tid) * t.num_tiles + blockIdx.x] = s;
+    }
 }
 
 // Block-wide exclusive scan of one value per thread (blockDim.x == 1024).
@@ -100,26 +102,29 @@ __device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t *lds /*[32]
 }
 
 // Single-block-per-segment scan: offs[d][tile] = seg_base + exclusive prefix
-// over the segment's entries in digit-major order.
+// over the segment's entries in digit-major order.  Thread 4d + q owns the
+// q-th quarter of digit d's row (contiguous tiles), so thread order is the
+// digit-major order and no index arithmetic divides.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_seg(const uint32_t *__restrict__ hist,
                                                            Tiling t,
                                                            uint32_t *__restrict__ offs) {
+    static_assert(SCAN_THREADS == 4 * RS_BINS, "four threads per digit row");
     __shared__ uint32_t lds[32];
     const int64_t s = blockIdx.x;
     const int64_t tps = t.tiles_per_seg;
-    const int64_t len = tps * RS_BINS;
-    const int64_t per = ceil_div(len, SCAN_THREADS);
-    const int64_t k0 = threadIdx.x * per;
-    const int64_t k1 = k0 + per < len ? k0 + per : len;
-    auto addr = [&](int64_t k) { return (k / tps) * t.num_tiles + s * tps + (k % tps); };
+    const int d = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const int64_t per = (tps + 3) >> 2;
+    const int64_t j0 = q * per < tps ? q * per : tps;
+    const int64_t j1 = j0 + per < tps ? j0 + per : tps;
+    const uint32_t *row = hist + static_cast<int64_t>(d) * t.num_tiles + s * tps;
+    uint32_t *orow = offs + static_cast<int64_t>(d) * t.num_tiles + s * tps;
     uint32_t sum = 0;
-    for (int64_t k = k0; k < k1; ++k) sum += hist[addr(k)];
+    for (int64_t j = j0; j < j1; ++j) sum += row[j];
     uint32_t total;
     uint32_t run = block_excl_scan_1024(sum, lds, total) + static_cast<uint32_t>(s * t.seg);
-    for (int64_t k = k0; k < k1; ++k) {
-        int64_t a = addr(k);
-        uint32_t v = hist[a];
-        offs[a] = run;
+    for (int64_t j = j0; j < j1; ++j) {
+        uint32_t v = row[j];
+        orow[j] = run;
         run += v;
     }
 }
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restri
     const int count = static_cast<int>(e - b);
 
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    goff[tid] = offs[static_cast<int64_t>(tid) * t.num_tiles + blockIdx.x];
+    if (tid < RS_BINS) goff[tid] = offs[static_cast<int64_t>(tid) * t.num_tiles + blockIdx.x];
     __syncthreads();
 
     uint64_t item[RS_ITEMS];
@@ -223,16 +228,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restri
         rank[j] = valid ? base + r : 0xFFFFFFFFu;
     }
     __syncthreads();
-    // per digit: prefix over waves, total, then block exclusive scan of totals
+    // per digit (threads 0..255): prefix over waves and total; then a
+    // block-wide exclusive scan of the totals (other threads contribute 0)
     {
-        const int d = tid;
         uint32_t run = 0;
-        for (int w = 0; w < RS_WAVES; ++w) {
-            uint32_t c = wcnt[w][d];
-            wcnt[w][d] = run;
-            run += c;
+        if (tid < RS_BINS) {
+            for (int w = 0; w < RS_WAVES; ++w) {
+                uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = run;
+                run += c;
+            }
         }
-        uint32_t x = run;  // inclusive scan over d across the block
+        uint32_t x = run;
         for (int o = 1; o < 64; o <<= 1) {
             uint32_t y = __shfl_up(x, o, 64);
             if (lane >= o) x += y;
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restri
         __syncthreads();
         uint32_t before = 0;
         for (int w = 0; w < wave; ++w) before += red[w];
-        dstart[d] = before + x - run;
+        if (tid < RS_BINS) dstart[tid] = before + x - run;
     }
     __syncthreads();
     for (int j = 0; j < RS_ITEMS; ++j) {

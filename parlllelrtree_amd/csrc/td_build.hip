@@ -20,7 +20,7 @@
 namespace prt {
 namespace {
 
-constexpr int LS_THREADS = 256;
+constexpr int LS_THREADS = 1024;
 constexpr int LS_SLOTS = 4096;
 
 __global__ void k_td_items(const float4 *__restrict__ rect_cur, int64_t n, int comp,
