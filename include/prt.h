@@ -155,6 +155,12 @@ int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA
                         const int64_t *offA, const double *xyB, const int64_t *offB,
                         int32_t max_verts, const prt_tune *tune, double *area, void *ws,
                         size_t ws_bytes, void *stream);
+/* Pairs of the last prt_overlay_batched call on workspace `ws` that went to
+ * the plain path because they did not fit the team's LDS (synchronises). */
+int prt_overlay_batched_overflow(const void *ws, size_t ws_bytes, int64_t npairs,
+                                 int32_t max_verts, int64_t *count_h, void *stream);
+/* Largest LDS allocation of one block on the current device (bytes). */
+int prt_device_lds_limit(void);
 
 #ifdef __cplusplus
 }

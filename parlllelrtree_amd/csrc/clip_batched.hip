@@ -645,3 +645,28 @@ extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const 
                              npairs, xyA, offA, xyB, offB, max_verts, area, w.plain_ws,
                              w.plain_bytes, s);
 }
+
+// Number of pairs of the last prt_overlay_batched call on this workspace that
+// took the plain path (LDS capacity exceeded); synchronises the stream.
+extern "C" int prt_overlay_batched_overflow(const void *ws, size_t ws_bytes, int64_t npairs,
+                                            int32_t max_verts, int64_t *count_h, void *stream) {
+    PRT_REQUIRE(ws && count_h, PRT_E_INVAL, "overlay_batched_overflow: null pointer");
+    Carver c(const_cast<void *>(ws), ws_bytes);
+    BatchWs w;
+    batch_layout(c, npairs, max_verts, &w);
+    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "overlay_batched_overflow: workspace too small");
+    unsigned long long v = 0;
+    PRT_HIP(hipMemcpyAsync(&v, w.ovf_count, sizeof v, hipMemcpyDeviceToHost, as_stream(stream)));
+    PRT_HIP(hipStreamSynchronize(as_stream(stream)));
+    *count_h = static_cast<int64_t>(v);
+    return PRT_OK;
+}
+
+// Largest LDS allocation one block may request on the current device.
+extern "C" int prt_device_lds_limit(void) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return -1;
+    return v;
+}

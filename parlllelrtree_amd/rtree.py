@@ -215,10 +215,11 @@ def default_tune() -> TuneStruct:
 
 
 def overlay_batched(pairs: torch.Tensor, A: PolyDev, B: PolyDev, tune: TuneStruct | None = None,
-                    area: torch.Tensor | None = None, workspace: torch.Tensor | None = None
-                    ) -> torch.Tensor:
+                    area: torch.Tensor | None = None, workspace: torch.Tensor | None = None,
+                    stats: dict | None = None) -> torch.Tensor:
     """Same areas as overlay_plain (bit-identical): bucketed, persistent,
-    work-stealing launches with the team shapes of `tune`."""
+    work-stealing launches with the team shapes of `tune`.  With `stats`
+    given, stats["overflow"] = pairs that took the plain path (syncs)."""
     pairs = pairs.contiguous()
     npairs = pairs.shape[0]
     if area is None:
@@ -231,6 +232,11 @@ def overlay_batched(pairs: torch.Tensor, A: PolyDev, B: PolyDev, tune: TuneStruc
     check(lib().prt_overlay_batched(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
                                     _ptr(B.off), mv, tp, _ptr(area), _ptr(ws), ws.numel(),
                                     _stream()))
+    if stats is not None:
+        ov = ctypes.c_int64(0)
+        check(lib().prt_overlay_batched_overflow(_ptr(ws), ws.numel(), npairs, mv,
+                                                 ctypes.byref(ov), _stream()))
+        stats["overflow"] = ov.value
     return area
 
 

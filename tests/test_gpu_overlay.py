@@ -35,9 +35,15 @@ def csr(polys):
     return PolyCSR(np.ascontiguousarray(xy), off)
 
 
+def nan_area(n):
+    # every output slot starts as NaN: a slot the kernel does not write fails
+    return torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+
+
 def run_plain(pairs, A, B):
     area = rtree.overlay_plain(torch.from_numpy(pairs.view(np.int32)).cuda(),
-                               rtree.to_device_polys(A), rtree.to_device_polys(B))
+                               rtree.to_device_polys(A), rtree.to_device_polys(B),
+                               nan_area(len(pairs)))
     return area.cpu().numpy()
 
 
@@ -104,9 +110,13 @@ def _grid_pairs():
     return np.stack([a, b], 1).astype(np.uint32), S, T
 
 
-def run_batched(pairs, A, B, tune=None):
+def run_batched(pairs, A, B, tune=None, max_overflow=0):
+    st = {}
     area = rtree.overlay_batched(torch.from_numpy(pairs.view(np.int32)).cuda(),
-                                 rtree.to_device_polys(A), rtree.to_device_polys(B), tune)
+                                 rtree.to_device_polys(A), rtree.to_device_polys(B), tune,
+                                 nan_area(len(pairs)), stats=st)
+    if max_overflow is not None:
+        assert st["overflow"] <= max_overflow, f"{st['overflow']} pairs took the plain path"
     return area.cpu().numpy()
 
 
@@ -130,7 +140,7 @@ def test_batched_routes_large_buckets_to_plain(dev):
         tune.bucket_max_verts[b] = bounds[b]
         tune.lanes_per_pair[b] = 8
         tune.waves_per_block[b] = 8
-    got = run_batched(pairs, S, T, tune)
+    got = run_batched(pairs, S, T, tune, max_overflow=None)
     ref = O.overlay(pairs, S, T, nthreads=8)
     assert np.array_equal(got, ref)
 
