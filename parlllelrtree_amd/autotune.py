@@ -30,7 +30,8 @@ def _team_bytes(nv: int) -> int:
     p2 = 1
     while p2 < nv:
         p2 <<= 1
-    return a16(40 * nv) + a16(8 * p2) + a16(8 * nv) + 2 * a16(4 * (nv + 1)) + a16(2 * 6 * nv)
+    scap = min(nv * nv // 2, max(6 * nv, 4096))
+    return a16(40 * nv) + a16(8 * p2) + a16(8 * nv) + 2 * a16(4 * (nv + 1)) + a16(2 * scap)
 
 
 def _time(fn, reps: int = 3) -> float:

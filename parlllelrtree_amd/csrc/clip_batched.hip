@@ -131,21 +131,33 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
 }
 
 // Doubled area of beam k (sum over sub-beams, left to right), recomputing
-// every edge x from the LDS edge table.
-__device__ double beam_sum(const TeamLds &t, int k, int nP) {
-    const int lo = t.boff[k], hi = t.boff[k + 1];
-    const int m = hi - lo;
-    if (m < 2) return 0.0;
+// every edge x from the LDS edge table.  LIST: the beam's active edges are
+// list[boff[k] .. boff[k+1]); otherwise every edge slot [0, n) is tested
+// (scan mode, used when the lists do not fit the team's LDS).
+template <bool LIST>
+__device__ double beam_sum(const TeamLds &t, int k, int nP, int n) {
     const double yb = t.ys[k], yt = t.ys[k + 1];
+    const int lo = LIST ? t.boff[k] : 0, hi = LIST ? t.boff[k + 1] : n;
+    auto edge_at = [&](int i) -> int { return LIST ? static_cast<int>(t.list[i]) : i; };
+    auto active = [&](int e) -> bool { return LIST || (t.by[e] <= yb && t.ty[e] >= yt); };
+    int m = 0;
+    if (LIST) {
+        m = hi - lo;
+    } else {
+        for (int e = 0; e < n; ++e) m += active(e) ? 1 : 0;
+    }
+    if (m < 2) return 0.0;
     const double dy = yt - yb;
     double ya = yb, bs = 0.0;
     for (;;) {
         double yc = yt;
         for (int i = lo; i < hi; ++i) {
-            const int ei = t.list[i];
+            const int ei = edge_at(i);
+            if (!active(ei)) continue;
             const double xbi = ex(t, ei, yb), xti = ex(t, ei, yt);
             for (int j = i + 1; j < hi; ++j) {
-                const int ej = t.list[j];
+                const int ej = edge_at(j);
+                if (!active(ej)) continue;
                 const double xbj = ex(t, ej, yb), xtj = ex(t, ej, yt);
                 double xbL, xtL, xbR, xtR;
                 if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
@@ -163,7 +175,8 @@ __device__ double beam_sum(const TeamLds &t, int k, int nP) {
         int cur = -1;
         double kc = 0.0, xac = 0.0, xcc = 0.0;
         for (int i = lo; i < hi; ++i) {
-            const int e = t.list[i];
+            const int e = edge_at(i);
+            if (!active(e)) continue;
             const double xa = ex(t, e, ya), xc = ex(t, e, yc), kk = xa + xc;
             if (cur < 0 || key_before(kk, xa, e, kc, xac, cur)) {
                 cur = e; kc = kk; xac = xa; xcc = xc;
@@ -176,7 +189,8 @@ __device__ double beam_sum(const TeamLds &t, int k, int nP) {
             int nxt = -1;
             double kn = 0.0, xan = 0.0, xcn = 0.0;
             for (int i = lo; i < hi; ++i) {
-                const int e = t.list[i];
+                const int e = edge_at(i);
+                if (!active(e)) continue;
                 const double xa = ex(t, e, ya), xc = ex(t, e, yc), kk = xa + xc;
                 if (!key_before(kc, xac, cur, kk, xa, e)) continue;  // not after cur
                 if (nxt < 0 || key_before(kk, xa, e, kn, xan, nxt)) {
@@ -324,7 +338,18 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         run += t.cnt[k];
     }
     if (tl == 0) t.boff[nb] = S;
-    if (S > A.SCAP) return 0;
+    if (S > A.SCAP) {
+        // lists do not fit: every lane scans the edge table for its beams
+        team_sync();
+        for (int k = tl; k < nb; k += TL) t.bsum[k] = beam_sum<false>(t, k, nP, n);
+        team_sync();
+        for (int wdt = 1; wdt < nb; wdt *= 2) {
+            for (int i = tl * 2 * wdt; i + wdt < nb; i += TL * 2 * wdt) t.bsum[i] += t.bsum[i + wdt];
+            team_sync();
+        }
+        result = 0.5 * t.bsum[0];
+        return 1;
+    }
     team_sync();
     for (int k = tl; k < nb; k += TL) t.cnt[k] = 0;
     team_sync();
@@ -338,7 +363,7 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
             static_cast<uint16_t>(e);
     }
     team_sync();
-    for (int k = tl; k < nb; k += TL) t.bsum[k] = beam_sum(t, k, nP);
+    for (int k = tl; k < nb; k += TL) t.bsum[k] = beam_sum<true>(t, k, nP, n);
     team_sync();
     for (int wdt = 1; wdt < nb; wdt *= 2) {
         for (int i = tl * 2 * wdt; i + wdt < nb; i += TL * 2 * wdt) t.bsum[i] += t.bsum[i + wdt];
@@ -461,7 +486,13 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t *__restri
     }
 }
 
-int scap_for(int NV) { return 6 * NV; }
+// list capacity per team: every possible entry for small pairs, a few per
+// beam for large ones (larger pairs fall back to scan mode)
+int scap_for(int NV) {
+    const int all = NV * NV / 2;
+    const int few = 6 * NV > 4096 ? 6 * NV : 4096;
+    return all < few ? all : few;
+}
 
 int device_cus() {
     static int cus = 0;
