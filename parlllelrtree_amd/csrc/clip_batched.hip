@@ -527,8 +527,8 @@ using namespace prt;
 extern "C" int prt_tune_default(prt_tune *t) {
     PRT_REQUIRE(t != nullptr, PRT_E_INVAL, "tune_default: null");
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
-    const int lanes[NBUCKET] = {8, 16, 32, 64, 64, 64};
-    const int waves[NBUCKET] = {4, 4, 4, 2, 1, 1};
+    const int lanes[NBUCKET] = {16, 32, 64, 64, 64, 64};
+    const int waves[NBUCKET] = {4, 2, 2, 2, 1, 1};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];
         t->lanes_per_pair[b] = lanes[b];

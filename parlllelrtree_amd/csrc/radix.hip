@@ -12,8 +12,9 @@
 //     input order (stable);
 //   * the scatter is staged through LDS so that each digit's run is written
 //     contiguously (coalesced);
-//   * segments: a tile never spans two segments and every segment is scanned
-//     on its own, so one launch sequence sorts many independent segments (the
+//   * segments: a tile never spans two segments and the tile histograms are
+//     stored segment-major, so one linear scan gives every segment its own
+//     offsets and one launch sequence sorts many independent segments (the
 //     top-down build's per-level segmented sort, TopDown-rtree.cu:577-588).
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
@@ -27,7 +28,7 @@ constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 items per tile
 constexpr int RS_BINS = 256;
 constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_CHUNK = 4096;  // digit-major entries per scan block (multi-block path)
+constexpr int SCAN_CHUNK = 4096;  // histogram entries per scan block
 
 struct Tiling {
     int64_t n, seg;
@@ -53,7 +54,16 @@ __device__ inline void tile_range(const Tiling &t, int64_t tile, int64_t &b, int
     if (b > e) b = e;
 }
 
-// hist[d * num_tiles + tile] = number of items of the tile with digit d
+// Histogram layout, segment-major: entry (segment s, digit d, tile j of the
+// segment) lives at (s * 256 + d) * tiles_per_seg + j.  A linear exclusive
+// scan of this array then yields every item's output position directly:
+// the prefix before segment s is the size of the segments before it.
+__device__ inline int64_t hist_index(const Tiling &t, int64_t tile, int d) {
+    const int64_t s = tile / t.tiles_per_seg, j = tile % t.tiles_per_seg;
+    return (s * RS_BINS + d) * t.tiles_per_seg + j;
+}
+
+// hist[hist_index(tile, d)] = number of items of the tile with digit d
 __global__ __launch_bounds__(RS_THREADS) void k_hist(const uint64_t *__restrict__ in, Tiling t,
                                                      int shift, uint32_t mask,
                                                      uint32_t *__restrict__ hist) {
@@ -71,7 +81,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist(const uint64_t *__restrict_
     if (tid < RS_BINS) {
         uint32_t s = 0;
         for (int w = 0; w < RS_WAVES; ++w) s += cnt[w][tid];
-        hist[static_cast<int64_t>(tid) * t.num_tiles + blockIdx.x] = s;
+        hist[hist_index(t, blockIdx.x, tid)] = s;
     }
 }
 
@@ -101,36 +111,9 @@ __device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t *lds /*[32]
     return r;
 }
 
-// Single-block-per-segment scan: offs[d][tile] = seg_base + exclusive prefix
-// over the segment's entries in digit-major order.  Thread 4d + q owns the
-// q-th quarter of digit d's row (contiguous tiles), so thread order is the
-// digit-major order and no index arithmetic divides.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_seg(const uint32_t *__restrict__ hist,
-                                                           Tiling t,
-                                                           uint32_t *__restrict__ offs) {
-    static_assert(SCAN_THREADS == 4 * RS_BINS, "four threads per digit row");
-    __shared__ uint32_t lds[32];
-    const int64_t s = blockIdx.x;
-    const int64_t tps = t.tiles_per_seg;
-    const int d = threadIdx.x >> 2, q = threadIdx.x & 3;
-    const int64_t per = (tps + 3) >> 2;
-    const int64_t j0 = q * per < tps ? q * per : tps;
-    const int64_t j1 = j0 + per < tps ? j0 + per : tps;
-    const uint32_t *row = hist + static_cast<int64_t>(d) * t.num_tiles + s * tps;
-    uint32_t *orow = offs + static_cast<int64_t>(d) * t.num_tiles + s * tps;
-    uint32_t sum = 0;
-    for (int64_t j = j0; j < j1; ++j) sum += row[j];
-    uint32_t total;
-    uint32_t run = block_excl_scan_1024(sum, lds, total) + static_cast<uint32_t>(s * t.seg);
-    for (int64_t j = j0; j < j1; ++j) {
-        uint32_t v = row[j];
-        orow[j] = run;
-        run += v;
-    }
-}
-
-// Multi-block scan for one big segment (num_segs == 1): phase 1 reduces
-// chunks of SCAN_CHUNK entries, phase 2 scans the partials, phase 3 writes.
+// Linear exclusive scan of the histogram: phase 1 reduces chunks of
+// SCAN_CHUNK entries (coalesced), phase 2 scans the partials in one block,
+// phase 3 rescans each chunk and writes the offsets.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *__restrict__ hist,
                                                               int64_t len,
                                                               uint32_t *__restrict__ partials) {
@@ -183,8 +166,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t *__re
     }
 }
 
-// Stable scatter of one tile.  Wave w ranks the items [w*1024, (w+1)*1024) of
-// the tile in input order; lane l of round j holds item w*1024 + j*64 + l.
+// Stable scatter of one tile.  Wave w ranks the items [w*256, (w+1)*256) of
+// the tile in input order; lane l of round j holds item w*256 + j*64 + l.
 __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restrict__ in,
                                                         uint64_t *__restrict__ out, Tiling t,
                                                         int shift, int bits,
@@ -202,7 +185,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restri
     const int count = static_cast<int>(e - b);
 
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    if (tid < RS_BINS) goff[tid] = offs[static_cast<int64_t>(tid) * t.num_tiles + blockIdx.x];
+    if (tid < RS_BINS) goff[tid] = offs[hist_index(t, blockIdx.x, tid)];
     __syncthreads();
 
     uint64_t item[RS_ITEMS];
@@ -294,12 +277,10 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
         return PRT_OK;
     }
     Tiling t = make_tiling(n, seg);
-    const int64_t len = RS_BINS * t.tiles_per_seg;
-    const bool multi = t.num_segs == 1 && len > int64_t(SCAN_THREADS) * 64;
-    if (multi) {
-        PRT_REQUIRE(ceil_div(len, SCAN_CHUNK) <= SCAN_THREADS * 4, PRT_E_UNSUPPORTED,
-                    "radix sort: n=%lld too large for the two-level scan", (long long)n);
-    }
+    const int64_t len = RS_BINS * t.num_tiles;
+    const int64_t np = ceil_div(len, SCAN_CHUNK);
+    PRT_REQUIRE(np <= SCAN_THREADS * 4, PRT_E_UNSUPPORTED,
+                "radix sort: n=%lld too large for the two-level scan", (long long)n);
     const uint64_t *src = in;
     uint64_t *dst = (in == buf0) ? buf1 : buf0;
     for (int shift = bit_lo; shift < bit_hi; shift += 8) {
@@ -308,17 +289,11 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
         PRT_LAUNCH(k_hist, dim3(static_cast<unsigned>(t.num_tiles)), dim3(RS_THREADS), 0, s, src, t,
                    shift, mask, w.hist);
         PRT_LAUNCHED();
-        if (multi) {
-            int64_t np = ceil_div(len, SCAN_CHUNK);
-            PRT_LAUNCH(k_scan_reduce, dim3(static_cast<unsigned>(np)), dim3(SCAN_THREADS), 0, s,
-                       w.hist, len, w.partials);
-            PRT_LAUNCH(k_scan_partials, 1, SCAN_THREADS, 0, s, w.partials, np);
-            PRT_LAUNCH(k_scan_down, dim3(static_cast<unsigned>(np)), dim3(SCAN_THREADS), 0, s,
-                       w.hist, len, w.partials, w.offs);
-        } else {
-            PRT_LAUNCH(k_scan_seg, dim3(static_cast<unsigned>(t.num_segs)), dim3(SCAN_THREADS), 0, s,
-                       w.hist, t, w.offs);
-        }
+        PRT_LAUNCH(k_scan_reduce, dim3(static_cast<unsigned>(np)), dim3(SCAN_THREADS), 0, s, w.hist,
+                   len, w.partials);
+        PRT_LAUNCH(k_scan_partials, 1, SCAN_THREADS, 0, s, w.partials, np);
+        PRT_LAUNCH(k_scan_down, dim3(static_cast<unsigned>(np)), dim3(SCAN_THREADS), 0, s, w.hist,
+                   len, w.partials, w.offs);
         PRT_LAUNCHED();
         PRT_LAUNCH(k_scatter, dim3(static_cast<unsigned>(t.num_tiles)), dim3(RS_THREADS), 0, s, src,
                    dst, t, shift, bits, w.offs);
