@@ -127,7 +127,9 @@ def test_batched_bit_exact_all_team_shapes(dev, lanes, waves):
     for b in range(6):
         tune.lanes_per_pair[b] = lanes
         tune.waves_per_block[b] = waves
-    got = run_batched(pairs, S, T, tune)
+    # shapes whose teams do not fit LDS for the larger buckets route those
+    # buckets to the plain path: results must still be identical
+    got = run_batched(pairs, S, T, tune, max_overflow=None)
     ref = O.overlay(pairs, S, T, nthreads=8)
     assert np.array_equal(got, ref), f"{(got != ref).sum()} areas differ"
 
