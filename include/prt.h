@@ -161,6 +161,8 @@ int prt_overlay_batched_overflow(const void *ws, size_t ws_bytes, int64_t npairs
                                  int32_t max_verts, int64_t *count_h, void *stream);
 /* Largest LDS allocation of one block on the current device (bytes). */
 int prt_device_lds_limit(void);
+/* LDS bytes of one batched-overlay team for pairs of up to nv vertices. */
+int64_t prt_overlay_team_lds(int32_t nv);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import rtree
-from ._lib import PRT_TUNE_BUCKETS, TuneStruct
+from ._lib import PRT_TUNE_BUCKETS, TuneStruct, lib
 
 LANES = (8, 16, 32, 64)
 WAVES = (1, 2, 4, 8)
@@ -25,13 +25,8 @@ LDS_BYTES = 160 * 1024
 
 
 def _team_bytes(nv: int) -> int:
-    """LDS bytes of one team (mirrors team_bytes() of clip_batched.hip)."""
-    a16 = lambda v: (v + 15) & ~15  # noqa: E731
-    p2 = 1
-    while p2 < nv:
-        p2 <<= 1
-    scap = min(nv * nv // 2, max(6 * nv, 4096))
-    return a16(40 * nv) + a16(8 * p2) + a16(8 * nv) + 2 * a16(4 * (nv + 1)) + a16(2 * scap)
+    """LDS bytes of one team for pairs of up to nv vertices."""
+    return int(lib().prt_overlay_team_lds(nv))
 
 
 def _time(fn, reps: int = 3) -> float:

@@ -10,14 +10,18 @@
 //     live in LDS;
 //   * the scanbeam table is sorted with a bitonic network across the team;
 //   * every edge registers itself in the lists of the beams it spans (LDS
-//     atomics + a team scan), so a beam finds its active edges without
-//     scanning the polygon;
+//     atomics + a team scan), with its x at the beam bottom and top, so a
+//     beam finds its active edges without scanning the polygon;
+//   * a beam's entries are sorted by the sub-beam key; a beam whose sorted
+//     entries show no crossing is summed in one linear pass;
 //   * lane k sweeps beams k, k+TL, ...; beams are independent, their sums
 //     are combined with the fixed pairwise tree the oracle uses;
 //   * teams take the next pair from a per-bucket ticket (work stealing), so
 //     polygons of very uneven size do not leave lanes idle.
-// Pairs whose active-edge lists do not fit the team's LDS are handed to the
-// plain kernel through an overflow list (same results, slower).
+// The lists of a pair are built for windows of consecutive beams that fit
+// the team's LDS.  Only a pair larger than its bucket's team (or a single
+// beam with more active edges than the list capacity) goes to the plain
+// kernel, through an overflow list (same results, slower).
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -41,11 +45,13 @@ __device__ inline void team_sync() {
 
 struct TeamLds {
     double *bx, *by, *tx, *ty, *dx;  // [NV] edge slot = vertex index (P: i, Q: nP + j)
+    int32_t *ekb, *ekt;              // [NV] beams [ekb, ekt) spanned by the edge
     double *ys;                      // [P2] scanbeam table (sorted, then unique)
     double *bsum;                    // [NV] beam sums (also the unique-compaction buffer)
     int32_t *cnt;                    // [NV + 1]
     int32_t *boff;                   // [NV + 1]
-    uint16_t *list;                  // [SCAP]
+    uint16_t *list;                  // [SCAP] active edges of the beams of a window
+    double *lxb, *lxt;               // [SCAP] their x at the beam bottom and top
 };
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
@@ -58,8 +64,9 @@ __host__ __device__ inline int pow2_ge(int v) {
 
 __host__ __device__ inline size_t team_bytes(int NV, int SCAP) {
     const int P2 = pow2_ge(NV);
-    return align16(5 * 8 * size_t(NV)) + align16(8 * size_t(P2)) + align16(8 * size_t(NV)) +
-           2 * align16(4 * size_t(NV + 1)) + align16(2 * size_t(SCAP));
+    return align16(5 * 8 * size_t(NV)) + 2 * align16(4 * size_t(NV)) + align16(8 * size_t(P2)) +
+           align16(8 * size_t(NV)) + 2 * align16(4 * size_t(NV + 1)) + align16(2 * size_t(SCAP)) +
+           2 * align16(8 * size_t(SCAP));
 }
 
 __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
@@ -71,6 +78,10 @@ __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
     t.ty = t.tx + NV;
     t.dx = t.ty + NV;
     p += align16(5 * 8 * size_t(NV));
+    t.ekb = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV));
+    t.ekt = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV));
     t.ys = reinterpret_cast<double *>(p);
     p += align16(8 * size_t(P2));
     t.bsum = reinterpret_cast<double *>(p);
@@ -80,6 +91,10 @@ __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
     t.boff = reinterpret_cast<int32_t *>(p);
     p += align16(4 * size_t(NV + 1));
     t.list = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(SCAP));
+    t.lxb = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(SCAP));
+    t.lxt = reinterpret_cast<double *>(p);
     return t;
 }
 
@@ -130,35 +145,20 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
     return ea < eb;
 }
 
-// Doubled area of beam k (sum over sub-beams, left to right), recomputing
-// every edge x from the LDS edge table.  LIST: the beam's active edges are
-// list[boff[k] .. boff[k+1]); otherwise every edge slot [0, n) is tested
-// (scan mode, used when the lists do not fit the team's LDS).
-template <bool LIST>
-__device__ double beam_sum(const TeamLds &t, int k, int nP, int n) {
-    const double yb = t.ys[k], yt = t.ys[k + 1];
-    const int lo = LIST ? t.boff[k] : 0, hi = LIST ? t.boff[k + 1] : n;
-    auto edge_at = [&](int i) -> int { return LIST ? static_cast<int>(t.list[i]) : i; };
-    auto active = [&](int e) -> bool { return LIST || (t.by[e] <= yb && t.ty[e] >= yt); };
-    int m = 0;
-    if (LIST) {
-        m = hi - lo;
-    } else {
-        for (int e = 0; e < n; ++e) m += active(e) ? 1 : 0;
-    }
-    if (m < 2) return 0.0;
+// Doubled area of one beam whose active edges are the list entries
+// [lo, hi) (with their x at the beam bottom/top cached in lxb/lxt): the
+// general sweep of the beam, recomputing x at crossing heights.  The order
+// of the entries does not matter.
+__device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP) {
+    const int m = hi - lo;
     const double dy = yt - yb;
     double ya = yb, bs = 0.0;
     for (;;) {
         double yc = yt;
         for (int i = lo; i < hi; ++i) {
-            const int ei = edge_at(i);
-            if (!active(ei)) continue;
-            const double xbi = ex(t, ei, yb), xti = ex(t, ei, yt);
+            const double xbi = t.lxb[i], xti = t.lxt[i];
             for (int j = i + 1; j < hi; ++j) {
-                const int ej = edge_at(j);
-                if (!active(ej)) continue;
-                const double xbj = ex(t, ej, yb), xtj = ex(t, ej, yt);
+                const double xbj = t.lxb[j], xtj = t.lxt[j];
                 double xbL, xtL, xbR, xtR;
                 if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
                 else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
@@ -171,37 +171,86 @@ __device__ double beam_sum(const TeamLds &t, int k, int nP, int n) {
             }
         }
         const double h = yc - ya;
+        // x at the sub-beam ends: cached at the beam ends, recomputed inside
+        auto xat = [&](int i, double y) -> double {
+            if (y == yb) return t.lxb[i];
+            if (y == yt) return t.lxt[i];
+            return ex(t, t.list[i], y);
+        };
         // selection walk in key order
-        int cur = -1;
+        int cur = -1, ce = 0;
         double kc = 0.0, xac = 0.0, xcc = 0.0;
         for (int i = lo; i < hi; ++i) {
-            const int e = edge_at(i);
-            if (!active(e)) continue;
-            const double xa = ex(t, e, ya), xc = ex(t, e, yc), kk = xa + xc;
-            if (cur < 0 || key_before(kk, xa, e, kc, xac, cur)) {
-                cur = e; kc = kk; xac = xa; xcc = xc;
+            const int e = t.list[i];
+            const double xa = xat(i, ya), xc = xat(i, yc), kk = xa + xc;
+            if (cur < 0 || key_before(kk, xa, e, kc, xac, ce)) {
+                cur = i; ce = e; kc = kk; xac = xa; xcc = xc;
             }
         }
         int pp = 0, pq = 0;
         for (int step = 0; step + 1 < m; ++step) {
-            if (cur < nP) pp ^= 1;
+            if (ce < nP) pp ^= 1;
             else pq ^= 1;
-            int nxt = -1;
+            int nxt = -1, ne = 0;
             double kn = 0.0, xan = 0.0, xcn = 0.0;
             for (int i = lo; i < hi; ++i) {
-                const int e = edge_at(i);
-                if (!active(e)) continue;
-                const double xa = ex(t, e, ya), xc = ex(t, e, yc), kk = xa + xc;
-                if (!key_before(kc, xac, cur, kk, xa, e)) continue;  // not after cur
-                if (nxt < 0 || key_before(kk, xa, e, kn, xan, nxt)) {
-                    nxt = e; kn = kk; xan = xa; xcn = xc;
+                const int e = t.list[i];
+                const double xa = xat(i, ya), xc = xat(i, yc), kk = xa + xc;
+                if (!key_before(kc, xac, ce, kk, xa, e)) continue;  // not after cur
+                if (nxt < 0 || key_before(kk, xa, e, kn, xan, ne)) {
+                    nxt = i; ne = e; kn = kk; xan = xa; xcn = xc;
                 }
             }
             if (pp && pq) bs += ((xan - xac) + (xcn - xcc)) * h;
-            cur = nxt; kc = kn; xac = xan; xcc = xcn;
+            cur = nxt; ce = ne; kc = kn; xac = xan; xcc = xcn;
         }
         if (yc == yt) break;
         ya = yc;
+    }
+    return bs;
+}
+
+// Doubled area of one beam from its list entries [lo, hi).  The entries are
+// sorted in place by the sub-beam key (xb + xt, xb, id) (shell sort); when
+// both xb and xt are then non-decreasing no two edges cross inside the beam
+// and the gaps are summed left to right directly -- the same terms, in the
+// same order, as the general sweep.  Otherwise the general sweep runs.
+__device__ double beam_sum(const TeamLds &t, int lo, int hi, double yb, double yt, int nP) {
+    const int m = hi - lo;
+    if (m < 2) return 0.0;
+    const int gaps[4] = {23, 10, 4, 1};
+    for (int gi = 0; gi < 4; ++gi) {
+        const int g = gaps[gi];
+        if (g >= m) continue;
+        for (int i = lo + g; i < hi; ++i) {
+            const int e = t.list[i];
+            const double xb = t.lxb[i], xt = t.lxt[i], k = xb + xt;
+            int j = i;
+            while (j - g >= lo) {
+                const int e2 = t.list[j - g];
+                const double xb2 = t.lxb[j - g], xt2 = t.lxt[j - g];
+                if (!key_before(k, xb, e, xb2 + xt2, xb2, e2)) break;
+                t.list[j] = static_cast<uint16_t>(e2);
+                t.lxb[j] = xb2;
+                t.lxt[j] = xt2;
+                j -= g;
+            }
+            t.list[j] = static_cast<uint16_t>(e);
+            t.lxb[j] = xb;
+            t.lxt[j] = xt;
+        }
+    }
+    bool mono = true;
+    for (int i = lo + 1; i < hi && mono; ++i)
+        mono = !(t.lxb[i] < t.lxb[i - 1]) && !(t.lxt[i] < t.lxt[i - 1]);
+    if (!mono) return beam_sum_general(t, lo, hi, yb, yt, nP);
+    const double h = yt - yb;
+    double bs = 0.0;
+    int pp = 0, pq = 0;
+    for (int i = lo; i + 1 < hi; ++i) {
+        if (t.list[i] < nP) pp ^= 1;
+        else pq ^= 1;
+        if (pp && pq) bs += ((t.lxb[i + 1] - t.lxb[i]) + (t.lxt[i + 1] - t.lxt[i])) * h;
     }
     return bs;
 }
@@ -315,13 +364,17 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
     const int nb = NY - 1;
     for (int k = tl; k <= nb; k += TL) t.cnt[k] = 0;
     team_sync();
-    // count list entries per beam
+    // beam span of every edge; list entries per beam
     for (int e = tl; e < n; e += TL) {
         const double ty = t.ty[e];
-        if (ty == -INF) continue;
-        const int kb = lower_bound(t.ys, NY, t.by[e]);
-        int kt = lower_bound(t.ys, NY, ty);
-        kt = kt < nb ? kt : nb;
+        int kb = 0, kt = 0;
+        if (ty != -INF) {
+            kb = lower_bound(t.ys, NY, t.by[e]);
+            kt = lower_bound(t.ys, NY, ty);
+            kt = kt < nb ? kt : nb;
+        }
+        t.ekb[e] = kb;
+        t.ekt[e] = kt;
         for (int k = kb; k < kt; ++k) atomicAdd(&t.cnt[k], 1);
     }
     team_sync();
@@ -338,33 +391,38 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         run += t.cnt[k];
     }
     if (tl == 0) t.boff[nb] = S;
-    if (S > A.SCAP) {
-        // lists do not fit: every lane scans the edge table for its beams
-        team_sync();
-        for (int k = tl; k < nb; k += TL) t.bsum[k] = beam_sum<false>(t, k, nP, n);
-        team_sync();
-        for (int wdt = 1; wdt < nb; wdt *= 2) {
-            for (int i = tl * 2 * wdt; i + wdt < nb; i += TL * 2 * wdt) t.bsum[i] += t.bsum[i + wdt];
-            team_sync();
+    team_sync();
+    // windows of consecutive beams whose entries fit the list capacity
+    for (int k0 = 0; k0 < nb;) {
+        // k1 = largest beam end with boff[k1] - boff[k0] <= SCAP (at least k0 + 1)
+        int lo = k0 + 1, hi = nb;
+        const int lim = t.boff[k0] + A.SCAP;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (t.boff[mid] <= lim) lo = mid;
+            else hi = mid - 1;
         }
-        result = 0.5 * t.bsum[0];
-        return 1;
+        const int k1 = lo;
+        const int wbase = t.boff[k0];
+        if (t.boff[k1] - wbase > A.SCAP) return 0;  // one beam alone exceeds the lists
+        for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
+        team_sync();
+        for (int e = tl; e < n; e += TL) {
+            const int kb = t.ekb[e] > k0 ? t.ekb[e] : k0;
+            const int kt = t.ekt[e] < k1 ? t.ekt[e] : k1;
+            for (int k = kb; k < kt; ++k) {
+                const int pos = t.boff[k] - wbase + atomicAdd(&t.cnt[k], 1);
+                t.list[pos] = static_cast<uint16_t>(e);
+                t.lxb[pos] = ex(t, e, t.ys[k]);
+                t.lxt[pos] = ex(t, e, t.ys[k + 1]);
+            }
+        }
+        team_sync();
+        for (int k = k0 + tl; k < k1; k += TL)
+            t.bsum[k] = beam_sum(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k], t.ys[k + 1], nP);
+        team_sync();
+        k0 = k1;
     }
-    team_sync();
-    for (int k = tl; k < nb; k += TL) t.cnt[k] = 0;
-    team_sync();
-    for (int e = tl; e < n; e += TL) {
-        const double ty = t.ty[e];
-        if (ty == -INF) continue;
-        const int kb = lower_bound(t.ys, NY, t.by[e]);
-        int kt = lower_bound(t.ys, NY, ty);
-        kt = kt < nb ? kt : nb;
-        for (int k = kb; k < kt; ++k) t.list[t.boff[k] + atomicAdd(&t.cnt[k], 1)] =
-            static_cast<uint16_t>(e);
-    }
-    team_sync();
-    for (int k = tl; k < nb; k += TL) t.bsum[k] = beam_sum<true>(t, k, nP, n);
-    team_sync();
     for (int wdt = 1; wdt < nb; wdt *= 2) {
         for (int i = tl * 2 * wdt; i + wdt < nb; i += TL * 2 * wdt) t.bsum[i] += t.bsum[i + wdt];
         team_sync();
@@ -486,12 +544,11 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t *__restri
     }
 }
 
-// list capacity per team: every possible entry for small pairs, a few per
-// beam for large ones (larger pairs fall back to scan mode)
+// list capacity per team (entries of a window of beams; a pair whose beams
+// need more is processed in several windows)
 int scap_for(int NV) {
-    const int all = NV * NV / 2;
-    const int few = 6 * NV > 4096 ? 6 * NV : 4096;
-    return all < few ? all : few;
+    const int c = 4 * NV;
+    return c < 2048 ? c : 2048;
 }
 
 int device_cus() {
@@ -700,4 +757,11 @@ extern "C" int prt_device_lds_limit(void) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         return -1;
     return v;
+}
+
+// LDS bytes of one team of the batched overlay for pairs of up to nv
+// vertices (n1 + n2).
+extern "C" int64_t prt_overlay_team_lds(int32_t nv) {
+    if (nv < 3) nv = 3;
+    return static_cast<int64_t>(team_bytes(nv, scap_for(nv)));
 }
