@@ -212,9 +212,17 @@ def main() -> int:
     avg_ms = tot / cnt
     alg_bytes = BYTES_PER_ITEM[dom] * n
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # (scaled by a copy-kernel calibration), when recorded for this kernel
+    traffic = None
+    tpath = ROOT / "profiles" / "hbm_traffic.json"
+    if tpath.exists():
+        rec = json.loads(tpath.read_text()).get(dom)
+        if rec and rec.get("items_per_launch") == n:
+            traffic = rec["traffic_bytes"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_us": round(avg_ms * 1e3, 2),
+                "traffic": traffic, "avg_us": round(avg_ms * 1e3, 2),
                 "bytes_per_launch": alg_bytes,
                 "kernels_ms_per_step": {k: round(v[1] / 3, 4) for k, v in
                                         sorted(kt.stats.items(), key=lambda kv: -kv[1][1])}}
