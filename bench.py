@@ -154,6 +154,72 @@ def grid_phase(dev, args, O):
     return out
 
 
+def grad_phase(dev, args, O):
+    """262,144 independent star pairs (16..128 vertices): FP64 forward area
+    (batched), FP64 adjoint, FP32 adjoint (+ Green's area), errors against the
+    FP64 CPU oracle gradient and FP32 against FP64."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import rtree, workloads
+    from parlllelrtree_amd._lib import lib
+
+    pairs, S = workloads.w3()
+    npairs = len(pairs)
+    pd = torch.from_numpy(pairs.view(np.int32)).to(dev)
+    S64 = rtree.to_device_polys(S, dev)
+    S32 = rtree.to_device_polys(S, dev, dtype=torch.float32)
+    area = torch.empty(npairs, dtype=torch.float64, device=dev)
+    mv = 2 * S64.max_verts
+    ws = torch.empty(lib().prt_overlay_batched_workspace_size(npairs, mv), dtype=torch.uint8,
+                     device=dev)
+    ms_f = timed(lambda: rtree.overlay_batched(pd, S64, S64, None, area, ws), 3, 1)
+    gA, gB = torch.zeros_like(S64.xy), torch.zeros_like(S64.xy)
+
+    def bwd64():
+        gA.zero_()
+        gB.zero_()
+        rtree.area_grad(pd, S64, S64, gA=gA, gB=gB)
+    ms_b = timed(bwd64, 3, 1)
+    hA, hB = torch.zeros_like(S32.xy), torch.zeros_like(S32.xy)
+    gr32 = {}
+
+    def fb32():
+        hA.zero_()
+        hB.zero_()
+        gr32["r"] = rtree.area_grad(pd, S32, S32, gA=hA, gB=hB, green=True)
+    ms_32 = timed(fb32, 3, 1)
+    g64 = (gA + gB).cpu().numpy()
+    g32 = (hA + hB).double().cpu().numpy()
+    # FP64 gradient against the CPU oracle on a sample of pairs
+    rng = np.random.default_rng(0)
+    sel = np.sort(rng.choice(npairs, 2000, replace=False))
+    worst, exact, tot = 0.0, 0, 0
+    for t in sel:
+        a, b = pairs[t]
+        gP, gQ = O.area_grad(S.poly(a), S.poly(b))
+        for G, idx in ((gP, a), (gQ, b)):
+            mine = g64[S.off[idx]:S.off[idx + 1]]
+            sc = max(np.abs(G).max(), 1e-300)
+            worst = max(worst, np.abs(mine - G).max() / sc)
+            exact += int((mine == G).sum())
+            tot += G.size
+    rel32 = np.abs(g32 - g64).max() / np.abs(g64).max()
+    a64 = area.cpu().numpy()
+    a32 = gr32["r"][2].double().cpu().numpy()
+    arel32 = float(np.median(np.abs(a32 - a64) / np.maximum(np.abs(a64), 1e-300)))
+    out = {"pairs": npairs, "fp64_forward_ms": round(ms_f, 3), "fp64_adjoint_ms": round(ms_b, 3),
+           "fp64_pairs_s": round(npairs / (ms_f + ms_b) * 1e3, 1),
+           "fp32_forward_adjoint_ms": round(ms_32, 3),
+           "fp32_pairs_s": round(npairs / ms_32 * 1e3, 1),
+           "grad_rel_err_vs_oracle_fp64": worst, "grad_bit_identical_fraction": exact / tot,
+           "grad_rel_err_fp32_vs_fp64": float(rel32), "area_median_rel_err_fp32": arel32,
+           "checked_pairs": int(len(sel))}
+    log(f"grad: fwd {ms_f:.2f} ms, adjoint {ms_b:.2f} ms (fp64), fp32 {ms_32:.2f} ms; "
+        f"rel err vs oracle {worst:.1e}, fp32 vs fp64 {rel32:.1e}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +322,8 @@ def main() -> int:
         phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, O)
         # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate pair
         phases["overlay_grid_2x500k"] = grid_phase(dev, args, O)
+        # differentiable overlay: 262,144 independent pairs, area + adjoint
+        phases["overlay_grad_262144"] = grad_phase(dev, args, O)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -159,6 +159,22 @@ int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA
  * the plain path because they did not fit the team's LDS (synchronises). */
 int prt_overlay_batched_overflow(const void *ws, size_t ws_bytes, int64_t npairs,
                                  int32_t max_verts, int64_t *count_h, void *stream);
+/* ---- differentiable overlay (no reference counterpart: the reference
+ * computes no area) ----------------------------------------------------
+ * Accumulates into gA / gB (caller zeroes them) weights[t] * dArea_t /
+ * d(x, y) of every vertex of A[pairs[2t]] and B[pairs[2t+1]], with the
+ * clipping topology fixed (weights NULL = all ones); green_area[t] (if not
+ * NULL) gets the area by Green's theorem from the same parts.
+ * max_poly_verts = the largest vertex count of any single polygon. */
+int prt_overlay_area_grad(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                          const int64_t *offA, const double *xyB, const int64_t *offB,
+                          int32_t max_poly_verts, const double *weights, double *gA, double *gB,
+                          double *green_area, void *stream);
+int prt_overlay_area_grad_f32(const uint32_t *pairs, int64_t npairs, const float *xyA,
+                              const int64_t *offA, const float *xyB, const int64_t *offB,
+                              int32_t max_poly_verts, const float *weights, float *gA, float *gB,
+                              float *green_area, void *stream);
+
 /* Largest LDS allocation of one block on the current device (bytes). */
 int prt_device_lds_limit(void);
 /* LDS bytes of one batched-overlay team for pairs of up to nv vertices. */

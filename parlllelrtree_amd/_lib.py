@@ -88,6 +88,11 @@ SIGNATURES = {
     "prt_overlay_batched_overflow": (c_int, [c_void_p, c_size_t, c_int64, c_int32,
                                              POINTER(c_int64), c_void_p]),
     "prt_device_lds_limit": (c_int, []),
+    "prt_overlay_area_grad": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "prt_overlay_area_grad_f32": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p]),
     "prt_overlay_team_lds": (c_int64, [c_int32]),
     # host-only generators (csrc/gen.cpp)
     "prt_gen_u01": (c_double, [c_uint64, c_uint64]),

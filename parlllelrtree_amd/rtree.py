@@ -203,9 +203,32 @@ class PolyDev:
     max_verts: int
 
 
-def to_device_polys(p, device="cuda") -> PolyDev:
-    return PolyDev(torch.from_numpy(p.xy).to(device), torch.from_numpy(p.off).to(device),
-                   p.max_verts())
+def to_device_polys(p, device="cuda", dtype=torch.float64) -> PolyDev:
+    return PolyDev(torch.from_numpy(p.xy).to(device=device, dtype=dtype),
+                   torch.from_numpy(p.off).to(device), p.max_verts())
+
+
+def area_grad(pairs: torch.Tensor, A: PolyDev, B: PolyDev, weights: torch.Tensor | None = None,
+              gA: torch.Tensor | None = None, gB: torch.Tensor | None = None,
+              green: bool = False):
+    """Adjoint of the pair areas: returns (gA, gB[, green_area]) with
+    gA[v] = sum_t weights[t] * dArea_t/dA[v] (topology fixed).  FP64 or FP32
+    follows the dtype of A.xy / B.xy."""
+    pairs = pairs.contiguous()
+    dt = A.xy.dtype
+    if B.xy.dtype != dt or dt not in (torch.float64, torch.float32):
+        raise ValueError("A and B must both be float64 or both float32")
+    if gA is None:
+        gA = torch.zeros_like(A.xy)
+    if gB is None:
+        gB = torch.zeros_like(B.xy)
+    ga = torch.empty(pairs.shape[0], dtype=dt, device=pairs.device) if green else None
+    w = None if weights is None else weights.to(dt).contiguous()
+    fn = lib().prt_overlay_area_grad if dt == torch.float64 else lib().prt_overlay_area_grad_f32
+    mv = max(A.max_verts, B.max_verts, 3)
+    check(fn(_ptr(pairs), pairs.shape[0], _ptr(A.xy), _ptr(A.off), _ptr(B.xy), _ptr(B.off), mv,
+             _ptr(w), _ptr(gA), _ptr(gB), _ptr(ga), _stream()))
+    return (gA, gB, ga) if green else (gA, gB)
 
 
 def default_tune() -> TuneStruct:
