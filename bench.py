@@ -220,6 +220,58 @@ def grad_phase(dev, args, O):
     return out
 
 
+def w4_phase(dev, args, O):
+    """16,777,216 Gaussian-mixture rects: bottom-up and top-down build
+    throughput and the dominant build kernel against HBM (the working set
+    exceeds the 256 MB Infinity Cache); overlay of two 4,000,000-star layers
+    (join + plain vs batched)."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import gen, rtree, workloads
+    from parlllelrtree_amd.profiling import KernelTimes
+
+    n = 1 << 24
+    r = workloads.w4_rects(n)
+    d = torch.from_numpy(r).to(dev)
+    tb = rtree.build_bottom_up(d, 16)
+    ms_bu = timed(lambda: rtree.rebuild(tb, d), 5, 1)
+    tt = rtree.build_top_down(d, 16)
+    ms_td = timed(lambda: rtree.rebuild(tt, d), 3, 1)
+    with KernelTimes() as kt:
+        rtree.rebuild(tb, d)
+        rtree.rebuild(tt, d)
+        torch.cuda.synchronize()
+    # k_scatter over all 16.7M items: 16 B per item algorithmic
+    sc = [v for k, v in kt.stats.items() if k == "k_scatter"]
+    out = {"rects": n, "bu_ms": round(ms_bu, 3), "bu_Mrect_s": round(n / ms_bu / 1e3, 1),
+           "td_ms": round(ms_td, 3), "td_Mrect_s": round(n / ms_td / 1e3, 1),
+           "kernels_ms": {k: round(v[1], 3) for k, v in
+                          sorted(kt.stats.items(), key=lambda kv: -kv[1][1])[:8]}}
+    if sc:
+        cnt, tot = sc[0]
+        # the unsegmented passes (BU and TD level 0) process all n items per launch
+        out["k_scatter_avg_us"] = round(tot / cnt * 1e3, 2)
+    log(f"w4 builds: bu {ms_bu:.3f} ms ({n / ms_bu / 1e3:.0f} Mrect/s), td {ms_td:.3f} ms")
+    del tb, tt, d
+    torch.cuda.empty_cache()
+    A, B = workloads.w4_polys()
+    ma, mb = gen.poly_mbr(A), gen.poly_mbr(B)
+    da, db = torch.from_numpy(ma).to(dev), torch.from_numpy(mb).to(dev)
+    t = rtree.build_bottom_up(da, 16)
+    cap = rtree.join(t, db).shape[0]
+    j = rtree.Joiner(t, mb.shape[0], cap, dev)
+    ms_j = timed(lambda: j.run(db), 3, 1)
+    pqr = j.result().cpu().numpy().view(np.uint32)
+    cp = np.ascontiguousarray(pqr[:, ::-1])
+    del j, t
+    log(f"w4 join {ms_j:.3f} ms ({cap} pairs)")
+    ov = overlay_phase(cp, A, B, dev, args, O, check_sample=200_000, plain_reps=(1, 0))
+    ov["join_ms"] = round(ms_j, 3)
+    out["overlay_2x4M"] = ov
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +376,8 @@ def main() -> int:
         phases["overlay_grid_2x500k"] = grid_phase(dev, args, O)
         # differentiable overlay: 262,144 independent pairs, area + adjoint
         phases["overlay_grad_262144"] = grad_phase(dev, args, O)
+        # largest workload: 16.7M-rect builds, 2 x 4M-star overlay
+        phases["w4_gmm"] = w4_phase(dev, args, O)
 
     cpu = None
     if not args.no_cpu_baseline:
