@@ -12,7 +12,8 @@
       offset U[-0.3, 0.3]), area and gradient.
   w4: 16,777,216 rects from a 64-component Gaussian mixture (sigma 0.01,
       sides U[1e-5, 1e-4]); two layers of 4,000,000 mixture stars (radius
-      U[1e-5, 1e-4], lognormal vertex counts).
+      U[1e-4, 3e-4], lognormal vertex counts; a few neighbours per polygon
+      in the cluster cores, none in the tails).
 """
 from __future__ import annotations
 
@@ -64,4 +65,5 @@ def w4_rects(n: int = 1 << 24):
 
 
 def w4_polys(n: int = 4_000_000):
-    return gen.stars_gmm(gen.seed(4), n), gen.stars_gmm(gen.seed(4, True), n)
+    return (gen.stars_gmm(gen.seed(4), n, rmin=1e-4, rmax=3e-4),
+            gen.stars_gmm(gen.seed(4, True), n, rmin=1e-4, rmax=3e-4))
