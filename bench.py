@@ -125,7 +125,7 @@ def overlay_phase(cp, A, B, dev, args, O, check_sample=None, plain_reps=(2, 1)):
         "total_area": float(got_b.sum()),
         "oracle_fp64_flops_per_pair": round(flops / len(sel), 1),
         "batched_fp64_tflops": round(flops_all / (ms_b * 1e-3) / 1e12, 4),
-        "tune": tune_to_json(tune, tstats)["buckets"]})
+        "tune": tune_to_json(tune, tstats)["buckets"], "_tune": tune})
     log(f"overlay {npairs} pairs: plain {ms_p:.2f} ms, batched {ms_b:.2f} ms "
         f"({ms_p / ms_b:.2f}x), max rel err {max(rel_p.max(), rel_b.max()):.1e}")
     return out
@@ -151,7 +151,39 @@ def grid_phase(dev, args, O):
     log(f"grid join {ms_j:.3f} ms ({cap} pairs)")
     out = overlay_phase(cp, A, B, dev, args, O, check_sample=300_000, plain_reps=(1, 0))
     out["join_ms"] = round(ms_j, 3)
+    out["pipeline"] = pipeline_phase(A, B, cap, out.pop("_tune"), out, dev)
     return out
+
+
+def pipeline_phase(A, B, cap, tune, ov, dev):
+    """The whole overlay pipeline (polygon MBRs -> bottom-up tree over B ->
+    join with A -> batched clip -> total area) as one hipGraph, replayed;
+    against the eager launch sequence and the separately timed stages."""
+    import torch
+
+    from parlllelrtree_amd import rtree
+    from parlllelrtree_amd.pipeline import OverlayPipeline
+
+    dA, dB = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
+    pipe = OverlayPipeline(dA, dB, cap=cap + 1024, tree="bu", tune=tune)
+    ms_e = timed(pipe.run, 3, 1)
+    e_pairs, e_area, e_total, e_nonempty = pipe.result()
+    e_pairs, e_area = e_pairs.clone(), e_area.clone()
+    pipe.capture()
+    ms_g = timed(pipe.replay, 5, 1)
+    pairs, area, total, nonempty = pipe.result()
+    same = bool(torch.equal(pairs, e_pairs) and torch.equal(area.view(torch.int64),
+                                                              e_area.view(torch.int64))
+                and total == e_total and nonempty == e_nonempty)
+    n = int(pairs.shape[0])
+    pipe.close()
+    log(f"pipeline graph {ms_g:.2f} ms, eager {ms_e:.2f} ms, {n} pairs, replay==eager {same}")
+    return {"graph_ms": round(ms_g, 3), "eager_ms": round(ms_e, 3), "pairs": n,
+            "pairs_s": round(n / ms_g * 1e3, 1), "total_area": total, "nonempty": nonempty,
+            "graph_equals_eager": same, "pairs_match_join": n == cap,
+            "nonempty_matches_stage_run": nonempty == ov["nonempty"] if ov["checked_pairs"] == n
+            else None,
+            "total_rel_diff_vs_stage_run": abs(total - ov["total_area"]) / max(abs(total), 1e-300)}
 
 
 def grad_phase(dev, args, O):
@@ -268,6 +300,7 @@ def w4_phase(dev, args, O):
     log(f"w4 join {ms_j:.3f} ms ({cap} pairs)")
     ov = overlay_phase(cp, A, B, dev, args, O, check_sample=200_000, plain_reps=(1, 0))
     ov["join_ms"] = round(ms_j, 3)
+    ov.pop("_tune")
     out["overlay_2x4M"] = ov
     return out
 
@@ -279,6 +312,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-phases", action="store_true")
+    ap.add_argument("--phases", default="bu,layers,grid,grad,w4",
+                    help="comma list of the phases to run (bu,layers,grid,grad,w4)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -347,8 +382,10 @@ def main() -> int:
     log(f"dominant kernel {dom}: {avg_ms * 1e3:.1f} us avg, {achieved:.0f} GB/s algorithmic")
 
     phases = {}
-    if not args.no_phases:
+    want = set() if args.no_phases else set(args.phases.split(","))
+    if want:
         import oracle as O
+    if "bu" in want:
         # bottom-up build of 1,048,576 rects + 65,536 windows
         r0, q0 = workloads.w0()
         d0, dq0 = torch.from_numpy(r0).to(dev), torch.from_numpy(q0).to(dev)
@@ -361,6 +398,7 @@ def main() -> int:
         phases["join_windows_65536"] = {"ms": round(ms_j0, 4), "pairs": cap0,
                                         "Mpairs_s": round(cap0 / ms_j0 / 1e3, 1)}
         log(f"bu build {ms_bu:.3f} ms, window join {ms_j0:.3f} ms ({cap0} pairs)")
+    if "layers" in want:
         # layer-layer join (top-down tree of A, queries = B)
         cap1 = rtree.join(ta, db).shape[0]
         j1 = rtree.Joiner(ta, n, cap1, dev)
@@ -372,10 +410,15 @@ def main() -> int:
         # plain and batched FP64 overlay of the first 100,000 sorted pairs
         cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
         phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, O)
-        # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate pair
+        phases["overlay_layers_100k"].pop("_tune")
+    if "grid" in want:
+        # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate
+        # pair, stage by stage and as one captured pipeline
         phases["overlay_grid_2x500k"] = grid_phase(dev, args, O)
+    if "grad" in want:
         # differentiable overlay: 262,144 independent pairs, area + adjoint
         phases["overlay_grad_262144"] = grad_phase(dev, args, O)
+    if "w4" in want:
         # largest workload: 16.7M-rect builds, 2 x 4M-star overlay
         phases["w4_gmm"] = w4_phase(dev, args, O)
 

@@ -6,7 +6,8 @@
  * pointers), a caller-owned workspace and a hipStream_t passed as void*.
  * Calls are stream-ordered and never synchronise the device, except the
  * reference-shaped shim prt_run(), which synchronises like the reference.
- * No entry point allocates device memory except prt_run().
+ * No entry point allocates device memory.  prt_pipeline_capture allocates
+ * the host-side graph objects it returns.
  * Return value: 0 (PRT_OK) or a negative PRT_E_* code; prt_last_error()
  * returns a message for the last failure on the calling thread.
  *
@@ -28,7 +29,12 @@
  *                                              BU:439-483, TD:56-76)
  *   prt_overlay_plain                       <- Execute OV:81-113 +
  *                                              gpc_polygon_clip OV:115-750
- *   prt_run                                 <- run OV:61-79
+ *   prt_overlay_total                       <- count1 of printfOutput OV:45-58
+ *   prt_pipeline_*                          <- the stage sequence of BU:836-987
+ *                                              / TD:629-758 feeding run()
+ *                                              OV:61-79 (no single reference
+ *                                              function: its stages are
+ *                                              separate programs)
  */
 #ifndef PRT_H
 #define PRT_H
@@ -174,6 +180,66 @@ int prt_overlay_area_grad_f32(const uint32_t *pairs, int64_t npairs, const float
                               const int64_t *offA, const float *xyB, const int64_t *offB,
                               int32_t max_poly_verts, const float *weights, float *gA, float *gB,
                               float *green_area, void *stream);
+
+/* As prt_overlay_batched for min(*npairs_d, cap) pairs, the count read on
+ * the device (written there by a join): no host round trip, so the call can
+ * follow a join on one stream and be captured with it into a graph.
+ * Workspace: prt_overlay_batched_workspace_size(cap, max_verts). */
+int prt_overlay_batched_dev(const uint32_t *pairs, const int64_t *npairs_d, int64_t cap,
+                            const double *xyA, const int64_t *offA, const double *xyB,
+                            const int64_t *offB, int32_t max_verts, const prt_tune *tune,
+                            double *area, void *ws, size_t ws_bytes, void *stream);
+
+/* Sum of area[0 .. n) and the number of entries > 0 (the reference's count1
+ * of non-empty results, OV:45-58), n = min(*npairs_d, cap) or cap when
+ * npairs_d is NULL.  Deterministic (fixed reduction order for a given n).
+ * total / nonempty may each be NULL. */
+size_t prt_overlay_total_workspace_size(void);
+int prt_overlay_total(const double *area, const int64_t *npairs_d, int64_t cap, double *total,
+                      int64_t *nonempty, void *ws, size_t ws_bytes, void *stream);
+
+/* fp32 MBRs (xmin, ymin, xmax, ymax) of FP64 CSR polygons, each bound
+ * rounded outward so the box contains the polygon (the reference builds its
+ * tree on (int)(v*1e5) boxes, OV:805-808). */
+int prt_poly_mbr_dev(const double *xy, const int64_t *off, int64_t n, float *mbr, void *stream);
+
+/* ---- overlay pipeline: MBRs of A and B -> R-tree over B's MBRs -> join
+ * with A's MBRs -> batched FP64 clip of every pair -> total area ----------
+ * Pairs come out as (A id, B id) sorted by (A id, B id); count[0] = pairs
+ * found (only the first cap are kept and clipped: count[0] > cap means the
+ * capacity was too small), count[1] = join status (nonzero: a query had more
+ * than 8192 hits and its pairs are unsorted).  area[t] is the FP64 area of
+ * pair t; total / nonempty (optional) get the sum and the count of area > 0.
+ * prt_pipeline_run enqueues the sequence; prt_pipeline_capture records it
+ * once into a hipGraph on `stream` (a created stream, not the null stream)
+ * and prt_pipeline_launch replays it on any stream.  The graph holds the
+ * pointers of io and ws: replays read whatever those buffers then hold. */
+typedef struct prt_pipeline_io {
+    const double *xyA;
+    const int64_t *offA;
+    int64_t nA;
+    const double *xyB;
+    const int64_t *offB;
+    int64_t nB;
+    int32_t max_verts; /* >= largest (n_A + n_B) vertex count of any pair */
+    int32_t fanout;    /* 2..64 */
+    int32_t tree;      /* 0 bottom-up, 1 top-down */
+    int32_t reserved;
+    int64_t cap;       /* pair capacity */
+    uint32_t *pairs;   /* [2*cap] */
+    double *area;      /* [cap] */
+    int64_t *count;    /* [2] */
+    double *total;     /* [1] or NULL */
+    int64_t *nonempty; /* [1] or NULL */
+} prt_pipeline_io;
+typedef struct prt_pipeline prt_pipeline;
+size_t prt_pipeline_workspace_size(const prt_pipeline_io *io);
+int prt_pipeline_run(const prt_pipeline_io *io, const prt_tune *tune, void *ws, size_t ws_bytes,
+                     void *stream);
+int prt_pipeline_capture(const prt_pipeline_io *io, const prt_tune *tune, void *ws,
+                         size_t ws_bytes, void *stream, prt_pipeline **out);
+int prt_pipeline_launch(prt_pipeline *p, void *stream);
+int prt_pipeline_destroy(prt_pipeline *p);
 
 /* Largest LDS allocation of one block on the current device (bytes). */
 int prt_device_lds_limit(void);

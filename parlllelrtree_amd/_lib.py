@@ -52,6 +52,16 @@ class TuneStruct(ctypes.Structure):
     ]
 
 
+class PipelineIO(ctypes.Structure):
+    _fields_ = [
+        ("xyA", c_void_p), ("offA", c_void_p), ("nA", c_int64),
+        ("xyB", c_void_p), ("offB", c_void_p), ("nB", c_int64),
+        ("max_verts", c_int32), ("fanout", c_int32), ("tree", c_int32), ("reserved", c_int32),
+        ("cap", c_int64), ("pairs", c_void_p), ("area", c_void_p), ("count", c_void_p),
+        ("total", c_void_p), ("nonempty", c_void_p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol of include/prt.h plus the host
 # generators of csrc/gen.cpp
 SIGNATURES = {
@@ -94,6 +104,20 @@ SIGNATURES = {
                                           c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p]),
     "prt_overlay_team_lds": (c_int64, [c_int32]),
+    "prt_overlay_batched_dev": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_int32, POINTER(TuneStruct),
+                                        c_void_p, c_void_p, c_size_t, c_void_p]),
+    "prt_overlay_total_workspace_size": (c_size_t, []),
+    "prt_overlay_total": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                  c_size_t, c_void_p]),
+    "prt_poly_mbr_dev": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "prt_pipeline_workspace_size": (c_size_t, [POINTER(PipelineIO)]),
+    "prt_pipeline_run": (c_int, [POINTER(PipelineIO), POINTER(TuneStruct), c_void_p, c_size_t,
+                                 c_void_p]),
+    "prt_pipeline_capture": (c_int, [POINTER(PipelineIO), POINTER(TuneStruct), c_void_p,
+                                     c_size_t, c_void_p, POINTER(c_void_p)]),
+    "prt_pipeline_launch": (c_int, [c_void_p, c_void_p]),
+    "prt_pipeline_destroy": (c_int, [c_void_p]),
     # host-only generators (csrc/gen.cpp)
     "prt_gen_u01": (c_double, [c_uint64, c_uint64]),
     "prt_gen_rects_uniform": (None, [c_uint64, c_int64, c_double, c_double, c_void_p]),

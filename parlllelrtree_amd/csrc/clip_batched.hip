@@ -483,10 +483,12 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 }
 
 // per-bucket counts (block-aggregated atomics)
-__global__ __launch_bounds__(256) void k_bucket_count(const uint32_t *__restrict__ pairs, int64_t np,
+__global__ __launch_bounds__(256) void k_bucket_count(const uint32_t *__restrict__ pairs,
+                                                      const int64_t *np_d, int64_t cap,
                                                       const int64_t *__restrict__ offA,
                                                       const int64_t *__restrict__ offB, Bounds bd,
                                                       unsigned long long *__restrict__ counts) {
+    const int64_t np = device_count(np_d, cap);
     __shared__ unsigned int c[NBUCKET];
     if (threadIdx.x < NBUCKET) c[threadIdx.x] = 0;
     __syncthreads();
@@ -515,11 +517,13 @@ __global__ void k_bucket_starts(const unsigned long long *counts, int64_t *start
 }
 
 __global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t *__restrict__ pairs,
-                                                        int64_t np, const int64_t *__restrict__ offA,
+                                                        const int64_t *np_d, int64_t cap,
+                                                        const int64_t *__restrict__ offA,
                                                         const int64_t *__restrict__ offB, Bounds bd,
                                                         const int64_t *__restrict__ starts,
                                                         unsigned long long *__restrict__ fill,
                                                         uint32_t *__restrict__ order) {
+    const int64_t np = device_count(np_d, cap);
     __shared__ unsigned int c[NBUCKET];
     __shared__ unsigned long long base[NBUCKET];
     for (int64_t p0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; p0 < np;
@@ -651,10 +655,13 @@ extern "C" size_t prt_overlay_batched_workspace_size(int64_t npairs, int32_t max
     return s.bytes();
 }
 
-extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA,
-                                   const int64_t *offA, const double *xyB, const int64_t *offB,
-                                   int32_t max_verts, const prt_tune *tune, double *area, void *ws,
-                                   size_t ws_bytes, void *stream) {
+namespace {
+// npairs_d == nullptr: npairs pairs; else min(*npairs_d, npairs) pairs, read
+// on the device (nothing here reads device memory on the host)
+int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
+                 const double *xyA, const int64_t *offA, const double *xyB, const int64_t *offB,
+                 int32_t max_verts, const prt_tune *tune, double *area, void *ws, size_t ws_bytes,
+                 void *stream) {
     PRT_REQUIRE(npairs >= 0 && npairs < (int64_t(1) << 32), PRT_E_INVAL,
                 "overlay_batched: npairs out of range");
     if (npairs == 0) return PRT_OK;
@@ -683,12 +690,12 @@ extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const 
     PRT_LAUNCH(k_zero_u64, 1, 64, 0, s, w.counts, 4 * NBUCKET + 1);
     PRT_LAUNCHED();
     const unsigned g = static_cast<unsigned>(ceil_div(npairs, 256) < 2048 ? ceil_div(npairs, 256) : 2048);
-    PRT_LAUNCH(k_bucket_count, g, 256, 0, s, pairs, npairs, offA, offB, bd, w.counts);
+    PRT_LAUNCH(k_bucket_count, g, 256, 0, s, pairs, npairs_d, npairs, offA, offB, bd, w.counts);
     PRT_LAUNCHED();
     PRT_LAUNCH(k_bucket_starts, 1, 64, 0, s, w.counts, w.starts, w.counts_i64, w.fill);
     PRT_LAUNCHED();
-    PRT_LAUNCH(k_bucket_scatter, g, 256, 0, s, pairs, npairs, offA, offB, bd, w.starts, w.fill,
-               w.order);
+    PRT_LAUNCH(k_bucket_scatter, g, 256, 0, s, pairs, npairs_d, npairs, offA, offB, bd, w.starts,
+               w.fill, w.order);
     PRT_LAUNCHED();
 
     // largest buckets first: the long pairs start early
@@ -732,6 +739,24 @@ extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const 
     return plain_list_launch(pairs, w.ovf_list, reinterpret_cast<const int64_t *>(w.ovf_count),
                              npairs, xyA, offA, xyB, offB, max_verts, area, w.plain_ws,
                              w.plain_bytes, s);
+}
+}  // namespace
+
+extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                   const int64_t *offA, const double *xyB, const int64_t *offB,
+                                   int32_t max_verts, const prt_tune *tune, double *area, void *ws,
+                                   size_t ws_bytes, void *stream) {
+    return batched_impl(pairs, nullptr, npairs, xyA, offA, xyB, offB, max_verts, tune, area, ws,
+                        ws_bytes, stream);
+}
+
+extern "C" int prt_overlay_batched_dev(const uint32_t *pairs, const int64_t *npairs_d, int64_t cap,
+                                       const double *xyA, const int64_t *offA, const double *xyB,
+                                       const int64_t *offB, int32_t max_verts, const prt_tune *tune,
+                                       double *area, void *ws, size_t ws_bytes, void *stream) {
+    PRT_REQUIRE(npairs_d != nullptr, PRT_E_INVAL, "overlay_batched_dev: null pair count");
+    return batched_impl(pairs, npairs_d, cap, xyA, offA, xyB, offB, max_verts, tune, area, ws,
+                        ws_bytes, stream);
 }
 
 // Number of pairs of the last prt_overlay_batched call on this workspace that

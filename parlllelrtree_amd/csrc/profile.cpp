@@ -27,6 +27,7 @@ std::mutex g_mu;
 bool g_on = false;
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_free;
+thread_local int t_paused = 0;
 
 hipEvent_t get_event() {
     if (!g_free.empty()) {
@@ -41,10 +42,13 @@ hipEvent_t get_event() {
 
 }  // namespace
 
-bool profiling() { return g_on; }
+bool profiling() { return g_on && t_paused == 0; }
+
+ProfPause::ProfPause() { ++t_paused; }
+ProfPause::~ProfPause() { --t_paused; }
 
 ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s), a_(nullptr) {
-    if (!g_on) return;
+    if (!g_on || t_paused) return;
     std::lock_guard<std::mutex> lk(g_mu);
     a_ = get_event();
     if (a_) (void)hipEventRecord(static_cast<hipEvent_t>(a_), s_);

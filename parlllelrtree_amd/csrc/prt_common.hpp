@@ -61,6 +61,14 @@ struct Sizer {
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// A count produced on the device by an earlier launch, clamped to [0, cap];
+// cap itself when n_d is null.
+__device__ inline int64_t device_count(const int64_t *n_d, int64_t cap) {
+    if (!n_d) return cap;
+    const int64_t v = *n_d;
+    return v < 0 ? 0 : (v < cap ? v : cap);
+}
+
 // ---- key helpers (host and device) --------------------------------------
 
 // Order-preserving fp32 -> u32; -0 folds onto +0.

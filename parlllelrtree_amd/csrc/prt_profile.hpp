@@ -8,6 +8,16 @@ namespace prt {
 
 bool profiling();
 
+// While one of these lives, launches on the calling thread record no events
+// (used around stream capture, where the events would become graph nodes).
+class ProfPause {
+  public:
+    ProfPause();
+    ~ProfPause();
+    ProfPause(const ProfPause &) = delete;
+    ProfPause &operator=(const ProfPause &) = delete;
+};
+
 class ProfScope {
   public:
     ProfScope(const char *name, hipStream_t s);
