@@ -118,10 +118,11 @@ __host__ __device__ inline float4 box_union(float4 a, float4 b) {
 // Stable LSD radix sort of 64-bit items on bits [bit_lo, bit_hi), segmented:
 // items never leave their segment [s*seg, (s+1)*seg).  Returns the buffer
 // holding the result (one of buf0/buf1; `in` may alias buf0).
+// Keys of at most 32 bits (bit_hi - bit_lo <= 32).
 struct RadixWs {
-    uint32_t *hist;      // [256 * tiles]
-    uint32_t *offs;      // [256 * tiles]
-    uint32_t *partials;  // scan partial sums
+    uint32_t *ghist;     // per pass, segment and digit: item counts; + tickets
+    uint64_t *status;    // per pass, tile and digit: look-back words
+    int64_t ghist_cap, status_cap;  // words carved
 };
 size_t radix_ws_bytes(int64_t n, int64_t seg);
 int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w);

@@ -12,10 +12,15 @@
 //     input order (stable);
 //   * the scatter is staged through LDS so that each digit's run is written
 //     contiguously (coalesced);
-//   * segments: a tile never spans two segments and the tile histograms are
-//     stored segment-major, so one linear scan gives every segment its own
-//     offsets and one launch sequence sorts many independent segments (the
-//     top-down build's per-level segmented sort, TopDown-rtree.cu:577-588).
+//   * onesweep: one launch histograms every digit pass at once; each pass is
+//     then a single launch whose tiles get their global offsets by decoupled
+//     look-back over the earlier tiles (no per-pass histogram or scan
+//     launches; the reference runs a histogram and two scan levels per
+//     4-bit pass);
+//   * segments: a tile never spans two segments and histograms and look-back
+//     stay inside a segment, so one launch sequence sorts many independent
+//     segments (the top-down build's per-level segmented sort,
+//     TopDown-rtree.cu:577-588).
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -27,8 +32,7 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 items per tile
 constexpr int RS_BINS = 256;
-constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_CHUNK = 4096;  // histogram entries per scan block
+constexpr int RS_MAX_PASSES = 4;  // keys of up to 32 bits
 
 struct Tiling {
     int64_t n, seg;
@@ -54,139 +58,111 @@ __device__ inline void tile_range(const Tiling &t, int64_t tile, int64_t &b, int
     if (b > e) b = e;
 }
 
-// Histogram layout, segment-major: entry (segment s, digit d, tile j of the
-// segment) lives at (s * 256 + d) * tiles_per_seg + j.  A linear exclusive
-// scan of this array then yields every item's output position directly:
-// the prefix before segment s is the size of the segments before it.
-__device__ inline int64_t hist_index(const Tiling &t, int64_t tile, int d) {
-    const int64_t s = tile / t.tiles_per_seg, j = tile % t.tiles_per_seg;
-    return (s * RS_BINS + d) * t.tiles_per_seg + j;
+// ---- onesweep: one histogram launch for all digit passes, then one scatter
+// launch per pass whose tiles find their global offsets by decoupled look-back
+//
+// Look-back state: one 64-bit word per (pass, tile, digit): the top two bits
+// are the flag (0 = not yet published, 1 = the tile's own count of the digit,
+// 2 = inclusive count over the segment's tiles up to and including this one),
+// the low 62 bits the count.  Tiles take their index from an atomic ticket, so
+// every tile a block waits for belongs to a block that is already running and
+// publishes its own count before it looks back: the waits always end.
+constexpr uint64_t LB_AGG = 1ull << 62;
+constexpr uint64_t LB_INC = 2ull << 62;
+constexpr uint64_t LB_VAL = (1ull << 62) - 1;
+constexpr int LB_SPIN_LIMIT = 1 << 22;  // safety net only (seconds of polling)
+
+__device__ inline uint64_t lb_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void lb_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// hist[hist_index(tile, d)] = number of items of the tile with digit d
-__global__ __launch_bounds__(RS_THREADS) void k_hist(const uint64_t *__restrict__ in, Tiling t,
-                                                     int shift, uint32_t mask,
-                                                     uint32_t *__restrict__ hist) {
-    __shared__ uint32_t cnt[RS_WAVES][RS_BINS];
-    const int tid = threadIdx.x, wave = tid >> 6;
-    for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+// Lanes of the wave holding the same digit as this lane (valid lanes only).
+__device__ inline uint64_t digit_peers(uint32_t d, bool valid, int bits) {
+    uint64_t peers = __ballot(valid);
+    for (int k = 0; k < bits; ++k) {
+        const uint64_t bal = __ballot((d >> k) & 1u);
+        peers &= ((d >> k) & 1u) ? bal : ~bal;
+    }
+    return peers;
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0u;
+}
+
+// ghist[(pass * num_segs + s) * 256 + d] += items of tile blockIdx.x (in
+// segment s) whose digit of `pass` is d, for every pass; also clears the
+// tile's look-back words of every pass.
+__global__ __launch_bounds__(RS_THREADS) void k_hist_all(const uint64_t *__restrict__ in, Tiling t,
+                                                         int bit_lo, int bit_hi, int passes,
+                                                         uint32_t *__restrict__ ghist,
+                                                         uint64_t *__restrict__ status) {
+    __shared__ uint32_t cnt[RS_MAX_PASSES][RS_BINS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tile = blockIdx.x;
+    for (int i = tid; i < RS_MAX_PASSES * RS_BINS; i += RS_THREADS) (&cnt[0][0])[i] = 0u;
+    for (int i = tid; i < passes * RS_BINS; i += RS_THREADS)
+        status[(static_cast<int64_t>(i / RS_BINS) * t.num_tiles + tile) * RS_BINS + i % RS_BINS] = 0ull;
     __syncthreads();
     int64_t b, e;
-    tile_range(t, blockIdx.x, b, e);
-    for (int64_t i = b + tid; i < e; i += RS_THREADS) {
-        uint32_t d = static_cast<uint32_t>(in[i] >> shift) & mask;
-        atomicAdd(&cnt[wave][d], 1u);
-    }
-    __syncthreads();
-    if (tid < RS_BINS) {
-        uint32_t s = 0;
-        for (int w = 0; w < RS_WAVES; ++w) s += cnt[w][tid];
-        hist[hist_index(t, blockIdx.x, tid)] = s;
-    }
-}
-
-// Block-wide exclusive scan of one value per thread (blockDim.x == 1024).
-__device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t *lds /*[32]*/, uint32_t &total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) lds[wave] = x;
-    __syncthreads();
-    if (wave == 0) {
-        uint32_t w = lane < 16 ? lds[lane] : 0u;
-        for (int o = 1; o < 16; o <<= 1) {
-            uint32_t y = __shfl_up(w, o, 64);
-            if (lane >= o) w += y;
+    tile_range(t, tile, b, e);
+    const int count = static_cast<int>(e - b);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
+        const bool valid = local < count;
+        const uint64_t v = valid ? in[b + local] : 0ull;
+        for (int p = 0; p < passes; ++p) {
+            const int shift = bit_lo + 8 * p;
+            const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
+            const uint32_t d = static_cast<uint32_t>(v >> shift) & ((1u << bits) - 1u);
+            const uint64_t peers = digit_peers(d, valid, bits);
+            if (valid && (peers & lt) == 0)
+                atomicAdd(&cnt[p][d], static_cast<uint32_t>(__popcll(peers)));
         }
-        if (lane < 16) lds[16 + lane] = w;
     }
     __syncthreads();
-    total = lds[16 + 15];
-    uint32_t before = wave > 0 ? lds[16 + wave - 1] : 0u;
-    uint32_t r = before + x - v;
-    __syncthreads();
-    return r;
-}
-
-// Linear exclusive scan of the histogram: phase 1 reduces chunks of
-// SCAN_CHUNK entries (coalesced), phase 2 scans the partials in one block,
-// phase 3 rescans each chunk and writes the offsets.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t *__restrict__ hist,
-                                                              int64_t len,
-                                                              uint32_t *__restrict__ partials) {
-    __shared__ uint32_t lds[32];
-    int64_t b = static_cast<int64_t>(blockIdx.x) * SCAN_CHUNK;
-    uint32_t s = 0;
-    for (int i = threadIdx.x; i < SCAN_CHUNK; i += SCAN_THREADS)
-        if (b + i < len) s += hist[b + i];
-    uint32_t total;
-    block_excl_scan_1024(s, lds, total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(uint32_t *__restrict__ partials,
-                                                                int64_t np) {
-    __shared__ uint32_t lds[32];
-    // np <= SCAN_THREADS * 4 (checked on the host)
-    uint32_t v[4], s = 0;
-    for (int j = 0; j < 4; ++j) {
-        int64_t k = threadIdx.x * 4 + j;
-        v[j] = k < np ? partials[k] : 0u;
-        s += v[j];
-    }
-    uint32_t total;
-    uint32_t run = block_excl_scan_1024(s, lds, total);
-    for (int j = 0; j < 4; ++j) {
-        int64_t k = threadIdx.x * 4 + j;
-        if (k < np) partials[k] = run;
-        run += v[j];
+    const int64_t s = tile / t.tiles_per_seg;
+    for (int i = tid; i < passes * RS_BINS; i += RS_THREADS) {
+        const uint32_t c = cnt[i / RS_BINS][i % RS_BINS];
+        if (c) atomicAdd(&ghist[(static_cast<int64_t>(i / RS_BINS) * t.num_segs + s) * RS_BINS +
+                                i % RS_BINS], c);
     }
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t *__restrict__ hist,
-                                                            int64_t len,
-                                                            const uint32_t *__restrict__ partials,
-                                                            uint32_t *__restrict__ offs) {
-    __shared__ uint32_t lds[32];
-    constexpr int PER = SCAN_CHUNK / SCAN_THREADS;
-    int64_t b = static_cast<int64_t>(blockIdx.x) * SCAN_CHUNK + threadIdx.x * PER;
-    uint32_t v[PER], s = 0;
-    for (int j = 0; j < PER; ++j) {
-        v[j] = b + j < len ? hist[b + j] : 0u;
-        s += v[j];
-    }
-    uint32_t total;
-    uint32_t run = block_excl_scan_1024(s, lds, total) + partials[blockIdx.x];
-    for (int j = 0; j < PER; ++j) {
-        if (b + j < len) offs[b + j] = run;
-        run += v[j];
-    }
-}
-
-// Stable scatter of one tile.  Wave w ranks the items [w*256, (w+1)*256) of
-// the tile in input order; lane l of round j holds item w*256 + j*64 + l.
-__global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restrict__ in,
-                                                        uint64_t *__restrict__ out, Tiling t,
-                                                        int shift, int bits,
-                                                        const uint32_t *__restrict__ offs) {
+// Stable scatter of one tile for one digit pass.  Wave w ranks the items
+// [w*256, (w+1)*256) of the tile in input order; lane l of round j holds item
+// w*256 + j*64 + l.  Output position of an item with digit d = segment start
+// + (items of the segment with a smaller digit: scan of ghist) + (items with
+// digit d in earlier tiles of the segment: look-back) + rank in the tile.
+__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restrict__ in,
+                                                         uint64_t *__restrict__ out, Tiling t,
+                                                         int shift, int bits,
+                                                         const uint32_t *__restrict__ ghist_p,
+                                                         uint64_t *status_p,
+                                                         uint32_t *__restrict__ ticket) {
     __shared__ uint64_t stage[RS_TILE];
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
     __shared__ uint32_t dstart[RS_BINS];
     __shared__ uint32_t goff[RS_BINS];
-    __shared__ uint32_t red[RS_WAVES];
+    __shared__ uint32_t red[2][RS_WAVES];
+    __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t mask = (1u << bits) - 1u;
-    int64_t b, e;
-    tile_range(t, blockIdx.x, b, e);
-    const int count = static_cast<int>(e - b);
-
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    if (tid < RS_BINS) goff[tid] = offs[hist_index(t, blockIdx.x, tid)];
     __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t seg = tile / t.tiles_per_seg, jt = tile % t.tiles_per_seg;
+    int64_t b, e;
+    tile_range(t, tile, b, e);
+    const int count = static_cast<int>(e - b);
 
     uint64_t item[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
@@ -194,16 +170,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restri
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
         const bool valid = local < count;
-        uint64_t v = valid ? in[b + local] : 0ull;
-        uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
-        uint64_t peers = __ballot(valid);
-        for (int k = 0; k < bits; ++k) {
-            uint64_t bal = __ballot((d >> k) & 1u);
-            peers &= ((d >> k) & 1u) ? bal : ~bal;
-        }
+        const uint64_t v = valid ? in[b + local] : 0ull;
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
+        const uint64_t peers = digit_peers(d, valid, bits);
         uint32_t base = 0;
         if (valid) base = wcnt[wave][d];
-        uint32_t r = static_cast<uint32_t>(__popcll(peers & lt));
+        const uint32_t r = static_cast<uint32_t>(__popcll(peers & lt));
         __builtin_amdgcn_wave_barrier();
         if (valid && r == 0) wcnt[wave][d] = base + static_cast<uint32_t>(__popcll(peers));
         __builtin_amdgcn_wave_barrier();
@@ -211,58 +183,103 @@ __global__ __launch_bounds__(RS_THREADS) void k_scatter(const uint64_t *__restri
         rank[j] = valid ? base + r : 0xFFFFFFFFu;
     }
     __syncthreads();
-    // per digit (threads 0..255): prefix over waves and total; then a
-    // block-wide exclusive scan of the totals (other threads contribute 0)
-    {
-        uint32_t run = 0;
-        if (tid < RS_BINS) {
-            for (int w = 0; w < RS_WAVES; ++w) {
-                uint32_t c = wcnt[w][tid];
-                wcnt[w][tid] = run;
-                run += c;
+    // threads 0..255 own digit tid: tile count (published at once), prefix
+    // over waves; block-wide exclusive scans of the tile counts (dstart) and
+    // of the segment's digit totals (gbase)
+    uint32_t run = 0, tot = 0;
+    if (tid < RS_BINS) {
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const uint32_t c = wcnt[w][tid];
+            wcnt[w][tid] = run;
+            run += c;
+        }
+        lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
+        tot = ghist_p[seg * RS_BINS + tid];
+    }
+    uint32_t x = run, g = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64), h = __shfl_up(g, o, 64);
+        if (lane >= o) {
+            x += y;
+            g += h;
+        }
+    }
+    if (lane == 63) {
+        red[0][wave] = x;
+        red[1][wave] = g;
+    }
+    __syncthreads();
+    if (tid < RS_BINS) {
+        uint32_t before = 0, gbefore = 0;
+        for (int w = 0; w < wave; ++w) {
+            before += red[0][w];
+            gbefore += red[1][w];
+        }
+        dstart[tid] = before + x - run;
+        const uint32_t gbase = gbefore + g - tot;
+        // look-back over the earlier tiles of this segment
+        uint64_t excl = 0;
+        if (jt > 0) {
+            int64_t k = tile - 1;
+            int spins = 0;
+            for (;;) {
+                const uint64_t v = lb_load(&status_p[k * RS_BINS + tid]);
+                const uint64_t flag = v >> 62;
+                if (flag == 0) {
+                    if (++spins > LB_SPIN_LIMIT) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & LB_VAL;
+                if (flag == 2 || k == tile - jt) break;
+                --k;
             }
+            lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
         }
-        uint32_t x = run;
-        for (int o = 1; o < 64; o <<= 1) {
-            uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) red[wave] = x;
-        __syncthreads();
-        uint32_t before = 0;
-        for (int w = 0; w < wave; ++w) before += red[w];
-        if (tid < RS_BINS) dstart[tid] = before + x - run;
+        goff[tid] = static_cast<uint32_t>(seg * t.seg) + gbase + static_cast<uint32_t>(excl);
     }
     __syncthreads();
     for (int j = 0; j < RS_ITEMS; ++j) {
         if (rank[j] == 0xFFFFFFFFu) continue;
-        uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
+        const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
         stage[dstart[d] + wcnt[wave][d] + rank[j]] = item[j];
     }
     __syncthreads();
     for (int i = tid; i < count; i += RS_THREADS) {
-        uint64_t v = stage[i];
-        uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
+        const uint64_t v = stage[i];
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
         out[goff[d] + (i - dstart[d])] = v;
     }
+}
+
+struct SortLayout {
+    int64_t ghist_words;   // passes * segs * 256 + tickets
+    int64_t status_words;  // passes * tiles * 256
+};
+
+SortLayout sort_layout(const Tiling &t) {
+    SortLayout l;
+    l.ghist_words = RS_MAX_PASSES * t.num_segs * RS_BINS + RS_MAX_PASSES;
+    l.status_words = RS_MAX_PASSES * t.num_tiles * RS_BINS;
+    return l;
 }
 
 }  // namespace
 
 size_t radix_ws_bytes(int64_t n, int64_t seg) {
-    Tiling t = make_tiling(n > 0 ? n : 1, seg);
+    const SortLayout l = sort_layout(make_tiling(n > 0 ? n : 1, seg));
     Sizer s;
-    s.take<uint32_t>(RS_BINS * t.num_tiles);
-    s.take<uint32_t>(RS_BINS * t.num_tiles);
-    s.take<uint32_t>(ceil_div(RS_BINS * t.num_tiles, SCAN_CHUNK) + 1);
+    s.take<uint32_t>(l.ghist_words);
+    s.take<uint64_t>(l.status_words);
     return s.bytes();
 }
 
 int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w) {
-    Tiling t = make_tiling(n > 0 ? n : 1, seg);
-    w.hist = c.take<uint32_t>(RS_BINS * t.num_tiles);
-    w.offs = c.take<uint32_t>(RS_BINS * t.num_tiles);
-    w.partials = c.take<uint32_t>(ceil_div(RS_BINS * t.num_tiles, SCAN_CHUNK) + 1);
+    const SortLayout l = sort_layout(make_tiling(n > 0 ? n : 1, seg));
+    w.ghist = c.take<uint32_t>(l.ghist_words);
+    w.status = c.take<uint64_t>(l.status_words);
+    w.ghist_cap = l.ghist_words;
+    w.status_cap = l.status_words;
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 
@@ -270,33 +287,35 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
                    int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s, uint64_t **result) {
     PRT_REQUIRE(n >= 0 && n < (int64_t(1) << 32), PRT_E_INVAL, "radix sort: n=%lld out of range",
                 (long long)n);
-    PRT_REQUIRE(bit_lo >= 0 && bit_hi <= 64 && bit_lo < bit_hi, PRT_E_INVAL,
-                "radix sort: bad bit range [%d,%d)", bit_lo, bit_hi);
+    PRT_REQUIRE(bit_lo >= 0 && bit_hi <= 64 && bit_lo < bit_hi && bit_hi - bit_lo <= 8 * RS_MAX_PASSES,
+                PRT_E_INVAL, "radix sort: bad bit range [%d,%d)", bit_lo, bit_hi);
     if (n == 0) {
         *result = buf0;
         return PRT_OK;
     }
-    Tiling t = make_tiling(n, seg);
-    const int64_t len = RS_BINS * t.num_tiles;
-    const int64_t np = ceil_div(len, SCAN_CHUNK);
-    PRT_REQUIRE(np <= SCAN_THREADS * 4, PRT_E_UNSUPPORTED,
-                "radix sort: n=%lld too large for the two-level scan", (long long)n);
+    const Tiling t = make_tiling(n, seg);
+    const SortLayout l = sort_layout(t);
+    PRT_REQUIRE(l.ghist_words <= w.ghist_cap && l.status_words <= w.status_cap, PRT_E_NOMEM,
+                "radix sort: workspace carved for a smaller tiling");
+    const int passes = static_cast<int>((bit_hi - bit_lo + 7) / 8);
+    uint32_t *ghist = w.ghist;
+    uint32_t *tickets = w.ghist + RS_MAX_PASSES * t.num_segs * RS_BINS;
+    const int64_t zero_words = l.ghist_words;
+    const unsigned zg = static_cast<unsigned>(ceil_div(zero_words, 1024) < 1024 ? ceil_div(zero_words, 1024) : 1024);
+    PRT_LAUNCH(k_zero_u32, zg, 1024, 0, s, ghist, zero_words);
+    PRT_LAUNCHED();
+    const dim3 grid(static_cast<unsigned>(t.num_tiles));
+    PRT_LAUNCH(k_hist_all, grid, dim3(RS_THREADS), 0, s, in, t, bit_lo, bit_hi, passes, ghist,
+               w.status);
+    PRT_LAUNCHED();
     const uint64_t *src = in;
     uint64_t *dst = (in == buf0) ? buf1 : buf0;
-    for (int shift = bit_lo; shift < bit_hi; shift += 8) {
-        int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
-        uint32_t mask = (1u << bits) - 1u;
-        PRT_LAUNCH(k_hist, dim3(static_cast<unsigned>(t.num_tiles)), dim3(RS_THREADS), 0, s, src, t,
-                   shift, mask, w.hist);
-        PRT_LAUNCHED();
-        PRT_LAUNCH(k_scan_reduce, dim3(static_cast<unsigned>(np)), dim3(SCAN_THREADS), 0, s, w.hist,
-                   len, w.partials);
-        PRT_LAUNCH(k_scan_partials, 1, SCAN_THREADS, 0, s, w.partials, np);
-        PRT_LAUNCH(k_scan_down, dim3(static_cast<unsigned>(np)), dim3(SCAN_THREADS), 0, s, w.hist,
-                   len, w.partials, w.offs);
-        PRT_LAUNCHED();
-        PRT_LAUNCH(k_scatter, dim3(static_cast<unsigned>(t.num_tiles)), dim3(RS_THREADS), 0, s, src,
-                   dst, t, shift, bits, w.offs);
+    for (int p = 0; p < passes; ++p) {
+        const int shift = bit_lo + 8 * p;
+        const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
+        PRT_LAUNCH(k_onesweep, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
+                   ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS,
+                   w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS, tickets + p);
         PRT_LAUNCHED();
         src = dst;
         dst = (dst == buf0) ? buf1 : buf0;

@@ -210,7 +210,7 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
     float4 *ra = c.take<float4>(n), *rb = c.take<float4>(n);
     uint32_t *da = c.take<uint32_t>(n), *db = c.take<uint32_t>(n);
     const int L = tree->num_levels, f = fanout;
-    RadixWs rw{nullptr, nullptr, nullptr};
+    RadixWs rw{};
     {
         int64_t worst = 0;
         size_t rad = 0;
