@@ -186,6 +186,41 @@ def pipeline_phase(A, B, cap, tune, ov, dev):
             "total_rel_diff_vs_stage_run": abs(total - ov["total_area"]) / max(abs(total), 1e-300)}
 
 
+def grid_batched_phase(dev, args, O):
+    """Batched FP64 overlay alone (default team shapes, no plain run, no
+    autotune): every candidate pair of the 2 x 500,000-star grid.  The target
+    of the FP64 VALU counter runs; FP64 flops per pair from the instrumented
+    oracle on a 50,000-pair sample."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import rtree, workloads
+    from parlllelrtree_amd._lib import lib
+    from parlllelrtree_amd.pipeline import OverlayPipeline
+
+    A, B = workloads.w2()
+    dA, dB = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
+    pipe = OverlayPipeline(dA, dB, cap=12_000_000, tree="bu")
+    pipe.run()
+    pairs, _, _, _ = pipe.result()
+    pd = pairs.contiguous()
+    npairs = int(pd.shape[0])
+    mv = dA.max_verts + dB.max_verts
+    area = torch.empty(npairs, dtype=torch.float64, device=dev)
+    ws = torch.empty(lib().prt_overlay_batched_workspace_size(npairs, mv), dtype=torch.uint8,
+                     device=dev)
+    ms = timed(lambda: rtree.overlay_batched(pd, dA, dB, None, area, ws), 3, 1)
+    sel = np.sort(np.random.default_rng(0).choice(npairs, 50_000, replace=False))
+    cp = pd.cpu().numpy().view(np.uint32)
+    ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16), with_flops=True)
+    same = bool(np.array_equal(area.cpu().numpy()[sel], ref))
+    fpp = flops / len(sel)
+    log(f"batched overlay {npairs} pairs {ms:.2f} ms, {fpp:.0f} oracle FP64 flops/pair")
+    return {"pairs": npairs, "batched_ms": round(ms, 3), "bit_identical_sample": same,
+            "oracle_fp64_flops_per_pair": round(fpp, 1),
+            "oracle_fp64_tflops": round(fpp * npairs / (ms * 1e-3) / 1e12, 4)}
+
+
 def grad_phase(dev, args, O):
     """262,144 independent star pairs (16..128 vertices): FP64 forward area
     (batched), FP64 adjoint, FP32 adjoint (+ Green's area), errors against the
@@ -313,7 +348,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--phases", default="bu,layers,grid,grad,w4",
-                    help="comma list of the phases to run (bu,layers,grid,grad,w4)")
+                    help="comma list of the phases to run (bu,layers,grid,grad,w4; "
+                         "also grid_batched: the batched overlay alone)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -415,6 +451,8 @@ def main() -> int:
         # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate
         # pair, stage by stage and as one captured pipeline
         phases["overlay_grid_2x500k"] = grid_phase(dev, args, O)
+    if "grid_batched" in want:
+        phases["overlay_grid_batched"] = grid_batched_phase(dev, args, O)
     if "grad" in want:
         # differentiable overlay: 262,144 independent pairs, area + adjoint
         phases["overlay_grad_262144"] = grad_phase(dev, args, O)

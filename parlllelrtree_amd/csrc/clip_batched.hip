@@ -52,6 +52,12 @@ struct TeamLds {
     int32_t *boff;                   // [NV + 1]
     uint16_t *list;                  // [SCAP] active edges of the beams of a window
     double *lxb, *lxt;               // [SCAP] their x at the beam bottom and top
+    // team-cooperative sweep of beams with crossings (GS entries, GC heights)
+    double *gxa, *gxc;               // [GS] x of the entries at the sub-beam ends
+    uint16_t *gidx;                  // [GS] their edge ids
+    double *giy;                     // [GC] crossing heights of the beam
+    int32_t *gmisc;                  // [4] deferred-beam count, crossing count
+    int GS, GC;
 };
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
@@ -62,11 +68,19 @@ __host__ __device__ inline int pow2_ge(int v) {
     return p;
 }
 
+// entries (and crossing heights) of one beam the team sweeps cooperatively
+__host__ __device__ inline int gen_cap(int NV) {
+    const int p = pow2_ge(NV);
+    return p < 512 ? p : 512;
+}
+
 __host__ __device__ inline size_t team_bytes(int NV, int SCAP) {
     const int P2 = pow2_ge(NV);
+    const int G = gen_cap(NV);
     return align16(5 * 8 * size_t(NV)) + 2 * align16(4 * size_t(NV)) + align16(8 * size_t(P2)) +
            align16(8 * size_t(NV)) + 2 * align16(4 * size_t(NV + 1)) + align16(2 * size_t(SCAP)) +
-           2 * align16(8 * size_t(SCAP));
+           2 * align16(8 * size_t(SCAP)) + 2 * align16(8 * size_t(G)) + align16(2 * size_t(G)) +
+           align16(8 * size_t(G)) + align16(4 * 4);
 }
 
 __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
@@ -95,6 +109,19 @@ __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
     t.lxb = reinterpret_cast<double *>(p);
     p += align16(8 * size_t(SCAP));
     t.lxt = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(SCAP));
+    const int G = gen_cap(NV);
+    t.GS = G;
+    t.GC = G;
+    t.gxa = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(G));
+    t.gxc = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(G));
+    t.gidx = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(G));
+    t.giy = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(G));
+    t.gmisc = reinterpret_cast<int32_t *>(p);
     return t;
 }
 
@@ -215,9 +242,15 @@ __device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, 
 // both xb and xt are then non-decreasing no two edges cross inside the beam
 // and the gaps are summed left to right directly -- the same terms, in the
 // same order, as the general sweep.  Otherwise the general sweep runs.
-__device__ double beam_sum(const TeamLds &t, int lo, int hi, double yb, double yt, int nP) {
+// Sorts the beam's entries by the sub-beam key and sums a crossing-free
+// beam in one linear pass.  A beam with crossings is summed here by the
+// serial general sweep when it has more than GS entries; otherwise the
+// function returns false and the team sweeps it (team_general).
+__device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                             double &out) {
     const int m = hi - lo;
-    if (m < 2) return 0.0;
+    out = 0.0;
+    if (m < 2) return true;
     const int gaps[4] = {23, 10, 4, 1};
     for (int gi = 0; gi < 4; ++gi) {
         const int g = gaps[gi];
@@ -243,7 +276,11 @@ __device__ double beam_sum(const TeamLds &t, int lo, int hi, double yb, double y
     bool mono = true;
     for (int i = lo + 1; i < hi && mono; ++i)
         mono = !(t.lxb[i] < t.lxb[i - 1]) && !(t.lxt[i] < t.lxt[i - 1]);
-    if (!mono) return beam_sum_general(t, lo, hi, yb, yt, nP);
+    if (!mono) {
+        if (m <= t.GS) return false;
+        out = beam_sum_general(t, lo, hi, yb, yt, nP);
+        return true;
+    }
     const double h = yt - yb;
     double bs = 0.0;
     int pp = 0, pq = 0;
@@ -252,7 +289,137 @@ __device__ double beam_sum(const TeamLds &t, int lo, int hi, double yb, double y
         else pq ^= 1;
         if (pp && pq) bs += ((t.lxb[i + 1] - t.lxb[i]) + (t.lxt[i + 1] - t.lxt[i])) * h;
     }
-    return bs;
+    out = bs;
+    return true;
+}
+
+// team-wide bitonic sort of M2 (power of two) entries (gxa, gxc, gidx) by
+// the strict key (xa + xc, xa, id), ascending
+template <int TL> __device__ inline void team_sort_entries(const TeamLds &t, int M2, int tl) {
+    for (int k = 2; k <= M2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tl; q < M2 / 2; q += TL) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i | j;
+                const bool up = (i & k) == 0;
+                const double ai = t.gxa[i], ci = t.gxc[i], al = t.gxa[l], cl = t.gxc[l];
+                const int ei = t.gidx[i], el = t.gidx[l];
+                // swap when the pair is out of order for this direction
+                const bool l_first = key_before(al + cl, al, el, ai + ci, ai, ei);
+                const bool i_first = key_before(ai + ci, ai, ei, al + cl, al, el);
+                if (up ? l_first : i_first) {
+                    t.gxa[i] = al; t.gxc[i] = cl; t.gidx[i] = static_cast<uint16_t>(el);
+                    t.gxa[l] = ai; t.gxc[l] = ci; t.gidx[l] = static_cast<uint16_t>(ei);
+                }
+            }
+            team_sync();
+        }
+    }
+}
+
+// team-wide bitonic sort of giy[0, M2) ascending
+template <int TL> __device__ inline void team_sort_heights(const TeamLds &t, int M2, int tl) {
+    for (int k = 2; k <= M2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tl; q < M2 / 2; q += TL) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i | j;
+                const bool up = (i & k) == 0;
+                const double x = t.giy[i], y = t.giy[l];
+                if ((x > y) == up) {
+                    t.giy[i] = y;
+                    t.giy[l] = x;
+                }
+            }
+            team_sync();
+        }
+    }
+}
+
+// The general sweep of one beam (list entries [lo, hi), m <= GS) by the
+// whole team, with the results of beam_sum_general bit for bit: the
+// sub-beam bounds are the distinct crossing heights in (yb, yt) in
+// increasing order (the serial sweep finds them one minimum at a time from
+// the same candidate set), and each sub-beam is summed by one lane in the
+// strict key order (the serial sweep's selection walk visits that order).
+// Returns the beam sum on every lane of the team.
+template <int TL>
+__device__ double team_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                               int tl, int base) {
+    const int m = hi - lo;
+    const double dy = yt - yb;
+    if (tl == 0) t.gmisc[1] = 0;
+    team_sync();
+    for (int i = lo + tl; i < hi; i += TL) {
+        const double xbi = t.lxb[i], xti = t.lxt[i];
+        for (int j = i + 1; j < hi; ++j) {
+            const double xbj = t.lxb[j], xtj = t.lxt[j];
+            double xbL, xtL, xbR, xtR;
+            if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+            else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+            else continue;
+            const double den = (xtL - xbL) - (xtR - xbR);
+            if (!(den > 0.0)) continue;
+            const double r = (xbR - xbL) / den;
+            const double iy = r * dy + yb;
+            if (iy > yb && iy < yt) {
+                const int pos = atomicAdd(&t.gmisc[1], 1);
+                if (pos < t.GC) t.giy[pos] = iy;
+            }
+        }
+    }
+    team_sync();
+    const int nc = t.gmisc[1];
+    if (nc > t.GC) {
+        // more crossing heights than the scratch holds: the serial sweep
+        double r = 0.0;
+        if (tl == 0) r = beam_sum_general(t, lo, hi, yb, yt, nP);
+        double v = __shfl(r, base, 64);
+        team_sync();
+        return v;
+    }
+    const double INF = __builtin_huge_val();
+    if (nc > 1) {
+        const int C2 = pow2_ge(nc);
+        for (int q = nc + tl; q < C2; q += TL) t.giy[q] = INF;
+        team_sync();
+        team_sort_heights<TL>(t, C2, tl);
+    }
+    const int M2 = pow2_ge(m);
+    double bs = 0.0;  // meaningful on lane 0 of the team
+    double ya = yb;
+    int ci = 0;
+    for (;;) {
+        while (ci < nc && !(t.giy[ci] > ya)) ++ci;
+        const double yc = ci < nc ? t.giy[ci] : yt;
+        const double h = yc - ya;
+        for (int q = tl; q < M2; q += TL) {
+            if (q < m) {
+                const int i = lo + q;
+                t.gxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+                t.gxc[q] = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
+                t.gidx[q] = t.list[i];
+            } else {
+                t.gxa[q] = INF;
+                t.gxc[q] = INF;
+                t.gidx[q] = 0xFFFFu;
+            }
+        }
+        team_sync();
+        team_sort_entries<TL>(t, M2, tl);
+        if (tl == 0) {
+            int pp = 0, pq = 0;
+            for (int q = 0; q + 1 < m; ++q) {
+                if (t.gidx[q] < nP) pp ^= 1;
+                else pq ^= 1;
+                if (pp && pq) bs += ((t.gxa[q + 1] - t.gxa[q]) + (t.gxc[q + 1] - t.gxc[q])) * h;
+            }
+        }
+        team_sync();
+        if (yc == yt) break;
+        ya = yc;
+    }
+    return __shfl(bs, base, 64);
 }
 
 struct BatchArgs {
@@ -417,9 +584,26 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
                 t.lxt[pos] = ex(t, e, t.ys[k + 1]);
             }
         }
+        if (tl == 0) t.gmisc[0] = 0;
         team_sync();
-        for (int k = k0 + tl; k < k1; k += TL)
-            t.bsum[k] = beam_sum(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k], t.ys[k + 1], nP);
+        // lane-parallel beams; beams with crossings are queued (cnt is free
+        // once the lists are built) and swept by the whole team afterwards
+        for (int k = k0 + tl; k < k1; k += TL) {
+            double bsk;
+            if (beam_sum_try(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k], t.ys[k + 1], nP,
+                             bsk))
+                t.bsum[k] = bsk;
+            else
+                t.cnt[atomicAdd(&t.gmisc[0], 1)] = k;
+        }
+        team_sync();
+        const int ndef = t.gmisc[0];
+        for (int q = 0; q < ndef; ++q) {
+            const int k = t.cnt[q];
+            const double bsk = team_general<TL>(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k],
+                                                t.ys[k + 1], nP, tl, base);
+            if (tl == 0) t.bsum[k] = bsk;
+        }
         team_sync();
         k0 = k1;
     }
