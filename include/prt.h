@@ -96,6 +96,9 @@ typedef struct prt_tune {
     int32_t bucket_max_verts[PRT_TUNE_BUCKETS]; /* upper bound of n1+n2 per bucket */
     int32_t lanes_per_pair[PRT_TUNE_BUCKETS];   /* 8, 16, 32 or 64                  */
     int32_t waves_per_block[PRT_TUNE_BUCKETS];  /* 1..8                             */
+    int32_t list_factor[PRT_TUNE_BUCKETS];      /* beam-list capacity per team, in
+                                                   units of the bucket's vertex
+                                                   bound: 1..8                      */
 } prt_tune;
 
 const char *prt_version(void);
@@ -243,8 +246,9 @@ int prt_pipeline_destroy(prt_pipeline *p);
 
 /* Largest LDS allocation of one block on the current device (bytes). */
 int prt_device_lds_limit(void);
-/* LDS bytes of one batched-overlay team for pairs of up to nv vertices. */
-int64_t prt_overlay_team_lds(int32_t nv);
+/* LDS bytes of one batched-overlay team for pairs of up to nv vertices
+ * with the beam-list capacity list_factor * nv. */
+int64_t prt_overlay_team_lds(int32_t nv, int32_t list_factor);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ class TuneStruct(ctypes.Structure):
         ("bucket_max_verts", c_int32 * PRT_TUNE_BUCKETS),
         ("lanes_per_pair", c_int32 * PRT_TUNE_BUCKETS),
         ("waves_per_block", c_int32 * PRT_TUNE_BUCKETS),
+        ("list_factor", c_int32 * PRT_TUNE_BUCKETS),
     ]
 
 
@@ -103,7 +104,7 @@ SIGNATURES = {
     "prt_overlay_area_grad_f32": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p]),
-    "prt_overlay_team_lds": (c_int64, [c_int32]),
+    "prt_overlay_team_lds": (c_int64, [c_int32, c_int32]),
     "prt_overlay_batched_dev": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_int32, POINTER(TuneStruct),
                                         c_void_p, c_void_p, c_size_t, c_void_p]),

@@ -2,11 +2,14 @@
 
 For every bucket of n1 + n2 (the bounds of prt_tune_default), the pairs of
 a workload that fall into it are timed under each feasible team shape
-(lanes per pair x waves per block) and the fastest is kept.  The result is
-a TuneStruct for prt_overlay_batched, and optionally a JSON cache:
+(lanes per pair x waves per block) and the fastest is kept; then the
+beam-list capacity (list_factor x bucket bound) is swept for that shape.
+The result is a TuneStruct for prt_overlay_batched, and optionally a JSON
+cache:
 
     {"device": "...", "buckets": [{"max_verts": 32, "lanes": 8, "waves": 4,
-                                   "pairs": 1234, "ms": 0.12}, ...]}
+                                   "list_factor": 1, "pairs": 1234,
+                                   "ms": 0.12}, ...]}
 """
 from __future__ import annotations
 
@@ -21,12 +24,13 @@ from ._lib import PRT_TUNE_BUCKETS, TuneStruct, lib
 
 LANES = (8, 16, 32, 64)
 WAVES = (1, 2, 4, 8)
+FACTORS = (1, 2, 4)
 LDS_BYTES = 160 * 1024
 
 
-def _team_bytes(nv: int) -> int:
+def _team_bytes(nv: int, factor: int) -> int:
     """LDS bytes of one team for pairs of up to nv vertices."""
-    return int(lib().prt_overlay_team_lds(nv))
+    return int(lib().prt_overlay_team_lds(nv, factor))
 
 
 def _time(fn, reps: int = 3) -> float:
@@ -45,7 +49,7 @@ def tune_to_json(t: TuneStruct, stats=None) -> dict:
     out = {"device": torch.cuda.get_device_name(0), "buckets": []}
     for b in range(PRT_TUNE_BUCKETS):
         e = {"max_verts": t.bucket_max_verts[b], "lanes": t.lanes_per_pair[b],
-             "waves": t.waves_per_block[b]}
+             "waves": t.waves_per_block[b], "list_factor": t.list_factor[b]}
         if stats and b in stats:
             e.update(stats[b])
         out["buckets"].append(e)
@@ -58,6 +62,7 @@ def tune_from_json(d: dict) -> TuneStruct:
         t.bucket_max_verts[b] = e["max_verts"]
         t.lanes_per_pair[b] = e["lanes"]
         t.waves_per_block[b] = e["waves"]
+        t.list_factor[b] = e.get("list_factor", 1)
     return t
 
 
@@ -82,21 +87,34 @@ def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000
         sub = torch.from_numpy(np.ascontiguousarray(pairs[idx]).view(np.int32)).cuda()
         area = torch.empty(len(idx), dtype=torch.float64, device="cuda")
         nv = min(hi, mv)
+
+        def trial(lanes, waves, factor):
+            if waves * (64 // lanes) * _team_bytes(nv, factor) > LDS_BYTES:
+                return None
+            cand = rtree.default_tune()
+            for k in range(PRT_TUNE_BUCKETS):
+                cand.lanes_per_pair[k] = lanes
+                cand.waves_per_block[k] = waves
+                cand.list_factor[k] = factor
+            return _time(lambda: rtree.overlay_batched(sub, Ad, Bd, cand, area), reps)
+
         best = None
+        f0 = tune.list_factor[b]
         for lanes in LANES:
             for waves in WAVES:
-                if waves * (64 // lanes) * _team_bytes(nv) > LDS_BYTES:
-                    continue
-                cand = rtree.default_tune()
-                for k in range(PRT_TUNE_BUCKETS):
-                    cand.lanes_per_pair[k] = lanes
-                    cand.waves_per_block[k] = waves
-                ms = _time(lambda: rtree.overlay_batched(sub, Ad, Bd, cand, area), reps)
-                if best is None or ms < best[0]:
-                    best = (ms, lanes, waves)
+                ms = trial(lanes, waves, f0)
+                if ms is not None and (best is None or ms < best[0]):
+                    best = (ms, lanes, waves, f0)
         if best is not None:
+            for factor in FACTORS:
+                if factor == f0:
+                    continue
+                ms = trial(best[1], best[2], factor)
+                if ms is not None and ms < best[0]:
+                    best = (ms, best[1], best[2], factor)
             tune.lanes_per_pair[b] = best[1]
             tune.waves_per_block[b] = best[2]
+            tune.list_factor[b] = best[3]
             stats[b] = {"pairs": int(len(idx)), "ms": round(best[0], 4)}
     if cache is not None:
         Path(cache).write_text(json.dumps(tune_to_json(tune, stats), indent=1) + "\n")

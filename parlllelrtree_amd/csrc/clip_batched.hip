@@ -68,8 +68,11 @@ __host__ __device__ inline int pow2_ge(int v) {
     return p;
 }
 
-// entries (and crossing heights) of one beam the team sweeps cooperatively
+// entries (and crossing heights) of one beam the team sweeps cooperatively;
+// none for the smallest pairs, whose serial sweep is short and whose teams
+// are limited by LDS
 __host__ __device__ inline int gen_cap(int NV) {
+    if (NV <= 32) return 0;
     const int p = pow2_ge(NV);
     return p < 512 ? p : 512;
 }
@@ -734,9 +737,10 @@ __global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t *__restri
 
 // list capacity per team (entries of a window of beams; a pair whose beams
 // need more is processed in several windows)
-int scap_for(int NV) {
-    const int c = 4 * NV;
-    return c < 2048 ? c : 2048;
+int scap_for(int NV, int factor) {
+    const int c = factor * NV;
+    const int cap = NV > 2048 ? NV : 2048;
+    return c < cap ? c : cap;
 }
 
 int device_cus() {
@@ -774,10 +778,12 @@ extern "C" int prt_tune_default(prt_tune *t) {
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
     const int lanes[NBUCKET] = {16, 32, 64, 64, 64, 64};
     const int waves[NBUCKET] = {4, 2, 2, 2, 1, 1};
+    const int factor[NBUCKET] = {1, 1, 1, 2, 2, 2};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];
         t->lanes_per_pair[b] = lanes[b];
         t->waves_per_block[b] = waves[b];
+        t->list_factor[b] = factor[b];
     }
     return PRT_OK;
 }
@@ -789,6 +795,8 @@ int check_tune(const prt_tune *t) {
         PRT_REQUIRE(L == 8 || L == 16 || L == 32 || L == 64, PRT_E_INVAL,
                     "tune: lanes_per_pair[%d] = %d not in {8,16,32,64}", b, L);
         PRT_REQUIRE(W >= 1 && W <= 8, PRT_E_INVAL, "tune: waves_per_block[%d] = %d", b, W);
+        PRT_REQUIRE(t->list_factor[b] >= 1 && t->list_factor[b] <= 8, PRT_E_INVAL,
+                    "tune: list_factor[%d] = %d not in [1, 8]", b, t->list_factor[b]);
         PRT_REQUIRE(t->bucket_max_verts[b] >= 3 && (b == 0 || t->bucket_max_verts[b] >
                                                                   t->bucket_max_verts[b - 1]),
                     PRT_E_INVAL, "tune: bucket bounds must increase");
@@ -903,7 +911,7 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         A.ovf_list = w.ovf_list;
         A.ovf_count = w.ovf_count;
         A.NV = tune->bucket_max_verts[b] < max_verts ? tune->bucket_max_verts[b] : max_verts;
-        A.SCAP = scap_for(A.NV);
+        A.SCAP = scap_for(A.NV, tune->list_factor[b]);
         const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
         const size_t lds = static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
         if (lds > static_cast<size_t>(max_lds)) {
@@ -970,7 +978,8 @@ extern "C" int prt_device_lds_limit(void) {
 
 // LDS bytes of one team of the batched overlay for pairs of up to nv
 // vertices (n1 + n2).
-extern "C" int64_t prt_overlay_team_lds(int32_t nv) {
+extern "C" int64_t prt_overlay_team_lds(int32_t nv, int32_t list_factor) {
     if (nv < 3) nv = 3;
-    return static_cast<int64_t>(team_bytes(nv, scap_for(nv)));
+    if (list_factor < 1 || list_factor > 8) return -1;
+    return static_cast<int64_t>(team_bytes(nv, scap_for(nv, list_factor)));
 }

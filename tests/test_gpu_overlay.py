@@ -120,13 +120,15 @@ def run_batched(pairs, A, B, tune=None, max_overflow=0):
     return area.cpu().numpy()
 
 
-@pytest.mark.parametrize("lanes,waves", [(8, 4), (16, 4), (32, 2), (64, 1), (64, 4)])
-def test_batched_bit_exact_all_team_shapes(dev, lanes, waves):
+@pytest.mark.parametrize("lanes,waves,factor", [(8, 4, 1), (16, 4, 2), (32, 2, 4), (64, 1, 1),
+                                                (64, 4, 2)])
+def test_batched_bit_exact_all_team_shapes(dev, lanes, waves, factor):
     pairs, S, T = _grid_pairs()
     tune = rtree.default_tune()
     for b in range(6):
         tune.lanes_per_pair[b] = lanes
         tune.waves_per_block[b] = waves
+        tune.list_factor[b] = factor
     # shapes whose teams do not fit LDS for the larger buckets route those
     # buckets to the plain path: results must still be identical
     got = run_batched(pairs, S, T, tune, max_overflow=None)
@@ -145,6 +147,34 @@ def test_batched_routes_large_buckets_to_plain(dev):
     got = run_batched(pairs, S, T, tune, max_overflow=None)
     ref = O.overlay(pairs, S, T, nthreads=8)
     assert np.array_equal(got, ref)
+
+
+def zigzag(k, lean, y0=0.0, y1=1.0, x0=0.0):
+    """Sawtooth of k teeth whose edges all span the band (y0, y1), leaning by
+    `lean`, closed below: the band is one beam holding 2k long edges."""
+    v = []
+    for i in range(k):
+        v.append((x0 + i, y0))
+        v.append((x0 + i + 0.5 + lean, y1))
+    v += [(x0 + k, y0), (x0 + k, y0 - 1.0), (x0, y0 - 1.0)]
+    return v
+
+
+@pytest.mark.parametrize("k", [6, 9, 12, 20, 40, 90])
+def test_batched_crossing_dense_beams(dev, k):
+    # two sawtooths leaning opposite ways: every tooth edge of one crosses
+    # many of the other inside the same beam (many crossing heights, from a
+    # few up to more than the team's scratch holds)
+    polys_a, polys_b = [], []
+    for j, lean in enumerate((0.3, 1.5, 3.7)):
+        polys_a.append(zigzag(k, lean, x0=0.1 * j))
+        polys_b.append(zigzag(k, -lean, y0=0.05, y1=1.1, x0=0.3))
+    A, B = csr(polys_a), csr(polys_b)
+    pairs = np.array([(i, j) for i in range(3) for j in range(3)], np.uint32)
+    got = run_batched(pairs, A, B, max_overflow=None)
+    ref = O.overlay(pairs, A, B)
+    assert np.array_equal(got, ref), (got, ref)
+    assert np.array_equal(run_plain(pairs, A, B), ref)
 
 
 def test_batched_golden_and_edges(dev):
