@@ -27,6 +27,9 @@
  *                                              TD:355-377, 450-463 (with the
  *                                              hit semantics of RtreeSearch
  *                                              BU:439-483, TD:56-76)
+ *   prt_search_stats_* / prt_sibling_overlap_*
+ *                                           <- RtreeSearch / RtreeSearch1
+ *                                              BU:439-483, 574-590, TD:56-93
  *   prt_overlay_plain                       <- Execute OV:81-113 +
  *                                              gpc_polygon_clip OV:115-750
  *   prt_overlay_total                       <- count1 of printfOutput OV:45-58
@@ -145,6 +148,20 @@ int prt_join_bu(const prt_bu_tree *tree, const float *queries, int64_t nq,
 int prt_join_td(const prt_td_tree *tree, const float *queries, int64_t nq,
                 uint32_t *pairs, int64_t cap, int64_t *count_d, void *ws,
                 size_t ws_bytes, void *stream);
+
+/* ---- the reference's search statistics --------------------------------
+ * Counters of the reference's CPU search over every query, as its main()
+ * prints them (BU:990-1027, TD:760-798): stats[0..5] = sum and max of
+ * `num` (entries tested), sum and max of the search's return value, sum and
+ * max of `num1` (leaf hits).  Replaces RtreeSearch BU:439-483 / TD:56-76.
+ * prt_sibling_overlap_*: *count = pairs of overlapping entries that share a
+ * node (RtreeSearch1 BU:574-590 / TD:78-93).  Outputs are device int64. */
+int prt_search_stats_bu(const prt_bu_tree *tree, const float *queries, int64_t nq,
+                        int64_t *stats, void *stream);
+int prt_search_stats_td(const prt_td_tree *tree, const float *queries, int64_t nq,
+                        int64_t *stats, void *stream);
+int prt_sibling_overlap_bu(const prt_bu_tree *tree, int64_t *count, void *stream);
+int prt_sibling_overlap_td(const prt_td_tree *tree, int64_t *count, void *stream);
 
 /* ---- overlay: area of A[pairs[2t]] intersect B[pairs[2t+1]], FP64 ----
  * Polygons are CSR: xy = interleaved (x, y) doubles, off[i]..off[i+1]. */

@@ -34,6 +34,12 @@ _SIG = {
     "orc_join_td": (c_int64, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                               c_int64, POINTER(c_int64)]),
     "orc_join_brute": (c_int64, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_int64]),
+    "orc_search_stats_bu": (None, [c_int64, c_int, c_int64, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "orc_search_stats_td": (None, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                   c_void_p, c_void_p]),
+    "orc_sibling_overlap_bu": (c_int64, [c_int64, c_int, c_int64, c_void_p, c_void_p, c_void_p]),
+    "orc_sibling_overlap_td": (c_int64, [c_int, c_void_p, c_void_p]),
     "orc_clip_area": (c_double, [c_void_p, c_int, c_void_p, c_int, POINTER(c_uint64)]),
     "orc_poly_area": (c_double, [c_void_p, c_int]),
     "orc_green_area": (c_double, [c_void_p, c_int, c_void_p, c_int]),
@@ -189,6 +195,31 @@ def join_brute(r4, q4):
     def run(pairs, cap):
         return lib().orc_join_brute(_p(r4), len(r4), _p(q4), len(q4), _p(pairs), cap), 0
     return _pairs_out(run, 64 * len(q4))[0]
+
+
+def search_stats(tree, q4):
+    """Per-query (num, count, num1) of the reference's CPU search over a
+    bu_build or td_build tree dict."""
+    q4 = _c(q4, np.float32)
+    nq = len(q4)
+    num, cnt, num1 = (np.zeros(nq, np.int64) for _ in range(3))
+    if "slot_child" in tree:
+        lib().orc_search_stats_td(tree["fanout"], _p(tree["slot_child"]), _p(tree["slot_mbr"]),
+                                  _p(q4), nq, _p(num), _p(cnt), _p(num1))
+    else:
+        lib().orc_search_stats_bu(tree["num_nodes"], tree["fanout"], tree["levels"][0][1],
+                                  _p(tree["node_count"]), _p(tree["entry_mbr"]),
+                                  _p(tree["entry_ref"]), _p(q4), nq, _p(num), _p(cnt), _p(num1))
+    return num, cnt, num1
+
+
+def sibling_overlap(tree):
+    if "slot_child" in tree:
+        return int(lib().orc_sibling_overlap_td(tree["fanout"], _p(tree["slot_child"]),
+                                                _p(tree["slot_mbr"])))
+    return int(lib().orc_sibling_overlap_bu(tree["num_nodes"], tree["fanout"],
+                                            tree["levels"][0][1], _p(tree["node_count"]),
+                                            _p(tree["entry_mbr"]), _p(tree["entry_ref"])))
 
 
 # ---------------------------------------------------------------- polygons

@@ -18,6 +18,13 @@
  *                                            lexsort restatement + invariants.
  *   orc_join_*                             : brute force (tree-independent
  *                                            hit set).
+ *   orc_search_stats_*                     : leaf hits = brute-force hit
+ *                                            count; hand-derived counters of a
+ *                                            tiny tree; closed forms for
+ *                                            single-node trees.
+ *   orc_sibling_overlap_*                  : closed forms (identical rects:
+ *                                            sum of C(entries, 2); a grid
+ *                                            whose nodes are disjoint: 0).
  *   orc_clip_area                          : closed forms (rect/rect, P/P,
  *                                            disjoint, nested), Sutherland-
  *                                            Hodgman on convex inputs, an
@@ -469,6 +476,115 @@ int64_t orc_join_brute(const float *r4, int64_t n, const float *q4, int64_t nq,
                 ++total;
             }
     return total;
+}
+
+/* ====================================================================== */
+/* Search counters and tree quality (the reference's printed statistics)  */
+/* ====================================================================== */
+
+/* RtreeSearchHelp, BU:455-483: every entry of a visited node counts in
+ * `num`; an overlapping entry adds one to the node's running value (which
+ * starts at -1) and, at an internal node, the child's result; a leaf hit
+ * also counts in `num1`; the node returns max(0, value). */
+static int64_t bu_stats_rec(int64_t node, int f, int64_t num_leaves, const int32_t *cnt,
+                            const float *mbr, const int32_t *ref, const float *q,
+                            int64_t *num, int64_t *num1)
+{
+    int64_t ans = -1;
+    for (int i = 0; i < cnt[node]; ++i) {
+        ++*num;
+        if (!orc_overlap(q, mbr + 4 * (node * f + i))) continue;
+        ++ans;
+        if (node < num_leaves) ++*num1;
+        else ans += bu_stats_rec(ref[node * f + i], f, num_leaves, cnt, mbr, ref, q, num, num1);
+    }
+    return ans > 0 ? ans : 0;
+}
+
+/* Per query q: num[q], count[q] (the search's return value) and num1[q] of
+ * RtreeSearch (BU:439-442) started at the root (the last node, BU:1008). */
+void orc_search_stats_bu(int64_t num_nodes, int f, int64_t num_leaves,
+                         const int32_t *node_count, const float *entry_mbr,
+                         const int32_t *entry_ref, const float *q4, int64_t nq,
+                         int64_t *num, int64_t *count, int64_t *num1)
+{
+    for (int64_t q = 0; q < nq; ++q) {
+        num[q] = 0;
+        num1[q] = 0;
+        count[q] = bu_stats_rec(num_nodes - 1, f, num_leaves, node_count, entry_mbr, entry_ref,
+                                q4 + 4 * q, &num[q], &num1[q]);
+    }
+}
+
+/* RtreeSearch, TD:56-76: all f slots of a visited block count in `num`
+ * (empty ones too); a non-empty overlapping slot adds one and the child
+ * block's result, or counts a leaf hit. */
+static int64_t td_stats_rec(int64_t index, int f, const int32_t *child, const float *mbr,
+                            const float *q, int64_t *num, int64_t *num1)
+{
+    int64_t ans = -1;
+    for (int i = 0; i < f; ++i) {
+        const int64_t s = index + i;
+        ++*num;
+        if (child[s] == 0) continue;
+        if (!orc_overlap(mbr + 4 * s, q)) continue;
+        ++ans;
+        if (child[s] > 0) ans += td_stats_rec(child[s], f, child, mbr, q, num, num1);
+        else ++*num1;
+    }
+    return ans > 0 ? ans : 0;
+}
+
+/* Per query, from the root block at slot 0 (TD:772). */
+void orc_search_stats_td(int f, const int32_t *slot_child, const float *slot_mbr,
+                         const float *q4, int64_t nq, int64_t *num, int64_t *count,
+                         int64_t *num1)
+{
+    for (int64_t q = 0; q < nq; ++q) {
+        num[q] = 0;
+        num1[q] = 0;
+        count[q] = td_stats_rec(0, f, slot_child, slot_mbr, q4 + 4 * q, &num[q], &num1[q]);
+    }
+}
+
+/* RtreeSearch1, BU:574-590: pairs of overlapping entries inside each node,
+ * summed over the nodes reachable from `node`. */
+static int64_t bu_sib_rec(int64_t node, int f, int64_t num_leaves, const int32_t *cnt,
+                          const float *mbr, const int32_t *ref)
+{
+    int64_t ans = 0;
+    for (int i = 1; i < cnt[node]; ++i)
+        for (int j = i - 1; j >= 0; --j)
+            if (orc_overlap(mbr + 4 * (node * f + j), mbr + 4 * (node * f + i))) ++ans;
+    if (node >= num_leaves)
+        for (int i = 0; i < cnt[node]; ++i)
+            ans += bu_sib_rec(ref[node * f + i], f, num_leaves, cnt, mbr, ref);
+    return ans;
+}
+
+int64_t orc_sibling_overlap_bu(int64_t num_nodes, int f, int64_t num_leaves,
+                               const int32_t *node_count, const float *entry_mbr,
+                               const int32_t *entry_ref)
+{
+    return bu_sib_rec(num_nodes - 1, f, num_leaves, node_count, entry_mbr, entry_ref);
+}
+
+/* RtreeSearch1, TD:78-93: within a block, pairs among the leading non-empty
+ * slots (the loop stops at the first empty slot), then the child blocks. */
+static int64_t td_sib_rec(int64_t index, int f, const int32_t *child, const float *mbr)
+{
+    int64_t ans = 0;
+    for (int i = 1; i < f && child[index + i] != 0; ++i)
+        for (int j = i - 1; j >= 0; --j)
+            if (orc_overlap(mbr + 4 * (index + i), mbr + 4 * (index + j))) ++ans;
+    for (int i = 0; i < f && child[index + i] != 0; ++i)
+        if (child[index + i] > 0) ans += td_sib_rec(child[index + i], f, child, mbr);
+    return ans;
+}
+
+int64_t orc_sibling_overlap_td(int f, const int32_t *slot_child, const float *slot_mbr)
+{
+    return td_sib_rec(0, f, slot_child, slot_mbr);
 }
 
 /* ====================================================================== */

@@ -64,6 +64,19 @@ int64_t orc_join_td(int f, const int32_t *slot_child, const float *slot_mbr,
 int64_t orc_join_brute(const float *r4, int64_t n, const float *q4, int64_t nq,
                        uint32_t *pairs, int64_t cap);
 
+/* ---------------- search counters, tree quality ---------------- */
+void orc_search_stats_bu(int64_t num_nodes, int f, int64_t num_leaves,
+                         const int32_t *node_count, const float *entry_mbr,
+                         const int32_t *entry_ref, const float *q4, int64_t nq,
+                         int64_t *num, int64_t *count, int64_t *num1);
+void orc_search_stats_td(int f, const int32_t *slot_child, const float *slot_mbr,
+                         const float *q4, int64_t nq, int64_t *num, int64_t *count,
+                         int64_t *num1);
+int64_t orc_sibling_overlap_bu(int64_t num_nodes, int f, int64_t num_leaves,
+                               const int32_t *node_count, const float *entry_mbr,
+                               const int32_t *entry_ref);
+int64_t orc_sibling_overlap_td(int f, const int32_t *slot_child, const float *slot_mbr);
+
 /* ---------------- polygon intersection area ---------------- */
 double orc_clip_area(const double *P, int nP, const double *Q, int nQ,
                      uint64_t *flops);

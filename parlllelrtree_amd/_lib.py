@@ -88,6 +88,12 @@ SIGNATURES = {
                             c_void_p, c_void_p, c_size_t, c_void_p]),
     "prt_join_td": (c_int, [POINTER(TDTreeStruct), c_void_p, c_int64, c_void_p, c_int64,
                             c_void_p, c_void_p, c_size_t, c_void_p]),
+    "prt_search_stats_bu": (c_int, [POINTER(BUTreeStruct), c_void_p, c_int64, c_void_p,
+                                    c_void_p]),
+    "prt_search_stats_td": (c_int, [POINTER(TDTreeStruct), c_void_p, c_int64, c_void_p,
+                                    c_void_p]),
+    "prt_sibling_overlap_bu": (c_int, [POINTER(BUTreeStruct), c_void_p, c_void_p]),
+    "prt_sibling_overlap_td": (c_int, [POINTER(TDTreeStruct), c_void_p, c_void_p]),
     "prt_overlay_plain_workspace_size": (c_size_t, [c_int64, c_int32]),
     "prt_overlay_plain": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),

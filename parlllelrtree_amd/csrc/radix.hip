@@ -112,18 +112,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(const uint64_t *__restr
     int64_t b, e;
     tile_range(t, tile, b, e);
     const int count = static_cast<int>(e - b);
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // plain LDS atomics: a digit shared by many lanes serialises only its
+    // own additions, which costs less than ranking every item by ballots
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
-        const bool valid = local < count;
-        const uint64_t v = valid ? in[b + local] : 0ull;
+        if (local >= count) continue;
+        const uint64_t v = in[b + local];
         for (int p = 0; p < passes; ++p) {
             const int shift = bit_lo + 8 * p;
             const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
-            const uint32_t d = static_cast<uint32_t>(v >> shift) & ((1u << bits) - 1u);
-            const uint64_t peers = digit_peers(d, valid, bits);
-            if (valid && (peers & lt) == 0)
-                atomicAdd(&cnt[p][d], static_cast<uint32_t>(__popcll(peers)));
+            atomicAdd(&cnt[p][static_cast<uint32_t>(v >> shift) & ((1u << bits) - 1u)], 1u);
         }
     }
     __syncthreads();

@@ -196,6 +196,39 @@ def join(tree: BottomUpTree | TopDownTree, queries: torch.Tensor, cap: int | Non
     return pairs[:total]
 
 
+def search_stats(tree: BottomUpTree | TopDownTree, queries: torch.Tensor) -> dict:
+    """The counters the reference prints after its CPU search over every
+    query (BotomUp-rtree.cu:990-1027, TopDown-rtree.cu:760-798): sum / ave /
+    max of `num` (entries tested), of the search's return value (sum1) and
+    of leaf hits (sum2 = total hits // 2, as printed)."""
+    queries = _cuda_f32_rects(queries)
+    nq = queries.shape[0]
+    out = torch.zeros(6, dtype=torch.int64, device=queries.device)
+    fn = lib().prt_search_stats_bu if isinstance(tree, BottomUpTree) else \
+        lib().prt_search_stats_td
+    check(fn(ctypes.byref(tree.st), _ptr(queries), nq, _ptr(out), _stream()))
+    v = [int(x) for x in out.tolist()]
+    d = max(nq, 1)
+    return {"sum": v[0], "ave": v[0] / d, "max": v[1], "sum1": v[2], "ave1": v[2] / d,
+            "max1": v[3], "hits": v[4], "sum2": v[4] // 2, "ave2": (v[4] // 2) / d, "max2": v[5]}
+
+
+def sibling_overlap(tree: BottomUpTree | TopDownTree) -> dict:
+    """sum3: pairs of overlapping entries that share a node (RtreeSearch1,
+    BotomUp-rtree.cu:574-590, TopDown-rtree.cu:78-93); ave3 divides by the
+    node slots as the reference prints it (BU:1026: num_nodes * m, TD:797:
+    all slots)."""
+    out = torch.zeros(1, dtype=torch.int64, device=tree.perm.device)
+    if isinstance(tree, BottomUpTree):
+        check(lib().prt_sibling_overlap_bu(ctypes.byref(tree.st), _ptr(out), _stream()))
+        slots = tree.st.num_nodes * tree.st.fanout
+    else:
+        check(lib().prt_sibling_overlap_td(ctypes.byref(tree.st), _ptr(out), _stream()))
+        slots = tree.st.num_slots
+    s3 = int(out.item())
+    return {"sum3": s3, "ave3": s3 / slots}
+
+
 @dataclass
 class PolyDev:
     xy: torch.Tensor    # (V, 2) float64 CUDA
