@@ -6,8 +6,8 @@
  * pointers), a caller-owned workspace and a hipStream_t passed as void*.
  * Calls are stream-ordered and never synchronise the device, except the
  * reference-shaped shim prt_run(), which synchronises like the reference.
- * No entry point allocates device memory.  prt_pipeline_capture allocates
- * the host-side graph objects it returns.
+ * No entry point allocates device memory except prt_run();
+ * prt_pipeline_capture allocates the host-side graph objects it returns.
  * Return value: 0 (PRT_OK) or a negative PRT_E_* code; prt_last_error()
  * returns a message for the last failure on the calling thread.
  *
@@ -260,6 +260,47 @@ int prt_pipeline_capture(const prt_pipeline_io *io, const prt_tune *tune, void *
                          size_t ws_bytes, void *stream, prt_pipeline **out);
 int prt_pipeline_launch(prt_pipeline *p, void *stream);
 int prt_pipeline_destroy(prt_pipeline *p);
+
+/* ---- the reference's text formats (host memory, no device work) -------
+ * Rect files (BU:668-691, TD:486-497): n, then n records "x1 x2 y1 y2 key";
+ * xyxy_h gets (x1, y1, x2, y2) per rect.  Polygon files (OV:766-818): n,
+ * then per polygon "num hole", a box of four doubles, num vertices "x y";
+ * off_h[n + 1] CSR offsets, xy_h[2 * nverts], hole_h[n], box_h[4 * n] =
+ * (int)(v * 100000) of the box values, min_xy_h[2] = smallest box[0] and
+ * box[1] (OV:805-814).  Readers: call with the arrays NULL for the sizes,
+ * then with arrays of those sizes. */
+int prt_read_rects_text(const char *path, int64_t *n, int64_t *xyxy_h, int32_t *key_h);
+int prt_write_rects_text(const char *path, int64_t n, const int64_t *xyxy_h,
+                         const int32_t *key_h);
+/* BU:682-721: shift rects by the smallest x1 / y1 (in place) and find the
+ * key scale 10^k (range / 10^k <= 1e4, integer division). */
+int prt_rects_normalize(int64_t *xyxy_h, int64_t n, int64_t *min_x, int64_t *min_y,
+                        int64_t *scale);
+int prt_read_polys_text(const char *path, int64_t *n, int64_t *nverts, int64_t *off_h,
+                        double *xy_h, int32_t *hole_h, int32_t *box_h, int32_t *min_xy_h);
+int prt_write_polys_text(const char *path, int64_t n, const int64_t *off_h, const double *xy_h,
+                         const int32_t *hole_h);
+
+/* ---- reference-shaped clip entry: run(), OV:61-79 ----------------------
+ * Record t pairs base polygon zorder_index[pairs[t].nodeid - offset] (the
+ * Z-order position -> id map, OV:89; a bottom-up tree's perm array) with
+ * overlay polygon pairs[t].queryid.  Polygons as the reference's readData
+ * holds them: num[] counts, prefix[] (running sums of num) and {x, y}
+ * double vertices; hole flags are accepted and do not change the area.
+ * area (optional, device) gets the FP64 area per record; *count1_h the
+ * number of non-empty intersections (printfOutput OV:45-58).  Allocates
+ * device memory and synchronises, like the reference.  The layout of the
+ * reference's rtree_search record is not in the reference (its common.h is
+ * absent): this one is {nodeid, queryid}. */
+typedef struct prt_rtree_search {
+    int32_t nodeid;
+    int32_t queryid;
+} prt_rtree_search;
+int prt_run(const prt_rtree_search *pairs, int32_t offset, const int32_t *num_base,
+            const int64_t *prefix_base, const double *vertex_base, const int32_t *num_overlay,
+            const int64_t *prefix_overlay, const double *vertex_overlay, int32_t num_output,
+            const int32_t *hole_base, const int32_t *hole_overlay, const int32_t *zorder_index,
+            double *area, int64_t *count1_h);
 
 /* Largest LDS allocation of one block on the current device (bytes). */
 int prt_device_lds_limit(void);

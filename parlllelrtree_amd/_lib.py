@@ -125,6 +125,17 @@ SIGNATURES = {
                                      c_size_t, c_void_p, POINTER(c_void_p)]),
     "prt_pipeline_launch": (c_int, [c_void_p, c_void_p]),
     "prt_pipeline_destroy": (c_int, [c_void_p]),
+    "prt_run": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                        c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                        POINTER(c_int64)]),
+    # host-only reference text formats (csrc/io.cpp)
+    "prt_read_rects_text": (c_int, [c_char_p, POINTER(c_int64), c_void_p, c_void_p]),
+    "prt_write_rects_text": (c_int, [c_char_p, c_int64, c_void_p, c_void_p]),
+    "prt_rects_normalize": (c_int, [c_void_p, c_int64, POINTER(c_int64), POINTER(c_int64),
+                                    POINTER(c_int64)]),
+    "prt_read_polys_text": (c_int, [c_char_p, POINTER(c_int64), POINTER(c_int64), c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_void_p]),
+    "prt_write_polys_text": (c_int, [c_char_p, c_int64, c_void_p, c_void_p, c_void_p]),
     # host-only generators (csrc/gen.cpp)
     "prt_gen_u01": (c_double, [c_uint64, c_uint64]),
     "prt_gen_rects_uniform": (None, [c_uint64, c_int64, c_double, c_double, c_void_p]),

@@ -310,3 +310,25 @@ def overlay_plain(pairs: torch.Tensor, A: PolyDev, B: PolyDev, area: torch.Tenso
     check(lib().prt_overlay_plain(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
                                   _ptr(B.off), mv, _ptr(area), _ptr(ws), ws.numel(), _stream()))
     return area
+
+
+def run_like_reference(records: torch.Tensor, offset: int, num_base: torch.Tensor,
+                       prefix_base: torch.Tensor, vertex_base: torch.Tensor,
+                       num_overlay: torch.Tensor, prefix_overlay: torch.Tensor,
+                       vertex_overlay: torch.Tensor, zorder_index: torch.Tensor,
+                       hole_base: torch.Tensor | None = None,
+                       hole_overlay: torch.Tensor | None = None) -> tuple[torch.Tensor, int]:
+    """prt_run: the clip stage with the reference's run() arguments
+    (vector_map_overlay_gpc_kernel.cu:61).  records: (np, 2) int32
+    (nodeid, queryid); num_*: int32 counts; prefix_*: int64 first-vertex
+    offsets; vertex_*: (V, 2) float64; zorder_index: int32 leaf position ->
+    base polygon id.  Returns (area (np,) float64, count1)."""
+    records = records.contiguous()
+    np_ = records.shape[0]
+    area = torch.empty(np_, dtype=torch.float64, device=records.device)
+    c1 = ctypes.c_int64(0)
+    check(lib().prt_run(_ptr(records), offset, _ptr(num_base), _ptr(prefix_base),
+                        _ptr(vertex_base), _ptr(num_overlay), _ptr(prefix_overlay),
+                        _ptr(vertex_overlay), np_, _ptr(hole_base), _ptr(hole_overlay),
+                        _ptr(zorder_index), _ptr(area), ctypes.byref(c1)))
+    return area, c1.value
