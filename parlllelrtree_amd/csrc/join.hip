@@ -107,9 +107,9 @@ __global__ __launch_bounds__(JN_THREADS) void k_join(TreeView tv, const float4 *
     if (!WRITE && valid && l == 0) counts[q] = found;
 }
 
-// Sort the r's of every query segment.  One wave per query; segments of at
-// most 1024 hits are sorted in LDS (bitonic), larger ones by one whole block
-// (up to 8192), beyond that a status bit is raised.
+// Sort the r's of every query segment: short segments by one thread, up to
+// 1024 hits by one wave (bitonic in LDS), up to 8192 by one block; beyond
+// that a status bit is raised.
 constexpr int SS_WAVE_MAX = 1024;
 constexpr int SS_BLOCK_MAX = 8192;
 
@@ -132,42 +132,87 @@ __device__ inline void bitonic_lds_u32(uint32_t *v, int n2, int tid, int nthread
     }
 }
 
-__global__ __launch_bounds__(JN_THREADS) void k_seg_sort_wave(const int64_t *__restrict__ counts,
-                                                              const int64_t *__restrict__ offsets,
-                                                              int64_t nq, int64_t cap,
-                                                              uint32_t *__restrict__ pairs) {
-    __shared__ uint32_t buf[JN_WAVES][SS_WAVE_MAX];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t q = static_cast<int64_t>(blockIdx.x) * JN_WAVES + wave;
+constexpr int SS_THREAD_MAX = 16;  // segments sorted by one thread
+
+// Per query: segments of up to SS_THREAD_MAX hits are sorted here by one
+// thread (insertion sort, in place); longer ones are appended to the list
+// of the wave sort (<= 1024) or of the block sort (<= 8192); beyond that the
+// status bit is raised and the segment stays in traversal order.
+// lists[0] / lists[nq]: the two lists; lcount[0..1]: their lengths.
+__global__ __launch_bounds__(JN_THREADS) void k_seg_classify(const int64_t *__restrict__ counts,
+                                                             const int64_t *__restrict__ offsets,
+                                                             int64_t nq, int64_t cap,
+                                                             uint32_t *__restrict__ pairs,
+                                                             uint32_t *__restrict__ lists,
+                                                             unsigned long long *lcount,
+                                                             int64_t *__restrict__ status) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * JN_THREADS + threadIdx.x;
     if (q >= nq) return;
     const int64_t o = offsets[q];
     int64_t len = counts[q];
     if (o + len > cap) len = cap - o;
-    if (len <= 1 || len > SS_WAVE_MAX) return;
-    int n2 = 1;
-    while (n2 < len) n2 <<= 1;
-    uint32_t *v = buf[wave];
-    for (int i = lane; i < n2; i += 64) v[i] = i < len ? pairs[2 * (o + i) + 1] : 0xFFFFFFFFu;
-    __builtin_amdgcn_wave_barrier();
-    bitonic_lds_u32(v, n2, lane, 64, false);
-    for (int i = lane; i < len; i += 64) pairs[2 * (o + i) + 1] = v[i];
+    if (len <= 1) return;
+    if (len <= SS_THREAD_MAX) {
+        uint32_t *r = pairs + 2 * o + 1;  // r of hit i at r[2 * i]
+        for (int i = 1; i < len; ++i) {
+            const uint32_t v = r[2 * i];
+            int j = i;
+            while (j > 0 && r[2 * (j - 1)] > v) {
+                r[2 * j] = r[2 * (j - 1)];
+                --j;
+            }
+            r[2 * j] = v;
+        }
+    } else if (len <= SS_WAVE_MAX) {
+        lists[atomicAdd(&lcount[0], 1ull)] = static_cast<uint32_t>(q);
+    } else if (len <= SS_BLOCK_MAX) {
+        lists[nq + atomicAdd(&lcount[1], 1ull)] = static_cast<uint32_t>(q);
+    } else {
+        atomicOr(reinterpret_cast<unsigned long long *>(status), 1ull);
+    }
 }
 
-__global__ __launch_bounds__(1024) void k_seg_sort_block(const int64_t *__restrict__ counts,
-                                                         const int64_t *__restrict__ offsets,
-                                                         int64_t nq, int64_t cap,
-                                                         uint32_t *__restrict__ pairs,
-                                                         int64_t *__restrict__ status) {
-    __shared__ uint32_t v[SS_BLOCK_MAX];
-    for (int64_t q = blockIdx.x; q < nq; q += gridDim.x) {
+// One wave per listed query (grid-stride over the list).
+__global__ __launch_bounds__(JN_THREADS) void k_seg_sort_wave(const int64_t *__restrict__ counts,
+                                                              const int64_t *__restrict__ offsets,
+                                                              int64_t cap,
+                                                              const uint32_t *__restrict__ list,
+                                                              const unsigned long long *lcount,
+                                                              uint32_t *__restrict__ pairs) {
+    __shared__ uint32_t buf[JN_WAVES][SS_WAVE_MAX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t nl = static_cast<int64_t>(*lcount);
+    for (int64_t x = static_cast<int64_t>(blockIdx.x) * JN_WAVES + wave; x < nl;
+         x += static_cast<int64_t>(gridDim.x) * JN_WAVES) {
+        const int64_t q = list[x];
         const int64_t o = offsets[q];
         int64_t len = counts[q];
         if (o + len > cap) len = cap - o;
-        if (len <= SS_WAVE_MAX) continue;
-        if (len > SS_BLOCK_MAX) {
-            if (threadIdx.x == 0) atomicOr(reinterpret_cast<unsigned long long *>(status), 1ull);
-            continue;
-        }
+        int n2 = 1;
+        while (n2 < len) n2 <<= 1;
+        uint32_t *v = buf[wave];
+        for (int i = lane; i < n2; i += 64) v[i] = i < len ? pairs[2 * (o + i) + 1] : 0xFFFFFFFFu;
+        __builtin_amdgcn_wave_barrier();
+        bitonic_lds_u32(v, n2, lane, 64, false);
+        for (int i = lane; i < len; i += 64) pairs[2 * (o + i) + 1] = v[i];
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One block per listed query (grid-stride over the list).
+__global__ __launch_bounds__(1024) void k_seg_sort_block(const int64_t *__restrict__ counts,
+                                                         const int64_t *__restrict__ offsets,
+                                                         int64_t cap,
+                                                         const uint32_t *__restrict__ list,
+                                                         const unsigned long long *lcount,
+                                                         uint32_t *__restrict__ pairs) {
+    __shared__ uint32_t v[SS_BLOCK_MAX];
+    const int64_t nl = static_cast<int64_t>(*lcount);
+    for (int64_t x = blockIdx.x; x < nl; x += gridDim.x) {
+        const int64_t q = list[x];
+        const int64_t o = offsets[q];
+        int64_t len = counts[q];
+        if (o + len > cap) len = cap - o;
         int n2 = 1;
         while (n2 < len) n2 <<= 1;
         for (int i = threadIdx.x; i < n2; i += blockDim.x)
@@ -193,6 +238,8 @@ int run_join(const TreeView &tv, int levels, const float *queries, int64_t nq, u
     int64_t *counts = c.take<int64_t>(nq + 1);
     int64_t *offs = c.take<int64_t>(nq + 1);
     void *scan_ws = c.take<char>(scan_i64_ws_bytes(nq));
+    uint32_t *lists = c.take<uint32_t>(2 * (nq > 0 ? nq : 1));
+    int64_t *lcount = c.take<int64_t>(2);
     PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "join: workspace too small");
     PRT_LAUNCH(k_zero_i64, 1, 64, 0, s, count_d, 2);
     PRT_LAUNCHED();
@@ -213,10 +260,18 @@ int run_join(const TreeView &tv, int levels, const float *queries, int64_t nq, u
     PRT_LAUNCH(k_join<true>, grid, JN_THREADS, lds, s, tv, q4, nq, stack_cap, counts, offs, pairs,
                cap);
     PRT_LAUNCHED();
-    PRT_LAUNCH(k_seg_sort_wave, dim3(static_cast<unsigned>(ceil_div(nq, JN_WAVES))), JN_THREADS, 0,
-               s, counts, offs, nq, cap, pairs);
+    PRT_LAUNCH(k_zero_i64, 1, 64, 0, s, lcount, 2);
     PRT_LAUNCHED();
-    PRT_LAUNCH(k_seg_sort_block, 256, 1024, 0, s, counts, offs, nq, cap, pairs, count_d + 1);
+    unsigned long long *lc = reinterpret_cast<unsigned long long *>(lcount);
+    PRT_LAUNCH(k_seg_classify, dim3(static_cast<unsigned>(ceil_div(nq, JN_THREADS))), JN_THREADS, 0,
+               s, counts, offs, nq, cap, pairs, lists, lc, count_d + 1);
+    PRT_LAUNCHED();
+    int64_t wgrid = ceil_div(nq, JN_WAVES);
+    if (wgrid > 4096) wgrid = 4096;
+    PRT_LAUNCH(k_seg_sort_wave, dim3(static_cast<unsigned>(wgrid)), JN_THREADS, 0, s, counts, offs,
+               cap, lists, lc, pairs);
+    PRT_LAUNCHED();
+    PRT_LAUNCH(k_seg_sort_block, 256, 1024, 0, s, counts, offs, cap, lists + nq, lc + 1, pairs);
     PRT_LAUNCHED();
     return PRT_OK;
 }
@@ -237,6 +292,8 @@ extern "C" size_t prt_join_workspace_size(int64_t nq) {
     s.take<int64_t>(nq + 1);
     s.take<int64_t>(nq + 1);
     s.take<char>(scan_i64_ws_bytes(nq));
+    s.take<uint32_t>(2 * (nq > 0 ? nq : 1));
+    s.take<int64_t>(2);
     return s.bytes();
 }
 

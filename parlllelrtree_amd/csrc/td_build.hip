@@ -87,6 +87,232 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_lds_sort(
     }
 }
 
+// Fused lower levels for power-of-two fanouts.  One group = one segment of
+// the first level whose segments fit LDS (a0 <= 4096 items, padded to
+// A2 = next_pow2(a0); a block holds 4096 / A2 groups).  The kernel
+//   1. gathers the group's rects in the order of the previous level's sort
+//      (src = low word of its sorted items, or the identity at level 0),
+//   2. sorts the group for every remaining level p0..L-1 in LDS: level p
+//      sorts each aligned block of a_p positions by (ord32(x1 or y1), current
+//      position) -- the stable per-segment sort of those levels -- carrying
+//      the group-local index of each rect,
+//   3. writes the leaf slots and perm in the final order, and
+//   4. reduces the boxes bottom-up in LDS into the slots of levels L-2..p0
+//      (entry of level q = union of its f children; empty slots get the
+//      empty box and child 0).
+// Levels above p0 are packed by k_td_level from these slots.  Same slots as
+// the level-by-level path (unions are exact, so bit-identical).
+struct SubtreeArgs {
+    const uint64_t *items;   // sorted items of level p0 - 1, or null (p0 == 0)
+    const float4 *rect_cur;  // rects in the order the items index
+    const uint32_t *id_cur;  // ids in that order, or null (identity)
+    int64_t n, a0;
+    int A2, L, p0, f, logf;
+    int64_t seg[PRT_MAX_LEVELS];        // a_p of the sort of level p
+    int64_t off[PRT_MAX_LEVELS + 1];    // slot offsets per level
+    int32_t *slot_child;
+    float4 *slot_mbr;
+    uint32_t *perm;
+    int64_t num_groups;
+};
+
+// Thread t holds the keys of positions 4t..4t+3 in registers.  A bitonic
+// compare-exchange at distance j pairs registers of one thread (j < 4),
+// lanes of one wave (4 <= j < 256: __shfl_xor by j/4), or positions of
+// different waves (j >= 256: through LDS with block barriers), so a 4096-key
+// sort needs 14 barriers instead of 78.
+constexpr int ST_PER = LS_SLOTS / LS_THREADS;  // 4 keys per thread
+
+// dynamic LDS: rect (64 KB) | id (16 KB) | keyx (32 KB); the rect array
+// holds the boxes in sorted order once the leaves are written
+constexpr size_t ST_LDS = LS_SLOTS * (16 + 4 + 8);
+
+__device__ inline uint64_t kmin(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ inline uint64_t kmax(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+// Sorts every aligned block of B positions (B a power of two <= 4096)
+// ascending; position of register r of thread t is 4t + r.
+__device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B, uint64_t *keyx) {
+    const int t = threadIdx.x;
+    for (int k = 2; k <= B; k <<= 1) {
+        int j = k >> 1;
+        if (j >= 256) {
+            for (int r = 0; r < ST_PER; ++r) keyx[ST_PER * t + r] = kk[r];
+            __syncthreads();
+            for (; j >= 256; j >>= 1) {
+                for (int q = t; q < LS_SLOTS / 2; q += LS_THREADS) {
+                    const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                    const int l = i | j;
+                    const bool up = (k == B) || ((i & k) == 0);
+                    const uint64_t x = keyx[i], y = keyx[l];
+                    if ((x > y) == up) {
+                        keyx[i] = y;
+                        keyx[l] = x;
+                    }
+                }
+                __syncthreads();
+            }
+            for (int r = 0; r < ST_PER; ++r) kk[r] = keyx[ST_PER * t + r];
+        }
+        for (; j >= ST_PER; j >>= 1) {
+            const int lm = j / ST_PER;
+#pragma unroll
+            for (int r = 0; r < ST_PER; ++r) {
+                const int i = ST_PER * t + r;
+                const uint64_t o = __shfl_xor(kk[r], lm, 64);
+                const bool lower = (i & j) == 0;
+                const bool up = (k == B) || ((i & k) == 0);
+                kk[r] = (lower == up) ? kmin(kk[r], o) : kmax(kk[r], o);
+            }
+        }
+        for (; j > 0; j >>= 1) {
+#pragma unroll
+            for (int r = 0; r < ST_PER; ++r) {
+                if (r & j) continue;
+                const int i = ST_PER * t + r;
+                const bool up = (k == B) || ((i & k) == 0);
+                const uint64_t x = kk[r], y = kk[r | j];
+                if ((x > y) == up) {
+                    kk[r] = y;
+                    kk[r | j] = x;
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
+    extern __shared__ __align__(16) char st_lds[];
+    float4 *rect = reinterpret_cast<float4 *>(st_lds);
+    uint32_t *ids = reinterpret_cast<uint32_t *>(st_lds + LS_SLOTS * 16);
+    uint64_t *keyx = reinterpret_cast<uint64_t *>(st_lds + LS_SLOTS * 20);
+    float4 *box = rect;
+    const int tid = threadIdx.x;
+    const int spb = LS_SLOTS / A.A2;
+    const int64_t g0 = static_cast<int64_t>(blockIdx.x) * spb;  // first group of the block
+    // 1. gather the groups' rects and ids into LDS (local index k), in the
+    //    order of the previous level's sort
+    for (int k = tid; k < LS_SLOTS; k += LS_THREADS) {
+        const int64_t gg = g0 + k / A.A2, w = k % A.A2, pos = gg * A.a0 + w;
+        float4 rc = empty_box();
+        uint32_t id = 0xFFFFFFFFu;
+        if (gg < A.num_groups && w < A.a0 && pos < A.n) {
+            const uint32_t sidx =
+                A.items ? static_cast<uint32_t>(A.items[pos]) : static_cast<uint32_t>(pos);
+            rc = A.rect_cur[sidx];
+            id = A.id_cur ? A.id_cur[sidx] : sidx;
+        }
+        rect[k] = rc;
+        ids[k] = id;
+    }
+    __syncthreads();
+    // key of position k = 4t + r: (ord32(coord), position, local index)
+    uint64_t kk[ST_PER];
+#pragma unroll
+    for (int r = 0; r < ST_PER; ++r) {
+        const int k = ST_PER * tid + r;
+        const int64_t gg = g0 + k / A.A2, w = k % A.A2, pos = gg * A.a0 + w;
+        const float4 rc = rect[k];
+        kk[r] = (gg < A.num_groups && w < A.a0 && pos < A.n)
+                    ? (static_cast<uint64_t>(ord32((A.p0 & 1) ? rc.y : rc.x)) << 32) |
+                          (static_cast<uint64_t>(k) << 12) | static_cast<uint64_t>(k)
+                    : ~0ull;
+    }
+    // 2. one sort per level, each within aligned blocks of B = a_p positions
+    for (int p = A.p0; p < A.L; ++p) {
+        if (p > A.p0) {
+            const int comp = p & 1;
+#pragma unroll
+            for (int r = 0; r < ST_PER; ++r) {
+                if (kk[r] == ~0ull) continue;
+                const uint32_t o = static_cast<uint32_t>(kk[r]) & 0xFFFu;
+                const float4 rc = rect[o];
+                kk[r] = (static_cast<uint64_t>(ord32(comp ? rc.y : rc.x)) << 32) |
+                        (static_cast<uint64_t>(ST_PER * tid + r) << 12) | o;
+            }
+        }
+        int B = 1;
+        while (B < A.seg[p]) B <<= 1;
+        if (B > A.A2) B = A.A2;
+        block_bitonic(kk, B, keyx);
+    }
+    // 3. leaves in the final order (position pos = tid + m * 1024, coalesced)
+#pragma unroll
+    for (int r = 0; r < ST_PER; ++r) keyx[ST_PER * tid + r] = kk[r];
+    __syncthreads();
+    const int64_t offL = A.off[A.L - 1], cntL = A.off[A.L] - A.off[A.L - 1];
+    float4 mine[ST_PER];
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        const int k = tid + m * LS_THREADS;
+        const uint64_t v = keyx[k];
+        const int64_t gg = g0 + k / A.A2, w = k % A.A2, dst = gg * A.a0 + w;
+        float4 rc = empty_box();
+        if (v != ~0ull) {
+            const uint32_t o = static_cast<uint32_t>(v) & 0xFFFu;
+            rc = rect[o];
+            A.perm[dst] = ids[o];
+        }
+        mine[m] = rc;
+        if (gg < A.num_groups && w < A.a0 && dst < cntL) {
+            A.slot_child[offL + dst] = dst < A.n ? static_cast<int32_t>(-dst - 1) : 0;
+            A.slot_mbr[offL + dst] = rc;
+        }
+    }
+    // leaf slots past the last group are empty
+    if (blockIdx.x == gridDim.x - 1) {
+        for (int64_t i = A.num_groups * A.a0 + tid; i < cntL; i += LS_THREADS) {
+            A.slot_child[offL + i] = 0;
+            A.slot_mbr[offL + i] = empty_box();
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) box[tid + m * LS_THREADS] = mine[m];
+    __syncthreads();
+    // 4. levels L-2 .. p0: box x of level q = union of boxes x*f .. x*f+f-1
+    //    of level q+1 (in place, compacted to the front); at most two
+    //    boxes per thread (f >= 2)
+    int64_t cover = 1;     // leaves under one entry of level q + 1
+    int width = LS_SLOTS;  // boxes of level q + 1 in LDS
+    for (int q = A.L - 2; q >= A.p0; --q) {
+        cover <<= A.logf;
+        width >>= A.logf;
+        float4 u0 = empty_box(), u1 = empty_box();
+        const int x0 = tid, x1 = tid + LS_THREADS;
+        if (x0 < width)
+            for (int c = 0; c < A.f; ++c) u0 = box_union(u0, box[x0 * A.f + c]);
+        if (x1 < width)
+            for (int c = 0; c < A.f; ++c) u1 = box_union(u1, box[x1 * A.f + c]);
+        __syncthreads();
+        const int epg = static_cast<int>(A.A2 / cover);
+        const int64_t cnt = A.off[q + 1] - A.off[q];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int x = h ? x1 : x0;
+            if (x >= width) continue;
+            const float4 u = h ? u1 : u0;
+            box[x] = u;
+            const int64_t gg = g0 + x / epg;
+            const int64_t i = (gg * A.a0) / cover + x % epg;  // slot index in level q
+            if (gg < A.num_groups && i < cnt && (x % epg) * cover < A.a0) {
+                const bool full = i * cover < A.n;
+                A.slot_child[A.off[q] + i] = full ? static_cast<int32_t>(A.off[q + 1] + i * A.f) : 0;
+                A.slot_mbr[A.off[q] + i] = full ? u : empty_box();
+            }
+        }
+        // slots of level q past the last group's entries are empty
+        if (blockIdx.x == gridDim.x - 1) {
+            const int64_t first = ceil_div(A.num_groups * A.a0, cover);
+            for (int64_t i = first + tid; i < cnt; i += LS_THREADS) {
+                A.slot_child[A.off[q] + i] = 0;
+                A.slot_mbr[A.off[q] + i] = empty_box();
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Last level: leaf slot i holds the rect at sorted position i (-i-1).
 __global__ void k_td_leaves(const float4 *__restrict__ rect_final, int64_t n, int64_t off,
                             int64_t count, int32_t *__restrict__ slot_child,
@@ -228,6 +454,67 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
     const float4 *cur_r = reinterpret_cast<const float4 *>(rects);
     const uint32_t *cur_i = nullptr;  // identity at level 0
     const unsigned grid = static_cast<unsigned>(ceil_div(n, 256));
+    if ((f & (f - 1)) == 0) {
+        // radix levels, then one fused launch for the LDS levels and their slots
+        int p0 = L - 1;
+        for (int p = 0; p < L; ++p)
+            if (td_seg(n, f, L, p) <= LS_SLOTS) {
+                p0 = p;
+                break;
+            }
+        const uint64_t *items = nullptr;
+        for (int p = 0; p < p0; ++p) {
+            const int64_t a = td_seg(n, f, L, p);
+            PRT_LAUNCH(k_td_items, grid, 256, 0, s, cur_r, n, p & 1, ia);
+            PRT_LAUNCHED();
+            uint64_t *sorted = nullptr;
+            rc = radix_sort_u64(ia, ia, ib, n, a, 32, 64, rw, s, &sorted);
+            if (rc) return rc;
+            if (p + 1 < p0) {
+                float4 *nxt_r = (cur_r == ra) ? rb : ra;
+                uint32_t *nxt_i = (cur_i == da) ? db : da;
+                PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i);
+                PRT_LAUNCHED();
+                cur_r = nxt_r;
+                cur_i = nxt_i;
+            } else {
+                items = sorted;  // the subtree kernel gathers through them
+            }
+        }
+        SubtreeArgs A;
+        A.items = items;
+        A.rect_cur = cur_r;
+        A.id_cur = cur_i;
+        A.n = n;
+        A.a0 = td_seg(n, f, L, p0);
+        A.A2 = static_cast<int>(next_pow2(A.a0));
+        A.L = L;
+        A.p0 = p0;
+        A.f = f;
+        A.logf = 0;
+        while ((1 << A.logf) < f) ++A.logf;
+        for (int p = 0; p < PRT_MAX_LEVELS; ++p) A.seg[p] = p < L ? td_seg(n, f, L, p) : 0;
+        for (int p = 0; p <= PRT_MAX_LEVELS; ++p) A.off[p] = tree->level_offset[p];
+        A.slot_child = tree->slot_child;
+        A.slot_mbr = reinterpret_cast<float4 *>(tree->slot_mbr);
+        A.perm = tree->perm;
+        A.num_groups = ceil_div(n, A.a0);
+        const int64_t blocks = ceil_div(A.num_groups, LS_SLOTS / A.A2);
+        PRT_LAUNCH(k_td_subtree, dim3(static_cast<unsigned>(blocks)), dim3(LS_THREADS), ST_LDS, s, A);
+        PRT_LAUNCHED();
+        float4 *sm = reinterpret_cast<float4 *>(tree->slot_mbr);
+        const int64_t *off = tree->level_offset;
+        int64_t a = 1;
+        for (int k = 0; k < L - p0; ++k) a *= f;  // entries of level p0 - 1 cover f^(L-p0)
+        for (int p = p0 - 1; p >= 0; --p) {
+            const int64_t cnt = off[p + 1] - off[p];
+            PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0, s,
+                       n, f, a, off[p], cnt, off[p + 1], tree->slot_child, sm);
+            PRT_LAUNCHED();
+            a *= f;
+        }
+        return PRT_OK;
+    }
     for (int p = 0; p < L; ++p) {
         const int comp = p & 1;  // x1 on even levels, y1 on odd (TopDown-rtree.cu:574-586)
         const int64_t a = td_seg(n, f, L, p);
