@@ -50,13 +50,13 @@ def cpu_model() -> str:
 # algorithmic HBM bytes per processed item, per kernel (minimal traffic:
 # every input read once, every output written once)
 BYTES_PER_ITEM = {
-    "k_scatter": 16,        # 8-B item read + 8-B item written
-    "k_hist": 8,            # 8-B item read
-    "k_td_items": 24,       # 16-B rect read, 8-B item written
+    "k_onesweep": 16,       # one digit pass: 8-B item read + 8-B item written
+    "k_hist_all<Src>": 24,  # 16-B rect read, 8-B item written (items made in the histogram launch)
     "k_td_gather": 48,      # 8-B item, 16-B rect + 4-B id read; 16 + 4 written
-    "k_td_lds_sort": 40,    # 16-B rect + 4-B id read, 16 + 4 written
-    "k_td_leaves": 20,      # 16-B rect read, 4-B child written (+16-B box per slot)
-    "k_morton_items": 24,   # 16-B rect read, 8-B item written
+    "k_td_subtree": 52,     # 8-B item, 16-B rect, 4-B id read; 16-B box + 4-B child + 4-B perm written
+    "k_td_lds_sort": 40,    # (fanouts that are not powers of two) 16 + 4 read, 16 + 4 written
+    "k_td_items": 24,       # (fanouts that are not powers of two) 16-B rect read, 8-B item written
+    "k_td_leaves": 20,      # (fanouts that are not powers of two) 16-B rect read, 4-B child written
     "k_bu_leaves": 52,      # 8-B item, 16-B rect read; 4+4 key/id, 16+4 entry written
 }
 

@@ -16,14 +16,6 @@ namespace {
 
 constexpr int PK_THREADS = 256;
 
-__global__ void k_morton_items(const float4 *__restrict__ rects, int64_t n,
-                               uint64_t *__restrict__ items) {
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t key = morton_center30(rects[i]);
-    items[i] = (static_cast<uint64_t>(key) << 32) | static_cast<uint64_t>(i);
-}
-
 __global__ void k_morton_keys(const float4 *__restrict__ rects, int64_t n,
                               uint32_t *__restrict__ keys) {
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -271,11 +263,8 @@ extern "C" int prt_bu_build(const float *rects, int64_t n, int fanout, prt_bu_tr
                 "bu_build: workspace too small");
     const float4 *r4 = reinterpret_cast<const float4 *>(rects);
 
-    PRT_LAUNCH(k_morton_items, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s, r4, n,
-               a);
-    PRT_LAUNCHED();
     uint64_t *sorted = nullptr;
-    rc = radix_sort_u64(a, a, b, n, 0, 32, 62, rw, s, &sorted);
+    rc = radix_sort_morton(r4, a, b, n, rw, s, &sorted);
     if (rc) return rc;
 
     const int f = fanout;

@@ -128,6 +128,16 @@ size_t radix_ws_bytes(int64_t n, int64_t seg);
 int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w);
 int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n, int64_t seg,
                    int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s, uint64_t **result);
+// The same sort of items made from rects by its histogram launch (written to
+// buf0 there): (ord32(x1 or y1) << 32 | position), segmented, bits 32..63;
+// or (morton_center30 << 32 | position), bits 32..61.  With zeroed = true the
+// caller has already cleared the words radix_zero_range names.
+int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t *buf1, int64_t n,
+                        int64_t seg, const RadixWs &w, hipStream_t s, uint64_t **result,
+                        bool zeroed);
+int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
+                      const RadixWs &w, hipStream_t s, uint64_t **result);
+void radix_zero_range(const RadixWs &w, int64_t n, int64_t seg, uint32_t **p, int64_t *words);
 
 // Exclusive scan of int64 counts; out[i] = sum_{j<i} in[j]; *total = sum.
 size_t scan_i64_ws_bytes(int64_t n);

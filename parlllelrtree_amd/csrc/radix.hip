@@ -71,6 +71,7 @@ constexpr uint64_t LB_AGG = 1ull << 62;
 constexpr uint64_t LB_INC = 2ull << 62;
 constexpr uint64_t LB_VAL = (1ull << 62) - 1;
 constexpr int LB_SPIN_LIMIT = 1 << 22;  // safety net only (seconds of polling)
+constexpr int LB_WIN = 8;                // look-back words loaded per step
 
 __device__ inline uint64_t lb_load(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -95,12 +96,41 @@ __global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n) {
         p[i] = 0u;
 }
 
+// Item producers of the histogram launch: read the items, or make them from
+// the rects (and store them) in the same pass.
+struct SrcItems {
+    const uint64_t *in;
+    __device__ uint64_t operator()(int64_t i) const { return in[i]; }
+};
+struct SrcTdCoord {  // (ord32(x1 or y1) << 32) | position
+    const float4 *rects;
+    uint64_t *out;
+    int comp;
+    __device__ uint64_t operator()(int64_t i) const {
+        const float4 r = rects[i];
+        const uint64_t v = (static_cast<uint64_t>(ord32(comp ? r.y : r.x)) << 32) |
+                           static_cast<uint64_t>(i);
+        out[i] = v;
+        return v;
+    }
+};
+struct SrcMorton {  // (morton_center30 << 32) | position
+    const float4 *rects;
+    uint64_t *out;
+    __device__ uint64_t operator()(int64_t i) const {
+        const uint64_t v = (static_cast<uint64_t>(morton_center30(rects[i])) << 32) |
+                           static_cast<uint64_t>(i);
+        out[i] = v;
+        return v;
+    }
+};
+
 // ghist[(pass * num_segs + s) * 256 + d] += items of tile blockIdx.x (in
 // segment s) whose digit of `pass` is d, for every pass; also clears the
 // tile's look-back words of every pass.
-__global__ __launch_bounds__(RS_THREADS) void k_hist_all(const uint64_t *__restrict__ in, Tiling t,
-                                                         int bit_lo, int bit_hi, int passes,
-                                                         uint32_t *__restrict__ ghist,
+template <class Src>
+__global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int bit_lo, int bit_hi,
+                                                         int passes, uint32_t *__restrict__ ghist,
                                                          uint64_t *__restrict__ status) {
     __shared__ uint32_t cnt[RS_MAX_PASSES][RS_BINS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -117,7 +147,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(const uint64_t *__restr
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
         if (local >= count) continue;
-        const uint64_t v = in[b + local];
+        const uint64_t v = src(b + local);
         for (int p = 0; p < passes; ++p) {
             const int shift = bit_lo + 8 * p;
             const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
@@ -215,22 +245,37 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         }
         dstart[tid] = before + x - run;
         const uint32_t gbase = gbefore + g - tot;
-        // look-back over the earlier tiles of this segment
+        // look-back over the earlier tiles of this segment, LB_WIN words in
+        // flight per step: counts are added from the nearest tile outwards up
+        // to the first inclusive word; a word not yet published ends the step
+        // there (the first tile of a segment publishes its inclusive count at
+        // once, so every walk ends inside the segment)
         uint64_t excl = 0;
         if (jt > 0) {
             int64_t k = tile - 1;
+            const int64_t first = tile - jt;
             int spins = 0;
             for (;;) {
-                const uint64_t v = lb_load(&status_p[k * RS_BINS + tid]);
-                const uint64_t flag = v >> 62;
-                if (flag == 0) {
+                uint64_t v[LB_WIN];
+#pragma unroll
+                for (int w = 0; w < LB_WIN; ++w)
+                    v[w] = k - w >= first ? lb_load(&status_p[(k - w) * RS_BINS + tid]) : LB_INC;
+                int used = 0;
+                bool done = false;
+#pragma unroll
+                for (int w = 0; w < LB_WIN; ++w) {
+                    if (done || used != w || (v[w] >> 62) == 0) continue;
+                    excl += v[w] & LB_VAL;
+                    used = w + 1;
+                    done = (v[w] >> 62) == 2;
+                }
+                if (done) break;
+                if (used == 0) {
                     if (++spins > LB_SPIN_LIMIT) break;
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += v & LB_VAL;
-                if (flag == 2 || k == tile - jt) break;
-                --k;
+                k -= used;
             }
             lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
         }
@@ -281,8 +326,17 @@ int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w) {
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 
-int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n, int64_t seg,
-                   int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s, uint64_t **result) {
+void radix_zero_range(const RadixWs &w, int64_t n, int64_t seg, uint32_t **p, int64_t *words) {
+    const Tiling t = make_tiling(n > 0 ? n : 1, seg);
+    *p = w.ghist;
+    *words = sort_layout(t).ghist_words;
+}
+
+namespace {
+template <class Src>
+int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n,
+                    int64_t seg, int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s,
+                    uint64_t **result, bool zeroed) {
     PRT_REQUIRE(n >= 0 && n < (int64_t(1) << 32), PRT_E_INVAL, "radix sort: n=%lld out of range",
                 (long long)n);
     PRT_REQUIRE(bit_lo >= 0 && bit_hi <= 64 && bit_lo < bit_hi && bit_hi - bit_lo <= 8 * RS_MAX_PASSES,
@@ -298,12 +352,14 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
     const int passes = static_cast<int>((bit_hi - bit_lo + 7) / 8);
     uint32_t *ghist = w.ghist;
     uint32_t *tickets = w.ghist + RS_MAX_PASSES * t.num_segs * RS_BINS;
-    const int64_t zero_words = l.ghist_words;
-    const unsigned zg = static_cast<unsigned>(ceil_div(zero_words, 1024) < 1024 ? ceil_div(zero_words, 1024) : 1024);
-    PRT_LAUNCH(k_zero_u32, zg, 1024, 0, s, ghist, zero_words);
-    PRT_LAUNCHED();
+    if (!zeroed) {
+        const int64_t zero_words = l.ghist_words;
+        const unsigned zg = static_cast<unsigned>(ceil_div(zero_words, 1024) < 1024 ? ceil_div(zero_words, 1024) : 1024);
+        PRT_LAUNCH(k_zero_u32, zg, 1024, 0, s, ghist, zero_words);
+        PRT_LAUNCHED();
+    }
     const dim3 grid(static_cast<unsigned>(t.num_tiles));
-    PRT_LAUNCH(k_hist_all, grid, dim3(RS_THREADS), 0, s, in, t, bit_lo, bit_hi, passes, ghist,
+    PRT_LAUNCH(k_hist_all<Src>, grid, dim3(RS_THREADS), 0, s, prod, t, bit_lo, bit_hi, passes, ghist,
                w.status);
     PRT_LAUNCHED();
     const uint64_t *src = in;
@@ -320,6 +376,26 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
     }
     *result = const_cast<uint64_t *>(src);
     return PRT_OK;
+}
+}  // namespace
+
+int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n, int64_t seg,
+                   int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s, uint64_t **result) {
+    return radix_sort_impl(SrcItems{in}, in, buf0, buf1, n, seg, bit_lo, bit_hi, w, s, result,
+                           false);
+}
+
+int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t *buf1, int64_t n,
+                        int64_t seg, const RadixWs &w, hipStream_t s, uint64_t **result,
+                        bool zeroed) {
+    return radix_sort_impl(SrcTdCoord{rects, buf0, comp}, buf0, buf0, buf1, n, seg, 32, 64, w, s,
+                           result, zeroed);
+}
+
+int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
+                      const RadixWs &w, hipStream_t s, uint64_t **result) {
+    return radix_sort_impl(SrcMorton{rects, buf0}, buf0, buf0, buf1, n, 0, 32, 62, w, s, result,
+                           false);
 }
 
 // ---------------------------------------------------------------------------

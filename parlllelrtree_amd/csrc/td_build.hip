@@ -31,10 +31,14 @@ __global__ void k_td_items(const float4 *__restrict__ rect_cur, int64_t n, int c
     items[i] = (static_cast<uint64_t>(ord32(comp ? r.y : r.x)) << 32) | static_cast<uint64_t>(i);
 }
 
+// also clears zero_words words at zero_p (the next sort's counters)
 __global__ void k_td_gather(const uint64_t *__restrict__ sorted, const float4 *__restrict__ rect_cur,
                             const uint32_t *__restrict__ id_cur, int64_t n,
-                            float4 *__restrict__ rect_next, uint32_t *__restrict__ id_next) {
+                            float4 *__restrict__ rect_next, uint32_t *__restrict__ id_next,
+                            uint32_t *__restrict__ zero_p, int64_t zero_words) {
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (int64_t z = i; z < zero_words; z += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        zero_p[z] = 0u;
     if (i >= n) return;
     uint32_t pos = static_cast<uint32_t>(sorted[i]);
     rect_next[i] = rect_cur[pos];
@@ -463,18 +467,21 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
                 break;
             }
         const uint64_t *items = nullptr;
+        bool zeroed = false;
         for (int p = 0; p < p0; ++p) {
             const int64_t a = td_seg(n, f, L, p);
-            PRT_LAUNCH(k_td_items, grid, 256, 0, s, cur_r, n, p & 1, ia);
-            PRT_LAUNCHED();
             uint64_t *sorted = nullptr;
-            rc = radix_sort_u64(ia, ia, ib, n, a, 32, 64, rw, s, &sorted);
+            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, rw, s, &sorted, zeroed);
             if (rc) return rc;
             if (p + 1 < p0) {
                 float4 *nxt_r = (cur_r == ra) ? rb : ra;
                 uint32_t *nxt_i = (cur_i == da) ? db : da;
-                PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i);
+                uint32_t *zp = nullptr;
+                int64_t zw = 0;
+                radix_zero_range(rw, n, td_seg(n, f, L, p + 1), &zp, &zw);
+                PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i, zp, zw);
                 PRT_LAUNCHED();
+                zeroed = true;
                 cur_r = nxt_r;
                 cur_i = nxt_i;
             } else {
@@ -526,7 +533,8 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
             uint64_t *sorted = nullptr;
             rc = radix_sort_u64(ia, ia, ib, n, a, 32, 64, rw, s, &sorted);
             if (rc) return rc;
-            PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i);
+            PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i, nullptr,
+                       int64_t(0));
             PRT_LAUNCHED();
         } else {
             const int A2 = static_cast<int>(next_pow2(a));
