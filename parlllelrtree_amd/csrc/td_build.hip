@@ -193,7 +193,13 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
     float4 *box = rect;
     const int tid = threadIdx.x;
     const int spb = LS_SLOTS / A.A2;
-    const int64_t g0 = static_cast<int64_t>(blockIdx.x) * spb;  // first group of the block
+    // XCD-aware block order: blocks are dealt to the 8 XCDs round-robin, so
+    // block b runs on XCD b % 8; XCD x gets a contiguous range of groups,
+    // whose random gathers then stay within a few segments that fit its L2
+    const int64_t nb = gridDim.x, xq = nb / 8, xr = nb % 8, xcd = blockIdx.x % 8;
+    const int64_t blk = xcd * xq + (xcd < xr ? xcd : xr) + blockIdx.x / 8;
+    const int64_t g0 = blk * spb;  // first group of the block
+    const bool last_block = blk == nb - 1;
     // 1. gather the groups' rects and ids into LDS (local index k), in the
     //    order of the previous level's sort
     for (int k = tid; k < LS_SLOTS; k += LS_THREADS) {
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
         }
     }
     // leaf slots past the last group are empty
-    if (blockIdx.x == gridDim.x - 1) {
+    if (last_block) {
         for (int64_t i = A.num_groups * A.a0 + tid; i < cntL; i += LS_THREADS) {
             A.slot_child[offL + i] = 0;
             A.slot_mbr[offL + i] = empty_box();
@@ -306,7 +312,7 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
             }
         }
         // slots of level q past the last group's entries are empty
-        if (blockIdx.x == gridDim.x - 1) {
+        if (last_block) {
             const int64_t first = ceil_div(A.num_groups * A.a0, cover);
             for (int64_t i = first + tid; i < cnt; i += LS_THREADS) {
                 A.slot_child[A.off[q] + i] = 0;
