@@ -133,6 +133,17 @@ size_t prt_bu_workspace_size(int64_t n, int fanout);
 int prt_bu_build(const float *rects, int64_t n, int fanout, prt_bu_tree *tree,
                  void *ws, size_t ws_bytes, void *stream);
 
+/* Bottom-up build in the reference's own key order (mode REF4D64 of
+ * CalcuateZOrderValue/Interleave BU:73-123, sorted BU:841-849): rects_i64 =
+ * int64 (x1, y1, x2, y2) per rect (device), keys = Interleave of rect/scale,
+ * leaf order stable by (64-bit key, id).  Node boxes are the rects in fp32,
+ * rounded outward (exact for |v| <= 2^24).  tree->keys gets the high word of
+ * each sorted key; keys_sorted (optional, device, n) the full keys. */
+size_t prt_bu_ref4d_workspace_size(int64_t n, int fanout);
+int prt_bu_build_ref4d(const int64_t *rects_i64, int64_t n, int fanout, int64_t scale,
+                       prt_bu_tree *tree, uint64_t *keys_sorted, void *ws, size_t ws_bytes,
+                       void *stream);
+
 /* ---- top-down build ------------------------------------------------- */
 int prt_td_plan(int64_t n, int fanout, prt_td_tree *tree);
 size_t prt_td_workspace_size(int64_t n, int fanout);

@@ -26,6 +26,8 @@ _SIG = {
     "orc_bu_levels": (c_int, [c_int64, c_int, c_void_p, c_void_p]),
     "orc_bu_build": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_void_p]),
+    "orc_bu_build_keys64": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_void_p]),
     "orc_td_levels": (c_int, [c_int64, c_int, c_void_p]),
     "orc_td_build": (c_int, [c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]),
     "orc_overlap": (c_int, [c_void_p, c_void_p]),
@@ -113,22 +115,23 @@ def bu_levels(n, f):
     return [(int(ls[i]), int(lc[i])) for i in range(L)]
 
 
-def bu_build(r4, f):
+def bu_build(r4, f, keys64=None):
     """Returns dict: perm, keys (sorted), node_count/start/end, entry_mbr
-    (nodes*f, 4), entry_ref, levels."""
+    (nodes*f, 4), entry_ref, levels.  Leaf order: CENTER30 keys, or the
+    given 64-bit keys (the reference's REF4D64 order)."""
     r4 = _c(r4, np.float32)
     n = len(r4)
     lv = bu_levels(n, f)
     nn = lv[-1][0] + lv[-1][1]
-    keys = morton_center30(r4)
+    keys = morton_center30(r4) if keys64 is None else _c(keys64, np.uint64)
     perm = np.empty(n, np.uint32)
     cnt = np.empty(nn, np.int32)
     st = np.empty(nn, np.int32)
     en = np.empty(nn, np.int32)
     mbr = np.empty((nn * f, 4), np.float32)
     ref = np.empty(nn * f, np.int32)
-    got = lib().orc_bu_build(_p(r4), n, f, _p(keys), _p(perm), _p(cnt), _p(st), _p(en), _p(mbr),
-                             _p(ref))
+    fn = lib().orc_bu_build if keys64 is None else lib().orc_bu_build_keys64
+    got = fn(_p(r4), n, f, _p(keys), _p(perm), _p(cnt), _p(st), _p(en), _p(mbr), _p(ref))
     assert got == nn
     return dict(perm=perm, keys=keys[perm], node_count=cnt, node_start=st, node_end=en,
                 entry_mbr=mbr, entry_ref=ref, levels=lv, num_nodes=nn, fanout=f)

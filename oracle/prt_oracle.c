@@ -205,14 +205,15 @@ int orc_bu_levels(int64_t n, int f, int64_t *level_start, int64_t *level_count)
  * rects (entry ref = original id, BU:794-801), each upper entry = union of
  * every entry of its child (BU:816-826, ref = child id).  Unused entry slots
  * hold the empty box (+inf,+inf,-inf,-inf) and ref -1.  Returns #nodes. */
-int orc_bu_build(const float *r4, int64_t n, int f, const uint32_t *keys,
-                 uint32_t *perm, int32_t *node_count, int32_t *node_start,
-                 int32_t *node_end, float *entry_mbr, int32_t *entry_ref)
+/* Packing of BU:788-830 for a given leaf order perm (sorted position ->
+ * rect id). */
+static int bu_pack(const float *r4, int64_t n, int f, const uint32_t *perm,
+                   int32_t *node_count, int32_t *node_start, int32_t *node_end,
+                   float *entry_mbr, int32_t *entry_ref)
 {
     int64_t ls[64], lc[64];
     int L = orc_bu_levels(n, f, ls, lc);
     if (L == 0) return -1;
-    orc_stable_order_u32(keys, n, perm);
     for (int64_t i = 0; i < lc[0]; ++i) {
         int64_t b = i * f, e = b + f - 1 < n - 1 ? b + f - 1 : n - 1;
         node_count[i] = (int32_t)(e - b + 1);
@@ -254,6 +255,26 @@ int orc_bu_build(const float *r4, int64_t n, int f, const uint32_t *keys,
         }
     }
     return (int)(ls[L - 1] + lc[L - 1]);
+}
+
+int orc_bu_build(const float *r4, int64_t n, int f, const uint32_t *keys,
+                 uint32_t *perm, int32_t *node_count, int32_t *node_start,
+                 int32_t *node_end, float *entry_mbr, int32_t *entry_ref)
+{
+    if (n <= 0) return -1;
+    orc_stable_order_u32(keys, n, perm);
+    return bu_pack(r4, n, f, perm, node_count, node_start, node_end, entry_mbr, entry_ref);
+}
+
+/* The reference's own key order (BU:841-849 sorting the REF4D64 keys of
+ * CalcuateZOrderValue, BU:73-94): stable by 64-bit key, then packed. */
+int orc_bu_build_keys64(const float *r4, int64_t n, int f, const uint64_t *keys,
+                        uint32_t *perm, int32_t *node_count, int32_t *node_start,
+                        int32_t *node_end, float *entry_mbr, int32_t *entry_ref)
+{
+    if (n <= 0) return -1;
+    orc_stable_order_u64(keys, n, perm);
+    return bu_pack(r4, n, f, perm, node_count, node_start, node_end, entry_mbr, entry_ref);
 }
 
 /* ====================================================================== */

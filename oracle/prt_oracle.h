@@ -47,6 +47,10 @@ int orc_bu_build(const float *r4, int64_t n, int f, const uint32_t *keys,
                  uint32_t *perm, int32_t *node_count, int32_t *node_start,
                  int32_t *node_end, float *entry_mbr, int32_t *entry_ref);
 
+int orc_bu_build_keys64(const float *r4, int64_t n, int f, const uint64_t *keys,
+                        uint32_t *perm, int32_t *node_count, int32_t *node_start,
+                        int32_t *node_end, float *entry_mbr, int32_t *entry_ref);
+
 /* ---------------- top-down build ---------------- */
 int orc_td_levels(int64_t n, int f, int64_t *level_offset);
 int orc_td_build(const float *r4, int64_t n, int f, int32_t *slot_child,

@@ -79,6 +79,9 @@ SIGNATURES = {
     "prt_bu_workspace_size": (c_size_t, [c_int64, c_int]),
     "prt_bu_build": (c_int, [c_void_p, c_int64, c_int, POINTER(BUTreeStruct), c_void_p,
                              c_size_t, c_void_p]),
+    "prt_bu_ref4d_workspace_size": (c_size_t, [c_int64, c_int]),
+    "prt_bu_build_ref4d": (c_int, [c_void_p, c_int64, c_int, c_int64, POINTER(BUTreeStruct),
+                                   c_void_p, c_void_p, c_size_t, c_void_p]),
     "prt_td_plan": (c_int, [c_int64, c_int, POINTER(TDTreeStruct)]),
     "prt_td_workspace_size": (c_size_t, [c_int64, c_int]),
     "prt_td_build": (c_int, [c_void_p, c_int64, c_int, POINTER(TDTreeStruct), c_void_p,

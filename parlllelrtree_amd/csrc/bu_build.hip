@@ -215,6 +215,31 @@ extern "C" int prt_sort_pairs_u32(const uint32_t *keys_in, int64_t n, int key_bi
     return PRT_OK;
 }
 
+namespace {
+// Leaves from the sorted items (key word << 32 | rect id), then every upper
+// level.
+int bu_pack(const uint64_t *sorted, const float4 *r4, int64_t n, prt_bu_tree *tree,
+            float4 *node_mbr, hipStream_t s) {
+    const int f = tree->fanout;
+    const int npb = PK_THREADS / f;
+    float4 *em = reinterpret_cast<float4 *>(tree->entry_mbr);
+    PRT_LAUNCH(k_bu_leaves, dim3(static_cast<unsigned>(ceil_div(tree->level_count[0], npb))),
+               dim3(PK_THREADS), 0, s, sorted, r4, n, f, npb, tree->level_count[0], tree->keys,
+               tree->perm, tree->node_count, tree->node_start, tree->node_end, em, tree->entry_ref,
+               node_mbr);
+    PRT_LAUNCHED();
+    for (int l = 1; l < tree->num_levels; ++l) {
+        PRT_LAUNCH(k_bu_upper, dim3(static_cast<unsigned>(ceil_div(tree->level_count[l], npb))),
+                   dim3(PK_THREADS), 0, s, f, npb, tree->level_start[l - 1],
+                   tree->level_count[l - 1], tree->level_start[l], tree->level_count[l],
+                   tree->node_count, tree->node_start, tree->node_end, em, tree->entry_ref,
+                   node_mbr);
+        PRT_LAUNCHED();
+    }
+    return PRT_OK;
+}
+}  // namespace
+
 extern "C" int prt_bu_plan(int64_t n, int fanout, prt_bu_tree *tree) {
     PRT_REQUIRE(tree != nullptr, PRT_E_INVAL, "bu_plan: null tree");
     PRT_REQUIRE(n >= 1 && n < (int64_t(1) << 31), PRT_E_INVAL, "bu_plan: n=%lld out of range",
@@ -266,22 +291,73 @@ extern "C" int prt_bu_build(const float *rects, int64_t n, int fanout, prt_bu_tr
     uint64_t *sorted = nullptr;
     rc = radix_sort_morton(r4, a, b, n, rw, s, &sorted);
     if (rc) return rc;
+    return bu_pack(sorted, r4, n, tree, node_mbr, s);
+}
 
-    const int f = fanout;
-    const int npb = PK_THREADS / f;
-    float4 *em = reinterpret_cast<float4 *>(tree->entry_mbr);
-    PRT_LAUNCH(k_bu_leaves, dim3(static_cast<unsigned>(ceil_div(tree->level_count[0], npb))),
-               dim3(PK_THREADS), 0, s, sorted, r4, n, f, npb, tree->level_count[0], tree->keys,
-               tree->perm, tree->node_count, tree->node_start, tree->node_end, em, tree->entry_ref,
-               node_mbr);
-    PRT_LAUNCHED();
-    for (int l = 1; l < tree->num_levels; ++l) {
-        PRT_LAUNCH(k_bu_upper, dim3(static_cast<unsigned>(ceil_div(tree->level_count[l], npb))),
-                   dim3(PK_THREADS), 0, s, f, npb, tree->level_start[l - 1],
-                   tree->level_count[l - 1], tree->level_start[l], tree->level_count[l],
-                   tree->node_count, tree->node_start, tree->node_end, em, tree->entry_ref,
-                   node_mbr);
-        PRT_LAUNCHED();
+namespace {
+__device__ inline float f32_down(int64_t v) {
+    const float f = static_cast<float>(v);
+    return static_cast<double>(f) > static_cast<double>(v) ? nextafterf(f, -__builtin_huge_valf()) : f;
+}
+__device__ inline float f32_up(int64_t v) {
+    const float f = static_cast<float>(v);
+    return static_cast<double>(f) < static_cast<double>(v) ? nextafterf(f, __builtin_huge_valf()) : f;
+}
+
+// Reference keys (as k_morton_ref4d) and the fp32 boxes of the integer rects
+// (rounded outward: exact for |v| <= 2^24, containing otherwise).
+__global__ void k_ref4d_prepare(const int64_t *__restrict__ rects, int64_t n, int64_t scale,
+                                uint64_t *__restrict__ keys, float4 *__restrict__ r4) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint64_t a = static_cast<uint64_t>(rects[4 * i + w] / scale);
+        k |= spread_ref4(a) << (6 - 2 * w);
     }
-    return PRT_OK;
+    keys[i] = k;
+    r4[i] = make_float4(f32_down(rects[4 * i]), f32_down(rects[4 * i + 1]), f32_up(rects[4 * i + 2]),
+                        f32_up(rects[4 * i + 3]));
+}
+}  // namespace
+
+extern "C" size_t prt_bu_ref4d_workspace_size(int64_t n, int fanout) {
+    prt_bu_tree t;
+    if (prt_bu_plan(n, fanout, &t) != PRT_OK) return 0;
+    Sizer s;
+    for (int k = 0; k < 6; ++k) s.take<uint64_t>(n);  // keys, items, 4 sort buffers
+    s.take<float4>(n);
+    s.take<float4>(t.num_nodes);
+    return s.bytes() + radix_ws_bytes(n, 0);
+}
+
+extern "C" int prt_bu_build_ref4d(const int64_t *rects, int64_t n, int fanout, int64_t scale,
+                                  prt_bu_tree *tree, uint64_t *keys_sorted, void *ws,
+                                  size_t ws_bytes, void *stream) {
+    int rc = prt_bu_plan(n, fanout, tree);
+    if (rc) return rc;
+    PRT_REQUIRE(scale > 0, PRT_E_INVAL, "bu_build_ref4d: scale must be positive");
+    PRT_REQUIRE(rects && tree->keys && tree->perm && tree->node_count && tree->node_start &&
+                    tree->node_end && tree->entry_mbr && tree->entry_ref,
+                PRT_E_INVAL, "bu_build_ref4d: null device pointer");
+    const size_t need = prt_bu_ref4d_workspace_size(n, fanout);
+    PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "bu_build_ref4d: workspace %zu < %zu bytes",
+                ws_bytes, need);
+    hipStream_t s = as_stream(stream);
+    Carver c(ws, ws_bytes);
+    uint64_t *keys = c.take<uint64_t>(n), *items = c.take<uint64_t>(n);
+    uint64_t *b0 = c.take<uint64_t>(n), *b1 = c.take<uint64_t>(n);
+    uint64_t *b2 = c.take<uint64_t>(n), *b3 = c.take<uint64_t>(n);
+    float4 *r4 = c.take<float4>(n);
+    float4 *node_mbr = c.take<float4>(tree->num_nodes);
+    RadixWs rw;
+    PRT_REQUIRE(radix_carve(c, n, 0, rw) == PRT_OK && c.ok(), PRT_E_NOMEM,
+                "bu_build_ref4d: workspace too small");
+    PRT_LAUNCH(k_ref4d_prepare, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
+               rects, n, scale, keys, r4);
+    PRT_LAUNCHED();
+    rc = radix_sort_keys64(keys, n, b0, b1, b2, b3, rw, s, items, keys_sorted);
+    if (rc) return rc;
+    return bu_pack(items, r4, n, tree, node_mbr, s);
 }

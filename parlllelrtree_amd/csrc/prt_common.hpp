@@ -138,6 +138,12 @@ int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t 
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
                       const RadixWs &w, hipStream_t s, uint64_t **result);
 void radix_zero_range(const RadixWs &w, int64_t n, int64_t seg, uint32_t **p, int64_t *words);
+// Stable sort by 64-bit keys (two 32-bit passes); items[k] = (key >> 32 << 32)
+// | id of the k-th rect, keys_sorted[k] (optional) its key.  b0..b3 and items
+// are distinct n-entry buffers.
+int radix_sort_keys64(const uint64_t *keys, int64_t n, uint64_t *b0, uint64_t *b1, uint64_t *b2,
+                      uint64_t *b3, const RadixWs &w, hipStream_t s, uint64_t *items,
+                      uint64_t *keys_sorted);
 
 // Exclusive scan of int64 counts; out[i] = sum_{j<i} in[j]; *total = sum.
 size_t scan_i64_ws_bytes(int64_t n);

@@ -125,6 +125,28 @@ struct SrcMorton {  // (morton_center30 << 32) | position
     }
 };
 
+struct SrcKeyLo {  // (low word of a 64-bit key << 32) | position
+    const uint64_t *keys;
+    uint64_t *out;
+    __device__ uint64_t operator()(int64_t i) const {
+        const uint64_t v = (keys[i] << 32) | static_cast<uint64_t>(i);
+        out[i] = v;
+        return v;
+    }
+};
+struct SrcKeyHi {  // (high word of the key of the rect at position j of the
+                   // low-word order << 32) | j
+    const uint64_t *keys;
+    const uint64_t *first;
+    uint64_t *out;
+    __device__ uint64_t operator()(int64_t j) const {
+        const uint32_t id = static_cast<uint32_t>(first[j]);
+        const uint64_t v = (keys[id] & 0xFFFFFFFF00000000ull) | static_cast<uint64_t>(j);
+        out[j] = v;
+        return v;
+    }
+};
+
 // ghist[(pass * num_segs + s) * 256 + d] += items of tile blockIdx.x (in
 // segment s) whose digit of `pass` is d, for every pass; also clears the
 // tile's look-back words of every pass.
@@ -396,6 +418,40 @@ int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64
                       const RadixWs &w, hipStream_t s, uint64_t **result) {
     return radix_sort_impl(SrcMorton{rects, buf0}, buf0, buf0, buf1, n, 0, 32, 62, w, s, result,
                            false);
+}
+
+namespace {
+// items[k] = (high word of key << 32) | id of the k-th rect in (key, id)
+// order; keys_sorted[k] (optional) = its full key
+__global__ void k_compose64(const uint64_t *__restrict__ first, const uint64_t *__restrict__ second,
+                            const uint64_t *__restrict__ keys, int64_t n,
+                            uint64_t *__restrict__ items, uint64_t *__restrict__ keys_sorted) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = static_cast<uint32_t>(second[k]);
+    const uint32_t id = static_cast<uint32_t>(first[j]);
+    const uint64_t key = keys[id];
+    items[k] = (key & 0xFFFFFFFF00000000ull) | id;
+    if (keys_sorted) keys_sorted[k] = key;
+}
+}  // namespace
+
+// Stable sort by a 64-bit key as two stable 32-bit sorts (low word, then
+// high word): the order is (key, index).
+int radix_sort_keys64(const uint64_t *keys, int64_t n, uint64_t *b0, uint64_t *b1, uint64_t *b2,
+                      uint64_t *b3, const RadixWs &w, hipStream_t s, uint64_t *items,
+                      uint64_t *keys_sorted) {
+    if (n == 0) return PRT_OK;
+    uint64_t *first = nullptr, *second = nullptr;
+    int rc = radix_sort_impl(SrcKeyLo{keys, b0}, b0, b0, b1, n, 0, 32, 64, w, s, &first, false);
+    if (rc) return rc;
+    rc = radix_sort_impl(SrcKeyHi{keys, first, b2}, b2, b2, b3, n, 0, 32, 64, w, s, &second,
+                         false);
+    if (rc) return rc;
+    PRT_LAUNCH(k_compose64, dim3(static_cast<unsigned>(ceil_div(n, 256))), 256, 0, s, first, second,
+               keys, n, items, keys_sorted);
+    PRT_LAUNCHED();
+    return PRT_OK;
 }
 
 // ---------------------------------------------------------------------------

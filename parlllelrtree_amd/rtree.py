@@ -119,6 +119,35 @@ def build_bottom_up(rects: torch.Tensor, fanout: int = 16, workspace: torch.Tens
     return t
 
 
+def build_bottom_up_ref4d(rects_i64: torch.Tensor, scale: int, fanout: int = 16
+                          ) -> tuple[BottomUpTree, torch.Tensor]:
+    """Bottom-up tree in the reference's own key order (REF4D64 keys of
+    BotomUp-rtree.cu:73-123 on rect // scale, stable by (key, id)).
+    rects_i64: (n, 4) int64 CUDA tensor (x1, y1, x2, y2), normalised as the
+    reference does (refio.normalize).  Returns (tree, sorted 64-bit keys)."""
+    if not (rects_i64.is_cuda and rects_i64.dtype == torch.int64 and rects_i64.dim() == 2
+            and rects_i64.shape[1] == 4):
+        raise ValueError("rects_i64 must be a (n, 4) int64 CUDA tensor (x1, y1, x2, y2)")
+    rects_i64 = rects_i64.contiguous()
+    n, dev = rects_i64.shape[0], rects_i64.device
+    st = BUTreeStruct()
+    check(lib().prt_bu_plan(n, fanout, ctypes.byref(st)))
+    nn = st.num_nodes
+    i32 = dict(dtype=torch.int32, device=dev)
+    t = BottomUpTree(st, torch.empty(n, **i32), torch.empty(n, **i32), torch.empty(nn, **i32),
+                     torch.empty(nn, **i32), torch.empty(nn, **i32),
+                     torch.empty((nn * fanout, 4), dtype=torch.float32, device=dev),
+                     torch.empty(nn * fanout, **i32))
+    for name in ("keys", "perm", "node_count", "node_start", "node_end", "entry_mbr", "entry_ref"):
+        setattr(st, name, getattr(t, name).data_ptr())
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    ws = _ws(lib().prt_bu_ref4d_workspace_size(n, fanout), dev)
+    check(lib().prt_bu_build_ref4d(_ptr(rects_i64), n, fanout, int(scale), ctypes.byref(st),
+                                   _ptr(keys), _ptr(ws), ws.numel(), _stream()))
+    t.workspace = None
+    return t, keys
+
+
 def rebuild(tree: BottomUpTree | TopDownTree, rects: torch.Tensor) -> None:
     """Rebuild `tree` in place from rects of the same count (reuses every
     output array and the workspace; launches only)."""

@@ -186,3 +186,26 @@ def test_workload1_layers_full(dev):
     opairs, _ = O.join_td(o, rb)
     assert len(pairs) == len(opairs) and len(pairs) > 1_000_000
     np.testing.assert_array_equal(pairs, opairs)
+
+
+@pytest.mark.parametrize("n,f", [(1, 10), (5, 2), (1000, 10), (20000, 16), (300001, 10)])
+def test_bu_build_ref4d_bit_exact(dev, n, f):
+    # integer rects in the reference's units, normalised as BotomUp-rtree.cu:682-721 does
+    from parlllelrtree_amd import refio
+    rng = np.random.default_rng(n)
+    x1 = rng.integers(-3_000_000, 9_000_000, n)
+    y1 = rng.integers(0, 12_000_000, n)
+    r = np.stack([x1, y1, x1 + rng.integers(0, 5000, n), y1 + rng.integers(0, 5000, n)], 1)
+    r[: n // 7] = r[0]                   # duplicate keys: ties must keep input order
+    rn, _, _, scale = refio.normalize(r)
+    keys = O.morton_ref4d(rn, scale)
+    ot = O.bu_build(refio.rects_to_f32(rn), f, keys64=keys)
+    t, k64 = rtree.build_bottom_up_ref4d(torch.from_numpy(rn).cuda(), scale, f)
+    np.testing.assert_array_equal(t.perm.cpu().numpy().view(np.uint32), ot["perm"])
+    np.testing.assert_array_equal(k64.cpu().numpy().view(np.uint64), keys[ot["perm"]])
+    np.testing.assert_array_equal(t.keys.cpu().numpy().view(np.uint32),
+                                  (keys[ot["perm"]] >> np.uint64(32)).astype(np.uint32))
+    for k in ("node_count", "node_start", "node_end", "entry_ref"):
+        np.testing.assert_array_equal(getattr(t, k).cpu().numpy(), ot[k], err_msg=k)
+    np.testing.assert_array_equal(t.entry_mbr.cpu().numpy().view(np.uint32),
+                                  ot["entry_mbr"].view(np.uint32))
