@@ -56,6 +56,13 @@ extern "C" {
 #define PRT_E_OVERFLOW (-4)
 #define PRT_E_UNSUPPORTED (-5)
 
+/* Polygon set operations, in the order of GPC's gpc_op (OV:264-306,
+ * 492-525): the area of A - B, A & B, A ^ B, A | B. */
+#define PRT_OP_DIFF 0
+#define PRT_OP_INT 1
+#define PRT_OP_XOR 2
+#define PRT_OP_UNION 3
+
 #define PRT_MAX_LEVELS 40
 #define PRT_MAX_FANOUT 64
 #define PRT_MAX_POLY_VERTS 4096
@@ -182,6 +189,16 @@ int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const double *xyA,
                       int32_t max_verts, double *area, void *ws, size_t ws_bytes,
                       void *stream);
 
+/* Area of A[pairs[2t]] op B[pairs[2t+1]] for the other GPC operations
+ * (op = PRT_OP_DIFF / INT / XOR / UNION; a polygon of fewer than 3
+ * vertices is empty): the same sweep with the operation's y range and gap
+ * predicate.  No reference counterpart beyond gpc_op (the reference only
+ * calls GPC_INT, OV:111). */
+int prt_overlay_plain_op(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                         const int64_t *offA, const double *xyB, const int64_t *offB,
+                         int32_t max_verts, int32_t op, double *area, void *ws, size_t ws_bytes,
+                         void *stream);
+
 /* Batched overlay: same areas as prt_overlay_plain (bit-identical), pairs
  * bucketed by n1+n2 and run by persistent work-stealing launches, one per
  * bucket, with the team shape of `tune` (NULL = prt_tune_default).  Also
@@ -192,6 +209,10 @@ int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const double *xyA
                         const int64_t *offA, const double *xyB, const int64_t *offB,
                         int32_t max_verts, const prt_tune *tune, double *area, void *ws,
                         size_t ws_bytes, void *stream);
+int prt_overlay_batched_op(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                           const int64_t *offA, const double *xyB, const int64_t *offB,
+                           int32_t max_verts, int32_t op, const prt_tune *tune, double *area,
+                           void *ws, size_t ws_bytes, void *stream);
 /* Pairs of the last prt_overlay_batched call on workspace `ws` that went to
  * the plain path because they did not fit the team's LDS (synchronises). */
 int prt_overlay_batched_overflow(const void *ws, size_t ws_bytes, int64_t npairs,

@@ -43,6 +43,9 @@ _SIG = {
     "orc_sibling_overlap_bu": (c_int64, [c_int64, c_int, c_int64, c_void_p, c_void_p, c_void_p]),
     "orc_sibling_overlap_td": (c_int64, [c_int, c_void_p, c_void_p]),
     "orc_clip_area": (c_double, [c_void_p, c_int, c_void_p, c_int, POINTER(c_uint64)]),
+    "orc_clip_area_op": (c_double, [c_void_p, c_int, c_void_p, c_int, c_int, POINTER(c_uint64)]),
+    "orc_overlay_op": (None, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                              c_void_p, POINTER(c_uint64), c_int]),
     "orc_poly_area": (c_double, [c_void_p, c_int]),
     "orc_green_area": (c_double, [c_void_p, c_int, c_void_p, c_int]),
     "orc_sh_convex_area": (c_double, [c_void_p, c_int, c_void_p, c_int]),
@@ -232,6 +235,25 @@ def clip_area(P, Q, with_flops=False):
     fl = c_uint64(0)
     a = lib().orc_clip_area(_p(P), len(P), _p(Q), len(Q), ctypes.byref(fl))
     return (a, fl.value) if with_flops else a
+
+
+OPS = {"diff": 0, "int": 1, "xor": 2, "union": 3}
+
+
+def clip_area_op(P, Q, op):
+    """Area of P op Q, op in OPS (names or gpc_op numbers)."""
+    P = _c(P, np.float64)
+    Q = _c(Q, np.float64)
+    return lib().orc_clip_area_op(_p(P), len(P), _p(Q), len(Q), OPS.get(op, op), None)
+
+
+def overlay_op(pairs, A, B, op, nthreads=1):
+    pairs = _c(pairs, np.uint32)
+    area = np.empty(len(pairs), np.float64)
+    lib().orc_overlay_op(_p(pairs), len(pairs), _p(_c(A.xy, np.float64)), _p(_c(A.off, np.int64)),
+                         _p(_c(B.xy, np.float64)), _p(_c(B.off, np.int64)), OPS.get(op, op),
+                         _p(area), None, nthreads)
+    return area
 
 
 def poly_area(P):

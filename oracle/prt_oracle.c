@@ -724,9 +724,48 @@ static void ws_free(clip_ws *w)
     free(w->xc); free(w->km); free(w->ord); free(w->cy); free(w->ys); free(w->bsum);
 }
 
+/* GPC's operations (gpc_op order DIFF, INT, XOR, UNION; OV:264-306, 492-525)
+ * as the set a gap belongs to, from the subject and clip parities. */
+static int op_in(int op, int pp, int pq)
+{
+    switch (op) {
+    case 0: return pp && !pq;   /* DIFF */
+    case 2: return pp != pq;    /* XOR */
+    case 3: return pp || pq;    /* UNION */
+    default: return pp && pq;   /* INT */
+    }
+}
+
+/* The y range the sweep covers: the common range for INT (empty when the
+ * boxes are disjoint, GPC's minimax pre-test OV:1656-1709), the subject's
+ * range for DIFF, the union of the ranges for XOR and UNION.  A polygon of
+ * fewer than 3 vertices is empty.  Returns 0 when the result is empty. */
+static int op_range(int op, int eP, int eQ, const double *bp, const double *bq,
+                    double *ylo, double *yhi)
+{
+    if (op == 1) {
+        if (eP || eQ) return 0;
+        if (bp[2] < bq[0] || bq[2] < bp[0] || bp[3] < bq[1] || bq[3] < bp[1]) return 0;
+        *ylo = bp[1] > bq[1] ? bp[1] : bq[1];
+        *yhi = bp[3] < bq[3] ? bp[3] : bq[3];
+    } else if (op == 0 || eQ) {
+        if (eP) return 0;
+        *ylo = bp[1];
+        *yhi = bp[3];
+    } else if (eP) {
+        *ylo = bq[1];
+        *yhi = bq[3];
+    } else {
+        *ylo = bp[1] < bq[1] ? bp[1] : bq[1];
+        *yhi = bp[3] > bq[3] ? bp[3] : bq[3];
+    }
+    return *ylo < *yhi;
+}
+
 /* One crossing-free sub-beam [ya, yc] with edge x's xa[], xc[] (indexed like
  * act[]).  Returns the doubled-area contribution, summed left to right. */
-static double sub_beam(clip_ws *w, int m, double ya, double yc, double acc, uint64_t *fl)
+static double sub_beam(clip_ws *w, int m, double ya, double yc, double acc, int op,
+                       uint64_t *fl)
 {
     double h = yc - ya;
     for (int i = 0; i < m; ++i) w->km[i] = w->xa[i] + w->xc[i];
@@ -744,7 +783,7 @@ static double sub_beam(clip_ws *w, int m, double ya, double yc, double acc, uint
     for (int t = 0; t + 1 < m; ++t) {
         int l = w->ord[t], r = w->ord[t + 1];
         if (w->act[l]->poly == 0) pp ^= 1; else pq ^= 1;
-        if (pp && pq) {
+        if (op_in(op, pp, pq)) {
             acc += ((w->xa[r] - w->xa[l]) + (w->xc[r] - w->xc[l])) * h;
             ops += 5;
         }
@@ -754,16 +793,16 @@ static double sub_beam(clip_ws *w, int m, double ya, double yc, double acc, uint
 }
 
 static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q, int nQ,
-                           uint64_t *fl)
+                           int op, uint64_t *fl)
 {
-    if (nP < 3 || nQ < 3) return 0.0;
-    double bp[4], bq[4];
-    bbox(P, nP, bp);
-    bbox(Q, nQ, bq);
-    if (bp[2] < bq[0] || bq[2] < bp[0] || bp[3] < bq[1] || bq[3] < bp[1]) return 0.0;
-    double ylo = bp[1] > bq[1] ? bp[1] : bq[1];
-    double yhi = bp[3] < bq[3] ? bp[3] : bq[3];
-    if (!(ylo < yhi)) return 0.0;
+    const int eP = nP < 3, eQ = nQ < 3;
+    double bp[4] = {0, 0, 0, 0}, bq[4] = {0, 0, 0, 0};
+    if (!eP) bbox(P, nP, bp);
+    if (!eQ) bbox(Q, nQ, bq);
+    double ylo, yhi;
+    if (!op_range(op, eP, eQ, bp, bq, &ylo, &yhi)) return 0.0;
+    nP = eP ? 0 : nP;   /* an empty polygon contributes no edge and no y */
+    nQ = eQ ? 0 : nQ;
     ws_reserve(w, nP + nQ, nP + nQ);
     uint64_t ops = 0;
 
@@ -842,7 +881,7 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
         }
         if (nc == 0) {
             for (int i = 0; i < m; ++i) { w->xa[i] = w->xb[i]; w->xc[i] = w->xt[i]; }
-            bsum[k] = sub_beam(w, m, yb, yt, 0.0, &ops);
+            bsum[k] = sub_beam(w, m, yb, yt, 0.0, op, &ops);
             continue;
         }
         qsort(w->cy, (size_t)nc, sizeof(double), cmp_double);
@@ -860,7 +899,7 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
             } else {
                 for (int i = 0; i < m; ++i) w->xc[i] = w->xt[i];
             }
-            bs = sub_beam(w, m, ya, yc, bs, &ops);
+            bs = sub_beam(w, m, ya, yc, bs, op, &ops);
             for (int i = 0; i < m; ++i) w->xa[i] = w->xc[i];
             ya = yc;
         }
@@ -876,9 +915,17 @@ static double clip_area_ws(clip_ws *w, const double *P, int nP, const double *Q,
 
 double orc_clip_area(const double *P, int nP, const double *Q, int nQ, uint64_t *flops)
 {
+    return orc_clip_area_op(P, nP, Q, nQ, 1, flops);
+}
+
+/* Area of P op Q for op = 0 DIFF, 1 INT, 2 XOR, 3 UNION: the same sweep with
+ * the op's y range and gap predicate. */
+double orc_clip_area_op(const double *P, int nP, const double *Q, int nQ, int op,
+                        uint64_t *flops)
+{
     clip_ws w;
     memset(&w, 0, sizeof w);
-    double a = clip_area_ws(&w, P, nP, Q, nQ, flops);
+    double a = clip_area_ws(&w, P, nP, Q, nQ, op, flops);
     ws_free(&w);
     return a;
 }
@@ -1088,6 +1135,7 @@ typedef struct {
     const int64_t *offA, *offB;
     double *area;
     int64_t lo, hi;
+    int op;
     uint64_t flops;
 } ov_job;
 
@@ -1103,7 +1151,7 @@ static void *ov_run(void *arg)
         const double *Q = j->xyB + 2 * j->offB[b];
         int nP = (int)(j->offA[a + 1] - j->offA[a]);
         int nQ = (int)(j->offB[b + 1] - j->offB[b]);
-        j->area[t] = clip_area_ws(&w, P, nP, Q, nQ, &fl);
+        j->area[t] = clip_area_ws(&w, P, nP, Q, nQ, j->op, &fl);
     }
     j->flops = fl;
     ws_free(&w);
@@ -1115,6 +1163,13 @@ void orc_overlay(const uint32_t *pairs, int64_t np, const double *xyA, const int
                  const double *xyB, const int64_t *offB, double *area, uint64_t *flops,
                  int nthreads)
 {
+    orc_overlay_op(pairs, np, xyA, offA, xyB, offB, 1, area, flops, nthreads);
+}
+
+void orc_overlay_op(const uint32_t *pairs, int64_t np, const double *xyA, const int64_t *offA,
+                    const double *xyB, const int64_t *offB, int op, double *area,
+                    uint64_t *flops, int nthreads)
+{
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     ov_job jobs[256];
@@ -1124,6 +1179,7 @@ void orc_overlay(const uint32_t *pairs, int64_t np, const double *xyA, const int
         jobs[k].offA = offA; jobs[k].offB = offB; jobs[k].area = area;
         jobs[k].lo = np * k / nthreads; jobs[k].hi = np * (k + 1) / nthreads;
         jobs[k].flops = 0;
+        jobs[k].op = op;
     }
     if (nthreads == 1) ov_run(&jobs[0]);
     else {

@@ -84,6 +84,8 @@ int64_t orc_sibling_overlap_td(int f, const int32_t *slot_child, const float *sl
 /* ---------------- polygon intersection area ---------------- */
 double orc_clip_area(const double *P, int nP, const double *Q, int nQ,
                      uint64_t *flops);
+double orc_clip_area_op(const double *P, int nP, const double *Q, int nQ, int op,
+                        uint64_t *flops);
 double orc_poly_area(const double *P, int n);
 double orc_green_area(const double *P, int nP, const double *Q, int nQ);
 double orc_sh_convex_area(const double *P, int nP, const double *Q, int nQ);
@@ -92,6 +94,9 @@ void orc_area_grad(const double *P, int nP, const double *Q, int nQ,
 void orc_overlay(const uint32_t *pairs, int64_t np, const double *xyA,
                  const int64_t *offA, const double *xyB, const int64_t *offB,
                  double *area, uint64_t *flops, int nthreads);
+void orc_overlay_op(const uint32_t *pairs, int64_t np, const double *xyA,
+                    const int64_t *offA, const double *xyB, const int64_t *offB, int op,
+                    double *area, uint64_t *flops, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,11 @@ SIGNATURES = {
     "prt_overlay_plain_workspace_size": (c_size_t, [c_int64, c_int32]),
     "prt_overlay_plain": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "prt_overlay_plain_op": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_int32, c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "prt_overlay_batched_op": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_int32, c_int32, POINTER(TuneStruct), c_void_p, c_void_p,
+                                       c_size_t, c_void_p]),
     "prt_tune_default": (c_int, [POINTER(TuneStruct)]),
     "prt_overlay_batched_workspace_size": (c_size_t, [c_int64, c_int32]),
     "prt_overlay_batched": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,

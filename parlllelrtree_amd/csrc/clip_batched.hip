@@ -29,7 +29,7 @@ namespace prt {
 
 int plain_list_launch(const uint32_t *pairs, const uint32_t *list, const int64_t *count,
                       int64_t max_count, const double *xyA, const int64_t *offA, const double *xyB,
-                      const int64_t *offB, int32_t max_verts, double *area, void *ws,
+                      const int64_t *offB, int32_t max_verts, int op, double *area, void *ws,
                       size_t ws_bytes, hipStream_t s);
 size_t plain_list_ws_bytes(int64_t max_count, int32_t max_verts);
 
@@ -179,7 +179,8 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
 // [lo, hi) (with their x at the beam bottom/top cached in lxb/lxt): the
 // general sweep of the beam, recomputing x at crossing heights.  The order
 // of the entries does not matter.
-__device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP) {
+__device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                                   int op) {
     const int m = hi - lo;
     const double dy = yt - yb;
     double ya = yb, bs = 0.0;
@@ -231,7 +232,7 @@ __device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, 
                     nxt = i; ne = e; kn = kk; xan = xa; xcn = xc;
                 }
             }
-            if (pp && pq) bs += ((xan - xac) + (xcn - xcc)) * h;
+            if (op_inside(op, pp, pq)) bs += ((xan - xac) + (xcn - xcc)) * h;
             cur = nxt; ce = ne; kc = kn; xac = xan; xcc = xcn;
         }
         if (yc == yt) break;
@@ -250,7 +251,7 @@ __device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, 
 // serial general sweep when it has more than GS entries; otherwise the
 // function returns false and the team sweeps it (team_general).
 __device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
-                             double &out) {
+                             int op, double &out) {
     const int m = hi - lo;
     out = 0.0;
     if (m < 2) return true;
@@ -281,7 +282,7 @@ __device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double
         mono = !(t.lxb[i] < t.lxb[i - 1]) && !(t.lxt[i] < t.lxt[i - 1]);
     if (!mono) {
         if (m <= t.GS) return false;
-        out = beam_sum_general(t, lo, hi, yb, yt, nP);
+        out = beam_sum_general(t, lo, hi, yb, yt, nP, op);
         return true;
     }
     const double h = yt - yb;
@@ -290,7 +291,7 @@ __device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double
     for (int i = lo; i + 1 < hi; ++i) {
         if (t.list[i] < nP) pp ^= 1;
         else pq ^= 1;
-        if (pp && pq) bs += ((t.lxb[i + 1] - t.lxb[i]) + (t.lxt[i + 1] - t.lxt[i])) * h;
+        if (op_inside(op, pp, pq)) bs += ((t.lxb[i + 1] - t.lxb[i]) + (t.lxt[i + 1] - t.lxt[i])) * h;
     }
     out = bs;
     return true;
@@ -348,7 +349,7 @@ template <int TL> __device__ inline void team_sort_heights(const TeamLds &t, int
 // Returns the beam sum on every lane of the team.
 template <int TL>
 __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
-                               int tl, int base) {
+                               int op, int tl, int base) {
     const int m = hi - lo;
     const double dy = yt - yb;
     if (tl == 0) t.gmisc[1] = 0;
@@ -376,7 +377,7 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
     if (nc > t.GC) {
         // more crossing heights than the scratch holds: the serial sweep
         double r = 0.0;
-        if (tl == 0) r = beam_sum_general(t, lo, hi, yb, yt, nP);
+        if (tl == 0) r = beam_sum_general(t, lo, hi, yb, yt, nP, op);
         double v = __shfl(r, base, 64);
         team_sync();
         return v;
@@ -415,7 +416,8 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
             for (int q = 0; q + 1 < m; ++q) {
                 if (t.gidx[q] < nP) pp ^= 1;
                 else pq ^= 1;
-                if (pp && pq) bs += ((t.gxa[q + 1] - t.gxa[q]) + (t.gxc[q + 1] - t.gxc[q])) * h;
+                if (op_inside(op, pp, pq))
+                    bs += ((t.gxa[q + 1] - t.gxa[q]) + (t.gxc[q + 1] - t.gxc[q])) * h;
             }
         }
         team_sync();
@@ -437,6 +439,7 @@ struct BatchArgs {
     uint32_t *ovf_list;
     unsigned long long *ovf_count;
     int NV, SCAP;
+    int op;  // PRT_OP_*
 };
 
 // Returns 1 when the pair was handled, 0 when it must go to the overflow list.
@@ -445,13 +448,16 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
                          double &result) {
     const uint32_t a = A.pairs[2 * p], b = A.pairs[2 * p + 1];
     const int64_t oa = A.offA[a], ob = A.offB[b];
-    const int nP = static_cast<int>(A.offA[a + 1] - oa);
-    const int nQ = static_cast<int>(A.offB[b + 1] - ob);
+    int nP = static_cast<int>(A.offA[a + 1] - oa);
+    int nQ = static_cast<int>(A.offB[b + 1] - ob);
     const double *P = A.xyA + 2 * oa, *Q = A.xyB + 2 * ob;
     result = 0.0;
-    if (nP < 3 || nQ < 3) return 1;
+    const bool eP = nP < 3, eQ = nQ < 3;  // empty operands
+    if (eP && eQ) return 1;
+    if (nP + nQ > A.NV) return 0;
+    if (eP) nP = 0;
+    if (eQ) nQ = 0;
     const int n = nP + nQ;
-    if (n > A.NV) return 0;
 
     // bounding boxes
     double bp0 = __builtin_huge_val(), bp1 = bp0, bp2 = -bp0, bp3 = -bp0;
@@ -466,10 +472,9 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
     }
     bp0 = team_min<TL>(bp0); bp1 = team_min<TL>(bp1); bp2 = team_max<TL>(bp2); bp3 = team_max<TL>(bp3);
     bq0 = team_min<TL>(bq0); bq1 = team_min<TL>(bq1); bq2 = team_max<TL>(bq2); bq3 = team_max<TL>(bq3);
-    if (bp2 < bq0 || bq2 < bp0 || bp3 < bq1 || bq3 < bp1) return 1;
-    const double ylo = bp1 > bq1 ? bp1 : bq1;
-    const double yhi = bp3 < bq3 ? bp3 : bq3;
-    if (!(ylo < yhi)) return 1;
+    const double bpa[4] = {bp0, bp1, bp2, bp3}, bqa[4] = {bq0, bq1, bq2, bq3};
+    double ylo = 0.0, yhi = 0.0;
+    if (!op_range(A.op, eP, eQ, bpa, bqa, ylo, yhi)) return 1;
 
     // edge table and raw scanbeam table
     const int P2 = pow2_ge(n);
@@ -594,7 +599,7 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         for (int k = k0 + tl; k < k1; k += TL) {
             double bsk;
             if (beam_sum_try(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k], t.ys[k + 1], nP,
-                             bsk))
+                             A.op, bsk))
                 t.bsum[k] = bsk;
             else
                 t.cnt[atomicAdd(&t.gmisc[0], 1)] = k;
@@ -604,7 +609,7 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         for (int q = 0; q < ndef; ++q) {
             const int k = t.cnt[q];
             const double bsk = team_general<TL>(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k],
-                                                t.ys[k + 1], nP, tl, base);
+                                                t.ys[k + 1], nP, A.op, tl, base);
             if (tl == 0) t.bsum[k] = bsk;
         }
         team_sync();
@@ -852,8 +857,9 @@ namespace {
 // on the device (nothing here reads device memory on the host)
 int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
                  const double *xyA, const int64_t *offA, const double *xyB, const int64_t *offB,
-                 int32_t max_verts, const prt_tune *tune, double *area, void *ws, size_t ws_bytes,
-                 void *stream) {
+                 int32_t max_verts, int op, const prt_tune *tune, double *area, void *ws,
+                 size_t ws_bytes, void *stream) {
+    PRT_REQUIRE(op >= PRT_OP_DIFF && op <= PRT_OP_UNION, PRT_E_INVAL, "overlay_batched: op %d", op);
     PRT_REQUIRE(npairs >= 0 && npairs < (int64_t(1) << 32), PRT_E_INVAL,
                 "overlay_batched: npairs out of range");
     if (npairs == 0) return PRT_OK;
@@ -912,6 +918,7 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         A.ovf_count = w.ovf_count;
         A.NV = tune->bucket_max_verts[b] < max_verts ? tune->bucket_max_verts[b] : max_verts;
         A.SCAP = scap_for(A.NV, tune->list_factor[b]);
+        A.op = op;
         const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
         const size_t lds = static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
         if (lds > static_cast<size_t>(max_lds)) {
@@ -929,7 +936,7 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         if (rc) return rc;
     }
     return plain_list_launch(pairs, w.ovf_list, reinterpret_cast<const int64_t *>(w.ovf_count),
-                             npairs, xyA, offA, xyB, offB, max_verts, area, w.plain_ws,
+                             npairs, xyA, offA, xyB, offB, max_verts, op, area, w.plain_ws,
                              w.plain_bytes, s);
 }
 }  // namespace
@@ -938,8 +945,16 @@ extern "C" int prt_overlay_batched(const uint32_t *pairs, int64_t npairs, const 
                                    const int64_t *offA, const double *xyB, const int64_t *offB,
                                    int32_t max_verts, const prt_tune *tune, double *area, void *ws,
                                    size_t ws_bytes, void *stream) {
-    return batched_impl(pairs, nullptr, npairs, xyA, offA, xyB, offB, max_verts, tune, area, ws,
-                        ws_bytes, stream);
+    return batched_impl(pairs, nullptr, npairs, xyA, offA, xyB, offB, max_verts, PRT_OP_INT, tune,
+                        area, ws, ws_bytes, stream);
+}
+
+extern "C" int prt_overlay_batched_op(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                      const int64_t *offA, const double *xyB, const int64_t *offB,
+                                      int32_t max_verts, int32_t op, const prt_tune *tune,
+                                      double *area, void *ws, size_t ws_bytes, void *stream) {
+    return batched_impl(pairs, nullptr, npairs, xyA, offA, xyB, offB, max_verts, op, tune, area,
+                        ws, ws_bytes, stream);
 }
 
 extern "C" int prt_overlay_batched_dev(const uint32_t *pairs, const int64_t *npairs_d, int64_t cap,
@@ -947,8 +962,8 @@ extern "C" int prt_overlay_batched_dev(const uint32_t *pairs, const int64_t *npa
                                        const int64_t *offB, int32_t max_verts, const prt_tune *tune,
                                        double *area, void *ws, size_t ws_bytes, void *stream) {
     PRT_REQUIRE(npairs_d != nullptr, PRT_E_INVAL, "overlay_batched_dev: null pair count");
-    return batched_impl(pairs, npairs_d, cap, xyA, offA, xyB, offB, max_verts, tune, area, ws,
-                        ws_bytes, stream);
+    return batched_impl(pairs, npairs_d, cap, xyA, offA, xyB, offB, max_verts, PRT_OP_INT, tune,
+                        area, ws, ws_bytes, stream);
 }
 
 // Number of pairs of the last prt_overlay_batched call on this workspace that

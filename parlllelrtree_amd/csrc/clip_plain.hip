@@ -115,7 +115,8 @@ __device__ inline bool sb_before(const Lane &L, int i, int j) {
     return L.at(L.w.id, L.at(L.w.act, i)) < L.at(L.w.id, L.at(L.w.act, j));
 }
 
-__device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, double acc) {
+__device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, double acc,
+                           int op) {
     const double h = yc - ya;
     for (int i = 0; i < m; ++i) L.at(L.w.ord, i) = i;
     for (int i = 1; i < m; ++i) {
@@ -131,17 +132,30 @@ __device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, d
         int l = L.at(L.w.ord, t), r = L.at(L.w.ord, t + 1);
         if (L.at(L.w.id, L.at(L.w.act, l)) < nP) pp ^= 1;
         else pq ^= 1;
-        if (pp && pq)
+        if (op_inside(op, pp, pq))
             acc += ((L.at(L.w.xa, r) - L.at(L.w.xa, l)) + (L.at(L.w.xc, r) - L.at(L.w.xc, l))) * h;
     }
     return acc;
 }
 
+// Area of P op Q (op: PRT_OP_DIFF / INT / XOR / UNION); a polygon of fewer
+// than 3 vertices is empty.
 __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
-                           const double *__restrict__ Q, int nQ) {
-    if (nP < 3 || nQ < 3) return 0.0;
+                           const double *__restrict__ Q, int nQ, int op) {
+    const bool eP = nP < 3, eQ = nQ < 3;
+    if (eP && eQ) return 0.0;
     if (nP + nQ > L.w.cap) return __builtin_nan("");
-    double bp[4] = {P[0], P[1], P[0], P[1]}, bq[4] = {Q[0], Q[1], Q[0], Q[1]};
+    if (eP) nP = 0;
+    if (eQ) nQ = 0;
+    double bp[4] = {0.0, 0.0, 0.0, 0.0}, bq[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!eP) {
+        bp[0] = bp[2] = P[0];
+        bp[1] = bp[3] = P[1];
+    }
+    if (!eQ) {
+        bq[0] = bq[2] = Q[0];
+        bq[1] = bq[3] = Q[1];
+    }
     for (int i = 1; i < nP; ++i) {
         double x = P[2 * i], y = P[2 * i + 1];
         bp[0] = x < bp[0] ? x : bp[0];
@@ -156,10 +170,8 @@ __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
         bq[1] = y < bq[1] ? y : bq[1];
         bq[3] = y > bq[3] ? y : bq[3];
     }
-    if (bp[2] < bq[0] || bq[2] < bp[0] || bp[3] < bq[1] || bq[3] < bp[1]) return 0.0;
-    const double ylo = bp[1] > bq[1] ? bp[1] : bq[1];
-    const double yhi = bp[3] < bq[3] ? bp[3] : bq[3];
-    if (!(ylo < yhi)) return 0.0;
+    double ylo = 0.0, yhi = 0.0;
+    if (!op_range(op, eP, eQ, bp, bq, ylo, yhi)) return 0.0;
 
     int ne = 0, ny = 0;
     for (int poly = 0; poly < 2; ++poly) {
@@ -239,7 +251,7 @@ __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
             } else {
                 for (int i = 0; i < m; ++i) L.at(L.w.xc, i) = edge_x(L, L.at(L.w.act, i), yc);
             }
-            bs = sub_beam(L, m, nP, ya, yc, bs);
+            bs = sub_beam(L, m, nP, ya, yc, bs, op);
             if (yc == yt) break;
             for (int i = 0; i < m; ++i) L.at(L.w.xa, i) = L.at(L.w.xc, i);
             ya = yc;
@@ -256,7 +268,7 @@ __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
 __global__ __launch_bounds__(CP_THREADS) void k_clip_plain(
     const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
     const int64_t *__restrict__ offA, const double *__restrict__ xyB,
-    const int64_t *__restrict__ offB, double *__restrict__ area, PlainWs w) {
+    const int64_t *__restrict__ offB, double *__restrict__ area, PlainWs w, int op) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= w.T) return;
     Lane L(w, t);
@@ -265,7 +277,7 @@ __global__ __launch_bounds__(CP_THREADS) void k_clip_plain(
         const int64_t oa = offA[a], ob = offB[b];
         const int nA = static_cast<int>(offA[a + 1] - oa);
         const int nB = static_cast<int>(offB[b + 1] - ob);
-        area[p] = clip_one(L, xyA + 2 * oa, nA, xyB + 2 * ob, nB);
+        area[p] = clip_one(L, xyA + 2 * oa, nA, xyB + 2 * ob, nB, op);
     }
 }
 
@@ -275,7 +287,7 @@ __global__ __launch_bounds__(CP_THREADS) void k_clip_plain_list(
     const uint32_t *__restrict__ pairs, const uint32_t *__restrict__ list,
     const int64_t *__restrict__ count, const double *__restrict__ xyA,
     const int64_t *__restrict__ offA, const double *__restrict__ xyB,
-    const int64_t *__restrict__ offB, double *__restrict__ area, PlainWs w) {
+    const int64_t *__restrict__ offB, double *__restrict__ area, PlainWs w, int op) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= w.T) return;
     Lane L(w, t);
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(CP_THREADS) void k_clip_plain_list(
         const int64_t oa = offA[a], ob = offB[b];
         const int nA = static_cast<int>(offA[a + 1] - oa);
         const int nB = static_cast<int>(offB[b + 1] - ob);
-        area[p] = clip_one(L, xyA + 2 * oa, nA, xyB + 2 * ob, nB);
+        area[p] = clip_one(L, xyA + 2 * oa, nA, xyB + 2 * ob, nB, op);
     }
 }
 
@@ -325,7 +337,7 @@ size_t plain_list_ws_bytes(int64_t max_count, int32_t max_verts) {
 
 int plain_list_launch(const uint32_t *pairs, const uint32_t *list, const int64_t *count,
                       int64_t max_count, const double *xyA, const int64_t *offA, const double *xyB,
-                      const int64_t *offB, int32_t max_verts, double *area, void *ws,
+                      const int64_t *offB, int32_t max_verts, int op, double *area, void *ws,
                       size_t ws_bytes, hipStream_t s) {
     int64_t T = max_count < CP_LIST_LANES ? max_count : CP_LIST_LANES;
     T = T < 1 ? 1 : T;
@@ -334,7 +346,7 @@ int plain_list_launch(const uint32_t *pairs, const uint32_t *list, const int64_t
                 "overlay: overflow workspace too small");
     const unsigned grid = static_cast<unsigned>(ceil_div(w.T, CP_THREADS));
     PRT_LAUNCH(k_clip_plain_list, grid, CP_THREADS, 0, s, pairs, list, count, xyA, offA, xyB, offB,
-               area, w);
+               area, w, op);
     PRT_LAUNCHED();
     return PRT_OK;
 }
@@ -352,6 +364,15 @@ extern "C" int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const do
                                  const int64_t *offA, const double *xyB, const int64_t *offB,
                                  int32_t max_verts, double *area, void *ws, size_t ws_bytes,
                                  void *stream) {
+    return prt_overlay_plain_op(pairs, npairs, xyA, offA, xyB, offB, max_verts, PRT_OP_INT, area,
+                                ws, ws_bytes, stream);
+}
+
+extern "C" int prt_overlay_plain_op(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                    const int64_t *offA, const double *xyB, const int64_t *offB,
+                                    int32_t max_verts, int32_t op, double *area, void *ws,
+                                    size_t ws_bytes, void *stream) {
+    PRT_REQUIRE(op >= PRT_OP_DIFF && op <= PRT_OP_UNION, PRT_E_INVAL, "overlay_plain: op %d", op);
     PRT_REQUIRE(npairs >= 0, PRT_E_INVAL, "overlay_plain: npairs < 0");
     if (npairs == 0) return PRT_OK;
     PRT_REQUIRE(pairs && xyA && offA && xyB && offB && area, PRT_E_INVAL,
@@ -366,7 +387,7 @@ extern "C" int prt_overlay_plain(const uint32_t *pairs, int64_t npairs, const do
                 "overlay_plain: workspace too small");
     const unsigned grid = static_cast<unsigned>(ceil_div(w.T, CP_THREADS));
     PRT_LAUNCH(k_clip_plain, grid, CP_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA, xyB,
-               offB, area, w);
+               offB, area, w, op);
     PRT_LAUNCHED();
     return PRT_OK;
 }

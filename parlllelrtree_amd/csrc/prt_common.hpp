@@ -114,6 +114,42 @@ __host__ __device__ inline float4 box_union(float4 a, float4 b) {
     return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
 }
 
+// ---- polygon set operations (gpc_op order: DIFF, INT, XOR, UNION) ------
+// A gap between neighbouring edges belongs to the result set by the even-odd
+// parities of A (subject, pp) and B (clip, pq).
+__host__ __device__ inline bool op_inside(int op, int pp, int pq) {
+    switch (op) {
+    case PRT_OP_DIFF: return pp && !pq;
+    case PRT_OP_XOR: return pp != pq;
+    case PRT_OP_UNION: return pp || pq;
+    default: return pp && pq;
+    }
+}
+
+// y range the sweep of `op` covers, from the bounding boxes (xmin, ymin,
+// xmax, ymax) of A and B; eP / eQ: the polygon has fewer than 3 vertices
+// and is empty.  Returns false when the result is empty.
+__host__ __device__ inline bool op_range(int op, bool eP, bool eQ, const double *bp,
+                                         const double *bq, double &ylo, double &yhi) {
+    if (op == PRT_OP_INT) {
+        if (eP || eQ) return false;
+        if (bp[2] < bq[0] || bq[2] < bp[0] || bp[3] < bq[1] || bq[3] < bp[1]) return false;
+        ylo = bp[1] > bq[1] ? bp[1] : bq[1];
+        yhi = bp[3] < bq[3] ? bp[3] : bq[3];
+    } else if (op == PRT_OP_DIFF || eQ) {
+        if (eP) return false;
+        ylo = bp[1];
+        yhi = bp[3];
+    } else if (eP) {
+        ylo = bq[1];
+        yhi = bq[3];
+    } else {
+        ylo = bp[1] < bq[1] ? bp[1] : bq[1];
+        yhi = bp[3] > bq[3] ? bp[3] : bq[3];
+    }
+    return ylo < yhi;
+}
+
 // ---- internal launchers shared between translation units ---------------
 // Stable LSD radix sort of 64-bit items on bits [bit_lo, bit_hi), segmented:
 // items never leave their segment [s*seg, (s+1)*seg).  Returns the buffer

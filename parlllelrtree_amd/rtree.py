@@ -299,12 +299,23 @@ def default_tune() -> TuneStruct:
     return t
 
 
+OPS = {"diff": 0, "int": 1, "xor": 2, "union": 3}  # PRT_OP_*, gpc_op order
+
+
+def _op(op) -> int:
+    v = OPS.get(op, op)
+    if v not in (0, 1, 2, 3):
+        raise ValueError(f"op must be one of {list(OPS)}")
+    return v
+
+
 def overlay_batched(pairs: torch.Tensor, A: PolyDev, B: PolyDev, tune: TuneStruct | None = None,
                     area: torch.Tensor | None = None, workspace: torch.Tensor | None = None,
-                    stats: dict | None = None) -> torch.Tensor:
+                    stats: dict | None = None, op="int") -> torch.Tensor:
     """Same areas as overlay_plain (bit-identical): bucketed, persistent,
     work-stealing launches with the team shapes of `tune`.  With `stats`
-    given, stats["overflow"] = pairs that took the plain path (syncs)."""
+    given, stats["overflow"] = pairs that took the plain path (syncs).
+    op: "int" (A & B), "diff" (A - B), "xor", "union"."""
     pairs = pairs.contiguous()
     npairs = pairs.shape[0]
     if area is None:
@@ -314,9 +325,9 @@ def overlay_batched(pairs: torch.Tensor, A: PolyDev, B: PolyDev, tune: TuneStruc
     ws = workspace if workspace is not None and workspace.numel() >= need else \
         _ws(need, pairs.device)
     tp = ctypes.byref(tune) if tune is not None else None
-    check(lib().prt_overlay_batched(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
-                                    _ptr(B.off), mv, tp, _ptr(area), _ptr(ws), ws.numel(),
-                                    _stream()))
+    check(lib().prt_overlay_batched_op(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
+                                       _ptr(B.off), mv, _op(op), tp, _ptr(area), _ptr(ws),
+                                       ws.numel(), _stream()))
     if stats is not None:
         ov = ctypes.c_int64(0)
         check(lib().prt_overlay_batched_overflow(_ptr(ws), ws.numel(), npairs, mv,
@@ -326,8 +337,9 @@ def overlay_batched(pairs: torch.Tensor, A: PolyDev, B: PolyDev, tune: TuneStruc
 
 
 def overlay_plain(pairs: torch.Tensor, A: PolyDev, B: PolyDev, area: torch.Tensor | None = None,
-                  workspace: torch.Tensor | None = None) -> torch.Tensor:
-    """FP64 area of A[pairs[t,0]] intersect B[pairs[t,1]], one lane per pair."""
+                  workspace: torch.Tensor | None = None, op="int") -> torch.Tensor:
+    """FP64 area of A[pairs[t,0]] op B[pairs[t,1]] (default: intersection),
+    one lane per pair."""
     pairs = pairs.contiguous()
     npairs = pairs.shape[0]
     if area is None:
@@ -336,8 +348,9 @@ def overlay_plain(pairs: torch.Tensor, A: PolyDev, B: PolyDev, area: torch.Tenso
     need = lib().prt_overlay_plain_workspace_size(npairs, mv)
     ws = workspace if workspace is not None and workspace.numel() >= need else \
         _ws(need, pairs.device)
-    check(lib().prt_overlay_plain(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
-                                  _ptr(B.off), mv, _ptr(area), _ptr(ws), ws.numel(), _stream()))
+    check(lib().prt_overlay_plain_op(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off), _ptr(B.xy),
+                                     _ptr(B.off), mv, _op(op), _ptr(area), _ptr(ws), ws.numel(),
+                                     _stream()))
     return area
 
 

@@ -212,3 +212,31 @@ def test_batched_lognormal_grid_sample(dev):
     assert (ref > 0).sum() > 10_000
     assert np.array_equal(got, ref)
     assert np.array_equal(run_plain(pairs, A, B), ref)
+
+
+@pytest.mark.parametrize("op", ["diff", "xor", "union", "int"])
+def test_set_operations_bit_exact(dev, op):
+    # GPC's other operations: areas of A - B, A ^ B, A | B by the same sweep;
+    # plain and batched (with windows and crossing-dense beams) against the
+    # oracle, including empty operands and a sawtooth pair
+    pairs, S, T = _grid_pairs()
+    ref = O.overlay_op(pairs, S, T, op, nthreads=8)
+    d = torch.from_numpy(pairs.view(np.int32)).cuda()
+    dS, dT = rtree.to_device_polys(S), rtree.to_device_polys(T)
+    got_p = rtree.overlay_plain(d, dS, dT, nan_area(len(pairs)), op=op).cpu().numpy()
+    got_b = rtree.overlay_batched(d, dS, dT, None, nan_area(len(pairs)), op=op).cpu().numpy()
+    assert np.array_equal(got_p, ref), f"plain: {(got_p != ref).sum()} differ"
+    assert np.array_equal(got_b, ref), f"batched: {(got_b != ref).sum()} differ"
+    tune = rtree.default_tune()
+    for b in range(6):
+        tune.lanes_per_pair[b], tune.waves_per_block[b], tune.list_factor[b] = 16, 2, 1
+    got_t = rtree.overlay_batched(d, dS, dT, tune, nan_area(len(pairs)), op=op).cpu().numpy()
+    assert np.array_equal(got_t, ref)
+    A = csr([zigzag(20, 0.3), [[0, 0], [1, 1]], zigzag(9, 1.5)])
+    B = csr([zigzag(20, -1.5, y0=0.05, y1=1.1, x0=0.3), zigzag(5, 0.5), [[2, 2]]])
+    pp = np.array([(i, j) for i in range(3) for j in range(3)], np.uint32)
+    ref2 = O.overlay_op(pp, A, B, op)
+    dp = torch.from_numpy(pp.view(np.int32)).cuda()
+    dA, dB = rtree.to_device_polys(A), rtree.to_device_polys(B)
+    assert np.array_equal(rtree.overlay_batched(dp, dA, dB, op=op).cpu().numpy(), ref2)
+    assert np.array_equal(rtree.overlay_plain(dp, dA, dB, op=op).cpu().numpy(), ref2)
