@@ -2,7 +2,8 @@
 
 ``python -m parlllelrtree_amd.build`` (or ``build()``) compiles every source
 under ``csrc/`` with hipcc into ``_build/`` and links ``libprt.so`` next to
-this file; the oracle (test infrastructure, never linked into libprt) is
+this file, then the command-line drivers under ``drivers/`` (host C++ over
+the C-ABI only) into ``bin/``, linked against that library; the oracle (test infrastructure, never linked into libprt) is
 built by ``oracle/Makefile`` into ``oracle/_build/liboracle.so``.
 """
 from __future__ import annotations
@@ -18,7 +19,10 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = PKG / "libprt.so"
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+DRIVERS = PKG / "drivers"
+BIN = PKG / "bin"
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", f"{ROCM}/bin/hipcc")
 ARCH = "gfx950"
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}"]
@@ -57,6 +61,25 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+def build_drivers(jobs: int = 4) -> list[Path]:
+    """prt_bu_rtree, prt_td_rtree, prt_overlay: host programs that include
+    only prt.h and the HIP runtime, found next to libprt.so by rpath."""
+    BIN.mkdir(exist_ok=True)
+
+    def one(src: Path) -> Path:
+        exe = BIN / f"prt_{src.stem}"
+        tmp = exe.with_suffix(".tmp")
+        _run([HIPCC, "-O2", "-std=c++17", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+              f"-I{ROOT / 'include'}", f"-I{ROCM}/include", f"-I{DRIVERS}", str(src),
+              "-o", str(tmp), f"-L{PKG}", "-lprt", "-Wl,-rpath,$ORIGIN/..",
+              f"-L{ROCM}/lib", "-lamdhip64"])
+        os.replace(tmp, exe)
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(one, sorted(DRIVERS.glob("*.cpp"))))
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     return ROOT / "oracle" / "_build" / "liboracle.so"
@@ -64,6 +87,7 @@ def build_oracle() -> Path:
 
 def build() -> None:
     build_lib()
+    build_drivers()
     build_oracle()
 
 
