@@ -167,13 +167,16 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
 // The same sort of items made from rects by its histogram launch (written to
 // buf0 there): (ord32(x1 or y1) << 32 | position), segmented, bits 32..63;
 // or (morton_center30 << 32 | position), bits 32..61.  With zeroed = true the
-// caller has already cleared the words radix_zero_range names.
+// caller has already cleared the sort's counter words (radix_ghist_words).
 int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t *buf1, int64_t n,
                         int64_t seg, const RadixWs &w, hipStream_t s, uint64_t **result,
                         bool zeroed);
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
                       const RadixWs &w, hipStream_t s, uint64_t **result);
-void radix_zero_range(const RadixWs &w, int64_t n, int64_t seg, uint32_t **p, int64_t *words);
+// Counter words (cleared before a sort unless it clears them itself) and
+// look-back words of a sort of n items in segments of seg.
+int64_t radix_ghist_words(int64_t n, int64_t seg);
+int64_t radix_status_words(int64_t n, int64_t seg);
 // Stable sort by 64-bit keys (two 32-bit passes); items[k] = (key >> 32 << 32)
 // | id of the k-th rect, keys_sorted[k] (optional) its key.  b0..b3 and items
 // are distinct n-entry buffers.

@@ -81,9 +81,12 @@ __device__ inline void lb_store(uint64_t *p, uint64_t v) {
 }
 
 // Lanes of the wave holding the same digit as this lane (valid lanes only).
-__device__ inline uint64_t digit_peers(uint32_t d, bool valid, int bits) {
+// Always 8 ballots: the bits of a narrower digit above its width are zero in
+// every lane and leave the mask unchanged.
+__device__ inline uint64_t digit_peers(uint32_t d, bool valid) {
     uint64_t peers = __ballot(valid);
-    for (int k = 0; k < bits; ++k) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
         const uint64_t bal = __ballot((d >> k) & 1u);
         peers &= ((d >> k) & 1u) ? bal : ~bal;
     }
@@ -164,12 +167,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
     int64_t b, e;
     tile_range(t, tile, b, e);
     const int count = static_cast<int>(e - b);
-    // plain LDS atomics: a digit shared by many lanes serialises only its
-    // own additions, which costs less than ranking every item by ballots
+    // all loads of the thread are issued before the first use (one memory
+    // latency per tile, not one per item); then plain LDS atomics: a digit
+    // shared by many lanes serialises only its own additions, which costs
+    // less than ranking every item by ballots
+    uint64_t vals[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
+        vals[j] = local < count ? src(b + local) : 0ull;
+    }
+#pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
         if (local >= count) continue;
-        const uint64_t v = src(b + local);
+        const uint64_t v = vals[j];
         for (int p = 0; p < passes; ++p) {
             const int shift = bit_lo + 8 * p;
             const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
@@ -213,30 +225,38 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
     int64_t b, e;
     tile_range(t, tile, b, e);
     const int count = static_cast<int>(e - b);
+    // the segment's digit total is loaded now and used after the ranking
+    const uint32_t tot = tid < RS_BINS ? ghist_p[seg * RS_BINS + tid] : 0u;
 
     uint64_t item[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // issue every load of the thread before ranking the first item
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
+        item[j] = local < count ? in[b + local] : 0ull;
+    }
+#pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
         const bool valid = local < count;
-        const uint64_t v = valid ? in[b + local] : 0ull;
+        const uint64_t v = item[j];
         const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
-        const uint64_t peers = digit_peers(d, valid, bits);
+        const uint64_t peers = digit_peers(d, valid);
         uint32_t base = 0;
         if (valid) base = wcnt[wave][d];
         const uint32_t r = static_cast<uint32_t>(__popcll(peers & lt));
         __builtin_amdgcn_wave_barrier();
         if (valid && r == 0) wcnt[wave][d] = base + static_cast<uint32_t>(__popcll(peers));
         __builtin_amdgcn_wave_barrier();
-        item[j] = v;
         rank[j] = valid ? base + r : 0xFFFFFFFFu;
     }
     __syncthreads();
     // threads 0..255 own digit tid: tile count (published at once), prefix
     // over waves; block-wide exclusive scans of the tile counts (dstart) and
     // of the segment's digit totals (gbase)
-    uint32_t run = 0, tot = 0;
+    uint32_t run = 0;
     if (tid < RS_BINS) {
         for (int w = 0; w < RS_WAVES; ++w) {
             const uint32_t c = wcnt[w][tid];
@@ -244,7 +264,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
             run += c;
         }
         lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
-        tot = ghist_p[seg * RS_BINS + tid];
     }
     uint32_t x = run, g = tot;
     for (int o = 1; o < 64; o <<= 1) {
@@ -348,11 +367,14 @@ int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w) {
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 
-void radix_zero_range(const RadixWs &w, int64_t n, int64_t seg, uint32_t **p, int64_t *words) {
-    const Tiling t = make_tiling(n > 0 ? n : 1, seg);
-    *p = w.ghist;
-    *words = sort_layout(t).ghist_words;
+int64_t radix_ghist_words(int64_t n, int64_t seg) {
+    return sort_layout(make_tiling(n > 0 ? n : 1, seg)).ghist_words;
 }
+
+int64_t radix_status_words(int64_t n, int64_t seg) {
+    return sort_layout(make_tiling(n > 0 ? n : 1, seg)).status_words;
+}
+
 
 namespace {
 template <class Src>

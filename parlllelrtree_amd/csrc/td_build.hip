@@ -31,14 +31,16 @@ __global__ void k_td_items(const float4 *__restrict__ rect_cur, int64_t n, int c
     items[i] = (static_cast<uint64_t>(ord32(comp ? r.y : r.x)) << 32) | static_cast<uint64_t>(i);
 }
 
-// also clears zero_words words at zero_p (the next sort's counters)
+__global__ void k_zero_words(uint32_t *__restrict__ p, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0u;
+}
+
 __global__ void k_td_gather(const uint64_t *__restrict__ sorted, const float4 *__restrict__ rect_cur,
                             const uint32_t *__restrict__ id_cur, int64_t n,
-                            float4 *__restrict__ rect_next, uint32_t *__restrict__ id_next,
-                            uint32_t *__restrict__ zero_p, int64_t zero_words) {
+                            float4 *__restrict__ rect_next, uint32_t *__restrict__ id_next) {
     int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (int64_t z = i; z < zero_words; z += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        zero_p[z] = 0u;
     if (i >= n) return;
     uint32_t pos = static_cast<uint32_t>(sorted[i]);
     rect_next[i] = rect_cur[pos];
@@ -201,19 +203,30 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
     const int64_t g0 = blk * spb;  // first group of the block
     const bool last_block = blk == nb - 1;
     // 1. gather the groups' rects and ids into LDS (local index k), in the
-    //    order of the previous level's sort
-    for (int k = tid; k < LS_SLOTS; k += LS_THREADS) {
+    //    order of the previous level's sort; every index load of the thread
+    //    is issued before the dependent rect and id loads (two memory
+    //    latencies per block, not two per position)
+    uint32_t sidx[ST_PER];
+    bool live[ST_PER];
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        const int k = tid + m * LS_THREADS;
         const int64_t gg = g0 + k / A.A2, w = k % A.A2, pos = gg * A.a0 + w;
-        float4 rc = empty_box();
-        uint32_t id = 0xFFFFFFFFu;
-        if (gg < A.num_groups && w < A.a0 && pos < A.n) {
-            const uint32_t sidx =
-                A.items ? static_cast<uint32_t>(A.items[pos]) : static_cast<uint32_t>(pos);
-            rc = A.rect_cur[sidx];
-            id = A.id_cur ? A.id_cur[sidx] : sidx;
-        }
-        rect[k] = rc;
-        ids[k] = id;
+        live[m] = gg < A.num_groups && w < A.a0 && pos < A.n;
+        sidx[m] = static_cast<uint32_t>(pos);
+        if (live[m] && A.items) sidx[m] = static_cast<uint32_t>(A.items[pos]);
+    }
+    float4 rcs[ST_PER];
+    uint32_t idv[ST_PER];
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        rcs[m] = live[m] ? A.rect_cur[sidx[m]] : empty_box();
+        idv[m] = live[m] ? (A.id_cur ? A.id_cur[sidx[m]] : sidx[m]) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        rect[tid + m * LS_THREADS] = rcs[m];
+        ids[tid + m * LS_THREADS] = idv[m];
     }
     __syncthreads();
     // key of position k = 4t + r: (ord32(coord), position, local index)
@@ -409,6 +422,23 @@ int64_t td_seg(int64_t n, int f, int L, int p) {
     }
     return a;
 }
+// Radix levels (segments larger than LS_SLOTS): look-back words of the
+// largest tiling, shared by the levels in turn, and counter words of every
+// level side by side, so that one launch clears all of them.
+struct TdRadixWords {
+    int64_t status = 0, ghist = 0;
+};
+TdRadixWords td_radix_words(int64_t n, int f, int L) {
+    TdRadixWords w;
+    for (int p = 0; p < L; ++p) {
+        const int64_t a = td_seg(n, f, L, p);
+        if (a <= LS_SLOTS) continue;
+        const int64_t st = radix_status_words(n, a);
+        if (st > w.status) w.status = st;
+        w.ghist += radix_ghist_words(n, a);
+    }
+    return w;
+}
 }  // namespace
 
 extern "C" size_t prt_td_workspace_size(int64_t n, int fanout) {
@@ -421,15 +451,10 @@ extern "C" size_t prt_td_workspace_size(int64_t n, int fanout) {
     s.take<float4>(n);
     s.take<uint32_t>(n);
     s.take<uint32_t>(n);
-    size_t rad = 0;
-    for (int p = 0; p < t.num_levels; ++p) {
-        int64_t a = td_seg(n, fanout, t.num_levels, p);
-        if (a > LS_SLOTS) {
-            size_t r = radix_ws_bytes(n, a);
-            if (r > rad) rad = r;
-        }
-    }
-    return s.bytes() + rad;
+    const TdRadixWords w = td_radix_words(n, fanout, t.num_levels);
+    s.take<uint64_t>(w.status);
+    s.take<uint32_t>(w.ghist);
+    return s.bytes();
 }
 
 extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tree *tree, void *ws,
@@ -446,20 +471,12 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
     float4 *ra = c.take<float4>(n), *rb = c.take<float4>(n);
     uint32_t *da = c.take<uint32_t>(n), *db = c.take<uint32_t>(n);
     const int L = tree->num_levels, f = fanout;
-    RadixWs rw{};
-    {
-        int64_t worst = 0;
-        size_t rad = 0;
-        for (int p = 0; p < L; ++p) {
-            int64_t a = td_seg(n, f, L, p);
-            if (a > LS_SLOTS && radix_ws_bytes(n, a) > rad) {
-                rad = radix_ws_bytes(n, a);
-                worst = a;
-            }
-        }
-        if (worst > 0) PRT_REQUIRE(radix_carve(c, n, worst, rw) == PRT_OK, PRT_E_NOMEM, "td_build: ws");
-    }
+    const TdRadixWords rwords = td_radix_words(n, f, L);
+    uint64_t *status = c.take<uint64_t>(rwords.status);
+    uint32_t *ghist = c.take<uint32_t>(rwords.ghist);
     PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "td_build: workspace too small");
+    // one sort at a time may use all counter words (the level-by-level path)
+    RadixWs rw{ghist, status, rwords.ghist, rwords.status};
 
     const float4 *cur_r = reinterpret_cast<const float4 *>(rects);
     const uint32_t *cur_i = nullptr;  // identity at level 0
@@ -472,22 +489,32 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
                 p0 = p;
                 break;
             }
+        // the counters of every radix level are cleared by one launch.
+        // Between two radix levels the rects and ids move into the order of
+        // the finished level (one streaming pass of many small blocks: the
+        // random reads overlap far better there than inside the sort's large
+        // tiles), so the next level's sort and the subtree kernel read
+        // neighbouring rects from a few segments.
+        if (rwords.ghist > 0) {
+            const int64_t zw = rwords.ghist;
+            const unsigned zg = static_cast<unsigned>(ceil_div(zw, 1024) < 1024 ? ceil_div(zw, 1024) : 1024);
+            PRT_LAUNCH(k_zero_words, zg, 1024, 0, s, ghist, zw);
+            PRT_LAUNCHED();
+        }
         const uint64_t *items = nullptr;
-        bool zeroed = false;
+        uint32_t *lg = ghist;
         for (int p = 0; p < p0; ++p) {
             const int64_t a = td_seg(n, f, L, p);
+            const RadixWs lw{lg, status, radix_ghist_words(n, a), rwords.status};
+            lg += lw.ghist_cap;
             uint64_t *sorted = nullptr;
-            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, rw, s, &sorted, zeroed);
+            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, lw, s, &sorted, true);
             if (rc) return rc;
             if (p + 1 < p0) {
                 float4 *nxt_r = (cur_r == ra) ? rb : ra;
                 uint32_t *nxt_i = (cur_i == da) ? db : da;
-                uint32_t *zp = nullptr;
-                int64_t zw = 0;
-                radix_zero_range(rw, n, td_seg(n, f, L, p + 1), &zp, &zw);
-                PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i, zp, zw);
+                PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i);
                 PRT_LAUNCHED();
-                zeroed = true;
                 cur_r = nxt_r;
                 cur_i = nxt_i;
             } else {
@@ -539,8 +566,7 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
             uint64_t *sorted = nullptr;
             rc = radix_sort_u64(ia, ia, ib, n, a, 32, 64, rw, s, &sorted);
             if (rc) return rc;
-            PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i, nullptr,
-                       int64_t(0));
+            PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i);
             PRT_LAUNCHED();
         } else {
             const int A2 = static_cast<int>(next_pow2(a));
