@@ -33,6 +33,9 @@
  *   prt_overlay_plain                       <- Execute OV:81-113 +
  *                                              gpc_polygon_clip OV:115-750
  *   prt_overlay_total                       <- count1 of printfOutput OV:45-58
+ *   prt_overlay_contours_count / _write     <- gpc_polygon_clip's result
+ *                                              polygon OV:527-750, copied
+ *                                              out OV:720-728
  *   prt_pipeline_*                          <- the stage sequence of BU:836-987
  *                                              / TD:629-758 feeding run()
  *                                              OV:61-79 (no single reference
@@ -254,6 +257,33 @@ int prt_overlay_total(const double *area, const int64_t *npairs_d, int64_t cap, 
  * rounded outward so the box contains the polygon (the reference builds its
  * tree on (int)(v*1e5) boxes, OV:805-808). */
 int prt_poly_mbr_dev(const double *xy, const int64_t *off, int64_t n, float *mbr, void *stream);
+
+/* ---- result contours of A[pairs[2t]] op B[pairs[2t+1]] ---------------
+ * GPC's output polygon (gpc_polygon_clip's result, OV:527-750, copied out
+ * OV:720-728), in two passes.  prt_overlay_contours_count writes per pair
+ * the number of contours and of vertices (-1 in both for a pair of more
+ * than max_verts vertices).  The caller scans them into cont_off / vert_off
+ * (np + 1 exclusive offsets, device) and calls prt_overlay_contours_write,
+ * which writes per contour its first vertex (cont_start, absolute index
+ * into xy) and hole flag, and the vertices (xy, interleaved doubles).
+ * Outer contours run counter-clockwise, holes clockwise; contours of a pair
+ * come out in the order the sweep closes them.  *status (device int32) is
+ * set to 1 when a pair's output did not match its counts.  The workspace of
+ * the count pass is prt_overlay_contours_workspace_size(np, max_verts, 0).
+ * The vertex order is this library's own (see the kernel's header), not
+ * GPC's. */
+size_t prt_overlay_contours_workspace_size(int64_t npairs, int32_t max_verts,
+                                           int64_t total_vertices);
+int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                               const int64_t *offA, const double *xyB, const int64_t *offB,
+                               int32_t max_verts, int32_t op, int64_t *ncont, int64_t *nvert,
+                               void *ws, size_t ws_bytes, void *stream);
+int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                               const int64_t *offA, const double *xyB, const int64_t *offB,
+                               int32_t max_verts, int32_t op, const int64_t *cont_off,
+                               const int64_t *vert_off, int64_t total_vertices,
+                               int64_t *cont_start, int32_t *cont_hole, double *xy,
+                               int32_t *status, void *ws, size_t ws_bytes, void *stream);
 
 /* ---- overlay pipeline: MBRs of A and B -> R-tree over B's MBRs -> join
  * with A's MBRs -> batched FP64 clip of every pair -> total area ----------

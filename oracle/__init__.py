@@ -46,6 +46,8 @@ _SIG = {
     "orc_clip_area_op": (c_double, [c_void_p, c_int, c_void_p, c_int, c_int, POINTER(c_uint64)]),
     "orc_overlay_op": (None, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                               c_void_p, POINTER(c_uint64), c_int]),
+    "orc_clip_contours": (None, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_int64,
+                                 c_void_p, c_void_p, c_int64, POINTER(c_int64), POINTER(c_int64)]),
     "orc_poly_area": (c_double, [c_void_p, c_int]),
     "orc_green_area": (c_double, [c_void_p, c_int, c_void_p, c_int]),
     "orc_sh_convex_area": (c_double, [c_void_p, c_int, c_void_p, c_int]),
@@ -245,6 +247,24 @@ def clip_area_op(P, Q, op):
     P = _c(P, np.float64)
     Q = _c(Q, np.float64)
     return lib().orc_clip_area_op(_p(P), len(P), _p(Q), len(Q), OPS.get(op, op), None)
+
+
+def clip_contours(P, Q, op="int"):
+    """Contours of P op Q: list of (k, 2) vertex arrays (outer counter-
+    clockwise, holes clockwise) and the hole flags, in closing order."""
+    P = _c(P, np.float64)
+    Q = _c(Q, np.float64)
+    nv, nc = c_int64(0), c_int64(0)
+    o = OPS.get(op, op)
+    lib().orc_clip_contours(_p(P), len(P), _p(Q), len(Q), o, None, 0, None, None, 0,
+                            ctypes.byref(nv), ctypes.byref(nc))
+    xy = np.empty((nv.value, 2), np.float64)
+    cnt = np.empty(nc.value, np.int32)
+    hole = np.empty(nc.value, np.int32)
+    lib().orc_clip_contours(_p(P), len(P), _p(Q), len(Q), o, _p(xy), nv.value, _p(cnt), _p(hole),
+                            nc.value, ctypes.byref(nv), ctypes.byref(nc))
+    cuts = np.cumsum(cnt)[:-1] if len(cnt) else []
+    return [c for c in np.split(xy, cuts)] if len(cnt) else [], hole.astype(bool)
 
 
 def overlay_op(pairs, A, B, op, nthreads=1):

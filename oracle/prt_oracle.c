@@ -931,6 +931,455 @@ double orc_clip_area_op(const double *P, int nP, const double *Q, int nQ, int op
 }
 
 /* ---------------------------------------------------------------------- */
+/* Result contours of P op Q (GPC's output polygon, OV:527-750)           */
+/* ---------------------------------------------------------------------- */
+/*
+ * The reference keeps GPC's result contours (vertex lists with hole flags,
+ * OV:720-728).  This restatement builds them from the same sweep as the
+ * area, with its own assembly rule (GPC's vertex-class tables are not
+ * followed; the vertex order is pinned by orientation, closed forms and the
+ * areas, not against GPC):
+ *   - inside intervals of a sub-beam: maximal runs of neighbouring gaps that
+ *     are inside (op_in) and not of zero width (a gap whose two edges have
+ *     equal x at both ends is skipped and does not end a run); the interval
+ *     keeps its left and right edge and their x at both ends;
+ *   - at every sub-beam boundary y the intervals below (x at their top) and
+ *     above (x at their bottom) give 2 points each; the points of the two
+ *     lists are merged by x (each list keeps its order; on equal x a right
+ *     end goes before a left end, then the one below first) and paired
+ *     consecutively: every point toggles one of the two inside states, so
+ *     consecutive points bound exactly the horizontal parts of the boundary;
+ *   - a pair is one of: the left (right) chain continuing upward, a local
+ *     minimum (left-right above: new contour), a hole bottom (right-left
+ *     above: new contour), a local maximum (left-right below) or a hole top
+ *     (right-left below); the last two close a contour when both chain ends
+ *     belong to it (outer at a maximum, hole at a hole top), else join the
+ *     two contours;
+ *   - a contour is a path head..tail: down its left chains, along its
+ *     bottoms, up its right chains, so outer contours run counter-clockwise
+ *     and holes clockwise (inside on the left);
+ *   - vertices: a chain continuing on the same edge adds no vertex; otherwise
+ *     the pair adds its two points in path order, or one when the two x's
+ *     differ by no more than the sum of their uncertainties: 0 for an edge
+ *     end (y is the edge's bottom or top), else
+ *     2^-44 * (|bx| + |tx| + |dx| * (|y| + |by|)), which covers the rounding
+ *     of x at a crossing height (a crossing is evaluated on each of its two
+ *     edges, at a height that is itself rounded).  The one vertex is the
+ *     point of smaller uncertainty; on a tie the point below for a chain,
+ *     the left point otherwise.
+ * Contours come out in closing order.
+ */
+typedef struct {
+    int eL, eR;         /* edge ids */
+    double xL, xR;      /* x of the edges at the scanline */
+    double uL, uR;      /* their uncertainties */
+    int lp, rp;         /* contour of the left / right chain */
+} civ;
+
+static double cb_unc(const cedge *e, double y)
+{
+    if (y == e->ty || y == e->by) return 0.0;
+    return 0x1p-44 * (fabs(e->bx) + fabs(e->tx) + fabs(e->dx) * (fabs(y) + fabs(e->by)));
+}
+
+typedef struct {
+    /* vertex arena and open contours */
+    double *vx, *vy;
+    int *vnext;
+    int64_t nv, cap_v;
+    int *phead, *ptail, *pcnt;
+    int npoly, cap_p;
+    /* interval lists */
+    civ *B, *A;
+    int nb, na, cap_i;
+    /* output */
+    double *xy;
+    int64_t cap_xy, nout_v;
+    int32_t *cnv, *chole;
+    int64_t cap_c, nout_c;
+} cbuild;
+
+static int cb_vertex(cbuild *b, double x, double y)
+{
+    if (b->nv == b->cap_v) {
+        b->cap_v = b->cap_v ? 2 * b->cap_v : 64;
+        b->vx = (double *)realloc(b->vx, (size_t)b->cap_v * sizeof(double));
+        b->vy = (double *)realloc(b->vy, (size_t)b->cap_v * sizeof(double));
+        b->vnext = (int *)realloc(b->vnext, (size_t)b->cap_v * sizeof(int));
+    }
+    b->vx[b->nv] = x;
+    b->vy[b->nv] = y;
+    b->vnext[b->nv] = -1;
+    return (int)b->nv++;
+}
+
+static int cb_poly(cbuild *b, int head, int tail, int cnt)
+{
+    if (b->npoly == b->cap_p) {
+        b->cap_p = b->cap_p ? 2 * b->cap_p : 16;
+        b->phead = (int *)realloc(b->phead, (size_t)b->cap_p * sizeof(int));
+        b->ptail = (int *)realloc(b->ptail, (size_t)b->cap_p * sizeof(int));
+        b->pcnt = (int *)realloc(b->pcnt, (size_t)b->cap_p * sizeof(int));
+    }
+    b->phead[b->npoly] = head;
+    b->ptail[b->npoly] = tail;
+    b->pcnt[b->npoly] = cnt;
+    return b->npoly++;
+}
+
+static void cb_prepend(cbuild *b, int p, double x, double y)
+{
+    int v = cb_vertex(b, x, y);
+    b->vnext[v] = b->phead[p];
+    b->phead[p] = v;
+    b->pcnt[p] += 1;
+}
+
+static void cb_append(cbuild *b, int p, double x, double y)
+{
+    int v = cb_vertex(b, x, y);
+    b->vnext[b->ptail[p]] = v;
+    b->ptail[p] = v;
+    b->pcnt[p] += 1;
+}
+
+/* path of p, then path of q; every chain end of q now belongs to p */
+static void cb_join(cbuild *b, int p, int q)
+{
+    b->vnext[b->ptail[p]] = b->phead[q];
+    b->ptail[p] = b->ptail[q];
+    b->pcnt[p] += b->pcnt[q];
+    for (int k = 0; k < b->nb; ++k) {
+        if (b->B[k].lp == q) b->B[k].lp = p;
+        if (b->B[k].rp == q) b->B[k].rp = p;
+    }
+    for (int k = 0; k < b->na; ++k) {
+        if (b->A[k].lp == q) b->A[k].lp = p;
+        if (b->A[k].rp == q) b->A[k].rp = p;
+    }
+}
+
+static void cb_close(cbuild *b, int p, int hole)
+{
+    if (b->nout_c < b->cap_c) {
+        b->cnv[b->nout_c] = b->pcnt[p];
+        b->chole[b->nout_c] = hole;
+    }
+    int v = b->phead[p];
+    for (int k = 0; k < b->pcnt[p]; ++k) {
+        if (b->nout_v < b->cap_xy) {
+            b->xy[2 * b->nout_v] = b->vx[v];
+            b->xy[2 * b->nout_v + 1] = b->vy[v];
+        }
+        b->nout_v++;
+        v = b->vnext[v];
+    }
+    b->nout_c++;
+}
+
+typedef struct {
+    int below, right, iv;
+    double x, unc;
+} cpt;
+
+static inline int cb_near(cpt u, cpt v)
+{
+    return fabs(u.x - v.x) <= u.unc + v.unc;
+}
+
+static cpt cb_point(const cbuild *b, int below, int k)
+{
+    cpt p;
+    const civ *I = below ? &b->B[k >> 1] : &b->A[k >> 1];
+    p.below = below;
+    p.right = k & 1;
+    p.iv = k >> 1;
+    p.x = p.right ? I->xR : I->xL;
+    p.unc = p.right ? I->uR : I->uL;
+    return p;
+}
+
+static void cb_pair(cbuild *b, double y, cpt u, cpt v)
+{
+    if (u.below != v.below) {                 /* a chain continues upward */
+        cpt lo = u.below ? u : v, hi = u.below ? v : u;
+        civ *I = &b->B[lo.iv], *J = &b->A[hi.iv];
+        int same = lo.right ? (I->eR == J->eR) : (I->eL == J->eL);
+        int p = lo.right ? I->rp : I->lp;
+        if (!same) {
+            const int two = !cb_near(lo, hi);
+            const double x1 = (two || !(hi.unc < lo.unc)) ? lo.x : hi.x;
+            if (lo.right) {
+                cb_append(b, p, x1, y);
+                if (two) cb_append(b, p, hi.x, y);
+            } else {
+                cb_prepend(b, p, x1, y);
+                if (two) cb_prepend(b, p, hi.x, y);
+            }
+        }
+        if (lo.right) J->rp = p; else J->lp = p;
+        return;
+    }
+    const int two = !cb_near(u, v);
+    const double x1 = v.unc < u.unc ? v.x : u.x;   /* the one vertex */
+    if (!u.below) {                           /* a contour starts */
+        int p;
+        if (!u.right) {                       /* local minimum: u -> v */
+            int h = cb_vertex(b, two ? u.x : x1, y), t = h;
+            if (two) { t = cb_vertex(b, v.x, y); b->vnext[h] = t; }
+            p = cb_poly(b, h, t, 1 + two);
+            b->A[u.iv].lp = p;
+            b->A[v.iv].rp = p;
+        } else {                              /* hole bottom: v -> u */
+            int h = cb_vertex(b, two ? v.x : x1, y), t = h;
+            if (two) { t = cb_vertex(b, u.x, y); b->vnext[h] = t; }
+            p = cb_poly(b, h, t, 1 + two);
+            b->A[v.iv].lp = p;
+            b->A[u.iv].rp = p;
+        }
+        return;
+    }
+    if (!u.right) {                           /* local maximum: tail -> v -> u -> head */
+        int p = b->B[u.iv].lp, q = b->B[v.iv].rp;
+        if (two) cb_append(b, q, v.x, y);
+        cb_append(b, q, two ? u.x : x1, y);
+        if (p == q) cb_close(b, q, 0);
+        else cb_join(b, q, p);
+    } else {                                  /* hole top: tail -> u -> v -> head */
+        int p = b->B[u.iv].rp, q = b->B[v.iv].lp;
+        cb_append(b, p, two ? u.x : x1, y);
+        if (two) cb_append(b, p, v.x, y);
+        if (p == q) cb_close(b, p, 1);
+        else cb_join(b, p, q);
+    }
+}
+
+/* intervals below (B, x at y) -> intervals above (A, x at y) */
+static void cb_scanline(cbuild *b, double y)
+{
+    int i = 0, j = 0, have = 0;
+    const int nbp = 2 * b->nb, nap = 2 * b->na;
+    cpt pend = {0, 0, 0, 0.0, 0.0};
+    while (i < nbp || j < nap) {
+        cpt p;
+        if (j >= nap) p = cb_point(b, 1, i++);
+        else if (i >= nbp) p = cb_point(b, 0, j++);
+        else {
+            cpt pb = cb_point(b, 1, i), pa = cb_point(b, 0, j);
+            int take_b;
+            if (pb.x != pa.x) take_b = pb.x < pa.x;
+            else take_b = pb.right || !pa.right;
+            if (take_b) { p = pb; ++i; } else { p = pa; ++j; }
+        }
+        if (!have) { pend = p; have = 1; }
+        else { cb_pair(b, y, pend, p); have = 0; }
+    }
+}
+
+static void cb_reserve_iv(cbuild *b, int n)
+{
+    if (n > b->cap_i) {
+        b->cap_i = n;
+        b->B = (civ *)realloc(b->B, (size_t)n * sizeof(civ));
+        b->A = (civ *)realloc(b->A, (size_t)n * sizeof(civ));
+    }
+}
+
+static void cb_add_interval(cbuild *b, const clip_ws *w, int s, int e, double ya)
+{
+    civ *J = &b->A[b->na++];
+    J->eL = w->act[s]->id;
+    J->eR = w->act[e]->id;
+    J->xL = w->xa[s];
+    J->xR = w->xa[e];
+    J->uL = cb_unc(w->act[s], ya);
+    J->uR = cb_unc(w->act[e], ya);
+    J->lp = J->rp = -1;
+}
+
+/* A <- inside intervals of the sub-beam (ord by sb_before), x at the bottom */
+static void cb_intervals(cbuild *b, clip_ws *w, int m, int op, double ya)
+{
+    for (int i = 0; i < m; ++i) w->km[i] = w->xa[i] + w->xc[i];
+    for (int i = 0; i < m; ++i) w->ord[i] = i;
+    for (int i = 1; i < m; ++i) {
+        int v = w->ord[i], j = i - 1;
+        while (j >= 0 && sb_before(w->km, w->xa, w->act, v, w->ord[j])) {
+            w->ord[j + 1] = w->ord[j];
+            --j;
+        }
+        w->ord[j + 1] = v;
+    }
+    int pp = 0, pq = 0, open = 0, start = 0, last = 0;
+    b->na = 0;
+    for (int t = 0; t + 1 < m; ++t) {
+        int l = w->ord[t], r = w->ord[t + 1];
+        if (w->act[l]->poly == 0) pp ^= 1; else pq ^= 1;
+        if (w->xa[l] == w->xa[r] && w->xc[l] == w->xc[r]) continue;
+        int in = op_in(op, pp, pq);
+        if (in) {
+            if (!open) { open = 1; start = l; }
+            last = r;
+        } else if (open) {
+            cb_add_interval(b, w, start, last, ya);
+            open = 0;
+        }
+    }
+    if (open) cb_add_interval(b, w, start, last, ya);
+}
+
+/* A (bottom x's) becomes B with the x's at the sub-beam top yc */
+static void cb_advance(cbuild *b, const clip_ws *w, int m, double yc)
+{
+    civ *t = b->B;
+    b->B = b->A;
+    b->A = t;
+    b->nb = b->na;
+    b->na = 0;
+    for (int k = 0; k < b->nb; ++k) {
+        civ *I = &b->B[k];
+        for (int i = 0; i < m; ++i) {
+            if (w->act[i]->id == I->eL) { I->xL = w->xc[i]; I->uL = cb_unc(w->act[i], yc); }
+            if (w->act[i]->id == I->eR) { I->xR = w->xc[i]; I->uR = cb_unc(w->act[i], yc); }
+        }
+    }
+}
+
+static void clip_contours_ws(clip_ws *w, cbuild *b, const double *P, int nP, const double *Q,
+                             int nQ, int op)
+{
+    const int eP = nP < 3, eQ = nQ < 3;
+    double bp[4] = {0, 0, 0, 0}, bq[4] = {0, 0, 0, 0};
+    if (!eP) bbox(P, nP, bp);
+    if (!eQ) bbox(Q, nQ, bq);
+    double ylo, yhi;
+    if (!op_range(op, eP, eQ, bp, bq, &ylo, &yhi)) return;
+    nP = eP ? 0 : nP;
+    nQ = eQ ? 0 : nQ;
+    ws_reserve(w, nP + nQ, nP + nQ);
+    cb_reserve_iv(b, (nP + nQ) / 2 + 1);
+
+    int ne = 0;
+    for (int poly = 0; poly < 2; ++poly) {
+        const double *V = poly ? Q : P;
+        int n = poly ? nQ : nP;
+        for (int i = 0; i < n; ++i) {
+            int j = i + 1 == n ? 0 : i + 1;
+            double x0 = V[2 * i], y0 = V[2 * i + 1], x1 = V[2 * j], y1 = V[2 * j + 1];
+            if (y0 == y1) continue;
+            cedge e;
+            if (y0 < y1) { e.bx = x0; e.by = y0; e.tx = x1; e.ty = y1; }
+            else { e.bx = x1; e.by = y1; e.tx = x0; e.ty = y0; }
+            if (e.ty <= ylo || e.by >= yhi) continue;
+            e.dx = (e.tx - e.bx) / (e.ty - e.by);
+            e.id = poly ? nP + i : i;
+            e.poly = poly;
+            w->edges[ne++] = e;
+        }
+    }
+    qsort(w->edges, (size_t)ne, sizeof(cedge), cmp_edge_by);
+    int ny = 0;
+    for (int poly = 0; poly < 2; ++poly) {
+        const double *V = poly ? Q : P;
+        int n = poly ? nQ : nP;
+        for (int i = 0; i < n; ++i) {
+            double y = V[2 * i + 1];
+            if (y >= ylo && y <= yhi) w->ys[ny++] = y;
+        }
+    }
+    qsort(w->ys, (size_t)ny, sizeof(double), cmp_double);
+    int u = 0;
+    for (int i = 0; i < ny; ++i)
+        if (u == 0 || w->ys[i] != w->ys[u - 1]) w->ys[u++] = w->ys[i];
+    ny = u;
+    if (ny < 2) return;
+
+    b->nb = b->na = 0;
+    int m = 0, next = 0;
+    for (int k = 0; k + 1 < ny; ++k) {
+        double yb = w->ys[k], yt = w->ys[k + 1];
+        while (next < ne && w->edges[next].by <= yb) w->act[m++] = &w->edges[next++];
+        int keep = 0;
+        for (int i = 0; i < m; ++i)
+            if (w->act[i]->ty > yb) w->act[keep++] = w->act[i];
+        m = keep;
+        if (m < 2) {                          /* nothing inside this beam */
+            b->na = 0;
+            cb_scanline(b, yb);
+            b->nb = 0;
+            continue;
+        }
+        for (int i = 0; i < m; ++i) {
+            w->xb[i] = edge_x(w->act[i], yb);
+            w->xt[i] = edge_x(w->act[i], yt);
+        }
+        int nc = 0;
+        double dy = yt - yb;
+        for (int i = 0; i < m; ++i) {
+            for (int j = i + 1; j < m; ++j) {
+                int L, R;
+                if (w->xb[i] < w->xb[j] && w->xt[i] > w->xt[j]) { L = i; R = j; }
+                else if (w->xb[j] < w->xb[i] && w->xt[j] > w->xt[i]) { L = j; R = i; }
+                else continue;
+                double den = (w->xt[L] - w->xb[L]) - (w->xt[R] - w->xb[R]);
+                if (!(den > 0.0)) continue;
+                double r = (w->xb[R] - w->xb[L]) / den;
+                double iy = r * dy + yb;
+                if (iy > yb && iy < yt) w->cy[nc++] = iy;
+            }
+        }
+        qsort(w->cy, (size_t)nc, sizeof(double), cmp_double);
+        u = 0;
+        for (int i = 0; i < nc; ++i)
+            if (u == 0 || w->cy[i] != w->cy[u - 1]) w->cy[u++] = w->cy[i];
+        nc = u;
+        double ya = yb;
+        for (int i = 0; i < m; ++i) w->xa[i] = w->xb[i];
+        for (int c = 0; c <= nc; ++c) {
+            double yc = c < nc ? w->cy[c] : yt;
+            if (c < nc) {
+                for (int i = 0; i < m; ++i) w->xc[i] = edge_x(w->act[i], yc);
+            } else {
+                for (int i = 0; i < m; ++i) w->xc[i] = w->xt[i];
+            }
+            cb_intervals(b, w, m, op, ya);
+            cb_scanline(b, ya);
+            cb_advance(b, w, m, yc);
+            for (int i = 0; i < m; ++i) w->xa[i] = w->xc[i];
+            ya = yc;
+        }
+    }
+    b->na = 0;
+    cb_scanline(b, w->ys[ny - 1]);
+    b->nb = 0;
+}
+
+/* Contours of P op Q: up to cap_v vertices into xy (interleaved), up to
+ * cap_c contours into nv[] (vertex counts) and hole[]; *nvert / *ncont get
+ * the totals (call with caps 0 to size the buffers). */
+void orc_clip_contours(const double *P, int nP, const double *Q, int nQ, int op, double *xy,
+                       int64_t cap_v, int32_t *nv, int32_t *hole, int64_t cap_c,
+                       int64_t *nvert, int64_t *ncont)
+{
+    clip_ws w;
+    memset(&w, 0, sizeof w);
+    cbuild b;
+    memset(&b, 0, sizeof b);
+    b.xy = xy;
+    b.cap_xy = xy ? cap_v : 0;
+    b.cnv = nv;
+    b.chole = hole;
+    b.cap_c = (nv && hole) ? cap_c : 0;
+    clip_contours_ws(&w, &b, P, nP, Q, nQ, op);
+    *nvert = b.nout_v;
+    *ncont = b.nout_c;
+    ws_free(&w);
+    free(b.vx); free(b.vy); free(b.vnext);
+    free(b.phead); free(b.ptail); free(b.pcnt);
+    free(b.B); free(b.A);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Independent checks of the area (different algorithms)                  */
 /* ---------------------------------------------------------------------- */
 

@@ -86,6 +86,9 @@ double orc_clip_area(const double *P, int nP, const double *Q, int nQ,
                      uint64_t *flops);
 double orc_clip_area_op(const double *P, int nP, const double *Q, int nQ, int op,
                         uint64_t *flops);
+void orc_clip_contours(const double *P, int nP, const double *Q, int nQ, int op, double *xy,
+                       int64_t cap_v, int32_t *nv, int32_t *hole, int64_t cap_c,
+                       int64_t *nvert, int64_t *ncont);
 double orc_poly_area(const double *P, int n);
 double orc_green_area(const double *P, int nP, const double *Q, int nQ);
 double orc_sh_convex_area(const double *P, int nP, const double *Q, int nQ);

@@ -23,6 +23,7 @@
 // in increasing order; inside a sub-beam, edges are ordered by
 // (xa + xc, xa, edge id); a gap is inside when both even-odd parities
 // (subject = A, clip = B) are odd.
+#include "clip_lane.hpp"
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -31,102 +32,11 @@ namespace {
 
 constexpr int CP_THREADS = 128;
 constexpr int64_t CP_MAX_LANES = 65536;
-constexpr int CP_BYTES_PER_SLOT = 11 * 8 + 4 * 4;  // 11 double + 4 int arrays
-
-struct PlainWs {
-    double *bx, *by, *tx, *ty, *dx, *ys, *xb, *xt, *xa, *xc, *bsum;
-    int32_t *id, *eord, *act, *ord;
-    int64_t T;
-    int cap;
-};
-
-struct Lane {
-    const PlainWs &w;
-    int64_t t;
-    __device__ Lane(const PlainWs &ws, int64_t lane) : w(ws), t(lane) {}
-    template <class T> __device__ T &at(T *base, int64_t k) const { return base[k * w.T + t]; }
-};
-
-__device__ inline double edge_x(const Lane &L, int e, double y) {
-    if (y == L.at(L.w.ty, e)) return L.at(L.w.tx, e);
-    return L.at(L.w.bx, e) + L.at(L.w.dx, e) * (y - L.at(L.w.by, e));
-}
-
-// heap sort of eord[0..n) by (by, id)
-__device__ inline bool edge_less(const Lane &L, int a, int b) {
-    double ya = L.at(L.w.by, a), yb = L.at(L.w.by, b);
-    if (ya != yb) return ya < yb;
-    return L.at(L.w.id, a) < L.at(L.w.id, b);
-}
-
-__device__ void sift_edges(const Lane &L, int root, int n) {
-    for (;;) {
-        int c = 2 * root + 1;
-        if (c >= n) return;
-        if (c + 1 < n && edge_less(L, L.at(L.w.eord, c), L.at(L.w.eord, c + 1))) ++c;
-        int rv = L.at(L.w.eord, root), cv = L.at(L.w.eord, c);
-        if (!edge_less(L, rv, cv)) return;
-        L.at(L.w.eord, root) = cv;
-        L.at(L.w.eord, c) = rv;
-        root = c;
-    }
-}
-
-__device__ void sort_edges(const Lane &L, int n) {
-    for (int i = n / 2 - 1; i >= 0; --i) sift_edges(L, i, n);
-    for (int end = n - 1; end > 0; --end) {
-        int v = L.at(L.w.eord, 0);
-        L.at(L.w.eord, 0) = L.at(L.w.eord, end);
-        L.at(L.w.eord, end) = v;
-        sift_edges(L, 0, end);
-    }
-}
-
-__device__ void sift_ys(const Lane &L, int root, int n) {
-    for (;;) {
-        int c = 2 * root + 1;
-        if (c >= n) return;
-        if (c + 1 < n && L.at(L.w.ys, c) < L.at(L.w.ys, c + 1)) ++c;
-        double rv = L.at(L.w.ys, root), cv = L.at(L.w.ys, c);
-        if (!(rv < cv)) return;
-        L.at(L.w.ys, root) = cv;
-        L.at(L.w.ys, c) = rv;
-        root = c;
-    }
-}
-
-__device__ void sort_ys(const Lane &L, int n) {
-    for (int i = n / 2 - 1; i >= 0; --i) sift_ys(L, i, n);
-    for (int end = n - 1; end > 0; --end) {
-        double v = L.at(L.w.ys, 0);
-        L.at(L.w.ys, 0) = L.at(L.w.ys, end);
-        L.at(L.w.ys, end) = v;
-        sift_ys(L, 0, end);
-    }
-}
-
-// strict order (xa + xc, xa, id) of active slots i, j
-__device__ inline bool sb_before(const Lane &L, int i, int j) {
-    double ki = L.at(L.w.xa, i) + L.at(L.w.xc, i);
-    double kj = L.at(L.w.xa, j) + L.at(L.w.xc, j);
-    if (ki != kj) return ki < kj;
-    double ai = L.at(L.w.xa, i), aj = L.at(L.w.xa, j);
-    if (ai != aj) return ai < aj;
-    return L.at(L.w.id, L.at(L.w.act, i)) < L.at(L.w.id, L.at(L.w.act, j));
-}
 
 __device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, double acc,
                            int op) {
     const double h = yc - ya;
-    for (int i = 0; i < m; ++i) L.at(L.w.ord, i) = i;
-    for (int i = 1; i < m; ++i) {
-        int v = L.at(L.w.ord, i), j = i - 1;
-        while (j >= 0 && sb_before(L, v, L.at(L.w.ord, j))) {
-            L.at(L.w.ord, j + 1) = L.at(L.w.ord, j);
-            --j;
-        }
-        L.at(L.w.ord, j + 1) = v;
-    }
+    sort_sub_beam(L, m);
     int pp = 0, pq = 0;
     for (int t = 0; t + 1 < m; ++t) {
         int l = L.at(L.w.ord, t), r = L.at(L.w.ord, t + 1);
@@ -142,70 +52,11 @@ __device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, d
 // than 3 vertices is empty.
 __device__ double clip_one(const Lane &L, const double *__restrict__ P, int nP,
                            const double *__restrict__ Q, int nQ, int op) {
-    const bool eP = nP < 3, eQ = nQ < 3;
-    if (eP && eQ) return 0.0;
+    if (nP < 3 && nQ < 3) return 0.0;
     if (nP + nQ > L.w.cap) return __builtin_nan("");
-    if (eP) nP = 0;
-    if (eQ) nQ = 0;
-    double bp[4] = {0.0, 0.0, 0.0, 0.0}, bq[4] = {0.0, 0.0, 0.0, 0.0};
-    if (!eP) {
-        bp[0] = bp[2] = P[0];
-        bp[1] = bp[3] = P[1];
-    }
-    if (!eQ) {
-        bq[0] = bq[2] = Q[0];
-        bq[1] = bq[3] = Q[1];
-    }
-    for (int i = 1; i < nP; ++i) {
-        double x = P[2 * i], y = P[2 * i + 1];
-        bp[0] = x < bp[0] ? x : bp[0];
-        bp[2] = x > bp[2] ? x : bp[2];
-        bp[1] = y < bp[1] ? y : bp[1];
-        bp[3] = y > bp[3] ? y : bp[3];
-    }
-    for (int i = 1; i < nQ; ++i) {
-        double x = Q[2 * i], y = Q[2 * i + 1];
-        bq[0] = x < bq[0] ? x : bq[0];
-        bq[2] = x > bq[2] ? x : bq[2];
-        bq[1] = y < bq[1] ? y : bq[1];
-        bq[3] = y > bq[3] ? y : bq[3];
-    }
-    double ylo = 0.0, yhi = 0.0;
-    if (!op_range(op, eP, eQ, bp, bq, ylo, yhi)) return 0.0;
-
+    double bp[4], bq[4];
     int ne = 0, ny = 0;
-    for (int poly = 0; poly < 2; ++poly) {
-        const double *V = poly ? Q : P;
-        const int n = poly ? nQ : nP;
-        for (int i = 0; i < n; ++i) {
-            const int j = i + 1 == n ? 0 : i + 1;
-            const double x0 = V[2 * i], y0 = V[2 * i + 1], x1 = V[2 * j], y1 = V[2 * j + 1];
-            if (y0 >= ylo && y0 <= yhi) L.at(L.w.ys, ny++) = y0;
-            if (y0 == y1) continue;
-            double bx, by, tx, ty;
-            if (y0 < y1) { bx = x0; by = y0; tx = x1; ty = y1; }
-            else { bx = x1; by = y1; tx = x0; ty = y0; }
-            if (ty <= ylo || by >= yhi) continue;
-            L.at(L.w.bx, ne) = bx;
-            L.at(L.w.by, ne) = by;
-            L.at(L.w.tx, ne) = tx;
-            L.at(L.w.ty, ne) = ty;
-            L.at(L.w.dx, ne) = (tx - bx) / (ty - by);
-            L.at(L.w.id, ne) = poly ? nP + i : i;
-            L.at(L.w.eord, ne) = ne;
-            ++ne;
-        }
-    }
-    sort_edges(L, ne);
-    sort_ys(L, ny);
-    int u = 0;
-    for (int i = 0; i < ny; ++i) {
-        double y = L.at(L.w.ys, i);
-        if (u == 0 || y != L.at(L.w.ys, u - 1)) L.at(L.w.ys, u++) = y;
-    }
-    ny = u;
-
-    if (ny < 2) return 0.0;
+    if (!lane_setup(L, P, nP, Q, nQ, op, bp, bq, ne, ny)) return 0.0;
     int m = 0, next = 0;
     for (int k = 0; k + 1 < ny; ++k) {
         const double yb = L.at(L.w.ys, k), yt = L.at(L.w.ys, k + 1);
@@ -308,19 +159,12 @@ int64_t plain_lanes(int64_t np) {
 }
 
 size_t plain_ws_bytes(int64_t T, int32_t max_verts) {
-    const int64_t cap = max_verts < 3 ? 3 : max_verts;
-    return static_cast<size_t>(T) * static_cast<size_t>(cap) * CP_BYTES_PER_SLOT + 16 * 256;
+    return lane_base_bytes(T, max_verts < 3 ? 3 : max_verts) + 16 * 256;
 }
 
 int carve_plain(PlainWs &w, int64_t T, int32_t max_verts, void *ws, size_t ws_bytes) {
-    w.T = T;
-    w.cap = max_verts < 3 ? 3 : max_verts;
     Carver c(ws, ws_bytes);
-    const size_t cnt = static_cast<size_t>(w.T) * w.cap;
-    double **dptr[] = {&w.bx, &w.by, &w.tx, &w.ty, &w.dx, &w.ys, &w.xb, &w.xt, &w.xa, &w.xc, &w.bsum};
-    for (double **p : dptr) *p = c.take<double>(cnt);
-    int32_t **iptr[] = {&w.id, &w.eord, &w.act, &w.ord};
-    for (int32_t **p : iptr) *p = c.take<int32_t>(cnt);
+    carve_lane_base(c, w, T, max_verts < 3 ? 3 : max_verts);
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 

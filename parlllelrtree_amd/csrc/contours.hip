@@ -1,0 +1,610 @@
+// contours.hip -- result contours of A[pairs[2t]] op B[pairs[2t+1]] (GPC's
+// output polygon, vector_map_overlay_gpc_kernel.cu:527-750 and its vertex
+// copy :720-728), one lane per pair, in two passes: count, then write.
+//
+// The sweep is the one of the plain area kernel (same edge table, beams,
+// crossing heights and sub-beam order).  The contours are assembled from it
+// by this rule (the CPU oracle restates it step by step):
+//   * inside intervals of a sub-beam: maximal runs of neighbouring gaps that
+//     are inside the operation and not of zero width (a gap whose two edges
+//     have equal x at both ends is skipped and does not end a run);
+//   * at each sub-beam boundary y, the 2 points of every interval below (x
+//     at its top) and above (x at its bottom) are merged by x -- each list
+//     keeps its own order; on equal x a right end goes first, then the point
+//     below -- and paired consecutively.  Every point toggles one of the two
+//     inside states, so the pairs are exactly the horizontal parts of the
+//     boundary.  A pair continues a left or right chain upward, starts a
+//     contour (local minimum: left-right above; hole bottom: right-left
+//     above), or ends two chains (local maximum: left-right below; hole top:
+//     right-left below) -- closing the contour when both chain ends belong to
+//     it (outer at a maximum, hole at a hole top), else joining two contours;
+//   * a contour is a path head..tail: down its left chains, along its
+//     bottoms, up its right chains, so outer contours run counter-clockwise
+//     and holes clockwise;
+//   * a chain that goes on along the same edge adds no vertex; otherwise a
+//     pair adds its two points, or one when their x's differ by no more than
+//     the sum of their uncertainties: 0 at an edge end (y is the edge's
+//     bottom or top), else 2^-44 * (|bx| + |tx| + |dx| * (|y| + |by|)),
+//     which covers a crossing evaluated on each of its two edges at a height
+//     that is itself rounded.  The one vertex is the point of smaller
+//     uncertainty; on a tie the point below of a chain, else the left one.
+// Contours come out in closing order; vertices, contour starts and hole
+// flags land at the offsets the caller scanned from the counts.
+//
+// Count pass: per pair the number of contours and of vertices (no vertex is
+// stored; open contours are tracked for their identity only).  Write pass:
+// every emitted vertex goes to an arena slot at the pair's vertex offset,
+// chained per contour; a closing contour is copied out in path order.
+#include "clip_lane.hpp"
+#include "prt_common.hpp"
+#include "prt_profile.hpp"
+
+namespace prt {
+namespace {
+
+constexpr int CT_THREADS = 128;
+constexpr int64_t CT_MAX_LANES = 65536;
+
+struct IvArrays {  // one interval list, icap slots per lane
+    int32_t *eL, *eR, *sL, *sR, *lp, *rp;
+    double *xL, *xR, *uL, *uR;  // x at the scanline and its uncertainty
+};
+
+struct ContourWs {
+    PlainWs base;
+    IvArrays iv[2];
+    int32_t *phead, *ptail, *pcnt, *pfree;  // open contours, pcap slots per lane
+    int icap, pcap;
+};
+
+struct ContourOut {  // write pass
+    double *vx, *vy;
+    int32_t *vnext;  // vertex arena, at the pairs' vertex offsets
+    double *xy;
+    int64_t *cstart;
+    int32_t *chole;
+};
+
+struct Pt {
+    bool below, right;
+    int iv;
+    double x, unc;
+};
+
+// uncertainty of edge e's x at y (see the header)
+__device__ inline double x_unc(const Lane &L, int e, double y) {
+    const double ty = L.at(L.w.ty, e), by = L.at(L.w.by, e);
+    if (y == ty || y == by) return 0.0;
+    return 0x1p-44 * (fabs(L.at(L.w.bx, e)) + fabs(L.at(L.w.tx, e)) +
+                      fabs(L.at(L.w.dx, e)) * (fabs(y) + fabs(by)));
+}
+
+template <bool W> struct Builder {
+    const Lane &L;
+    const ContourWs &cw;
+    const ContourOut &o;
+    int sb = 0;  // iv[sb] holds the intervals below the scanline
+    int nb = 0, na = 0, nfree = 0, nnew = 0;
+    int64_t nv = 0, nc = 0, written = 0;
+    int64_t vbase = 0, vlim = 0, cbase = 0, clim = 0;
+    bool bad = false;
+
+    __device__ Builder(const Lane &l, const ContourWs &w, const ContourOut &out)
+        : L(l), cw(w), o(out) {}
+
+    __device__ int32_t &B(int32_t *const IvArrays::*f, int k) const { return L.at(cw.iv[sb].*f, k); }
+    __device__ int32_t &A(int32_t *const IvArrays::*f, int k) const {
+        return L.at(cw.iv[1 - sb].*f, k);
+    }
+    __device__ double &Bx(double *const IvArrays::*f, int k) const { return L.at(cw.iv[sb].*f, k); }
+    __device__ double &Ax(double *const IvArrays::*f, int k) const {
+        return L.at(cw.iv[1 - sb].*f, k);
+    }
+
+    __device__ int vertex(double x, double y) {
+        const int64_t v = nv++;
+        if (W) {
+            if (v < vlim) {
+                o.vx[vbase + v] = x;
+                o.vy[vbase + v] = y;
+                o.vnext[vbase + v] = -1;
+            } else {
+                bad = true;
+            }
+        }
+        return static_cast<int>(v);
+    }
+    __device__ int32_t &ph(int p) const { return L.at(cw.phead, p); }
+    __device__ int32_t &pt(int p) const { return L.at(cw.ptail, p); }
+    __device__ int32_t &pc(int p) const { return L.at(cw.pcnt, p); }
+    __device__ bool arena_ok(int v) const { return v >= 0 && v < vlim; }
+
+    __device__ int poly_new(int h, int t, int cnt) {
+        int p;
+        if (nfree > 0) {
+            p = L.at(cw.pfree, --nfree);
+        } else if (nnew < cw.pcap) {
+            p = nnew++;
+        } else {
+            bad = true;
+            return 0;
+        }
+        if (W) {
+            ph(p) = h;
+            pt(p) = t;
+            pc(p) = cnt;
+        }
+        return p;
+    }
+    __device__ void poly_free(int p) {
+        if (nfree < cw.pcap) L.at(cw.pfree, nfree++) = p;
+    }
+    __device__ void prepend(int p, double x, double y) {
+        const int v = vertex(x, y);
+        if (W && arena_ok(v)) {
+            o.vnext[vbase + v] = ph(p);
+            ph(p) = v;
+            pc(p) += 1;
+        }
+    }
+    __device__ void append(int p, double x, double y) {
+        const int v = vertex(x, y);
+        if (W && arena_ok(v) && arena_ok(pt(p))) {
+            o.vnext[vbase + pt(p)] = v;
+            pt(p) = v;
+            pc(p) += 1;
+        }
+    }
+    // path of p, then path of q; every chain end of q now belongs to p
+    __device__ void join(int p, int q) {
+        if (W && arena_ok(pt(p))) {
+            o.vnext[vbase + pt(p)] = ph(q);
+            pt(p) = pt(q);
+            pc(p) += pc(q);
+        }
+        for (int k = 0; k < nb; ++k) {
+            if (B(&IvArrays::lp, k) == q) B(&IvArrays::lp, k) = p;
+            if (B(&IvArrays::rp, k) == q) B(&IvArrays::rp, k) = p;
+        }
+        for (int k = 0; k < na; ++k) {
+            if (A(&IvArrays::lp, k) == q) A(&IvArrays::lp, k) = p;
+            if (A(&IvArrays::rp, k) == q) A(&IvArrays::rp, k) = p;
+        }
+        poly_free(q);
+    }
+    __device__ void close(int p, bool hole) {
+        if (W) {
+            if (nc < clim) {
+                o.cstart[cbase + nc] = vbase + written;
+                o.chole[cbase + nc] = hole ? 1 : 0;
+            } else {
+                bad = true;
+            }
+            int v = ph(p);
+            for (int k = 0; k < pc(p); ++k) {
+                if (!arena_ok(v) || written >= vlim) {
+                    bad = true;
+                    break;
+                }
+                o.xy[2 * (vbase + written)] = o.vx[vbase + v];
+                o.xy[2 * (vbase + written) + 1] = o.vy[vbase + v];
+                ++written;
+                v = o.vnext[vbase + v];
+            }
+        }
+        ++nc;
+        poly_free(p);
+    }
+    __device__ static bool near(const Pt &u, const Pt &v) { return fabs(u.x - v.x) <= u.unc + v.unc; }
+    // a chain end's contour; every interval side is assigned by the pairing
+    // of the scanline it was made at, so this only guards the arena
+    __device__ bool live(int p) {
+        if (p >= 0 && p < cw.pcap) return true;
+        bad = true;
+        return false;
+    }
+
+    __device__ Pt point(bool below, int k) const {
+        Pt p;
+        p.below = below;
+        p.right = (k & 1) != 0;
+        p.iv = k >> 1;
+        if (below) {
+            p.x = p.right ? Bx(&IvArrays::xR, p.iv) : Bx(&IvArrays::xL, p.iv);
+            p.unc = p.right ? Bx(&IvArrays::uR, p.iv) : Bx(&IvArrays::uL, p.iv);
+        } else {
+            p.x = p.right ? Ax(&IvArrays::xR, p.iv) : Ax(&IvArrays::xL, p.iv);
+            p.unc = p.right ? Ax(&IvArrays::uR, p.iv) : Ax(&IvArrays::uL, p.iv);
+        }
+        return p;
+    }
+
+    __device__ void pair(double y, const Pt &u, const Pt &v) {
+        if (u.below != v.below) {  // a chain continues upward
+            const Pt &lo = u.below ? u : v, &hi = u.below ? v : u;
+            const bool same = lo.right ? B(&IvArrays::eR, lo.iv) == A(&IvArrays::eR, hi.iv)
+                                       : B(&IvArrays::eL, lo.iv) == A(&IvArrays::eL, hi.iv);
+            const int p = lo.right ? B(&IvArrays::rp, lo.iv) : B(&IvArrays::lp, lo.iv);
+            if (!live(p)) return;
+            if (!same) {
+                const bool two = !near(lo, hi);
+                const double x1 = (two || !(hi.unc < lo.unc)) ? lo.x : hi.x;
+                if (lo.right) {
+                    append(p, x1, y);
+                    if (two) append(p, hi.x, y);
+                } else {
+                    prepend(p, x1, y);
+                    if (two) prepend(p, hi.x, y);
+                }
+            }
+            if (lo.right) A(&IvArrays::rp, hi.iv) = p;
+            else A(&IvArrays::lp, hi.iv) = p;
+            return;
+        }
+        const bool two = !near(u, v);
+        const double x1 = v.unc < u.unc ? v.x : u.x;  // the one vertex
+        if (!u.below) {  // a contour starts
+            int h, t;
+            if (!u.right) {  // local minimum: u -> v
+                h = t = vertex(two ? u.x : x1, y);
+                if (two) t = vertex(v.x, y);
+            } else {         // hole bottom: v -> u
+                h = t = vertex(two ? v.x : x1, y);
+                if (two) t = vertex(u.x, y);
+            }
+            if (W && two && arena_ok(h)) o.vnext[vbase + h] = t;
+            const int p = poly_new(h, t, two ? 2 : 1);
+            if (!u.right) {
+                A(&IvArrays::lp, u.iv) = p;
+                A(&IvArrays::rp, v.iv) = p;
+            } else {
+                A(&IvArrays::lp, v.iv) = p;
+                A(&IvArrays::rp, u.iv) = p;
+            }
+            return;
+        }
+        if (!u.right) {  // local maximum: tail -> v -> u -> head
+            const int p = B(&IvArrays::lp, u.iv), q = B(&IvArrays::rp, v.iv);
+            if (!live(p) || !live(q)) return;
+            if (two) append(q, v.x, y);
+            append(q, two ? u.x : x1, y);
+            if (p == q) close(q, false);
+            else join(q, p);
+        } else {         // hole top: tail -> u -> v -> head
+            const int p = B(&IvArrays::rp, u.iv), q = B(&IvArrays::lp, v.iv);
+            if (!live(p) || !live(q)) return;
+            append(p, two ? u.x : x1, y);
+            if (two) append(p, v.x, y);
+            if (p == q) close(p, true);
+            else join(p, q);
+        }
+    }
+
+    __device__ void scanline(double y) {
+        int i = 0, j = 0;
+        bool have = false;
+        const int nbp = 2 * nb, nap = 2 * na;
+        Pt pend{false, false, 0, 0.0, 0.0};
+        while (i < nbp || j < nap) {
+            Pt p;
+            if (j >= nap) {
+                p = point(true, i++);
+            } else if (i >= nbp) {
+                p = point(false, j++);
+            } else {
+                const Pt pb = point(true, i), pa = point(false, j);
+                const bool take_b = pb.x != pa.x ? pb.x < pa.x : (pb.right || !pa.right);
+                if (take_b) {
+                    p = pb;
+                    ++i;
+                } else {
+                    p = pa;
+                    ++j;
+                }
+            }
+            if (!have) {
+                pend = p;
+                have = true;
+            } else {
+                pair(y, pend, p);
+                have = false;
+            }
+        }
+    }
+
+    // A <- inside intervals of the sub-beam (ord sorted), x at its bottom
+    __device__ void intervals(int m, int nP, int op, double ya) {
+        int pp = 0, pq = 0, start = 0, last = 0;
+        bool open = false;
+        na = 0;
+        for (int t = 0; t + 1 < m; ++t) {
+            const int l = L.at(L.w.ord, t), r = L.at(L.w.ord, t + 1);
+            if (L.at(L.w.id, L.at(L.w.act, l)) < nP) pp ^= 1;
+            else pq ^= 1;
+            if (L.at(L.w.xa, l) == L.at(L.w.xa, r) && L.at(L.w.xc, l) == L.at(L.w.xc, r)) continue;
+            if (op_inside(op, pp, pq)) {
+                if (!open) {
+                    open = true;
+                    start = l;
+                }
+                last = r;
+            } else if (open) {
+                add_interval(start, last, ya);
+                open = false;
+            }
+        }
+        if (open) add_interval(start, last, ya);
+    }
+    __device__ void add_interval(int s, int e, double ya) {
+        if (na >= cw.icap) {
+            bad = true;
+            return;
+        }
+        A(&IvArrays::eL, na) = L.at(L.w.id, L.at(L.w.act, s));
+        A(&IvArrays::eR, na) = L.at(L.w.id, L.at(L.w.act, e));
+        A(&IvArrays::sL, na) = s;
+        A(&IvArrays::sR, na) = e;
+        Ax(&IvArrays::xL, na) = L.at(L.w.xa, s);
+        Ax(&IvArrays::xR, na) = L.at(L.w.xa, e);
+        Ax(&IvArrays::uL, na) = x_unc(L, L.at(L.w.act, s), ya);
+        Ax(&IvArrays::uR, na) = x_unc(L, L.at(L.w.act, e), ya);
+        A(&IvArrays::lp, na) = -1;
+        A(&IvArrays::rp, na) = -1;
+        ++na;
+    }
+    // the intervals above become the ones below, x at the sub-beam top yc
+    __device__ void advance(double yc) {
+        sb = 1 - sb;
+        nb = na;
+        na = 0;
+        for (int k = 0; k < nb; ++k) {
+            const int sl = B(&IvArrays::sL, k), sr = B(&IvArrays::sR, k);
+            Bx(&IvArrays::xL, k) = L.at(L.w.xc, sl);
+            Bx(&IvArrays::xR, k) = L.at(L.w.xc, sr);
+            Bx(&IvArrays::uL, k) = x_unc(L, L.at(L.w.act, sl), yc);
+            Bx(&IvArrays::uR, k) = x_unc(L, L.at(L.w.act, sr), yc);
+        }
+    }
+};
+
+template <bool W>
+__device__ void contours_one(Builder<W> &cb, const double *__restrict__ P, int nP,
+                             const double *__restrict__ Q, int nQ, int op) {
+    const Lane &L = cb.L;
+    double bp[4], bq[4];
+    int ne = 0, ny = 0;
+    if (!lane_setup(L, P, nP, Q, nQ, op, bp, bq, ne, ny)) return;
+    int m = 0, next = 0;
+    for (int k = 0; k + 1 < ny; ++k) {
+        const double yb = L.at(L.w.ys, k), yt = L.at(L.w.ys, k + 1);
+        while (next < ne && L.at(L.w.by, L.at(L.w.eord, next)) <= yb)
+            L.at(L.w.act, m++) = L.at(L.w.eord, next++);
+        int keep = 0;
+        for (int i = 0; i < m; ++i) {
+            const int e = L.at(L.w.act, i);
+            if (L.at(L.w.ty, e) > yb) L.at(L.w.act, keep++) = e;
+        }
+        m = keep;
+        if (m < 2) {  // nothing inside this beam
+            cb.na = 0;
+            cb.scanline(yb);
+            cb.nb = 0;
+            continue;
+        }
+        for (int i = 0; i < m; ++i) {
+            const int e = L.at(L.w.act, i);
+            const double b = edge_x(L, e, yb), t = edge_x(L, e, yt);
+            L.at(L.w.xb, i) = b;
+            L.at(L.w.xt, i) = t;
+            L.at(L.w.xa, i) = b;
+        }
+        const double dy = yt - yb;
+        double ya = yb;
+        for (;;) {
+            double yc = yt;  // next distinct crossing height above ya
+            for (int i = 0; i < m; ++i) {
+                const double xbi = L.at(L.w.xb, i), xti = L.at(L.w.xt, i);
+                for (int j = i + 1; j < m; ++j) {
+                    const double xbj = L.at(L.w.xb, j), xtj = L.at(L.w.xt, j);
+                    double xbL, xtL, xbR, xtR;
+                    if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+                    else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+                    else continue;
+                    const double den = (xtL - xbL) - (xtR - xbR);
+                    if (!(den > 0.0)) continue;
+                    const double r = (xbR - xbL) / den;
+                    const double iy = r * dy + yb;
+                    if (iy > ya && iy < yc) yc = iy;
+                }
+            }
+            if (yc == yt) {
+                for (int i = 0; i < m; ++i) L.at(L.w.xc, i) = L.at(L.w.xt, i);
+            } else {
+                for (int i = 0; i < m; ++i) L.at(L.w.xc, i) = edge_x(L, L.at(L.w.act, i), yc);
+            }
+            sort_sub_beam(L, m);
+            cb.intervals(m, nP, op, ya);
+            cb.scanline(ya);
+            cb.advance(yc);
+            if (yc == yt) break;
+            for (int i = 0; i < m; ++i) L.at(L.w.xa, i) = L.at(L.w.xc, i);
+            ya = yc;
+        }
+    }
+    cb.na = 0;
+    cb.scanline(L.at(L.w.ys, ny - 1));
+    cb.nb = 0;
+}
+
+// Count pass: ncont[p], nvert[p]; -1 in both when the pair does not fit the
+// lane (more than max_verts vertices) or its arena bounds.
+__global__ __launch_bounds__(CT_THREADS) void k_contours_count(
+    const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
+    const int64_t *__restrict__ offA, const double *__restrict__ xyB,
+    const int64_t *__restrict__ offB, int op, ContourWs cw, int64_t *__restrict__ ncont,
+    int64_t *__restrict__ nvert) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= cw.base.T) return;
+    Lane L(cw.base, t);
+    const ContourOut none{};
+    for (int64_t p = t; p < np; p += cw.base.T) {
+        const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int64_t oa = offA[a], ob = offB[b];
+        const int nA = static_cast<int>(offA[a + 1] - oa), nB = static_cast<int>(offB[b + 1] - ob);
+        Builder<false> cb(L, cw, none);
+        if (nA + nB > cw.base.cap && !(nA < 3 && nB < 3)) cb.bad = true;
+        else contours_one(cb, xyA + 2 * oa, nA, xyB + 2 * ob, nB, op);
+        ncont[p] = cb.bad ? -1 : cb.nc;
+        nvert[p] = cb.bad ? -1 : cb.nv;
+    }
+}
+
+// Write pass at the scanned offsets; *status (cleared by the launch before)
+// gets 1 when a pair did not produce what its counts promised.
+__global__ __launch_bounds__(CT_THREADS) void k_contours_write(
+    const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
+    const int64_t *__restrict__ offA, const double *__restrict__ xyB,
+    const int64_t *__restrict__ offB, int op, ContourWs cw, const int64_t *__restrict__ cont_off,
+    const int64_t *__restrict__ vert_off, ContourOut out, int32_t *status) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= cw.base.T) return;
+    Lane L(cw.base, t);
+    for (int64_t p = t; p < np; p += cw.base.T) {
+        const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int64_t oa = offA[a], ob = offB[b];
+        const int nA = static_cast<int>(offA[a + 1] - oa), nB = static_cast<int>(offB[b + 1] - ob);
+        Builder<true> cb(L, cw, out);
+        cb.vbase = vert_off[p];
+        cb.vlim = vert_off[p + 1] - cb.vbase;
+        cb.cbase = cont_off[p];
+        cb.clim = cont_off[p + 1] - cb.cbase;
+        if (nA + nB > cw.base.cap && !(nA < 3 && nB < 3)) cb.bad = true;
+        else contours_one(cb, xyA + 2 * oa, nA, xyB + 2 * ob, nB, op);
+        if (cb.bad || cb.nv != cb.vlim || cb.nc != cb.clim || cb.written != cb.vlim)
+            atomicOr(status, 1);
+    }
+}
+
+__global__ void k_status_clear(int32_t *status) {
+    if (threadIdx.x == 0) *status = 0;
+}
+
+int64_t contour_lanes(int64_t np) {
+    int64_t T = np < CT_MAX_LANES ? np : CT_MAX_LANES;
+    return T < 1 ? 1 : T;
+}
+
+int contour_cap(int32_t max_verts) { return max_verts < 3 ? 3 : max_verts; }
+
+// lane arrays: base, 2 interval lists of icap slots, 4 contour arrays of pcap
+size_t contour_lane_bytes(int64_t T, int cap) {
+    const int64_t icap = cap / 2 + 1, pcap = cap + 2;
+    return lane_base_bytes(T, cap) +
+           static_cast<size_t>(T) * (2 * icap * (6 * 4 + 4 * 8) + pcap * 4 * 4) + 32 * 256;
+}
+
+int carve_contours(ContourWs &cw, int64_t T, int cap, Carver &c) {
+    carve_lane_base(c, cw.base, T, cap);
+    cw.icap = cap / 2 + 1;
+    cw.pcap = cap + 2;
+    const size_t ni = static_cast<size_t>(T) * cw.icap, npc = static_cast<size_t>(T) * cw.pcap;
+    for (IvArrays &v : cw.iv) {
+        int32_t **ip[] = {&v.eL, &v.eR, &v.sL, &v.sR, &v.lp, &v.rp};
+        for (int32_t **p : ip) *p = c.take<int32_t>(ni);
+        double **dp[] = {&v.xL, &v.xR, &v.uL, &v.uR};
+        for (double **p : dp) *p = c.take<double>(ni);
+    }
+    int32_t **pp[] = {&cw.phead, &cw.ptail, &cw.pcnt, &cw.pfree};
+    for (int32_t **p : pp) *p = c.take<int32_t>(npc);
+    return c.ok() ? PRT_OK : PRT_E_NOMEM;
+}
+
+}  // namespace
+}  // namespace prt
+
+using namespace prt;
+
+extern "C" size_t prt_overlay_contours_workspace_size(int64_t npairs, int32_t max_verts,
+                                                      int64_t total_vertices) {
+    if (npairs < 0 || max_verts < 0 || total_vertices < 0) return 0;
+    Sizer s;
+    s.take<char>(contour_lane_bytes(contour_lanes(npairs), contour_cap(max_verts)));
+    s.take<double>(total_vertices);
+    s.take<double>(total_vertices);
+    s.take<int32_t>(total_vertices);
+    s.take<int32_t>(64);  // status
+    return s.bytes();
+}
+
+namespace {
+int contours_args(const uint32_t *pairs, int64_t npairs, const double *xyA, const int64_t *offA,
+                  const double *xyB, const int64_t *offB, int32_t max_verts, int32_t op) {
+    PRT_REQUIRE(op >= PRT_OP_DIFF && op <= PRT_OP_UNION, PRT_E_INVAL, "overlay_contours: op %d", op);
+    PRT_REQUIRE(npairs >= 0, PRT_E_INVAL, "overlay_contours: npairs < 0");
+    PRT_REQUIRE(npairs == 0 || (pairs && xyA && offA && xyB && offB), PRT_E_INVAL,
+                "overlay_contours: null device pointer");
+    PRT_REQUIRE(max_verts >= 0 && max_verts <= 2 * PRT_MAX_POLY_VERTS, PRT_E_INVAL,
+                "overlay_contours: max_verts %d out of range", max_verts);
+    return PRT_OK;
+}
+}  // namespace
+
+extern "C" int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                          const int64_t *offA, const double *xyB,
+                                          const int64_t *offB, int32_t max_verts, int32_t op,
+                                          int64_t *ncont, int64_t *nvert, void *ws,
+                                          size_t ws_bytes, void *stream) {
+    int rc = contours_args(pairs, npairs, xyA, offA, xyB, offB, max_verts, op);
+    if (rc) return rc;
+    if (npairs == 0) return PRT_OK;
+    PRT_REQUIRE(ncont && nvert && ws, PRT_E_INVAL, "overlay_contours_count: null pointer");
+    const size_t need = prt_overlay_contours_workspace_size(npairs, max_verts, 0);
+    PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "overlay_contours_count: workspace %zu < %zu bytes",
+                ws_bytes, need);
+    Carver c(ws, ws_bytes);
+    ContourWs cw;
+    PRT_REQUIRE(carve_contours(cw, contour_lanes(npairs), contour_cap(max_verts), c) == PRT_OK,
+                PRT_E_NOMEM, "overlay_contours_count: workspace too small");
+    const unsigned grid = static_cast<unsigned>(ceil_div(cw.base.T, CT_THREADS));
+    PRT_LAUNCH(k_contours_count, grid, CT_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA,
+               xyB, offB, static_cast<int>(op), cw, ncont, nvert);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}
+
+extern "C" int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs, const double *xyA,
+                                          const int64_t *offA, const double *xyB,
+                                          const int64_t *offB, int32_t max_verts, int32_t op,
+                                          const int64_t *cont_off, const int64_t *vert_off,
+                                          int64_t total_vertices, int64_t *cont_start,
+                                          int32_t *cont_hole, double *xy, int32_t *status,
+                                          void *ws, size_t ws_bytes, void *stream) {
+    int rc = contours_args(pairs, npairs, xyA, offA, xyB, offB, max_verts, op);
+    if (rc) return rc;
+    PRT_REQUIRE(total_vertices >= 0, PRT_E_INVAL, "overlay_contours_write: total_vertices < 0");
+    if (npairs == 0) return PRT_OK;
+    PRT_REQUIRE(cont_off && vert_off && status && ws && (total_vertices == 0 || xy),
+                PRT_E_INVAL, "overlay_contours_write: null pointer");
+    const size_t need = prt_overlay_contours_workspace_size(npairs, max_verts, total_vertices);
+    PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "overlay_contours_write: workspace %zu < %zu bytes",
+                ws_bytes, need);
+    Carver c(ws, ws_bytes);
+    ContourWs cw;
+    PRT_REQUIRE(carve_contours(cw, contour_lanes(npairs), contour_cap(max_verts), c) == PRT_OK,
+                PRT_E_NOMEM, "overlay_contours_write: workspace too small");
+    ContourOut out;
+    out.vx = c.take<double>(total_vertices);
+    out.vy = c.take<double>(total_vertices);
+    out.vnext = c.take<int32_t>(total_vertices);
+    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "overlay_contours_write: workspace too small");
+    out.xy = xy;
+    out.cstart = cont_start;
+    out.chole = cont_hole;
+    const unsigned grid = static_cast<unsigned>(ceil_div(cw.base.T, CT_THREADS));
+    PRT_LAUNCH(k_status_clear, 1, 64, 0, as_stream(stream), status);
+    PRT_LAUNCHED();
+    PRT_LAUNCH(k_contours_write, grid, CT_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA,
+               xyB, offB, static_cast<int>(op), cw, cont_off, vert_off, out, status);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}

@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import torch
 
-from ._lib import BUTreeStruct, TDTreeStruct, TuneStruct, check, lib
+from ._lib import BUTreeStruct, PrtError, TDTreeStruct, TuneStruct, check, lib
 
 __all__ = ["BottomUpTree", "TopDownTree", "build_bottom_up", "build_top_down", "join",
            "overlay_plain", "PolyDev", "to_device_polys", "morton_center30", "sort_pairs_u32"]
@@ -352,6 +352,62 @@ def overlay_plain(pairs: torch.Tensor, A: PolyDev, B: PolyDev, area: torch.Tenso
                                      _ptr(B.off), mv, _op(op), _ptr(area), _ptr(ws), ws.numel(),
                                      _stream()))
     return area
+
+
+@dataclass
+class Contours:
+    """Result contours of a pair list: pair t owns contours
+    cont_off[t]..cont_off[t+1] and vertices vert_off[t]..vert_off[t+1];
+    contour c starts at vertex cont_start[c] (and ends where the next one of
+    its pair starts, or at its pair's end); hole[c] = 1 for a hole.  Outer
+    contours run counter-clockwise, holes clockwise."""
+    cont_off: torch.Tensor   # (np + 1,) int64
+    vert_off: torch.Tensor   # (np + 1,) int64
+    cont_start: torch.Tensor  # (nc,) int64
+    hole: torch.Tensor       # (nc,) int32
+    xy: torch.Tensor         # (nv, 2) float64
+
+    def pair(self, t: int) -> list[tuple]:
+        """[(vertices (k, 2) numpy, is_hole)] of pair t, on the host."""
+        c0, c1 = int(self.cont_off[t]), int(self.cont_off[t + 1])
+        v1 = int(self.vert_off[t + 1])
+        starts = self.cont_start[c0:c1].tolist() + [v1]
+        xy = self.xy.cpu().numpy()
+        holes = self.hole[c0:c1].tolist()
+        return [(xy[starts[i]:starts[i + 1]], bool(holes[i])) for i in range(c1 - c0)]
+
+
+def overlay_contours(pairs: torch.Tensor, A: PolyDev, B: PolyDev, op="int") -> Contours:
+    """Contours of A[pairs[t,0]] op B[pairs[t,1]]: the count pass, a scan of
+    the counts, then the write pass (one lane per pair)."""
+    pairs = pairs.contiguous()
+    npairs, dev = pairs.shape[0], pairs.device
+    mv = A.max_verts + B.max_verts
+    ncont = torch.zeros(npairs, dtype=torch.int64, device=dev)
+    nvert = torch.zeros(npairs, dtype=torch.int64, device=dev)
+    ws = _ws(lib().prt_overlay_contours_workspace_size(npairs, mv, 0), dev)
+    check(lib().prt_overlay_contours_count(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off),
+                                           _ptr(B.xy), _ptr(B.off), mv, _op(op), _ptr(ncont),
+                                           _ptr(nvert), _ptr(ws), ws.numel(), _stream()))
+    if npairs and int(torch.minimum(ncont.min(), nvert.min())) < 0:
+        raise PrtError(-5, "overlay_contours: a pair exceeds max_verts")
+    zero = torch.zeros(1, dtype=torch.int64, device=dev)
+    cont_off = torch.cat([zero, torch.cumsum(ncont, 0)])
+    vert_off = torch.cat([zero, torch.cumsum(nvert, 0)])
+    nc, nv = int(cont_off[-1]), int(vert_off[-1])
+    cont_start = torch.empty(nc, dtype=torch.int64, device=dev)
+    hole = torch.empty(nc, dtype=torch.int32, device=dev)
+    xy = torch.empty((nv, 2), dtype=torch.float64, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    ws = _ws(lib().prt_overlay_contours_workspace_size(npairs, mv, nv), dev)
+    check(lib().prt_overlay_contours_write(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off),
+                                           _ptr(B.xy), _ptr(B.off), mv, _op(op), _ptr(cont_off),
+                                           _ptr(vert_off), nv, _ptr(cont_start), _ptr(hole),
+                                           _ptr(xy), _ptr(status), _ptr(ws), ws.numel(),
+                                           _stream()))
+    if int(status.item()) != 0:
+        raise PrtError(-4, "overlay_contours: write pass disagrees with the counts")
+    return Contours(cont_off, vert_off, cont_start, hole, xy)
 
 
 def run_like_reference(records: torch.Tensor, offset: int, num_base: torch.Tensor,

@@ -1,0 +1,94 @@
+"""GPU: result contours (count pass, scan, write pass; one lane per pair)
+equal to the oracle's, vertex for vertex, with the same hole flags and the
+same contour order, for the four operations on grid-overlay pairs, random
+star pairs of uneven sizes, and closed-form cases (rects, nested holes,
+disjoint and empty operands).  The oracle's contours are pinned in
+test_oracle_contours.py."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from parlllelrtree_amd import gen, rtree
+from parlllelrtree_amd._lib import PrtError
+
+pytestmark = pytest.mark.gpu
+
+
+def check_all(pairs, A, B, op):
+    dA, dB = rtree.to_device_polys(A), rtree.to_device_polys(B)
+    got = rtree.overlay_contours(torch.from_numpy(pairs.view(np.int32)).cuda(), dA, dB, op)
+    assert got.cont_off.shape[0] == len(pairs) + 1
+    for t, (a, b) in enumerate(pairs):
+        ref, holes = O.clip_contours(A.poly(int(a)), B.poly(int(b)), op)
+        mine = got.pair(t)
+        assert len(mine) == len(ref), (t, op)
+        for (c, h), r, rh in zip(mine, ref, holes):
+            assert h == bool(rh), (t, op)
+            assert np.array_equal(c, r), (t, op)
+    return got
+
+
+@pytest.mark.parametrize("op", ["int", "union", "diff", "xor"])
+def test_grid_pairs_match_oracle(dev, op):
+    A = gen.stars_grid(gen.seed(2), 3000, 55)
+    B = gen.stars_grid(gen.seed(2, True), 3000, 55)
+    pairs = O.join_brute(gen.poly_mbr(B), gen.poly_mbr(A))[:2500]
+    got = check_all(np.ascontiguousarray(pairs), A, B, op)
+    assert int(got.cont_off[-1]) > 1000
+
+
+def test_uneven_star_pairs(dev):
+    S = gen.star_pairs(gen.seed(3), 600, kmin=3, kmax=300)   # pair i = (2i, 2i + 1)
+    pairs = np.stack([np.arange(0, 1200, 2), np.arange(1, 1200, 2)], 1).astype(np.uint32)
+    for op in ("int", "xor"):
+        check_all(pairs, S, S, op)
+
+
+def csr(polys):
+    off = np.zeros(len(polys) + 1, np.int64)
+    off[1:] = np.cumsum([len(p) for p in polys])
+    xy = np.concatenate([np.asarray(p, float).reshape(-1, 2) for p in polys]) if len(polys) else \
+        np.zeros((0, 2))
+    return gen.PolyCSR(np.ascontiguousarray(xy), off)
+
+
+def rect(x0, y0, x1, y1):
+    return [[x0, y0], [x1, y0], [x1, y1], [x0, y1]]
+
+
+def test_closed_forms_and_empty(dev):
+    A = csr([rect(0, 0, 4, 4), rect(0, 0, 2, 2), rect(0, 0, 1, 1), [[0, 0], [1, 1]]])
+    B = csr([rect(1, 1, 2, 2), rect(1, 1, 3, 3), rect(5, 5, 6, 6), rect(0, 0, 1, 1)])
+    pairs = np.array([[0, 0], [1, 1], [2, 2], [3, 3], [0, 2]], np.uint32)
+    for op in ("int", "union", "diff", "xor"):
+        got = check_all(pairs, A, B, op)
+    got = rtree.overlay_contours(torch.from_numpy(pairs.view(np.int32)).cuda(),
+                                 rtree.to_device_polys(A), rtree.to_device_polys(B), "diff")
+    (outer, h0), (hole, h1) = sorted(got.pair(0), key=lambda c: c[1])
+    assert not h0 and h1
+    np.testing.assert_array_equal(outer, np.array(rect(0, 0, 4, 4), float))
+    assert hole.shape == (4, 2)
+    empty = rtree.overlay_contours(torch.zeros((0, 2), dtype=torch.int32, device="cuda"),
+                                   rtree.to_device_polys(A), rtree.to_device_polys(B))
+    assert empty.xy.shape == (0, 2) and empty.cont_off.tolist() == [0]
+
+
+def test_rejects_bad_arguments(dev):
+    from parlllelrtree_amd._lib import lib
+    A = rtree.to_device_polys(csr([rect(0, 0, 1, 1)]))
+    pairs = torch.zeros((1, 2), dtype=torch.int32, device="cuda")
+    n = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ws = torch.empty(lib().prt_overlay_contours_workspace_size(1, 8, 0), dtype=torch.uint8,
+                     device="cuda")
+    args = (pairs.data_ptr(), 1, A.xy.data_ptr(), A.off.data_ptr(), A.xy.data_ptr(),
+            A.off.data_ptr())
+    s = torch.cuda.current_stream().cuda_stream
+    assert lib().prt_overlay_contours_count(*args, 8, 7, n.data_ptr(), n.data_ptr(),
+                                            ws.data_ptr(), ws.numel(), s) == -1
+    assert lib().prt_overlay_contours_count(*args, 8, 1, n.data_ptr(), n.data_ptr(),
+                                            ws.data_ptr(), 16, s) == -2
+    # a pair larger than max_verts is reported, not clipped
+    assert lib().prt_overlay_contours_count(*args, 3, 1, n.data_ptr(), n.data_ptr(),
+                                            ws.data_ptr(), ws.numel(), s) == 0
+    assert int(n.item()) == -1
