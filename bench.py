@@ -131,6 +131,33 @@ def overlay_phase(cp, A, B, dev, args, O, check_sample=None, plain_reps=(2, 1)):
     return out
 
 
+def contours_phase(cp, A, B, dev, O, sample=2000):
+    """Result contours (intersection) of the pairs cp: count pass, scan,
+    write pass; the first `sample` pairs compared with the oracle vertex for
+    vertex."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import rtree
+
+    pd = torch.from_numpy(np.ascontiguousarray(cp).view(np.int32)).to(dev)
+    Ad, Bd = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
+    ms = timed(lambda: rtree.overlay_contours(pd, Ad, Bd, "int"), 3, 1)
+    got = rtree.overlay_contours(pd, Ad, Bd, "int")
+    same = True
+    for t in range(min(sample, len(cp))):
+        ref, holes = O.clip_contours(A.poly(int(cp[t, 0])), B.poly(int(cp[t, 1])), "int")
+        mine = got.pair(t)
+        same &= len(mine) == len(ref) and all(
+            h == bool(rh) and np.array_equal(c, r) for (c, h), r, rh in zip(mine, ref, holes))
+    out = {"pairs": len(cp), "ms": round(ms, 3), "pairs_s": round(len(cp) / ms * 1e3, 1),
+           "contours": int(got.cont_off[-1]), "vertices": int(got.vert_off[-1]),
+           "checked_pairs": int(min(sample, len(cp))), "equal_to_oracle": bool(same)}
+    log(f"contours {len(cp)} pairs: {ms:.2f} ms, {out['contours']} contours, "
+        f"{out['vertices']} vertices, equal to oracle: {same}")
+    return out
+
+
 def grid_phase(dev, args, O):
     """2 x 500,000 stars on the jittered 708x708 grid: bottom-up tree of
     layer A's MBRs, join with layer B's MBRs, overlay of every pair."""
@@ -446,6 +473,7 @@ def main() -> int:
         # plain and batched FP64 overlay of the first 100,000 sorted pairs
         cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
         phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, O)
+        phases["contours_layers_100k"] = contours_phase(cp, A, B, dev, O)
         phases["overlay_layers_100k"].pop("_tune")
     if "grid" in want:
         # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate

@@ -367,14 +367,18 @@ class Contours:
     hole: torch.Tensor       # (nc,) int32
     xy: torch.Tensor         # (nv, 2) float64
 
+    _host: tuple | None = field(default=None, repr=False)
+
     def pair(self, t: int) -> list[tuple]:
-        """[(vertices (k, 2) numpy, is_hole)] of pair t, on the host."""
-        c0, c1 = int(self.cont_off[t]), int(self.cont_off[t + 1])
-        v1 = int(self.vert_off[t + 1])
-        starts = self.cont_start[c0:c1].tolist() + [v1]
-        xy = self.xy.cpu().numpy()
-        holes = self.hole[c0:c1].tolist()
-        return [(xy[starts[i]:starts[i + 1]], bool(holes[i])) for i in range(c1 - c0)]
+        """[(vertices (k, 2) numpy, is_hole)] of pair t, on the host (the
+        arrays are copied to the host once)."""
+        if self._host is None:
+            self._host = tuple(a.cpu().numpy() for a in
+                               (self.cont_off, self.vert_off, self.cont_start, self.hole, self.xy))
+        co, vo, cs, hole, xy = self._host
+        c0, c1 = int(co[t]), int(co[t + 1])
+        starts = list(cs[c0:c1]) + [int(vo[t + 1])]
+        return [(xy[starts[i]:starts[i + 1]], bool(hole[c0 + i])) for i in range(c1 - c0)]
 
 
 def overlay_contours(pairs: torch.Tensor, A: PolyDev, B: PolyDev, op="int") -> Contours:
