@@ -22,6 +22,8 @@
 // the team's LDS.  Only a pair larger than its bucket's team (or a single
 // beam with more active edges than the list capacity) goes to the plain
 // kernel, through an overflow list (same results, slower).
+#include <mutex>
+
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -773,6 +775,39 @@ int launch_bucket(const BatchArgs &A, int waves, size_t lds, hipStream_t s) {
     return PRT_OK;
 }
 
+// Helper streams of the bucket launches, created once per device (default
+// priority, non-blocking).  A call forks them from its stream with an event
+// and joins them back before it returns to that stream's order, so the
+// calls stay stream-ordered (and capturable: the fork and join become graph
+// edges).  Calls from several host threads share them; that only orders
+// their bucket launches.
+constexpr int AUX_STREAMS = 3;
+constexpr int AUX_DEVICES = 16;
+
+int aux_streams(hipStream_t *out) {
+    static std::mutex mu;
+    static hipStream_t streams[AUX_DEVICES][AUX_STREAMS] = {};
+    int dev = 0;
+    PRT_HIP(hipGetDevice(&dev));
+    PRT_REQUIRE(dev >= 0 && dev < AUX_DEVICES, PRT_E_UNSUPPORTED, "overlay_batched: device %d", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < AUX_STREAMS; ++i) {
+        if (!streams[dev][i])
+            PRT_HIP(hipStreamCreateWithFlags(&streams[dev][i], hipStreamNonBlocking));
+        out[i] = streams[dev][i];
+    }
+    return PRT_OK;
+}
+
+struct EventSet {  // the fork event and one join event per helper stream
+    hipEvent_t fork = nullptr, join[AUX_STREAMS] = {};
+    ~EventSet() {
+        if (fork) (void)hipEventDestroy(fork);
+        for (hipEvent_t e : join)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 }  // namespace
 }  // namespace prt
 
@@ -896,10 +931,20 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
                w.fill, w.order);
     PRT_LAUNCHED();
 
-    // largest buckets first: the long pairs start early
+    // The three buckets of the largest pairs run on the helper streams,
+    // concurrently with the others on the caller's stream: their launches are
+    // small (LDS-limited teams) and end in long tails, which the concurrent
+    // launches fill.  Largest first, so the long pairs start early.
     int max_lds = 0, dev = 0;
     PRT_HIP(hipGetDevice(&dev));
     PRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    hipStream_t aux[AUX_STREAMS];
+    rc = aux_streams(aux);
+    if (rc) return rc;
+    EventSet ev;
+    PRT_HIP(hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
+    PRT_HIP(hipEventRecord(ev.fork, s));
+    bool forked[AUX_STREAMS] = {};
     for (int b = NBUCKET - 1; b >= 0; --b) {
         const int lo = b == 0 ? 0 : tune->bucket_max_verts[b - 1];
         if (max_verts <= lo) continue;  // no pair can fall in this bucket
@@ -919,21 +964,36 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         A.NV = tune->bucket_max_verts[b] < max_verts ? tune->bucket_max_verts[b] : max_verts;
         A.SCAP = scap_for(A.NV, tune->list_factor[b]);
         A.op = op;
+        const int slot = NBUCKET - 1 - b;  // 0, 1, 2 for the three largest buckets
+        hipStream_t bs = s;
+        if (slot < AUX_STREAMS) {
+            bs = aux[slot];
+            if (!forked[slot]) {
+                PRT_HIP(hipStreamWaitEvent(bs, ev.fork, 0));
+                forked[slot] = true;
+            }
+        }
         const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
         const size_t lds = static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
         if (lds > static_cast<size_t>(max_lds)) {
             // too large for LDS: the whole bucket takes the plain path
-            PRT_LAUNCH(k_route_bucket, 256, 256, 0, s, A);
+            PRT_LAUNCH(k_route_bucket, 256, 256, 0, bs, A);
             PRT_LAUNCHED();
             continue;
         }
         switch (L) {
-        case 8: rc = launch_bucket<8>(A, W, lds, s); break;
-        case 16: rc = launch_bucket<16>(A, W, lds, s); break;
-        case 32: rc = launch_bucket<32>(A, W, lds, s); break;
-        default: rc = launch_bucket<64>(A, W, lds, s); break;
+        case 8: rc = launch_bucket<8>(A, W, lds, bs); break;
+        case 16: rc = launch_bucket<16>(A, W, lds, bs); break;
+        case 32: rc = launch_bucket<32>(A, W, lds, bs); break;
+        default: rc = launch_bucket<64>(A, W, lds, bs); break;
         }
         if (rc) return rc;
+    }
+    for (int i = 0; i < AUX_STREAMS; ++i) {
+        if (!forked[i]) continue;
+        PRT_HIP(hipEventCreateWithFlags(&ev.join[i], hipEventDisableTiming));
+        PRT_HIP(hipEventRecord(ev.join[i], aux[i]));
+        PRT_HIP(hipStreamWaitEvent(s, ev.join[i], 0));
     }
     return plain_list_launch(pairs, w.ovf_list, reinterpret_cast<const int64_t *>(w.ovf_count),
                              npairs, xyA, offA, xyB, offB, max_verts, op, area, w.plain_ws,

@@ -124,17 +124,15 @@ struct SubtreeArgs {
 
 // Thread t holds the keys of positions 4t..4t+3 in registers.  A bitonic
 // compare-exchange at distance j pairs registers of one thread (j < 4),
-// lanes of one wave (4 <= j < 256: __shfl_xor by j/4), or positions of
-// different waves (j >= 256: through LDS with block barriers), so a 4096-key
-// sort needs 14 barriers instead of 78.
+// lanes of one wave (4 <= j < 256: exchanged through the wave's own LDS
+// slots), or positions of different waves (j >= 256: through LDS with block
+// barriers), so a 4096-key sort needs 14 block barriers instead of 78.
 constexpr int ST_PER = LS_SLOTS / LS_THREADS;  // 4 keys per thread
 
 // dynamic LDS: rect (64 KB) | id (16 KB) | keyx (32 KB); the rect array
 // holds the boxes in sorted order once the leaves are written
 constexpr size_t ST_LDS = LS_SLOTS * (16 + 4 + 8);
 
-__device__ inline uint64_t kmin(uint64_t a, uint64_t b) { return a < b ? a : b; }
-__device__ inline uint64_t kmax(uint64_t a, uint64_t b) { return a < b ? b : a; }
 
 // Sorts every aligned block of B positions (B a power of two <= 4096)
 // ascending; position of register r of thread t is 4t + r.
@@ -160,15 +158,28 @@ __device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B, uint64_t *ke
             }
             for (int r = 0; r < ST_PER; ++r) kk[r] = keyx[ST_PER * t + r];
         }
+        // 4 <= j < 256: the partners of positions 4t..4t+3 are the positions
+        // 4t'..4t'+3 of lane t' = t ^ (j/4) of the same wave, so the keys are
+        // exchanged through the wave's own 256 LDS slots (two 16-byte stores
+        // and loads per lane, no block barrier); the direction bits come from
+        // t alone, and each key takes its partner when the partner belongs on
+        // its side (one 64-bit compare and one select per key)
         for (; j >= ST_PER; j >>= 1) {
-            const int lm = j / ST_PER;
+            const int i0 = ST_PER * t;
+            const bool want_max = ((i0 & j) == 0) != ((k == B) || ((i0 & k) == 0));
+            ulonglong2 *mine = reinterpret_cast<ulonglong2 *>(keyx + i0);
+            const ulonglong2 *theirs =
+                reinterpret_cast<const ulonglong2 *>(keyx + ST_PER * (t ^ (j / ST_PER)));
+            mine[0] = make_ulonglong2(kk[0], kk[1]);
+            mine[1] = make_ulonglong2(kk[2], kk[3]);
+            __builtin_amdgcn_wave_barrier();
+            const ulonglong2 p0 = theirs[0], p1 = theirs[1];
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t o[ST_PER] = {p0.x, p0.y, p1.x, p1.y};
 #pragma unroll
             for (int r = 0; r < ST_PER; ++r) {
-                const int i = ST_PER * t + r;
-                const uint64_t o = __shfl_xor(kk[r], lm, 64);
-                const bool lower = (i & j) == 0;
-                const bool up = (k == B) || ((i & k) == 0);
-                kk[r] = (lower == up) ? kmin(kk[r], o) : kmax(kk[r], o);
+                const bool take = (o[r] < kk[r]) != want_max;
+                kk[r] = take ? o[r] : kk[r];
             }
         }
         for (; j > 0; j >>= 1) {
