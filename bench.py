@@ -50,7 +50,7 @@ def cpu_model() -> str:
 # algorithmic HBM bytes per processed item, per kernel (minimal traffic:
 # every input read once, every output written once)
 BYTES_PER_ITEM = {
-    "k_onesweep": 16,       # one digit pass: 8-B item read + 8-B item written
+    "k_onesweep": 16,       # one digit pass: 8-B item read + 8-B item written (+ GATHER_BYTES)
     "k_hist_all<Src>": 24,  # 16-B rect read, 8-B item written (items made in the histogram launch)
     "k_td_gather": 48,      # 8-B item, 16-B rect + 4-B id read; 16 + 4 written
     "k_td_subtree": 52,     # 8-B item, 16-B rect, 4-B id read; 16-B box + 4-B child + 4-B perm written
@@ -59,6 +59,28 @@ BYTES_PER_ITEM = {
     "k_td_leaves": 20,      # (fanouts that are not powers of two) 16-B rect read, 4-B child written
     "k_bu_leaves": 52,      # 8-B item, 16-B rect read; 4+4 key/id, 16+4 entry written
 }
+
+
+# the last digit pass of a top-down radix level that feeds another radix
+# level also moves the rects and ids into the sorted order: 16-B rect + 4-B
+# id read, 16 + 4 written per item
+GATHER_BYTES = 40
+TD_LDS_SLOTS = 4096  # segments up to this size are sorted in LDS by the subtree kernel
+
+
+def td_gather_passes(n: int, f: int, levels: int) -> int:
+    """Digit passes per top-down build that also gather (one per radix level
+    followed by another radix level)."""
+    radix_levels = 0
+    for p in range(levels):
+        a = 1
+        for _ in range(levels - p):
+            a *= f
+            if a >= n:
+                a = n
+                break
+        radix_levels += a > TD_LDS_SLOTS
+    return max(0, radix_levels - 1)
 
 
 def timed(fn, steps: int, warmup: int) -> float:
@@ -427,6 +449,11 @@ def main() -> int:
     cnt, tot = kt.stats[dom]
     avg_ms = tot / cnt
     alg_bytes = BYTES_PER_ITEM[dom] * n
+    if dom == "k_onesweep":
+        # mean over the step's launches, gathering passes included
+        per_step = cnt / 3
+        gathers = 2 * td_gather_passes(n, 16, ta.st.num_levels)
+        alg_bytes = int(n * (16 * per_step + GATHER_BYTES * gathers) / per_step)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (scaled by a copy-kernel calibration), when recorded for this kernel

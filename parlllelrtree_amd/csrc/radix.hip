@@ -197,6 +197,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
     }
 }
 
+// Optional gather of the last digit pass: with rin set, the rect (and id)
+// of every item's low word is copied to its sorted position as well, so the
+// rects of the next level come in sorted order without a pass of their own.
+struct PassGather {
+    const float4 *rin;
+    const uint32_t *iin;  // null: the id is the low word itself
+    float4 *rout;
+    uint32_t *iout;
+};
+
 // Stable scatter of one tile for one digit pass.  Wave w ranks the items
 // [w*256, (w+1)*256) of the tile in input order; lane l of round j holds item
 // w*256 + j*64 + l.  Output position of an item with digit d = segment start
@@ -207,7 +217,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
                                                          int shift, int bits,
                                                          const uint32_t *__restrict__ ghist_p,
                                                          uint64_t *status_p,
-                                                         uint32_t *__restrict__ ticket) {
+                                                         uint32_t *__restrict__ ticket,
+                                                         PassGather gat) {
     __shared__ uint64_t stage[RS_TILE];
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
     __shared__ uint32_t dstart[RS_BINS];
@@ -329,10 +340,37 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         stage[dstart[d] + wcnt[wave][d] + rank[j]] = item[j];
     }
     __syncthreads();
-    for (int i = tid; i < count; i += RS_THREADS) {
+    if (gat.rin == nullptr) {
+        for (int i = tid; i < count; i += RS_THREADS) {
+            const uint64_t v = stage[i];
+            const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
+            out[goff[d] + (i - dstart[d])] = v;
+        }
+        return;
+    }
+    // the items' rects and ids are read (at random) before the first store
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int i = tid + j * RS_THREADS;
+        if (i >= count) continue;
         const uint64_t v = stage[i];
-        const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
-        out[goff[d] + (i - dstart[d])] = v;
+        const uint32_t id = static_cast<uint32_t>(v);
+        item[j] = v;
+        rank[j] = gat.iin ? gat.iin[id] : id;
+    }
+    float4 rc[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j)
+        if (tid + j * RS_THREADS < count) rc[j] = gat.rin[static_cast<uint32_t>(item[j])];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int i = tid + j * RS_THREADS;
+        if (i >= count) continue;
+        const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
+        const uint32_t pos = goff[d] + (i - dstart[d]);
+        out[pos] = item[j];
+        gat.rout[pos] = rc[j];
+        gat.iout[pos] = rank[j];
     }
 }
 
@@ -380,7 +418,7 @@ namespace {
 template <class Src>
 int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n,
                     int64_t seg, int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s,
-                    uint64_t **result, bool zeroed) {
+                    uint64_t **result, bool zeroed, const PassGather &gat = PassGather{}) {
     PRT_REQUIRE(n >= 0 && n < (int64_t(1) << 32), PRT_E_INVAL, "radix sort: n=%lld out of range",
                 (long long)n);
     PRT_REQUIRE(bit_lo >= 0 && bit_hi <= 64 && bit_lo < bit_hi && bit_hi - bit_lo <= 8 * RS_MAX_PASSES,
@@ -411,9 +449,11 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
     for (int p = 0; p < passes; ++p) {
         const int shift = bit_lo + 8 * p;
         const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
+        const PassGather none{nullptr, nullptr, nullptr, nullptr};
         PRT_LAUNCH(k_onesweep, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
                    ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS,
-                   w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS, tickets + p);
+                   w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS, tickets + p,
+                   p + 1 == passes ? gat : none);
         PRT_LAUNCHED();
         src = dst;
         dst = (dst == buf0) ? buf1 : buf0;
@@ -431,9 +471,10 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
 
 int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t *buf1, int64_t n,
                         int64_t seg, const RadixWs &w, hipStream_t s, uint64_t **result,
-                        bool zeroed) {
+                        bool zeroed, const uint32_t *ids, float4 *rects_out, uint32_t *ids_out) {
+    const PassGather g{rects_out ? rects : nullptr, ids, rects_out, ids_out};
     return radix_sort_impl(SrcTdCoord{rects, buf0, comp}, buf0, buf0, buf1, n, seg, 32, 64, w, s,
-                           result, zeroed);
+                           result, zeroed, g);
 }
 
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,

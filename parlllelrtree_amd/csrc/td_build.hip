@@ -379,6 +379,36 @@ __global__ void k_td_level(int64_t n, int f, int64_t a, int64_t off, int64_t cou
     slot_mbr[off + i] = u;
 }
 
+// Levels p_top .. 0 in one block, level by level (a block barrier between
+// two levels): the same slots as k_td_level launched once per level.  a =
+// leaves under one entry of level p_top.
+constexpr int TOP_THREADS = 1024;
+struct TopArgs {
+    int64_t n, a;
+    int f, p_top;
+    int64_t off[PRT_MAX_LEVELS + 1];
+};
+__global__ __launch_bounds__(TOP_THREADS) void k_td_top(TopArgs T, int32_t *__restrict__ slot_child,
+                                                        float4 *__restrict__ slot_mbr) {
+    int64_t a = T.a;
+    for (int p = T.p_top; p >= 0; --p) {
+        const int64_t off = T.off[p], cnt = T.off[p + 1] - T.off[p], off_next = T.off[p + 1];
+        for (int64_t i = threadIdx.x; i < cnt; i += TOP_THREADS) {
+            float4 u = empty_box();
+            int32_t child = 0;
+            if (i * a < T.n) {
+                const int64_t c = off_next + i * T.f;
+                child = static_cast<int32_t>(c);
+                for (int k = 0; k < T.f; ++k) u = box_union(u, slot_mbr[c + k]);
+            }
+            slot_child[off + i] = child;
+            slot_mbr[off + i] = u;
+        }
+        a *= T.f;
+        __syncthreads();
+    }
+}
+
 int td_levels(int64_t n, int f, int64_t *off) {
     int64_t pushed[PRT_MAX_LEVELS];
     int L = 0;
@@ -502,10 +532,9 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
             }
         // the counters of every radix level are cleared by one launch.
         // Between two radix levels the rects and ids move into the order of
-        // the finished level (one streaming pass of many small blocks: the
-        // random reads overlap far better there than inside the sort's large
-        // tiles), so the next level's sort and the subtree kernel read
-        // neighbouring rects from a few segments.
+        // the finished level (in the last digit pass), so the next level's
+        // sort and the subtree kernel read neighbouring rects from a few
+        // segments.
         if (rwords.ghist > 0) {
             const int64_t zw = rwords.ghist;
             const unsigned zg = static_cast<unsigned>(ceil_div(zw, 1024) < 1024 ? ceil_div(zw, 1024) : 1024);
@@ -519,16 +548,19 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
             const RadixWs lw{lg, status, radix_ghist_words(n, a), rwords.status};
             lg += lw.ghist_cap;
             uint64_t *sorted = nullptr;
-            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, lw, s, &sorted, true);
-            if (rc) return rc;
             if (p + 1 < p0) {
+                // the last digit pass also moves the rects and ids into this
+                // level's order for the next level
                 float4 *nxt_r = (cur_r == ra) ? rb : ra;
                 uint32_t *nxt_i = (cur_i == da) ? db : da;
-                PRT_LAUNCH(k_td_gather, grid, 256, 0, s, sorted, cur_r, cur_i, n, nxt_r, nxt_i);
-                PRT_LAUNCHED();
+                rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, lw, s, &sorted, true, cur_i,
+                                         nxt_r, nxt_i);
+                if (rc) return rc;
                 cur_r = nxt_r;
                 cur_i = nxt_i;
             } else {
+                rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, lw, s, &sorted, true);
+                if (rc) return rc;
                 items = sorted;  // the subtree kernel gathers through them
             }
         }
@@ -557,12 +589,24 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
         const int64_t *off = tree->level_offset;
         int64_t a = 1;
         for (int k = 0; k < L - p0; ++k) a *= f;  // entries of level p0 - 1 cover f^(L-p0)
-        for (int p = p0 - 1; p >= 0; --p) {
-            const int64_t cnt = off[p + 1] - off[p];
-            PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0, s,
-                       n, f, a, off[p], cnt, off[p + 1], tree->slot_child, sm);
+        if (p0 > 0 && off[p0] <= 64 * TOP_THREADS) {
+            // levels p0-1 .. 0 hold few slots: one block packs them all
+            TopArgs T;
+            T.n = n;
+            T.a = a;
+            T.f = f;
+            T.p_top = p0 - 1;
+            for (int p = 0; p <= PRT_MAX_LEVELS; ++p) T.off[p] = off[p];
+            PRT_LAUNCH(k_td_top, dim3(1), dim3(TOP_THREADS), 0, s, T, tree->slot_child, sm);
             PRT_LAUNCHED();
-            a *= f;
+        } else {
+            for (int p = p0 - 1; p >= 0; --p) {
+                const int64_t cnt = off[p + 1] - off[p];
+                PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0,
+                           s, n, f, a, off[p], cnt, off[p + 1], tree->slot_child, sm);
+                PRT_LAUNCHED();
+                a *= f;
+            }
         }
         return PRT_OK;
     }
