@@ -8,7 +8,7 @@
  * reference-shaped shim prt_run(), which synchronises like the reference.
  * No entry point allocates device memory except prt_run();
  * prt_pipeline_capture allocates the host-side graph objects it returns;
- * the batched overlay creates, once per device, three helper streams.
+ * the batched overlay creates, once per device, two helper streams.
  * Return value: 0 (PRT_OK) or a negative PRT_E_* code; prt_last_error()
  * returns a message for the last failure on the calling thread.
  *

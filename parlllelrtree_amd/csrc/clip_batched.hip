@@ -781,7 +781,7 @@ int launch_bucket(const BatchArgs &A, int waves, size_t lds, hipStream_t s) {
 // calls stay stream-ordered (and capturable: the fork and join become graph
 // edges).  Calls from several host threads share them; that only orders
 // their bucket launches.
-constexpr int AUX_STREAMS = 3;
+constexpr int AUX_STREAMS = 2;
 constexpr int AUX_DEVICES = 16;
 
 int aux_streams(hipStream_t *out) {
@@ -931,10 +931,12 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
                w.fill, w.order);
     PRT_LAUNCHED();
 
-    // The three buckets of the largest pairs run on the helper streams,
+    // The two buckets of the largest pairs run on the helper streams,
     // concurrently with the others on the caller's stream: their launches are
     // small (LDS-limited teams) and end in long tails, which the concurrent
-    // launches fill.  Largest first, so the long pairs start early.
+    // launches fill.  (A third one, the 256-vertex bucket, slows workloads
+    // where that bucket is heavy.)  Largest first, so the long pairs start
+    // early.
     int max_lds = 0, dev = 0;
     PRT_HIP(hipGetDevice(&dev));
     PRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -964,7 +966,7 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         A.NV = tune->bucket_max_verts[b] < max_verts ? tune->bucket_max_verts[b] : max_verts;
         A.SCAP = scap_for(A.NV, tune->list_factor[b]);
         A.op = op;
-        const int slot = NBUCKET - 1 - b;  // 0, 1, 2 for the three largest buckets
+        const int slot = NBUCKET - 1 - b;  // 0, 1 for the two largest buckets
         hipStream_t bs = s;
         if (slot < AUX_STREAMS) {
             bs = aux[slot];
