@@ -6,7 +6,7 @@
  * pointers), a caller-owned workspace and a hipStream_t passed as void*.
  * Calls are stream-ordered and never synchronise the device, except the
  * reference-shaped shim prt_run(), which synchronises like the reference.
- * No entry point allocates device memory except prt_run();
+ * No entry point allocates device memory except prt_run() and prt_read_data();
  * prt_pipeline_capture allocates the host-side graph objects it returns;
  * the batched overlay creates, once per device, two helper streams.
  * Return value: 0 (PRT_OK) or a negative PRT_E_* code; prt_last_error()
@@ -343,6 +343,23 @@ int prt_read_polys_text(const char *path, int64_t *n, int64_t *nverts, int64_t *
                         double *xy_h, int32_t *hole_h, int32_t *box_h, int32_t *min_xy_h);
 int prt_write_polys_text(const char *path, int64_t n, const int64_t *off_h, const double *xy_h,
                          const int32_t *hole_h);
+
+/* ---- reference-shaped reader: readData, OV:752-888 ---------------------
+ * The same argument list as the reference's readData (gpc_vertex arrays as
+ * interleaved {x, y} doubles, long as int64): reads the base and overlay
+ * polygon files; rect1..4 / rect*_query = (int)(v * 100000) of each
+ * polygon's box, num_* vertex counts, prefix_* first-vertex offsets, hole_*
+ * flags (host arrays from malloc: release with free()); vertex_* = device
+ * arrays from hipMalloc (release with hipFree).  *min_x / *min_y are lowered
+ * to the smallest base box values (the caller initialises them, OV:805-810).
+ * Unlike the reference it returns a status instead of exiting. */
+int prt_read_data(const char *base_filename, const char *overlay_filename, int32_t **rect1,
+                  int32_t **rect2, int32_t **rect3, int32_t **rect4, int32_t *num_contours,
+                  int32_t **rect1_query, int32_t **rect2_query, int32_t **rect3_query,
+                  int32_t **rect4_query, int32_t *num_querys, int32_t **num_base,
+                  int64_t **prefix_base, double **vertex_base, int32_t **num_overlay,
+                  int64_t **prefix_overlay, double **vertex_overlay, int32_t *min_x,
+                  int32_t *min_y, int32_t **hole_base, int32_t **hole_overlay);
 
 /* ---- reference-shaped clip entry: run(), OV:61-79 ----------------------
  * Record t pairs base polygon zorder_index[pairs[t].nodeid - offset] (the

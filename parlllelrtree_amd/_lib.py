@@ -141,6 +141,9 @@ SIGNATURES = {
                                      c_size_t, c_void_p, POINTER(c_void_p)]),
     "prt_pipeline_launch": (c_int, [c_void_p, c_void_p]),
     "prt_pipeline_destroy": (c_int, [c_void_p]),
+    # 2 file names, 20 output pointers (rect1..4, num_contours, rect*_query,
+    # num_querys, num/prefix/vertex of both layers, min_x, min_y, holes)
+    "prt_read_data": (c_int, [c_char_p, c_char_p] + [c_void_p] * 20),
     "prt_run": (c_int, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                         c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                         POINTER(c_int64)]),

@@ -232,3 +232,115 @@ extern "C" int prt_write_polys_text(const char *path, int64_t n, const int64_t *
     PRT_REQUIRE(ok, PRT_E_INVAL, "write_polys_text: write to %s failed", path);
     return PRT_OK;
 }
+
+// ---- readData (vector_map_overlay_gpc_kernel.cu:752-888), same arguments --
+namespace {
+struct LayerText {
+    int64_t n = 0, nv = 0;
+    std::vector<int64_t> off;
+    std::vector<double> xy;
+    std::vector<int32_t> hole, box;
+    int32_t mn[2] = {INT_MAX, INT_MAX};
+};
+
+int read_layer(const char *path, LayerText &L) {
+    int rc = prt_read_polys_text(path, &L.n, &L.nv, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    L.off.resize(L.n + 1);
+    L.xy.resize(2 * L.nv);
+    L.hole.resize(L.n);
+    L.box.resize(4 * L.n);
+    return prt_read_polys_text(path, &L.n, &L.nv, L.off.data(), L.xy.data(), L.hole.data(),
+                               L.box.data(), L.mn);
+}
+
+template <class T> T *host_array(int64_t n) {
+    return static_cast<T *>(std::malloc(static_cast<size_t>(n > 0 ? n : 1) * sizeof(T)));
+}
+
+// the host arrays of one layer in the reference's layout; the vertices to a
+// new device array
+int layer_out(const LayerText &L, int32_t **r1, int32_t **r2, int32_t **r3, int32_t **r4,
+              int32_t **num, int64_t **prefix, double **vertex, int32_t **hole) {
+    int32_t *a[4] = {host_array<int32_t>(L.n), host_array<int32_t>(L.n), host_array<int32_t>(L.n),
+                     host_array<int32_t>(L.n)};
+    int32_t *nm = host_array<int32_t>(L.n), *ho = host_array<int32_t>(L.n);
+    int64_t *pf = host_array<int64_t>(L.n);
+    double *dv = nullptr;
+    bool ok = a[0] && a[1] && a[2] && a[3] && nm && ho && pf;
+    if (ok && L.nv > 0) {
+        ok = hipMalloc(&dv, static_cast<size_t>(L.nv) * 2 * sizeof(double)) == hipSuccess &&
+             hipMemcpy(dv, L.xy.data(), static_cast<size_t>(L.nv) * 2 * sizeof(double),
+                       hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
+        for (int32_t *p : a) std::free(p);
+        std::free(nm);
+        std::free(ho);
+        std::free(pf);
+        if (dv) (void)hipFree(dv);
+        return PRT_E_NOMEM;
+    }
+    for (int64_t i = 0; i < L.n; ++i) {
+        for (int j = 0; j < 4; ++j) a[j][i] = L.box[4 * i + j];
+        nm[i] = static_cast<int32_t>(L.off[i + 1] - L.off[i]);
+        pf[i] = L.off[i];
+        ho[i] = L.hole[i];
+    }
+    *r1 = a[0];
+    *r2 = a[1];
+    *r3 = a[2];
+    *r4 = a[3];
+    *num = nm;
+    *prefix = pf;
+    *vertex = dv;
+    *hole = ho;
+    return PRT_OK;
+}
+
+void free_layer(int32_t **r1, int32_t **r2, int32_t **r3, int32_t **r4, int32_t **num,
+                int64_t **prefix, double **vertex, int32_t **hole) {
+    for (int32_t **p : {r1, r2, r3, r4, num, hole}) {
+        std::free(*p);
+        *p = nullptr;
+    }
+    std::free(*prefix);
+    *prefix = nullptr;
+    if (*vertex) (void)hipFree(*vertex);
+    *vertex = nullptr;
+}
+}  // namespace
+
+extern "C" int prt_read_data(const char *base_filename, const char *overlay_filename,
+                             int32_t **rect1, int32_t **rect2, int32_t **rect3, int32_t **rect4,
+                             int32_t *num_contours, int32_t **rect1_query,
+                             int32_t **rect2_query, int32_t **rect3_query,
+                             int32_t **rect4_query, int32_t *num_querys, int32_t **num_base,
+                             int64_t **prefix_base, double **vertex_base, int32_t **num_overlay,
+                             int64_t **prefix_overlay, double **vertex_overlay, int32_t *min_x,
+                             int32_t *min_y, int32_t **hole_base, int32_t **hole_overlay) {
+    PRT_REQUIRE(base_filename && overlay_filename && rect1 && rect2 && rect3 && rect4 &&
+                    num_contours && rect1_query && rect2_query && rect3_query && rect4_query &&
+                    num_querys && num_base && prefix_base && vertex_base && num_overlay &&
+                    prefix_overlay && vertex_overlay && min_x && min_y && hole_base && hole_overlay,
+                PRT_E_INVAL, "read_data: null argument");
+    LayerText B, Q;
+    int rc = read_layer(base_filename, B);
+    if (rc) return rc;
+    rc = read_layer(overlay_filename, Q);
+    if (rc) return rc;
+    rc = layer_out(B, rect1, rect2, rect3, rect4, num_base, prefix_base, vertex_base, hole_base);
+    PRT_REQUIRE(rc == PRT_OK, rc, "read_data: out of memory");
+    rc = layer_out(Q, rect1_query, rect2_query, rect3_query, rect4_query, num_overlay,
+                   prefix_overlay, vertex_overlay, hole_overlay);
+    if (rc) {
+        free_layer(rect1, rect2, rect3, rect4, num_base, prefix_base, vertex_base, hole_base);
+        PRT_REQUIRE(false, rc, "read_data: out of memory");
+    }
+    *num_contours = static_cast<int32_t>(B.n);
+    *num_querys = static_cast<int32_t>(Q.n);
+    // only the base layer's boxes lower the caller's minimum (OV:805-810)
+    if (B.n > 0 && B.mn[0] < *min_x) *min_x = B.mn[0];
+    if (B.n > 0 && B.mn[1] < *min_y) *min_y = B.mn[1];
+    return PRT_OK;
+}

@@ -3,6 +3,8 @@
 positions mapped through the bottom-up tree's Z-order permutation, polygons
 as num / prefix / vertex arrays.  Areas bit-identical to the oracle for the
 mapped pairs, count1 = non-empty intersections."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -59,3 +61,51 @@ def test_run_errors_and_empty(dev):
     area, c1 = rtree.run_like_reference(rec, 0, nA, pA, vA, nA, pA, vA, z)
     ref = O.overlay(np.array([[5, 1], [3, 2]], np.uint32), A, A)
     assert np.array_equal(area.cpu().numpy(), ref) and c1 == int((ref > 0).sum())
+
+
+def test_read_data_matches_the_readers(dev, tmp_path):
+    """prt_read_data (readData's argument list, OV:752-888): host arrays equal
+    to the text readers' output, vertices in device memory, minX / minY only
+    lowered by the base layer."""
+    from parlllelrtree_amd import refio
+    from parlllelrtree_amd._lib import lib
+    A = gen.stars_grid(gen.seed(5), 400, 21)
+    B = gen.stars_grid(gen.seed(5, True), 300, 18)
+    pa, pb = tmp_path / "base.txt", tmp_path / "overlay.txt"
+    refio.write_polys_text(pa, A, np.arange(A.n, dtype=np.int32) % 2)
+    refio.write_polys_text(pb, B)
+    ra, rb = refio.read_polys_text(pa), refio.read_polys_text(pb)
+    i32p = lambda: ctypes.POINTER(ctypes.c_int32)()
+    r = [i32p() for _ in range(4)]
+    rq = [i32p() for _ in range(4)]
+    nb, no, hb, ho = i32p(), i32p(), i32p(), i32p()
+    pfb, pfo = ctypes.POINTER(ctypes.c_int64)(), ctypes.POINTER(ctypes.c_int64)()
+    vb, vo = ctypes.c_void_p(), ctypes.c_void_p()
+    nc, nq = ctypes.c_int32(0), ctypes.c_int32(0)
+    mx, my = ctypes.c_int32(10**9), ctypes.c_int32(-10**9)
+    by = ctypes.byref
+    rc = lib().prt_read_data(str(pa).encode(), str(pb).encode(), *[by(x) for x in r], by(nc),
+                             *[by(x) for x in rq], by(nq), by(nb), by(pfb), by(vb), by(no), by(pfo),
+                             by(vo), by(mx), by(my), by(hb), by(ho))
+    assert rc == 0
+    hip = ctypes.CDLL("libamdhip64.so")
+    try:
+        assert (nc.value, nq.value) == (A.n, B.n)
+        for L, f, rr, num, pf, v, hole in ((A, ra, r, nb, pfb, vb, hb), (B, rb, rq, no, pfo, vo, ho)):
+            for j in range(4):
+                assert np.array_equal(np.ctypeslib.as_array(rr[j], (L.n,)), f.box[:, j])
+            assert np.array_equal(np.ctypeslib.as_array(num, (L.n,)), np.diff(L.off))
+            assert np.array_equal(np.ctypeslib.as_array(pf, (L.n,)), L.off[:-1])
+            assert np.array_equal(np.ctypeslib.as_array(hole, (L.n,)), f.hole)
+            host = np.empty_like(L.xy)
+            assert hip.hipMemcpy(ctypes.c_void_p(host.ctypes.data), v, host.nbytes, 2) == 0
+            assert np.array_equal(host, L.xy)
+        assert mx.value == ra.min_xy[0]      # lowered
+        assert my.value == -10**9            # already smaller: kept
+    finally:
+        libc = ctypes.CDLL(None)
+        libc.free.argtypes = [ctypes.c_void_p]
+        for x in r + rq + [nb, no, hb, ho, pfb, pfo]:
+            libc.free(ctypes.cast(x, ctypes.c_void_p))
+        hip.hipFree(vb)
+        hip.hipFree(vo)
