@@ -179,11 +179,15 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
 
 // Doubled area of one beam whose active edges are the list entries
 // [lo, hi) (with their x at the beam bottom/top cached in lxb/lxt): the
-// general sweep of the beam, recomputing x at crossing heights.  The order
-// of the entries does not matter.
-__device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
-                                   int op) {
-    const int m = hi - lo;
+// general sweep of the beam.  Sub-beam ends are the distinct crossing
+// heights, found one minimum at a time; at every sub-beam the list region
+// (still in the previous sub-beam's order, so nearly sorted) is
+// insertion-sorted in place by the strict key (xa + xc, xa, edge id), with
+// the x at the sub-beam ends recomputed as the entries are compared, and the
+// gaps are summed left to right -- the terms and order of the plain kernel's
+// sweep.  The region's order changes; its entries do not.
+__device__ double beam_sum_serial(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                                  int op) {
     const double dy = yt - yb;
     double ya = yb, bs = 0.0;
     for (;;) {
@@ -204,38 +208,37 @@ __device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, 
             }
         }
         const double h = yc - ya;
-        // x at the sub-beam ends: cached at the beam ends, recomputed inside
         auto xat = [&](int i, double y) -> double {
             if (y == yb) return t.lxb[i];
             if (y == yt) return t.lxt[i];
             return ex(t, t.list[i], y);
         };
-        // selection walk in key order
-        int cur = -1, ce = 0;
-        double kc = 0.0, xac = 0.0, xcc = 0.0;
-        for (int i = lo; i < hi; ++i) {
+        for (int i = lo + 1; i < hi; ++i) {
             const int e = t.list[i];
-            const double xa = xat(i, ya), xc = xat(i, yc), kk = xa + xc;
-            if (cur < 0 || key_before(kk, xa, e, kc, xac, ce)) {
-                cur = i; ce = e; kc = kk; xac = xa; xcc = xc;
+            const double xa = xat(i, ya), k = xa + xat(i, yc);
+            const double xb = t.lxb[i], xt = t.lxt[i];
+            int j = i;
+            while (j > lo) {
+                const double xa2 = xat(j - 1, ya), k2 = xa2 + xat(j - 1, yc);
+                if (!key_before(k, xa, e, k2, xa2, t.list[j - 1])) break;
+                t.list[j] = t.list[j - 1];
+                t.lxb[j] = t.lxb[j - 1];
+                t.lxt[j] = t.lxt[j - 1];
+                --j;
             }
+            t.list[j] = static_cast<uint16_t>(e);
+            t.lxb[j] = xb;
+            t.lxt[j] = xt;
         }
         int pp = 0, pq = 0;
-        for (int step = 0; step + 1 < m; ++step) {
-            if (ce < nP) pp ^= 1;
+        double xa0 = xat(lo, ya), xc0 = xat(lo, yc);
+        for (int i = lo; i + 1 < hi; ++i) {
+            if (t.list[i] < nP) pp ^= 1;
             else pq ^= 1;
-            int nxt = -1, ne = 0;
-            double kn = 0.0, xan = 0.0, xcn = 0.0;
-            for (int i = lo; i < hi; ++i) {
-                const int e = t.list[i];
-                const double xa = xat(i, ya), xc = xat(i, yc), kk = xa + xc;
-                if (!key_before(kc, xac, ce, kk, xa, e)) continue;  // not after cur
-                if (nxt < 0 || key_before(kk, xa, e, kn, xan, ne)) {
-                    nxt = i; ne = e; kn = kk; xan = xa; xcn = xc;
-                }
-            }
-            if (op_inside(op, pp, pq)) bs += ((xan - xac) + (xcn - xcc)) * h;
-            cur = nxt; ce = ne; kc = kn; xac = xan; xcc = xcn;
+            const double xa1 = xat(i + 1, ya), xc1 = xat(i + 1, yc);
+            if (op_inside(op, pp, pq)) bs += ((xa1 - xa0) + (xc1 - xc0)) * h;
+            xa0 = xa1;
+            xc0 = xc1;
         }
         if (yc == yt) break;
         ya = yc;
@@ -250,8 +253,8 @@ __device__ double beam_sum_general(const TeamLds &t, int lo, int hi, double yb, 
 // same order, as the general sweep.  Otherwise the general sweep runs.
 // Sorts the beam's entries by the sub-beam key and sums a crossing-free
 // beam in one linear pass.  A beam with crossings is summed here by the
-// serial general sweep when it has more than GS entries; otherwise the
-// function returns false and the team sweeps it (team_general).
+// serial sweep when it has more than GS entries; otherwise the function
+// returns false and the team sweeps it (team_general).
 __device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
                              int op, double &out) {
     const int m = hi - lo;
@@ -283,8 +286,10 @@ __device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double
     for (int i = lo + 1; i < hi && mono; ++i)
         mono = !(t.lxb[i] < t.lxb[i - 1]) && !(t.lxt[i] < t.lxt[i - 1]);
     if (!mono) {
-        if (m <= t.GS) return false;
-        out = beam_sum_general(t, lo, hi, yb, yt, nP, op);
+        if (m <= t.GS) return false;  // the team sweeps it (measured: sweeping the
+                                      // small ones in their lanes, 16 or 32
+                                      // entries at most, is 8-20 % slower)
+        out = beam_sum_serial(t, lo, hi, yb, yt, nP, op);
         return true;
     }
     const double h = yt - yb;
@@ -343,11 +348,11 @@ template <int TL> __device__ inline void team_sort_heights(const TeamLds &t, int
 }
 
 // The general sweep of one beam (list entries [lo, hi), m <= GS) by the
-// whole team, with the results of beam_sum_general bit for bit: the
+// whole team, with the results of beam_sum_serial bit for bit: the
 // sub-beam bounds are the distinct crossing heights in (yb, yt) in
 // increasing order (the serial sweep finds them one minimum at a time from
 // the same candidate set), and each sub-beam is summed by one lane in the
-// strict key order (the serial sweep's selection walk visits that order).
+// strict key order (the order the serial sweep sorts the entries into).
 // Returns the beam sum on every lane of the team.
 template <int TL>
 __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
@@ -379,7 +384,7 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
     if (nc > t.GC) {
         // more crossing heights than the scratch holds: the serial sweep
         double r = 0.0;
-        if (tl == 0) r = beam_sum_general(t, lo, hi, yb, yt, nP, op);
+        if (tl == 0) r = beam_sum_serial(t, lo, hi, yb, yt, nP, op);
         double v = __shfl(r, base, 64);
         team_sync();
         return v;
