@@ -4,13 +4,16 @@ MI355X, printed as one JSON line.
 
 Workload (the step): top-down R-tree build, fanout 16, of both layers of
 1,048,576 fp32 rects (centres U[0,1]^2, sides U[1e-4, 1e-3]), inputs
-resident in HBM; value = rects built per second (Mrect/s).  The same line
-carries, measured separately: the bottom-up build of 1,048,576 rects, the
-65,536-window join, the layer-layer join, and the plain FP64 overlay of the
-first 100,000 sorted layer pairs (with its area error against the FP64 CPU
-oracle).  roofline: the dominant kernel of the step, its average duration
-from HIP events recorded on its launch stream.  cpu_baseline: the oracle's
-serial top-down build on the host.
+resident in HBM, both layers in one shared launch sequence
+(prt_td_build_layers); value = rects built per second (Mrect/s).  The same
+line carries, measured separately: the same builds one layer per call, the
+bottom-up build of 1,048,576 rects, the 65,536-window join, the layer-layer
+join, and the plain FP64 overlay of the first 100,000 sorted layer pairs.
+roofline: the dominant kernel of the step, its average duration from HIP
+events that hipExtLaunchKernel stamps with each launch's own start and end.
+cpu_baseline: the oracle's serial top-down build on the host; the same CPU
+leg then compares the phases' GPU results (copied to the host when each
+phase ends) with the oracle and adds the error fields to each phase.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -100,9 +103,10 @@ def timed(fn, steps: int, warmup: int) -> float:
     return a.elapsed_time(b) / steps
 
 
-def overlay_phase(cp, A, B, dev, args, O, check_sample=None, plain_reps=(2, 1)):
-    """Plain vs batched FP64 overlay of pairs cp over CSR sets A, B; the
-    areas are compared with the oracle (on `check_sample` pairs if given)."""
+def overlay_phase(cp, A, B, dev, args, checks, check_sample=None, plain_reps=(2, 1)):
+    """Plain vs batched FP64 overlay of pairs cp over CSR sets A, B.  Appends
+    to `checks` the comparison of the areas with the oracle (on
+    `check_sample` pairs if given), run after every GPU phase."""
     import numpy as np
     import torch
 
@@ -126,37 +130,46 @@ def overlay_phase(cp, A, B, dev, args, O, check_sample=None, plain_reps=(2, 1)):
                        device=dev)
     ms_b = timed(lambda: rtree.overlay_batched(pd, Ad, Bd, tune, area_b, ws_b), 5, 1)
     got_p, got_b = area_p.cpu().numpy(), area_b.cpu().numpy()
-    sel = np.arange(npairs) if check_sample is None or check_sample >= npairs else \
-        np.sort(np.random.default_rng(0).choice(npairs, check_sample, replace=False))
-    ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16), with_flops=True)
-    pa = np.abs([O.poly_area(A.poly(i)) for i in cp[sel, 0]])
-    pb = np.abs([O.poly_area(B.poly(i)) for i in cp[sel, 1]])
-    den = np.maximum(np.abs(ref), 1e-12 * np.minimum(pa, pb))
-    den = np.where(den == 0, 1.0, den)
-    rel_p = np.abs(got_p[sel] - ref) / den
-    rel_b = np.abs(got_b[sel] - ref) / den
-    flops_all = flops * npairs / len(sel)
     out.update({
         "plain_ms": round(ms_p, 3), "plain_pairs_s": round(npairs / ms_p * 1e3, 1),
         "batched_ms": round(ms_b, 3), "batched_pairs_s": round(npairs / ms_b * 1e3, 1),
         "batched_vs_plain": round(ms_p / ms_b, 2),
-        "checked_pairs": int(len(sel)), "nonempty": int((ref > 0).sum()),
-        "max_rel_err_plain": float(rel_p.max()), "max_rel_err_batched": float(rel_b.max()),
-        "bit_identical": bool(np.array_equal(got_p[sel], ref) and np.array_equal(got_b[sel], ref)),
+        "nonempty": int((got_b > 0).sum()),
         "plain_equals_batched": bool(np.array_equal(got_p, got_b)),
         "total_area": float(got_b.sum()),
-        "oracle_fp64_flops_per_pair": round(flops / len(sel), 1),
-        "batched_fp64_tflops": round(flops_all / (ms_b * 1e-3) / 1e12, 4),
         "tune": tune_to_json(tune, tstats)["buckets"], "_tune": tune})
     log(f"overlay {npairs} pairs: plain {ms_p:.2f} ms, batched {ms_b:.2f} ms "
-        f"({ms_p / ms_b:.2f}x), max rel err {max(rel_p.max(), rel_b.max()):.1e}")
+        f"({ms_p / ms_b:.2f}x)")
+
+    def check(O):
+        sel = np.arange(npairs) if check_sample is None or check_sample >= npairs else \
+            np.sort(np.random.default_rng(0).choice(npairs, check_sample, replace=False))
+        ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16),
+                               with_flops=True)
+        pa = np.abs([O.poly_area(A.poly(i)) for i in cp[sel, 0]])
+        pb = np.abs([O.poly_area(B.poly(i)) for i in cp[sel, 1]])
+        den = np.maximum(np.abs(ref), 1e-12 * np.minimum(pa, pb))
+        den = np.where(den == 0, 1.0, den)
+        rel_p = np.abs(got_p[sel] - ref) / den
+        rel_b = np.abs(got_b[sel] - ref) / den
+        flops_all = flops * npairs / len(sel)
+        out.update({
+            "checked_pairs": int(len(sel)),
+            "max_rel_err_plain": float(rel_p.max()), "max_rel_err_batched": float(rel_b.max()),
+            "bit_identical": bool(np.array_equal(got_p[sel], ref)
+                                  and np.array_equal(got_b[sel], ref)),
+            "oracle_fp64_flops_per_pair": round(flops / len(sel), 1),
+            "batched_fp64_tflops": round(flops_all / (ms_b * 1e-3) / 1e12, 4)})
+        log(f"overlay {npairs} pairs against the oracle: max rel err "
+            f"{max(rel_p.max(), rel_b.max()):.1e} on {len(sel)} pairs")
+    checks.append(check)
     return out
 
 
-def contours_phase(cp, A, B, dev, O, sample=2000):
+def contours_phase(cp, A, B, dev, checks, sample=2000):
     """Result contours (intersection) of the pairs cp: count pass, scan,
-    write pass; the first `sample` pairs compared with the oracle vertex for
-    vertex."""
+    write pass.  Appends to `checks` the vertex-for-vertex comparison of the
+    first `sample` pairs with the oracle."""
     import numpy as np
     import torch
 
@@ -166,21 +179,27 @@ def contours_phase(cp, A, B, dev, O, sample=2000):
     Ad, Bd = rtree.to_device_polys(A, dev), rtree.to_device_polys(B, dev)
     ms = timed(lambda: rtree.overlay_contours(pd, Ad, Bd, "int"), 3, 1)
     got = rtree.overlay_contours(pd, Ad, Bd, "int")
-    same = True
-    for t in range(min(sample, len(cp))):
-        ref, holes = O.clip_contours(A.poly(int(cp[t, 0])), B.poly(int(cp[t, 1])), "int")
-        mine = got.pair(t)
-        same &= len(mine) == len(ref) and all(
-            h == bool(rh) and np.array_equal(c, r) for (c, h), r, rh in zip(mine, ref, holes))
+    k = min(sample, len(cp))
+    mine = [got.pair(t) for t in range(k)]
     out = {"pairs": len(cp), "ms": round(ms, 3), "pairs_s": round(len(cp) / ms * 1e3, 1),
-           "contours": int(got.cont_off[-1]), "vertices": int(got.vert_off[-1]),
-           "checked_pairs": int(min(sample, len(cp))), "equal_to_oracle": bool(same)}
+           "contours": int(got.cont_off[-1]), "vertices": int(got.vert_off[-1])}
     log(f"contours {len(cp)} pairs: {ms:.2f} ms, {out['contours']} contours, "
-        f"{out['vertices']} vertices, equal to oracle: {same}")
+        f"{out['vertices']} vertices")
+
+    def check(O):
+        same = True
+        for t in range(k):
+            ref, holes = O.clip_contours(A.poly(int(cp[t, 0])), B.poly(int(cp[t, 1])), "int")
+            same &= len(mine[t]) == len(ref) and all(
+                h == bool(rh) and np.array_equal(c, r)
+                for (c, h), r, rh in zip(mine[t], ref, holes))
+        out.update({"checked_pairs": int(k), "equal_to_oracle": bool(same)})
+        log(f"contours against the oracle on {k} pairs: equal {same}")
+    checks.append(check)
     return out
 
 
-def grid_phase(dev, args, O):
+def grid_phase(dev, args, checks):
     """2 x 500,000 stars on the jittered 708x708 grid: bottom-up tree of
     layer A's MBRs, join with layer B's MBRs, overlay of every pair."""
     import numpy as np
@@ -198,7 +217,7 @@ def grid_phase(dev, args, O):
     pqr = j.result().cpu().numpy().view(np.uint32)
     cp = np.ascontiguousarray(pqr[:, ::-1])      # (A index, B index)
     log(f"grid join {ms_j:.3f} ms ({cap} pairs)")
-    out = overlay_phase(cp, A, B, dev, args, O, check_sample=300_000, plain_reps=(1, 0))
+    out = overlay_phase(cp, A, B, dev, args, checks, check_sample=300_000, plain_reps=(1, 0))
     out["join_ms"] = round(ms_j, 3)
     out["pipeline"] = pipeline_phase(A, B, cap, out.pop("_tune"), out, dev)
     return out
@@ -230,16 +249,15 @@ def pipeline_phase(A, B, cap, tune, ov, dev):
     return {"graph_ms": round(ms_g, 3), "eager_ms": round(ms_e, 3), "pairs": n,
             "pairs_s": round(n / ms_g * 1e3, 1), "total_area": total, "nonempty": nonempty,
             "graph_equals_eager": same, "pairs_match_join": n == cap,
-            "nonempty_matches_stage_run": nonempty == ov["nonempty"] if ov["checked_pairs"] == n
-            else None,
+            "nonempty_matches_stage_run": nonempty == ov["nonempty"],
             "total_rel_diff_vs_stage_run": abs(total - ov["total_area"]) / max(abs(total), 1e-300)}
 
 
-def grid_batched_phase(dev, args, O):
+def grid_batched_phase(dev, args, checks):
     """Batched FP64 overlay alone (default team shapes, no plain run, no
     autotune): every candidate pair of the 2 x 500,000-star grid.  The target
-    of the FP64 VALU counter runs; FP64 flops per pair from the instrumented
-    oracle on a 50,000-pair sample."""
+    of the FP64 VALU counter runs.  Appends to `checks` the comparison with
+    the oracle and its FP64 flops per pair on a 50,000-pair sample."""
     import numpy as np
     import torch
 
@@ -261,16 +279,25 @@ def grid_batched_phase(dev, args, O):
     ms = timed(lambda: rtree.overlay_batched(pd, dA, dB, None, area, ws), 3, 1)
     sel = np.sort(np.random.default_rng(0).choice(npairs, 50_000, replace=False))
     cp = pd.cpu().numpy().view(np.uint32)
-    ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16), with_flops=True)
-    same = bool(np.array_equal(area.cpu().numpy()[sel], ref))
-    fpp = flops / len(sel)
-    log(f"batched overlay {npairs} pairs {ms:.2f} ms, {fpp:.0f} oracle FP64 flops/pair")
-    return {"pairs": npairs, "batched_ms": round(ms, 3), "bit_identical_sample": same,
-            "oracle_fp64_flops_per_pair": round(fpp, 1),
-            "oracle_fp64_tflops": round(fpp * npairs / (ms * 1e-3) / 1e12, 4)}
+    got = area.cpu().numpy()[sel]
+    pipe.close()
+    log(f"batched overlay {npairs} pairs {ms:.2f} ms")
+    out = {"pairs": npairs, "batched_ms": round(ms, 3)}
+
+    def check(O):
+        ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16),
+                               with_flops=True)
+        fpp = flops / len(sel)
+        out.update({"bit_identical_sample": bool(np.array_equal(got, ref)),
+                    "oracle_fp64_flops_per_pair": round(fpp, 1),
+                    "oracle_fp64_tflops": round(fpp * npairs / (ms * 1e-3) / 1e12, 4)})
+        log(f"batched overlay: {fpp:.0f} oracle FP64 flops/pair, sample bit-identical "
+            f"{out['bit_identical_sample']}")
+    checks.append(check)
+    return out
 
 
-def grad_phase(dev, args, O):
+def grad_phase(dev, args, checks):
     """262,144 independent star pairs (16..128 vertices): FP64 forward area
     (batched), FP64 adjoint, FP32 adjoint (+ Green's area), errors against the
     FP64 CPU oracle gradient and FP32 against FP64."""
@@ -307,19 +334,6 @@ def grad_phase(dev, args, O):
     ms_32 = timed(fb32, 3, 1)
     g64 = (gA + gB).cpu().numpy()
     g32 = (hA + hB).double().cpu().numpy()
-    # FP64 gradient against the CPU oracle on a sample of pairs
-    rng = np.random.default_rng(0)
-    sel = np.sort(rng.choice(npairs, 2000, replace=False))
-    worst, exact, tot = 0.0, 0, 0
-    for t in sel:
-        a, b = pairs[t]
-        gP, gQ = O.area_grad(S.poly(a), S.poly(b))
-        for G, idx in ((gP, a), (gQ, b)):
-            mine = g64[S.off[idx]:S.off[idx + 1]]
-            sc = max(np.abs(G).max(), 1e-300)
-            worst = max(worst, np.abs(mine - G).max() / sc)
-            exact += int((mine == G).sum())
-            tot += G.size
     rel32 = np.abs(g32 - g64).max() / np.abs(g64).max()
     a64 = area.cpu().numpy()
     a32 = gr32["r"][2].double().cpu().numpy()
@@ -328,15 +342,31 @@ def grad_phase(dev, args, O):
            "fp64_pairs_s": round(npairs / (ms_f + ms_b) * 1e3, 1),
            "fp32_forward_adjoint_ms": round(ms_32, 3),
            "fp32_pairs_s": round(npairs / ms_32 * 1e3, 1),
-           "grad_rel_err_vs_oracle_fp64": worst, "grad_bit_identical_fraction": exact / tot,
-           "grad_rel_err_fp32_vs_fp64": float(rel32), "area_median_rel_err_fp32": arel32,
-           "checked_pairs": int(len(sel))}
+           "grad_rel_err_fp32_vs_fp64": float(rel32), "area_median_rel_err_fp32": arel32}
     log(f"grad: fwd {ms_f:.2f} ms, adjoint {ms_b:.2f} ms (fp64), fp32 {ms_32:.2f} ms; "
-        f"rel err vs oracle {worst:.1e}, fp32 vs fp64 {rel32:.1e}")
+        f"fp32 vs fp64 {rel32:.1e}")
+
+    def check(O):
+        # FP64 gradient against the CPU oracle on a sample of pairs
+        sel = np.sort(np.random.default_rng(0).choice(npairs, 2000, replace=False))
+        worst, exact, tot = 0.0, 0, 0
+        for t in sel:
+            a, b = pairs[t]
+            gP, gQ = O.area_grad(S.poly(a), S.poly(b))
+            for G, idx in ((gP, a), (gQ, b)):
+                mine = g64[S.off[idx]:S.off[idx + 1]]
+                sc = max(np.abs(G).max(), 1e-300)
+                worst = max(worst, np.abs(mine - G).max() / sc)
+                exact += int((mine == G).sum())
+                tot += G.size
+        out.update({"grad_rel_err_vs_oracle_fp64": worst,
+                    "grad_bit_identical_fraction": exact / tot, "checked_pairs": int(len(sel))})
+        log(f"grad against the oracle on {len(sel)} pairs: rel err {worst:.1e}")
+    checks.append(check)
     return out
 
 
-def w4_phase(dev, args, O):
+def w4_phase(dev, args, checks):
     """16,777,216 Gaussian-mixture rects: bottom-up and top-down build
     throughput and the dominant build kernel against HBM (the working set
     exceeds the 256 MB Infinity Cache); overlay of two 4,000,000-star layers
@@ -382,7 +412,7 @@ def w4_phase(dev, args, O):
     cp = np.ascontiguousarray(pqr[:, ::-1])
     del j, t
     log(f"w4 join {ms_j:.3f} ms ({cap} pairs)")
-    ov = overlay_phase(cp, A, B, dev, args, O, check_sample=200_000, plain_reps=(1, 0))
+    ov = overlay_phase(cp, A, B, dev, args, checks, check_sample=200_000, plain_reps=(1, 0))
     ov["join_ms"] = round(ms_j, 3)
     ov.pop("_tune")
     out["overlay_2x4M"] = ov
@@ -394,7 +424,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU leg: the oracle's serial build and the oracle "
+                         "comparisons of the phases' results")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--phases", default="bu,layers,grid,grad,w4",
                     help="comma list of the phases to run (bu,layers,grid,grad,w4; "
@@ -427,20 +459,21 @@ def main() -> int:
     n = 1 << 20
     ra, rb = workloads.w1(n)
     da, db = torch.from_numpy(ra).to(dev), torch.from_numpy(rb).to(dev)
-    ta = rtree.build_top_down(da, 16)
-    tb = rtree.build_top_down(db, 16)
+    # both layers in one shared launch sequence (every launch covers both)
+    TL = rtree.TopDownLayers([da, db], 16)
+    ta = TL.trees[0]
     torch.cuda.synchronize()
 
     def step():
-        rtree.rebuild(ta, da)
-        rtree.rebuild(tb, db)
+        TL.rebuild([da, db])
 
     log(f"timing top-down build of 2 x {n} rects: {args.warmup} warmup + {args.steps} steps")
     ms = timed(step, args.steps, args.warmup)
     value = 2 * n / (ms * 1e-3) / 1e6
     log(f"step {ms:.3f} ms -> {value:.1f} Mrect/s")
 
-    # dominant kernel of the step, durations from HIP events on its stream
+    # dominant kernel of the step; durations from HIP events that
+    # hipExtLaunchKernel stamps with each launch's own start and end
     with KernelTimes() as kt:
         for _ in range(3):
             step()
@@ -448,12 +481,13 @@ def main() -> int:
     dom = max((k for k in kt.stats if k in BYTES_PER_ITEM), key=lambda k: kt.stats[k][1])
     cnt, tot = kt.stats[dom]
     avg_ms = tot / cnt
-    alg_bytes = BYTES_PER_ITEM[dom] * n
+    items = 2 * n                      # every launch of the step covers both layers
+    alg_bytes = BYTES_PER_ITEM[dom] * items
     if dom == "k_onesweep":
         # mean over the step's launches, gathering passes included
         per_step = cnt / 3
-        gathers = 2 * td_gather_passes(n, 16, ta.st.num_levels)
-        alg_bytes = int(n * (16 * per_step + GATHER_BYTES * gathers) / per_step)
+        gathers = td_gather_passes(n, 16, ta.st.num_levels)
+        alg_bytes = int(items * (16 * per_step + GATHER_BYTES * gathers) / per_step)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (scaled by a copy-kernel calibration), when recorded for this kernel
@@ -461,7 +495,7 @@ def main() -> int:
     tpath = ROOT / "profiles" / "hbm_traffic.json"
     if tpath.exists():
         rec = json.loads(tpath.read_text()).get(dom)
-        if rec and rec.get("items_per_launch") == n:
+        if rec and rec.get("items_per_launch") == items:
             traffic = rec["traffic_bytes"]
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -472,9 +506,19 @@ def main() -> int:
     log(f"dominant kernel {dom}: {avg_ms * 1e3:.1f} us avg, {achieved:.0f} GB/s algorithmic")
 
     phases = {}
+    checks = []     # oracle comparisons of the phases' results, run with the CPU baseline
+    # the same two builds, one prt_td_build call per layer
+    t1a, t1b = rtree.build_top_down(da, 16), rtree.build_top_down(db, 16)
+
+    def step_per_layer():
+        rtree.rebuild(t1a, da)
+        rtree.rebuild(t1b, db)
+    ms1 = timed(step_per_layer, args.steps, args.warmup)
+    phases["td_build_one_layer_per_call"] = {"ms": round(ms1, 4),
+                                             "Mrect_s": round(2 * n / ms1 / 1e3, 1)}
+    log(f"one layer per call: {ms1:.3f} ms -> {2 * n / ms1 / 1e3:.1f} Mrect/s")
+    del t1a, t1b
     want = set() if args.no_phases else set(args.phases.split(","))
-    if want:
-        import oracle as O
     if "bu" in want:
         # bottom-up build of 1,048,576 rects + 65,536 windows
         r0, q0 = workloads.w0()
@@ -499,21 +543,21 @@ def main() -> int:
         log(f"layer join {ms_j1:.3f} ms ({cap1} pairs)")
         # plain and batched FP64 overlay of the first 100,000 sorted pairs
         cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
-        phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, O)
-        phases["contours_layers_100k"] = contours_phase(cp, A, B, dev, O)
+        phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, checks)
+        phases["contours_layers_100k"] = contours_phase(cp, A, B, dev, checks)
         phases["overlay_layers_100k"].pop("_tune")
     if "grid" in want:
         # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate
         # pair, stage by stage and as one captured pipeline
-        phases["overlay_grid_2x500k"] = grid_phase(dev, args, O)
+        phases["overlay_grid_2x500k"] = grid_phase(dev, args, checks)
     if "grid_batched" in want:
-        phases["overlay_grid_batched"] = grid_batched_phase(dev, args, O)
+        phases["overlay_grid_batched"] = grid_batched_phase(dev, args, checks)
     if "grad" in want:
         # differentiable overlay: 262,144 independent pairs, area + adjoint
-        phases["overlay_grad_262144"] = grad_phase(dev, args, O)
+        phases["overlay_grad_262144"] = grad_phase(dev, args, checks)
     if "w4" in want:
         # largest workload: 16.7M-rect builds, 2 x 4M-star overlay
-        phases["w4_gmm"] = w4_phase(dev, args, O)
+        phases["w4_gmm"] = w4_phase(dev, args, checks)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -528,6 +572,8 @@ def main() -> int:
                "sample": f"serial top-down build of one 1,048,576-rect layer (oracle, C), "
                          f"median of 3 = {t_med:.3f} s; host {cpu_model()}"}
         log(f"cpu baseline {cpu['value']} Mrect/s")
+        for chk in checks:
+            chk(O)
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mrect/s", "n_gpus": 1,
@@ -535,7 +581,8 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 boxes, u64 sort keys", "data": "synthetic",
         "config": {"workload": "top-down R-tree build, fanout 16, two layers of 1,048,576 "
-                               "uniform fp32 rects (sides U[1e-4,1e-3])",
+                               "uniform fp32 rects (sides U[1e-4,1e-3]), both layers in one "
+                               "launch sequence",
                    "rects_per_step": 2 * n, "fanout": 16, "parallelism": "1 GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "phases": phases,
     }

@@ -70,6 +70,7 @@ extern "C" {
 #define PRT_MAX_LEVELS 40
 #define PRT_MAX_FANOUT 64
 #define PRT_MAX_POLY_VERTS 4096
+#define PRT_MAX_LAYERS 4      /* layers of one prt_td_build_layers call */
 
 /* Bottom-up tree, SoA.  Node ids run level by level, leaves first, root
  * last (BU:788-830).  Entry slot j of node i lives at i*fanout + j. */
@@ -160,6 +161,16 @@ int prt_td_plan(int64_t n, int fanout, prt_td_tree *tree);
 size_t prt_td_workspace_size(int64_t n, int fanout);
 int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tree *tree,
                  void *ws, size_t ws_bytes, void *stream);
+/* Top-down trees of 1..PRT_MAX_LAYERS layers of n rects each (rects[l]: the
+ * device rects of layer l; trees[l]: its tree, output arrays set as for
+ * prt_td_build), built by one shared launch sequence: every sort, subtree
+ * and top launch covers all layers, so one launch holds num_layers times the
+ * work.  trees[l] is exactly what prt_td_build gives for rects[l].  No
+ * counterpart in the reference, which builds one layer per call
+ * (TopDown-rtree.cu:465-481 per layer); an overlay builds two. */
+size_t prt_td_layers_workspace_size(int num_layers, int64_t n, int fanout);
+int prt_td_build_layers(int num_layers, const float *const *rects, int64_t n, int fanout,
+                        prt_td_tree *trees, void *ws, size_t ws_bytes, void *stream);
 
 /* ---- join: pairs (q, r) sorted by (q, r); *count_d gets the total,
  *      only the first cap pairs are written --------------------------- */

@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libprt.so"
 
 PRT_MAX_LEVELS = 40
 PRT_MAX_FANOUT = 64
+PRT_MAX_LAYERS = 4
 PRT_TUNE_BUCKETS = 6
 
 ERRORS = {0: "OK", -1: "PRT_E_INVAL", -2: "PRT_E_NOMEM", -3: "PRT_E_HIP",
@@ -86,6 +87,9 @@ SIGNATURES = {
     "prt_td_workspace_size": (c_size_t, [c_int64, c_int]),
     "prt_td_build": (c_int, [c_void_p, c_int64, c_int, POINTER(TDTreeStruct), c_void_p,
                              c_size_t, c_void_p]),
+    "prt_td_layers_workspace_size": (c_size_t, [c_int, c_int64, c_int]),
+    "prt_td_build_layers": (c_int, [c_int, POINTER(c_void_p), c_int64, c_int,
+                                    POINTER(TDTreeStruct), c_void_p, c_size_t, c_void_p]),
     "prt_join_workspace_size": (c_size_t, [c_int64]),
     "prt_join_bu": (c_int, [POINTER(BUTreeStruct), c_void_p, c_int64, c_void_p, c_int64,
                             c_void_p, c_void_p, c_size_t, c_void_p]),

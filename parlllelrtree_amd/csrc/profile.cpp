@@ -1,10 +1,10 @@
 // profile.cpp -- optional per-kernel timing with HIP events.
 //
 // When enabled (prt_profile_enable(1)), every kernel launch site of libprt
-// records a start and an end event on the launch stream; prt_profile_read()
-// synchronises those events and reports, per kernel name, the number of
-// launches and the summed duration.  Disabled (the default) it costs one
-// branch per launch.
+// launches through hipExtLaunchKernel with a start and a stop event, which
+// take the kernel's own start and end times; prt_profile_read() synchronises
+// those events and reports, per kernel name, the number of launches and the
+// summed duration.  Disabled (the default) it costs one branch per launch.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -47,20 +47,23 @@ bool profiling() { return g_on && t_paused == 0; }
 ProfPause::ProfPause() { ++t_paused; }
 ProfPause::~ProfPause() { --t_paused; }
 
-ProfScope::ProfScope(const char *name, hipStream_t s) : name_(name), s_(s), a_(nullptr) {
+ProfScope::ProfScope(const char *name) : name_(name), a_(nullptr), b_(nullptr) {
     if (!g_on || t_paused) return;
     std::lock_guard<std::mutex> lk(g_mu);
-    a_ = get_event();
-    if (a_) (void)hipEventRecord(static_cast<hipEvent_t>(a_), s_);
+    hipEvent_t a = get_event();
+    hipEvent_t b = a ? get_event() : nullptr;
+    if (!b) {
+        if (a) g_free.push_back(a);
+        return;
+    }
+    a_ = a;
+    b_ = b;
 }
 
 ProfScope::~ProfScope() {
     if (!a_) return;
     std::lock_guard<std::mutex> lk(g_mu);
-    hipEvent_t b = get_event();
-    if (!b) return;
-    (void)hipEventRecord(b, s_);
-    g_recs.push_back({name_, static_cast<hipEvent_t>(a_), b});
+    g_recs.push_back({name_, a_, b_});
 }
 
 }  // namespace prt

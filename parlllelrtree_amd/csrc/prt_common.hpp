@@ -110,6 +110,33 @@ __host__ __device__ inline float4 empty_box() {
                        -__builtin_huge_valf());
 }
 
+// The rects of up to PRT_MAX_LAYERS layers of `len` rects each, read as one
+// array: position i is rect i - l*len of layer l (l = i / len, found by
+// compares and selects, so that no pointer is indexed by a lane value).
+struct RectLayers {
+    const float4 *p[PRT_MAX_LAYERS];
+    int64_t len;
+    int count;
+    __device__ float4 at(int64_t i) const {
+        const float4 *b = p[0];
+        int64_t o = i;
+#pragma unroll
+        for (int l = 1; l < PRT_MAX_LAYERS; ++l)
+            if (l < count && i >= l * len) {
+                b = p[l];
+                o = i - l * len;
+            }
+        return b[o];
+    }
+};
+inline RectLayers one_layer(const float4 *r, int64_t n) {
+    RectLayers L{};
+    L.p[0] = r;
+    L.len = n;
+    L.count = 1;
+    return L;
+}
+
 __host__ __device__ inline float4 box_union(float4 a, float4 b) {
     return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
 }
@@ -154,7 +181,9 @@ __host__ __device__ inline bool op_range(int op, bool eP, bool eQ, const double 
 // Stable LSD radix sort of 64-bit items on bits [bit_lo, bit_hi), segmented:
 // items never leave their segment [s*seg, (s+1)*seg).  Returns the buffer
 // holding the result (one of buf0/buf1; `in` may alias buf0).
-// Keys of at most 32 bits (bit_hi - bit_lo <= 32).
+// Keys of at most 32 bits (bit_hi - bit_lo <= 32).  With layer > 0 the n
+// items are n / layer layers of `layer` items, each segmented on its own
+// (segments [l*layer + s*seg, l*layer + min((s+1)*seg, layer))).
 struct RadixWs {
     uint32_t *ghist;     // per pass, segment and digit: item counts; + tickets
     uint64_t *status;    // per pass, tile and digit: look-back words
@@ -170,16 +199,16 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
 // caller has already cleared the sort's counter words (radix_ghist_words).
 // With rects_out set, the last digit pass also writes, at every item's
 // sorted position, its rect and id (ids[i], or i when ids is null).
-int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t *buf1, int64_t n,
-                        int64_t seg, const RadixWs &w, hipStream_t s, uint64_t **result,
-                        bool zeroed, const uint32_t *ids = nullptr, float4 *rects_out = nullptr,
-                        uint32_t *ids_out = nullptr);
+int radix_sort_td_coord(const RectLayers &rects, int comp, uint64_t *buf0, uint64_t *buf1,
+                        int64_t n, int64_t seg, int64_t layer, const RadixWs &w, hipStream_t s,
+                        uint64_t **result, bool zeroed, const uint32_t *ids = nullptr,
+                        float4 *rects_out = nullptr, uint32_t *ids_out = nullptr);
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
                       const RadixWs &w, hipStream_t s, uint64_t **result);
 // Counter words (cleared before a sort unless it clears them itself) and
 // look-back words of a sort of n items in segments of seg.
-int64_t radix_ghist_words(int64_t n, int64_t seg);
-int64_t radix_status_words(int64_t n, int64_t seg);
+int64_t radix_ghist_words(int64_t n, int64_t seg, int64_t layer = 0);
+int64_t radix_status_words(int64_t n, int64_t seg, int64_t layer = 0);
 // Stable sort by 64-bit keys (two 32-bit passes); items[k] = (key >> 32 << 32)
 // | id of the k-th rect, keys_sorted[k] (optional) its key.  b0..b3 and items
 // are distinct n-entry buffers.

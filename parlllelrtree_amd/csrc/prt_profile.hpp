@@ -1,7 +1,8 @@
-// prt_profile.hpp -- scope object that brackets one kernel launch with HIP
+// prt_profile.hpp -- scope object that times one kernel launch with HIP
 // events when profiling is enabled (see profile.cpp).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 namespace prt {
@@ -18,25 +19,35 @@ class ProfPause {
     ProfPause &operator=(const ProfPause &) = delete;
 };
 
+// While profiling, holds the start and stop events of one launch; they are
+// handed to hipExtLaunchKernel, which stamps them with the kernel's own
+// start and end on the GPU (not with the times of markers queued around it).
 class ProfScope {
   public:
-    ProfScope(const char *name, hipStream_t s);
+    explicit ProfScope(const char *name);
     ~ProfScope();
     ProfScope(const ProfScope &) = delete;
     ProfScope &operator=(const ProfScope &) = delete;
+    bool active() const { return a_ != nullptr; }
+    hipEvent_t start() const { return a_; }
+    hipEvent_t stop() const { return b_; }
 
   private:
     const char *name_;
-    hipStream_t s_;
-    void *a_;
+    hipEvent_t a_, b_;
 };
 
 }  // namespace prt
 
-// Launch `kernel<<<grid, block, lds, stream>>>(args...)` inside a profiling
-// scope named after the kernel.
-#define PRT_LAUNCH(kernel, grid, block, lds, stream, ...)                       \
-    do {                                                                        \
-        ::prt::ProfScope _ps(#kernel, (stream));                                \
-        kernel<<<(grid), (block), (lds), (stream)>>>(__VA_ARGS__);              \
+// Launch `kernel<<<grid, block, lds, stream>>>(args...)`; while profiling,
+// through hipExtLaunchKernelGGL with the scope's events.
+#define PRT_LAUNCH(kernel, grid, block, lds, stream, ...)                              \
+    do {                                                                               \
+        ::prt::ProfScope _ps(#kernel);                                                 \
+        if (_ps.active())                                                              \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block),                     \
+                                  static_cast<std::uint32_t>(lds), (stream), _ps.start(), \
+                                  _ps.stop(), 0u, __VA_ARGS__);                        \
+        else                                                                           \
+            kernel<<<(grid), (block), (lds), (stream)>>>(__VA_ARGS__);                 \
     } while (0)

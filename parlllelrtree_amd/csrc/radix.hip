@@ -34,26 +34,37 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 items per tile
 constexpr int RS_BINS = 256;
 constexpr int RS_MAX_PASSES = 4;  // keys of up to 32 bits
 
+// n items in n / layer layers of `layer` items; each layer in segments of
+// seg items (the last one of a layer may be shorter), each segment in tiles.
 struct Tiling {
-    int64_t n, seg;
-    int64_t tiles_per_seg, num_segs, num_tiles;
+    int64_t n, seg, layer;
+    int64_t tiles_per_seg, segs_per_layer, num_segs, num_tiles;
 };
 
-Tiling make_tiling(int64_t n, int64_t seg) {
+Tiling make_tiling(int64_t n, int64_t seg, int64_t layer = 0) {
     Tiling t;
     t.n = n;
-    t.seg = (seg <= 0 || seg > n) ? n : seg;
+    t.layer = (layer <= 0 || layer > n) ? n : layer;
+    t.seg = (seg <= 0 || seg > t.layer) ? t.layer : seg;
     t.tiles_per_seg = ceil_div(t.seg, RS_TILE);
-    t.num_segs = ceil_div(n, t.seg);
+    t.segs_per_layer = ceil_div(t.layer, t.seg);
+    t.num_segs = t.segs_per_layer * (n / t.layer);
     t.num_tiles = t.tiles_per_seg * t.num_segs;
     return t;
 }
 
+// first item and end of segment s
+__device__ inline void seg_range(const Tiling &t, int64_t s, int64_t &b, int64_t &e) {
+    const int64_t l = s / t.segs_per_layer, ls = s - l * t.segs_per_layer;
+    b = l * t.layer + ls * t.seg;
+    e = l * t.layer + ((ls + 1) * t.seg < t.layer ? (ls + 1) * t.seg : t.layer);
+}
+
 __device__ inline void tile_range(const Tiling &t, int64_t tile, int64_t &b, int64_t &e) {
-    int64_t s = tile / t.tiles_per_seg, j = tile % t.tiles_per_seg;
-    b = s * t.seg + j * RS_TILE;
-    int64_t se = (s + 1) * t.seg;
-    if (se > t.n) se = t.n;
+    const int64_t s = tile / t.tiles_per_seg, j = tile % t.tiles_per_seg;
+    int64_t sb, se;
+    seg_range(t, s, sb, se);
+    b = sb + j * RS_TILE;
     e = b + RS_TILE < se ? b + RS_TILE : se;
     if (b > e) b = e;
 }
@@ -106,11 +117,11 @@ struct SrcItems {
     __device__ uint64_t operator()(int64_t i) const { return in[i]; }
 };
 struct SrcTdCoord {  // (ord32(x1 or y1) << 32) | position
-    const float4 *rects;
+    RectLayers rects;
     uint64_t *out;
     int comp;
     __device__ uint64_t operator()(int64_t i) const {
-        const float4 r = rects[i];
+        const float4 r = rects.at(i);
         const uint64_t v = (static_cast<uint64_t>(ord32(comp ? r.y : r.x)) << 32) |
                            static_cast<uint64_t>(i);
         out[i] = v;
@@ -201,7 +212,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
 // of every item's low word is copied to its sorted position as well, so the
 // rects of the next level come in sorted order without a pass of their own.
 struct PassGather {
-    const float4 *rin;
+    RectLayers rin;       // rin.count == 0: no gather
     const uint32_t *iin;  // null: the id is the low word itself
     float4 *rout;
     uint32_t *iout;
@@ -331,7 +342,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
             }
             lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
         }
-        goff[tid] = static_cast<uint32_t>(seg * t.seg) + gbase + static_cast<uint32_t>(excl);
+        int64_t sb, se;
+        seg_range(t, seg, sb, se);
+        goff[tid] = static_cast<uint32_t>(sb) + gbase + static_cast<uint32_t>(excl);
     }
     __syncthreads();
     for (int j = 0; j < RS_ITEMS; ++j) {
@@ -340,7 +353,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         stage[dstart[d] + wcnt[wave][d] + rank[j]] = item[j];
     }
     __syncthreads();
-    if (gat.rin == nullptr) {
+    if (gat.rin.count == 0) {
         for (int i = tid; i < count; i += RS_THREADS) {
             const uint64_t v = stage[i];
             const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
@@ -348,30 +361,42 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         }
         return;
     }
-    // the items' rects and ids are read (at random) before the first store
+    // the items' rects and ids are read (at random) before the first store,
+    // unconditionally (positions past the tile's count read entry 0, which
+    // exists since n >= 1, and store nothing) and into named registers:
+    // loads under a branch or into an array of rects were issued one at a
+    // time and parked in scratch memory
+    uint32_t src[RS_ITEMS], idv[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int i = tid + j * RS_THREADS;
-        if (i >= count) continue;
-        const uint64_t v = stage[i];
-        const uint32_t id = static_cast<uint32_t>(v);
-        item[j] = v;
-        rank[j] = gat.iin ? gat.iin[id] : id;
+        item[j] = i < count ? stage[i] : 0ull;
+        src[j] = static_cast<uint32_t>(item[j]);
     }
-    float4 rc[RS_ITEMS];
+    static_assert(RS_ITEMS == 4, "the gather names four rects per thread");
+    const float4 rc0 = gat.rin.at(src[0]), rc1 = gat.rin.at(src[1]), rc2 = gat.rin.at(src[2]),
+                 rc3 = gat.rin.at(src[3]);
+    if (gat.iin) {
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j)
-        if (tid + j * RS_THREADS < count) rc[j] = gat.rin[static_cast<uint32_t>(item[j])];
+        for (int j = 0; j < RS_ITEMS; ++j) idv[j] = gat.iin[src[j]];
+    } else {
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
+        for (int j = 0; j < RS_ITEMS; ++j) idv[j] = src[j];
+    }
+    auto put = [&](int j, const float4 &rc) {
         const int i = tid + j * RS_THREADS;
-        if (i >= count) continue;
-        const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
-        const uint32_t pos = goff[d] + (i - dstart[d]);
-        out[pos] = item[j];
-        gat.rout[pos] = rc[j];
-        gat.iout[pos] = rank[j];
-    }
+        if (i < count) {
+            const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
+            const uint32_t pos = goff[d] + (i - dstart[d]);
+            out[pos] = item[j];
+            gat.rout[pos] = rc;
+            gat.iout[pos] = idv[j];
+        }
+    };
+    put(0, rc0);
+    put(1, rc1);
+    put(2, rc2);
+    put(3, rc3);
 }
 
 struct SortLayout {
@@ -405,12 +430,12 @@ int radix_carve(Carver &c, int64_t n, int64_t seg, RadixWs &w) {
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 
-int64_t radix_ghist_words(int64_t n, int64_t seg) {
-    return sort_layout(make_tiling(n > 0 ? n : 1, seg)).ghist_words;
+int64_t radix_ghist_words(int64_t n, int64_t seg, int64_t layer) {
+    return sort_layout(make_tiling(n > 0 ? n : 1, seg, layer)).ghist_words;
 }
 
-int64_t radix_status_words(int64_t n, int64_t seg) {
-    return sort_layout(make_tiling(n > 0 ? n : 1, seg)).status_words;
+int64_t radix_status_words(int64_t n, int64_t seg, int64_t layer) {
+    return sort_layout(make_tiling(n > 0 ? n : 1, seg, layer)).status_words;
 }
 
 
@@ -418,7 +443,8 @@ namespace {
 template <class Src>
 int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n,
                     int64_t seg, int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s,
-                    uint64_t **result, bool zeroed, const PassGather &gat = PassGather{}) {
+                    uint64_t **result, bool zeroed, const PassGather &gat = PassGather{},
+                    int64_t layer = 0) {
     PRT_REQUIRE(n >= 0 && n < (int64_t(1) << 32), PRT_E_INVAL, "radix sort: n=%lld out of range",
                 (long long)n);
     PRT_REQUIRE(bit_lo >= 0 && bit_hi <= 64 && bit_lo < bit_hi && bit_hi - bit_lo <= 8 * RS_MAX_PASSES,
@@ -427,7 +453,10 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
         *result = buf0;
         return PRT_OK;
     }
-    const Tiling t = make_tiling(n, seg);
+    PRT_REQUIRE(layer <= 0 || n % layer == 0, PRT_E_INVAL,
+                "radix sort: %lld items are not whole layers of %lld", (long long)n,
+                (long long)layer);
+    const Tiling t = make_tiling(n, seg, layer);
     const SortLayout l = sort_layout(t);
     PRT_REQUIRE(l.ghist_words <= w.ghist_cap && l.status_words <= w.status_cap, PRT_E_NOMEM,
                 "radix sort: workspace carved for a smaller tiling");
@@ -449,7 +478,7 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
     for (int p = 0; p < passes; ++p) {
         const int shift = bit_lo + 8 * p;
         const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
-        const PassGather none{nullptr, nullptr, nullptr, nullptr};
+        const PassGather none{};
         PRT_LAUNCH(k_onesweep, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
                    ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS,
                    w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS, tickets + p,
@@ -469,12 +498,14 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
                            false);
 }
 
-int radix_sort_td_coord(const float4 *rects, int comp, uint64_t *buf0, uint64_t *buf1, int64_t n,
-                        int64_t seg, const RadixWs &w, hipStream_t s, uint64_t **result,
-                        bool zeroed, const uint32_t *ids, float4 *rects_out, uint32_t *ids_out) {
-    const PassGather g{rects_out ? rects : nullptr, ids, rects_out, ids_out};
+int radix_sort_td_coord(const RectLayers &rects, int comp, uint64_t *buf0, uint64_t *buf1,
+                        int64_t n, int64_t seg, int64_t layer, const RadixWs &w, hipStream_t s,
+                        uint64_t **result, bool zeroed, const uint32_t *ids, float4 *rects_out,
+                        uint32_t *ids_out) {
+    PassGather g{};
+    if (rects_out) g = PassGather{rects, ids, rects_out, ids_out};
     return radix_sort_impl(SrcTdCoord{rects, buf0, comp}, buf0, buf0, buf1, n, seg, 32, 64, w, s,
-                           result, zeroed, g);
+                           result, zeroed, g, layer);
 }
 
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,

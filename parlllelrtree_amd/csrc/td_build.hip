@@ -95,7 +95,9 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_lds_sort(
 
 // Fused lower levels for power-of-two fanouts.  One group = one segment of
 // the first level whose segments fit LDS (a0 <= 4096 items, padded to
-// A2 = next_pow2(a0); a block holds 4096 / A2 groups).  The kernel
+// A2 = next_pow2(a0); a block holds 4096 / A2 groups of one layer).  Items,
+// rects and ids of all layers sit side by side (layer l at positions
+// l*n ..), ids are positions in that joint order.  The kernel
 //   1. gathers the group's rects in the order of the previous level's sort
 //      (src = low word of its sorted items, or the identity at level 0),
 //   2. sorts the group for every remaining level p0..L-1 in LDS: level p
@@ -110,16 +112,17 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_lds_sort(
 // the level-by-level path (unions are exact, so bit-identical).
 struct SubtreeArgs {
     const uint64_t *items;   // sorted items of level p0 - 1, or null (p0 == 0)
-    const float4 *rect_cur;  // rects in the order the items index
+    RectLayers rect_cur;     // rects in the order the items index
     const uint32_t *id_cur;  // ids in that order, or null (identity)
-    int64_t n, a0;
+    int64_t n, a0;           // n: rects per layer
     int A2, L, p0, f, logf;
     int64_t seg[PRT_MAX_LEVELS];        // a_p of the sort of level p
     int64_t off[PRT_MAX_LEVELS + 1];    // slot offsets per level
-    int32_t *slot_child;
-    float4 *slot_mbr;
-    uint32_t *perm;
-    int64_t num_groups;
+    int32_t *slot_child[PRT_MAX_LAYERS];
+    float4 *slot_mbr[PRT_MAX_LAYERS];
+    uint32_t *perm[PRT_MAX_LAYERS];
+    int64_t num_groups;                 // per layer
+    int64_t blocks_per_layer;
 };
 
 // Thread t holds the keys of positions 4t..4t+3 in registers.  A bitonic
@@ -312,9 +315,15 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
     // block b runs on XCD b % 8; XCD x gets a contiguous range of groups,
     // whose random gathers then stay within a few segments that fit its L2
     const int64_t nb = gridDim.x, xq = nb / 8, xr = nb % 8, xcd = blockIdx.x % 8;
-    const int64_t blk = xcd * xq + (xcd < xr ? xcd : xr) + blockIdx.x / 8;
-    const int64_t g0 = blk * spb;  // first group of the block
-    const bool last_block = blk == nb - 1;
+    const int64_t blk_all = xcd * xq + (xcd < xr ? xcd : xr) + blockIdx.x / 8;
+    const int layer = static_cast<int>(blk_all / A.blocks_per_layer);
+    const int64_t blk = blk_all - layer * A.blocks_per_layer;
+    const int64_t base = layer * A.n;  // first joint position of the layer
+    int32_t *const slot_child = A.slot_child[layer];
+    float4 *const slot_mbr = A.slot_mbr[layer];
+    uint32_t *const perm = A.perm[layer];
+    const int64_t g0 = blk * spb;  // first group of the block (in its layer)
+    const bool last_block = blk == A.blocks_per_layer - 1;
     // 1. gather the groups' rects and ids into LDS (local index k), in the
     //    order of the previous level's sort; every index load of the thread
     //    is issued before the dependent rect and id loads (two memory
@@ -326,14 +335,14 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
         const int k = tid + m * LS_THREADS;
         const int64_t gg = g0 + k / A.A2, w = k % A.A2, pos = gg * A.a0 + w;
         live[m] = gg < A.num_groups && w < A.a0 && pos < A.n;
-        sidx[m] = static_cast<uint32_t>(pos);
-        if (live[m] && A.items) sidx[m] = static_cast<uint32_t>(A.items[pos]);
+        sidx[m] = static_cast<uint32_t>(base + pos);
+        if (live[m] && A.items) sidx[m] = static_cast<uint32_t>(A.items[base + pos]);
     }
     float4 rcs[ST_PER];
     uint32_t idv[ST_PER];
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
-        rcs[m] = live[m] ? A.rect_cur[sidx[m]] : empty_box();
+        rcs[m] = live[m] ? A.rect_cur.at(sidx[m]) : empty_box();
         idv[m] = live[m] ? (A.id_cur ? A.id_cur[sidx[m]] : sidx[m]) : 0xFFFFFFFFu;
     }
 #pragma unroll
@@ -399,19 +408,19 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
         float4 rc = empty_box();
         if (o != ST_NONE) {
             rc = rect[o];
-            A.perm[dst] = ids[o];
+            perm[dst] = ids[o] - static_cast<uint32_t>(base);
         }
         mine[m] = rc;
         if (gg < A.num_groups && w < A.a0 && dst < cntL) {
-            A.slot_child[offL + dst] = dst < A.n ? static_cast<int32_t>(-dst - 1) : 0;
-            A.slot_mbr[offL + dst] = rc;
+            slot_child[offL + dst] = dst < A.n ? static_cast<int32_t>(-dst - 1) : 0;
+            slot_mbr[offL + dst] = rc;
         }
     }
     // leaf slots past the last group are empty
     if (last_block) {
         for (int64_t i = A.num_groups * A.a0 + tid; i < cntL; i += LS_THREADS) {
-            A.slot_child[offL + i] = 0;
-            A.slot_mbr[offL + i] = empty_box();
+            slot_child[offL + i] = 0;
+            slot_mbr[offL + i] = empty_box();
         }
     }
     __syncthreads();
@@ -445,16 +454,16 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
             const int64_t i = (gg * A.a0) / cover + x % epg;  // slot index in level q
             if (gg < A.num_groups && i < cnt && (x % epg) * cover < A.a0) {
                 const bool full = i * cover < A.n;
-                A.slot_child[A.off[q] + i] = full ? static_cast<int32_t>(A.off[q + 1] + i * A.f) : 0;
-                A.slot_mbr[A.off[q] + i] = full ? u : empty_box();
+                slot_child[A.off[q] + i] = full ? static_cast<int32_t>(A.off[q + 1] + i * A.f) : 0;
+                slot_mbr[A.off[q] + i] = full ? u : empty_box();
             }
         }
         // slots of level q past the last group's entries are empty
         if (last_block) {
             const int64_t first = ceil_div(A.num_groups * A.a0, cover);
             for (int64_t i = first + tid; i < cnt; i += LS_THREADS) {
-                A.slot_child[A.off[q] + i] = 0;
-                A.slot_mbr[A.off[q] + i] = empty_box();
+                slot_child[A.off[q] + i] = 0;
+                slot_mbr[A.off[q] + i] = empty_box();
             }
         }
         __syncthreads();
@@ -501,9 +510,12 @@ struct TopArgs {
     int64_t n, a;
     int f, p_top;
     int64_t off[PRT_MAX_LEVELS + 1];
+    int32_t *slot_child[PRT_MAX_LAYERS];  // block l packs layer l
+    float4 *slot_mbr[PRT_MAX_LAYERS];
 };
-__global__ __launch_bounds__(TOP_THREADS) void k_td_top(TopArgs T, int32_t *__restrict__ slot_child,
-                                                        float4 *__restrict__ slot_mbr) {
+__global__ __launch_bounds__(TOP_THREADS) void k_td_top(TopArgs T) {
+    int32_t *const slot_child = T.slot_child[blockIdx.x];
+    float4 *const slot_mbr = T.slot_mbr[blockIdx.x];
     int64_t a = T.a;
     for (int p = T.p_top; p >= 0; --p) {
         const int64_t off = T.off[p], cnt = T.off[p + 1] - T.off[p], off_next = T.off[p + 1];
@@ -579,37 +591,168 @@ int64_t td_seg(int64_t n, int f, int L, int p) {
 }
 // Radix levels (segments larger than LS_SLOTS): look-back words of the
 // largest tiling, shared by the levels in turn, and counter words of every
-// level side by side, so that one launch clears all of them.
+// level side by side, so that one launch clears all of them.  `nl` layers of
+// n rects sort together.
 struct TdRadixWords {
     int64_t status = 0, ghist = 0;
 };
-TdRadixWords td_radix_words(int64_t n, int f, int L) {
+TdRadixWords td_radix_words(int64_t n, int f, int L, int nl) {
     TdRadixWords w;
     for (int p = 0; p < L; ++p) {
         const int64_t a = td_seg(n, f, L, p);
         if (a <= LS_SLOTS) continue;
-        const int64_t st = radix_status_words(n, a);
+        const int64_t st = radix_status_words(nl * n, a, n);
         if (st > w.status) w.status = st;
-        w.ghist += radix_ghist_words(n, a);
+        w.ghist += radix_ghist_words(nl * n, a, n);
     }
     return w;
+}
+
+// Workspace of a build of nl layers of n rects (L levels): items, rects and
+// ids twice each, look-back and counter words.
+size_t td_ws_bytes(int nl, int64_t n, int f, int L) {
+    const int64_t N = nl * n;
+    Sizer s;
+    s.take<uint64_t>(N);
+    s.take<uint64_t>(N);
+    s.take<float4>(N);
+    s.take<float4>(N);
+    s.take<uint32_t>(N);
+    s.take<uint32_t>(N);
+    const TdRadixWords w = td_radix_words(n, f, L, nl);
+    s.take<uint64_t>(w.status);
+    s.take<uint32_t>(w.ghist);
+    return s.bytes();
+}
+
+// Power-of-two fanouts: the radix levels, then one fused launch for the LDS
+// levels and their slots, then the levels above.  `in` holds nl layers of n
+// rects (planned trees[0..nl-1], all alike); every launch covers all layers.
+int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *trees, void *ws,
+                  size_t ws_bytes, hipStream_t s) {
+    const int64_t N = nl * n;
+    const int L = trees[0].num_levels;
+    Carver c(ws, ws_bytes);
+    uint64_t *ia = c.take<uint64_t>(N), *ib = c.take<uint64_t>(N);
+    float4 *ra = c.take<float4>(N), *rb = c.take<float4>(N);
+    uint32_t *da = c.take<uint32_t>(N), *db = c.take<uint32_t>(N);
+    const TdRadixWords rwords = td_radix_words(n, f, L, nl);
+    uint64_t *status = c.take<uint64_t>(rwords.status);
+    uint32_t *ghist = c.take<uint32_t>(rwords.ghist);
+    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "td_build: workspace too small");
+
+    RectLayers cur_r = in;
+    const uint32_t *cur_i = nullptr;  // identity at level 0
+    int p0 = L - 1;
+    for (int p = 0; p < L; ++p)
+        if (td_seg(n, f, L, p) <= LS_SLOTS) {
+            p0 = p;
+            break;
+        }
+    // the counters of every radix level are cleared by one launch.  Between
+    // two radix levels the rects and ids move into the order of the finished
+    // level (in the last digit pass), so the next level's sort and the
+    // subtree kernel read neighbouring rects from a few segments.
+    if (rwords.ghist > 0) {
+        const int64_t zw = rwords.ghist;
+        const unsigned zg = static_cast<unsigned>(ceil_div(zw, 1024) < 1024 ? ceil_div(zw, 1024) : 1024);
+        PRT_LAUNCH(k_zero_words, zg, 1024, 0, s, ghist, zw);
+        PRT_LAUNCHED();
+    }
+    const uint64_t *items = nullptr;
+    uint32_t *lg = ghist;
+    for (int p = 0; p < p0; ++p) {
+        const int64_t a = td_seg(n, f, L, p);
+        const RadixWs lw{lg, status, radix_ghist_words(N, a, n), rwords.status};
+        lg += lw.ghist_cap;
+        uint64_t *sorted = nullptr;
+        int rc;
+        if (p + 1 < p0) {
+            // the last digit pass also moves the rects and ids into this
+            // level's order for the next level
+            float4 *nxt_r = (cur_r.p[0] == ra) ? rb : ra;
+            uint32_t *nxt_i = (cur_i == da) ? db : da;
+            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, N, a, n, lw, s, &sorted, true, cur_i,
+                                     nxt_r, nxt_i);
+            if (rc) return rc;
+            cur_r = one_layer(nxt_r, N);
+            cur_i = nxt_i;
+        } else {
+            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, N, a, n, lw, s, &sorted, true);
+            if (rc) return rc;
+            items = sorted;  // the subtree kernel gathers through them
+        }
+    }
+    SubtreeArgs A{};
+    A.items = items;
+    A.rect_cur = cur_r;
+    A.id_cur = cur_i;
+    A.n = n;
+    A.a0 = td_seg(n, f, L, p0);
+    A.A2 = static_cast<int>(next_pow2(A.a0));
+    A.L = L;
+    A.p0 = p0;
+    A.f = f;
+    A.logf = 0;
+    while ((1 << A.logf) < f) ++A.logf;
+    for (int p = 0; p < PRT_MAX_LEVELS; ++p) A.seg[p] = p < L ? td_seg(n, f, L, p) : 0;
+    const int64_t *off = trees[0].level_offset;
+    for (int p = 0; p <= PRT_MAX_LEVELS; ++p) A.off[p] = off[p];
+    for (int l = 0; l < nl; ++l) {
+        A.slot_child[l] = trees[l].slot_child;
+        A.slot_mbr[l] = reinterpret_cast<float4 *>(trees[l].slot_mbr);
+        A.perm[l] = trees[l].perm;
+    }
+    A.num_groups = ceil_div(n, A.a0);
+    A.blocks_per_layer = ceil_div(A.num_groups, LS_SLOTS / A.A2);
+    PRT_LAUNCH(k_td_subtree, dim3(static_cast<unsigned>(nl * A.blocks_per_layer)),
+               dim3(LS_THREADS), ST_LDS, s, A);
+    PRT_LAUNCHED();
+    int64_t a = 1;
+    for (int k = 0; k < L - p0; ++k) a *= f;  // entries of level p0 - 1 cover f^(L-p0)
+    if (p0 > 0 && off[p0] <= 64 * TOP_THREADS) {
+        // levels p0-1 .. 0 hold few slots: one block per layer packs them all
+        TopArgs T{};
+        T.n = n;
+        T.a = a;
+        T.f = f;
+        T.p_top = p0 - 1;
+        for (int p = 0; p <= PRT_MAX_LEVELS; ++p) T.off[p] = off[p];
+        for (int l = 0; l < nl; ++l) {
+            T.slot_child[l] = trees[l].slot_child;
+            T.slot_mbr[l] = reinterpret_cast<float4 *>(trees[l].slot_mbr);
+        }
+        PRT_LAUNCH(k_td_top, dim3(static_cast<unsigned>(nl)), dim3(TOP_THREADS), 0, s, T);
+        PRT_LAUNCHED();
+    } else {
+        for (int l = 0; l < nl; ++l) {
+            float4 *sm = reinterpret_cast<float4 *>(trees[l].slot_mbr);
+            int64_t al = a;
+            for (int p = p0 - 1; p >= 0; --p) {
+                const int64_t cnt = off[p + 1] - off[p];
+                PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0,
+                           s, n, f, al, off[p], cnt, off[p + 1], trees[l].slot_child, sm);
+                PRT_LAUNCHED();
+                al *= f;
+            }
+        }
+    }
+    return PRT_OK;
 }
 }  // namespace
 
 extern "C" size_t prt_td_workspace_size(int64_t n, int fanout) {
     prt_td_tree t;
     if (prt_td_plan(n, fanout, &t) != PRT_OK) return 0;
-    Sizer s;
-    s.take<uint64_t>(n);
-    s.take<uint64_t>(n);
-    s.take<float4>(n);
-    s.take<float4>(n);
-    s.take<uint32_t>(n);
-    s.take<uint32_t>(n);
-    const TdRadixWords w = td_radix_words(n, fanout, t.num_levels);
-    s.take<uint64_t>(w.status);
-    s.take<uint32_t>(w.ghist);
-    return s.bytes();
+    return td_ws_bytes(1, n, fanout, t.num_levels);
+}
+
+extern "C" size_t prt_td_layers_workspace_size(int num_layers, int64_t n, int fanout) {
+    prt_td_tree t;
+    if (num_layers < 1 || num_layers > PRT_MAX_LAYERS) return 0;
+    if (prt_td_plan(n, fanout, &t) != PRT_OK) return 0;
+    if ((fanout & (fanout - 1)) != 0) return td_ws_bytes(1, n, fanout, t.num_levels);
+    return td_ws_bytes(num_layers, n, fanout, t.num_levels);
 }
 
 extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tree *tree, void *ws,
@@ -621,12 +764,15 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
     size_t need = prt_td_workspace_size(n, fanout);
     PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "td_build: workspace %zu < %zu bytes", ws_bytes, need);
     hipStream_t s = as_stream(stream);
+    if ((fanout & (fanout - 1)) == 0)
+        return td_build_pow2(1, one_layer(reinterpret_cast<const float4 *>(rects), n), n, fanout,
+                             tree, ws, ws_bytes, s);
     Carver c(ws, ws_bytes);
     uint64_t *ia = c.take<uint64_t>(n), *ib = c.take<uint64_t>(n);
     float4 *ra = c.take<float4>(n), *rb = c.take<float4>(n);
     uint32_t *da = c.take<uint32_t>(n), *db = c.take<uint32_t>(n);
     const int L = tree->num_levels, f = fanout;
-    const TdRadixWords rwords = td_radix_words(n, f, L);
+    const TdRadixWords rwords = td_radix_words(n, f, L, 1);
     uint64_t *status = c.take<uint64_t>(rwords.status);
     uint32_t *ghist = c.take<uint32_t>(rwords.ghist);
     PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "td_build: workspace too small");
@@ -636,94 +782,6 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
     const float4 *cur_r = reinterpret_cast<const float4 *>(rects);
     const uint32_t *cur_i = nullptr;  // identity at level 0
     const unsigned grid = static_cast<unsigned>(ceil_div(n, 256));
-    if ((f & (f - 1)) == 0) {
-        // radix levels, then one fused launch for the LDS levels and their slots
-        int p0 = L - 1;
-        for (int p = 0; p < L; ++p)
-            if (td_seg(n, f, L, p) <= LS_SLOTS) {
-                p0 = p;
-                break;
-            }
-        // the counters of every radix level are cleared by one launch.
-        // Between two radix levels the rects and ids move into the order of
-        // the finished level (in the last digit pass), so the next level's
-        // sort and the subtree kernel read neighbouring rects from a few
-        // segments.
-        if (rwords.ghist > 0) {
-            const int64_t zw = rwords.ghist;
-            const unsigned zg = static_cast<unsigned>(ceil_div(zw, 1024) < 1024 ? ceil_div(zw, 1024) : 1024);
-            PRT_LAUNCH(k_zero_words, zg, 1024, 0, s, ghist, zw);
-            PRT_LAUNCHED();
-        }
-        const uint64_t *items = nullptr;
-        uint32_t *lg = ghist;
-        for (int p = 0; p < p0; ++p) {
-            const int64_t a = td_seg(n, f, L, p);
-            const RadixWs lw{lg, status, radix_ghist_words(n, a), rwords.status};
-            lg += lw.ghist_cap;
-            uint64_t *sorted = nullptr;
-            if (p + 1 < p0) {
-                // the last digit pass also moves the rects and ids into this
-                // level's order for the next level
-                float4 *nxt_r = (cur_r == ra) ? rb : ra;
-                uint32_t *nxt_i = (cur_i == da) ? db : da;
-                rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, lw, s, &sorted, true, cur_i,
-                                         nxt_r, nxt_i);
-                if (rc) return rc;
-                cur_r = nxt_r;
-                cur_i = nxt_i;
-            } else {
-                rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, n, a, lw, s, &sorted, true);
-                if (rc) return rc;
-                items = sorted;  // the subtree kernel gathers through them
-            }
-        }
-        SubtreeArgs A;
-        A.items = items;
-        A.rect_cur = cur_r;
-        A.id_cur = cur_i;
-        A.n = n;
-        A.a0 = td_seg(n, f, L, p0);
-        A.A2 = static_cast<int>(next_pow2(A.a0));
-        A.L = L;
-        A.p0 = p0;
-        A.f = f;
-        A.logf = 0;
-        while ((1 << A.logf) < f) ++A.logf;
-        for (int p = 0; p < PRT_MAX_LEVELS; ++p) A.seg[p] = p < L ? td_seg(n, f, L, p) : 0;
-        for (int p = 0; p <= PRT_MAX_LEVELS; ++p) A.off[p] = tree->level_offset[p];
-        A.slot_child = tree->slot_child;
-        A.slot_mbr = reinterpret_cast<float4 *>(tree->slot_mbr);
-        A.perm = tree->perm;
-        A.num_groups = ceil_div(n, A.a0);
-        const int64_t blocks = ceil_div(A.num_groups, LS_SLOTS / A.A2);
-        PRT_LAUNCH(k_td_subtree, dim3(static_cast<unsigned>(blocks)), dim3(LS_THREADS), ST_LDS, s, A);
-        PRT_LAUNCHED();
-        float4 *sm = reinterpret_cast<float4 *>(tree->slot_mbr);
-        const int64_t *off = tree->level_offset;
-        int64_t a = 1;
-        for (int k = 0; k < L - p0; ++k) a *= f;  // entries of level p0 - 1 cover f^(L-p0)
-        if (p0 > 0 && off[p0] <= 64 * TOP_THREADS) {
-            // levels p0-1 .. 0 hold few slots: one block packs them all
-            TopArgs T;
-            T.n = n;
-            T.a = a;
-            T.f = f;
-            T.p_top = p0 - 1;
-            for (int p = 0; p <= PRT_MAX_LEVELS; ++p) T.off[p] = off[p];
-            PRT_LAUNCH(k_td_top, dim3(1), dim3(TOP_THREADS), 0, s, T, tree->slot_child, sm);
-            PRT_LAUNCHED();
-        } else {
-            for (int p = p0 - 1; p >= 0; --p) {
-                const int64_t cnt = off[p + 1] - off[p];
-                PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0,
-                           s, n, f, a, off[p], cnt, off[p + 1], tree->slot_child, sm);
-                PRT_LAUNCHED();
-                a *= f;
-            }
-        }
-        return PRT_OK;
-    }
     for (int p = 0; p < L; ++p) {
         const int comp = p & 1;  // x1 on even levels, y1 on odd (TopDown-rtree.cu:574-586)
         const int64_t a = td_seg(n, f, L, p);
@@ -765,4 +823,35 @@ extern "C" int prt_td_build(const float *rects, int64_t n, int fanout, prt_td_tr
         PRT_LAUNCHED();
     }
     return PRT_OK;
+}
+
+extern "C" int prt_td_build_layers(int num_layers, const float *const *rects, int64_t n, int fanout,
+                                   prt_td_tree *trees, void *ws, size_t ws_bytes, void *stream) {
+    PRT_REQUIRE(num_layers >= 1 && num_layers <= PRT_MAX_LAYERS, PRT_E_INVAL,
+                "td_build_layers: %d layers not in [1,%d]", num_layers, PRT_MAX_LAYERS);
+    PRT_REQUIRE(rects && trees, PRT_E_INVAL, "td_build_layers: null host array");
+    PRT_REQUIRE(n >= 1 && n * num_layers < (int64_t(1) << 31), PRT_E_INVAL,
+                "td_build_layers: %d layers of %lld rects out of range", num_layers, (long long)n);
+    for (int l = 0; l < num_layers; ++l) {
+        const int rc = prt_td_plan(n, fanout, &trees[l]);
+        if (rc) return rc;
+        PRT_REQUIRE(rects[l] && trees[l].slot_child && trees[l].slot_mbr && trees[l].perm,
+                    PRT_E_INVAL, "td_build_layers: null device pointer in layer %d", l);
+    }
+    const size_t need = prt_td_layers_workspace_size(num_layers, n, fanout);
+    PRT_REQUIRE(ws_bytes >= need, PRT_E_NOMEM, "td_build_layers: workspace %zu < %zu bytes",
+                ws_bytes, need);
+    if ((fanout & (fanout - 1)) != 0) {
+        // the level-by-level path builds one layer after another
+        for (int l = 0; l < num_layers; ++l) {
+            const int rc = prt_td_build(rects[l], n, fanout, &trees[l], ws, ws_bytes, stream);
+            if (rc) return rc;
+        }
+        return PRT_OK;
+    }
+    RectLayers in{};
+    for (int l = 0; l < num_layers; ++l) in.p[l] = reinterpret_cast<const float4 *>(rects[l]);
+    in.len = n;
+    in.count = num_layers;
+    return td_build_pow2(num_layers, in, n, fanout, trees, ws, ws_bytes, as_stream(stream));
 }

@@ -179,6 +179,45 @@ def build_top_down(rects: torch.Tensor, fanout: int = 16, workspace: torch.Tenso
     return t
 
 
+class TopDownLayers:
+    """Top-down trees of several layers of equal size, built by one shared
+    launch sequence (prt_td_build_layers): every launch covers all layers.
+    trees[l] equals build_top_down(layers[l])."""
+
+    def __init__(self, layers: list[torch.Tensor], fanout: int = 16):
+        from ._lib import PRT_MAX_LAYERS
+        if not 1 <= len(layers) <= PRT_MAX_LAYERS:
+            raise ValueError(f"1..{PRT_MAX_LAYERS} layers, got {len(layers)}")
+        layers = [_cuda_f32_rects(r) for r in layers]
+        n, dev = layers[0].shape[0], layers[0].device
+        if any(r.shape[0] != n for r in layers):
+            raise ValueError("layers of unequal size")
+        nl = len(layers)
+        self.fanout = fanout
+        self.n = n
+        self.sts = (TDTreeStruct * nl)()
+        self.trees = []
+        for l in range(nl):
+            check(lib().prt_td_plan(n, fanout, ctypes.byref(self.sts[l])))
+            ns = self.sts[l].num_slots
+            t = TopDownTree(self.sts[l], torch.empty(ns, dtype=torch.int32, device=dev),
+                            torch.empty((ns, 4), dtype=torch.float32, device=dev),
+                            torch.empty(n, dtype=torch.int32, device=dev))
+            for name in ("slot_child", "slot_mbr", "perm"):
+                setattr(self.sts[l], name, getattr(t, name).data_ptr())
+            self.trees.append(t)
+        self.workspace = _ws(lib().prt_td_layers_workspace_size(nl, n, fanout), dev)
+        self.rebuild(layers)
+
+    def rebuild(self, layers: list[torch.Tensor]) -> None:
+        """Rebuild every tree in place from layers of the same sizes."""
+        if len(layers) != len(self.trees) or any(r.shape[0] != self.n for r in layers):
+            raise ValueError("layers differ from the ones the trees were planned for")
+        ptrs = (ctypes.c_void_p * len(layers))(*[r.data_ptr() for r in layers])
+        check(lib().prt_td_build_layers(len(layers), ptrs, self.n, self.fanout, self.sts,
+                                        _ptr(self.workspace), self.workspace.numel(), _stream()))
+
+
 class Joiner:
     """Preallocated join of a fixed query count: run() only launches."""
 

@@ -56,6 +56,26 @@ def test_plans_match_oracle(n, f):
     assert lib.prt_td_workspace_size(n, f) > 0
 
 
+def test_td_layers_workspace_and_argument_checks():
+    lib = _lib.lib()
+    for n, f in ((1, 16), (70001, 16), (1 << 20, 16), (1000, 10), (300000, 2)):
+        one = lib.prt_td_workspace_size(n, f)
+        assert lib.prt_td_layers_workspace_size(1, n, f) == one
+        two = lib.prt_td_layers_workspace_size(2, n, f)
+        # power-of-two fanouts sort the layers together; others reuse one
+        # layer's workspace layer after layer
+        assert two >= 2 * one - 4096 if f & (f - 1) == 0 else two == one
+    assert lib.prt_td_layers_workspace_size(0, 100, 16) == 0
+    assert lib.prt_td_layers_workspace_size(_lib.PRT_MAX_LAYERS + 1, 100, 16) == 0
+    trees = (_lib.TDTreeStruct * 2)()
+    ptrs = (ctypes.c_void_p * 2)()
+    assert lib.prt_td_build_layers(0, ptrs, 100, 16, trees, None, 0, None) == -1
+    assert b"layers not in" in lib.prt_last_error()
+    assert lib.prt_td_build_layers(2, ptrs, 1 << 30, 16, trees, None, 0, None) == -1
+    assert lib.prt_td_build_layers(2, ptrs, 100, 16, trees, None, 0, None) == -1
+    assert b"null device pointer" in lib.prt_last_error()
+
+
 def test_known_layout_sizes():
     lib = _lib.lib()
     st = _lib.BUTreeStruct()

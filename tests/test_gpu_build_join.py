@@ -122,6 +122,57 @@ def test_td_build_parity(dev, n, f):
     check_td(t, O.td_build(r, f))
 
 
+@pytest.mark.parametrize("nl,n,f", [(1, 1, 16), (2, 1, 16), (2, 17, 16), (3, 300, 4), (2, 5000, 16),
+                                    (4, 70000, 16), (2, 70001, 16), (3, 1000, 10), (2, 200000, 8),
+                                    (2, 300000, 2), (2, 1 << 20, 16)])
+def test_td_layers_parity(dev, nl, n, f):
+    """Layers built by one shared launch sequence: each tree bit-exact
+    against the oracle's build of that layer alone; then rebuilt in place
+    from other layers."""
+    layers = [rects(n, 11 * n + l) for l in range(nl)]
+    TL = rtree.TopDownLayers([dev_rects(r) for r in layers], f)
+    for t, r in zip(TL.trees, layers):
+        check_td(t, O.td_build(r, f))
+    other = [rects(n, 13 * n + 5 + l) for l in range(nl)]
+    TL.rebuild([dev_rects(r) for r in other])
+    for t, r in zip(TL.trees, other):
+        check_td(t, O.td_build(r, f))
+
+
+def test_td_layers_equal_single_builds_16M(dev):
+    """Two layers of 16,777,216 clustered rects (three radix levels, the
+    first two moving the rects): equal to the single-layer builds."""
+    from parlllelrtree_amd import workloads
+    a = torch.from_numpy(workloads.w4_rects(1 << 24)).cuda()
+    b = a.flip(0).contiguous()
+    TL = rtree.TopDownLayers([a, b], 16)
+    for t, r in zip(TL.trees, (a, b)):
+        s = rtree.build_top_down(r, 16)
+        assert torch.equal(t.perm, s.perm)
+        assert torch.equal(t.slot_child, s.slot_child)
+        assert torch.equal(t.slot_mbr.view(torch.int32), s.slot_mbr.view(torch.int32))
+        del s
+    with pytest.raises(ValueError):
+        rtree.TopDownLayers([a, b[:10]], 16)
+
+
+def test_kernel_times_of_layered_build(dev):
+    """Per-launch timing (hipExtLaunchKernel events): one launch of each
+    kernel covers both layers; the profiled launches build the same trees."""
+    from parlllelrtree_amd.profiling import KernelTimes
+    layers = [rects(1 << 20, s) for s in (21, 22)]
+    TL = rtree.TopDownLayers([dev_rects(r) for r in layers], 16)
+    with KernelTimes() as kt:
+        TL.rebuild([dev_rects(r) for r in layers])
+        torch.cuda.synchronize()
+    assert kt.stats["k_onesweep"][0] == 8          # 4 digit passes x 2 radix levels
+    assert kt.stats["k_hist_all<Src>"][0] == 2
+    assert kt.stats["k_td_subtree"][0] == 1 and kt.stats["k_td_top"][0] == 1
+    assert all(0 < ms < 50 for _, ms in kt.stats.values())
+    for t, r in zip(TL.trees, layers):
+        check_td(t, O.td_build(r, 16))
+
+
 def test_td_build_duplicate_coords(dev):
     r = rects(30000, 5)
     r[:, 0] = np.round(r[:, 0] * 50) / 50   # many equal x1
