@@ -126,67 +126,41 @@ struct SubtreeArgs {
 };
 
 // Thread t holds the keys of positions 4t..4t+3 in registers.  A bitonic
-// compare-exchange at distance j pairs registers of one thread (j < 4),
-// lanes of one wave (4 <= j < 256: exchanged through the wave's own LDS
-// slots), or positions of different waves (j >= 256: through LDS with block
-// barriers), so a 4096-key sort needs 14 block barriers instead of 78.
+// compare-exchange at distance j pairs registers of one thread (j < 4) or
+// lanes of one wave (4 <= j < 256: lane t ^ (j/4), by cross-lane shuffles),
+// so blocks of up to 256 positions sort without LDS or block barriers;
+// larger blocks go to block_radix.
 constexpr int ST_PER = LS_SLOTS / LS_THREADS;  // 4 keys per thread
 
-// dynamic LDS: rect (64 KB) | id (16 KB) | keyx (32 KB; the radix sort's two
-// key arrays alias it) | ord and its radix twin (2 x 8 KB) | per-wave digit
-// counters (16 KB); the rect array holds the boxes in sorted order once the
-// leaves are written
+// dynamic LDS, 72 KB so that two blocks share a CU: kx | ky (ord32 of x1 and
+// y1 per local index, 2 x 16 KB) | src (joint position of each local index,
+// 16 KB) | ord and its radix twin (2 x 8 KB) | per-wave 16-bit digit
+// counters (8 KB).  Once the sorts are done the kx/ky region holds the boxes
+// of the block's upper levels (at most 2048).
 constexpr int ST_WAVES = LS_THREADS / 64;
-constexpr size_t ST_LDS = LS_SLOTS * (16 + 4 + 8 + 2 + 2) + ST_WAVES * 256 * 4;
+constexpr size_t ST_LDS = LS_SLOTS * (4 + 4 + 4 + 2 + 2) + ST_WAVES * 256 * 2;
 constexpr uint16_t ST_NONE = 0xFFFFu;  // ord entry of a padding position
+constexpr int ST_RADIX_MIN = 512;       // blocks of this many positions or more: block_radix
 
-
-// Sorts every aligned block of B positions (B a power of two <= 4096)
+// Sorts every aligned block of B positions (B a power of two <= 256)
 // ascending; position of register r of thread t is 4t + r.
-__device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B, uint64_t *keyx) {
+__device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B) {
     const int t = threadIdx.x;
     for (int k = 2; k <= B; k <<= 1) {
         int j = k >> 1;
-        if (j >= 256) {
-            for (int r = 0; r < ST_PER; ++r) keyx[ST_PER * t + r] = kk[r];
-            __syncthreads();
-            for (; j >= 256; j >>= 1) {
-                for (int q = t; q < LS_SLOTS / 2; q += LS_THREADS) {
-                    const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                    const int l = i | j;
-                    const bool up = (k == B) || ((i & k) == 0);
-                    const uint64_t x = keyx[i], y = keyx[l];
-                    if ((x > y) == up) {
-                        keyx[i] = y;
-                        keyx[l] = x;
-                    }
-                }
-                __syncthreads();
-            }
-            for (int r = 0; r < ST_PER; ++r) kk[r] = keyx[ST_PER * t + r];
-        }
         // 4 <= j < 256: the partners of positions 4t..4t+3 are the positions
-        // 4t'..4t'+3 of lane t' = t ^ (j/4) of the same wave, so the keys are
-        // exchanged through the wave's own 256 LDS slots (two 16-byte stores
-        // and loads per lane, no block barrier); the direction bits come from
-        // t alone, and each key takes its partner when the partner belongs on
-        // its side (one 64-bit compare and one select per key)
+        // 4t'..4t'+3 of lane t' = t ^ (j/4) of the same wave; the direction
+        // bits come from t alone, and each key takes its partner's when the
+        // partner belongs on its side (one 64-bit compare and one select)
         for (; j >= ST_PER; j >>= 1) {
             const int i0 = ST_PER * t;
             const bool want_max = ((i0 & j) == 0) != ((k == B) || ((i0 & k) == 0));
-            ulonglong2 *mine = reinterpret_cast<ulonglong2 *>(keyx + i0);
-            const ulonglong2 *theirs =
-                reinterpret_cast<const ulonglong2 *>(keyx + ST_PER * (t ^ (j / ST_PER)));
-            mine[0] = make_ulonglong2(kk[0], kk[1]);
-            mine[1] = make_ulonglong2(kk[2], kk[3]);
-            __builtin_amdgcn_wave_barrier();
-            const ulonglong2 p0 = theirs[0], p1 = theirs[1];
-            __builtin_amdgcn_wave_barrier();
-            const uint64_t o[ST_PER] = {p0.x, p0.y, p1.x, p1.y};
+            const int lx = j / ST_PER;
 #pragma unroll
             for (int r = 0; r < ST_PER; ++r) {
-                const bool take = (o[r] < kk[r]) != want_max;
-                kk[r] = take ? o[r] : kk[r];
+                const uint64_t o = __shfl_xor(kk[r], lx, 64);
+                const bool take = (o < kk[r]) != want_max;
+                kk[r] = take ? o : kk[r];
             }
         }
         for (; j > 0; j >>= 1) {
@@ -206,17 +180,17 @@ __device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B, uint64_t *ke
 }
 
 // Stable LSD radix sort (four 8-bit passes) of the positions of every
-// aligned segment of B positions (B a multiple of 256) by the 32-bit key of
-// the local index at each position: ord[pos] -> ord[pos] with keys in ka.
-// Wave w ranks positions w*256 + r*64 + lane (r = 0..3, in order) with
-// ballots over the digit bits and per-wave counters; one exclusive scan over
-// the counters in (segment, digit, wave) order gives every (wave, digit) its
-// first destination, since segments are consecutive in that order.
-// Padding positions carry key 0xFFFFFFFF and stay behind the others of their
-// segment (they come last in position order).  The same order as a sort by
-// (key, position).
-__device__ void block_radix(uint16_t *ord, uint16_t *ord2, uint32_t *ka, uint32_t *kb,
-                            uint32_t *wcnt, int B) {
+// aligned segment of B positions (B a multiple of 256) by the 32-bit key
+// key_of[local index] of the index at each position: ord -> ord (ord2 is
+// the twin buffer).  Wave w ranks positions w*256 + r*64 + lane (r = 0..3,
+// in order) with ballots over the digit bits and per-wave counters; one
+// exclusive scan over the counters in (segment, digit, wave) order gives
+// every (wave, digit) its first destination, since segments are
+// consecutive in that order.  Padding positions carry key 0xFFFFFFFF and
+// stay behind the others of their segment (they come last in position
+// order).  The same order as a sort by (key, position).
+__device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_of, uint16_t *wcnt,
+                            int B) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wps = B / 256;  // waves per segment
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -224,14 +198,14 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, uint32_t *ka, uint32_
     __syncthreads();
     for (int pass = 0; pass < 4; ++pass) {
         const int shift = 8 * pass;
-        uint32_t key[ST_PER], dig[ST_PER], rank[ST_PER];
+        uint32_t dig[ST_PER], rank[ST_PER];
         uint16_t o[ST_PER];
 #pragma unroll
         for (int r = 0; r < ST_PER; ++r) {
             const int pos = wave * 256 + r * 64 + lane;
-            key[r] = ka[pos];
             o[r] = ord[pos];
-            dig[r] = (key[r] >> shift) & 0xFFu;
+            const uint32_t key = o[r] == ST_NONE ? 0xFFFFFFFFu : key_of[o[r]];
+            dig[r] = (key >> shift) & 0xFFu;
         }
 #pragma unroll
         for (int r = 0; r < ST_PER; ++r) {
@@ -245,7 +219,8 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, uint32_t *ka, uint32_
             const uint32_t base = wcnt[wave * 256 + d];
             const uint32_t before = static_cast<uint32_t>(__popcll(peers & lt));
             __builtin_amdgcn_wave_barrier();
-            if (before == 0) wcnt[wave * 256 + d] = base + static_cast<uint32_t>(__popcll(peers));
+            if (before == 0)
+                wcnt[wave * 256 + d] = static_cast<uint16_t>(base + static_cast<uint32_t>(__popcll(peers)));
             __builtin_amdgcn_wave_barrier();
             rank[r] = base + before;
         }
@@ -276,21 +251,14 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, uint32_t *ka, uint32_
             const int e = ST_PER * tid + q;
             const int seg = e / (256 * wps), rem = e % (256 * wps);
             const int d = rem / wps, w = seg * wps + rem % wps;
-            wcnt[w * 256 + d] = before;  // first destination of (wave, digit)
+            wcnt[w * 256 + d] = static_cast<uint16_t>(before);  // first destination of (wave, digit)
             before += c[q];
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < ST_PER; ++r) {
-            const uint32_t dst = wcnt[wave * 256 + dig[r]] + rank[r];
-            kb[dst] = key[r];
-            ord2[dst] = o[r];
-        }
+        for (int r = 0; r < ST_PER; ++r) ord2[wcnt[wave * 256 + dig[r]] + rank[r]] = o[r];
         __syncthreads();
         for (int i = tid; i < ST_WAVES * 256; i += LS_THREADS) wcnt[i] = 0;
-        uint32_t *tk = ka;
-        ka = kb;
-        kb = tk;
         uint16_t *to = ord;
         ord = ord2;
         ord2 = to;
@@ -298,18 +266,48 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, uint32_t *ka, uint32_
     }
 }
 
-__global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
+__device__ inline float4 shfl_xor_box(float4 v, int x) {
+    return make_float4(__shfl_xor(v.x, x, 64), __shfl_xor(v.y, x, 64), __shfl_xor(v.z, x, 64),
+                       __shfl_xor(v.w, x, 64));
+}
+
+// Slot i of level q (entries of `cover` leaves) for box x of the block:
+// group g0 + x / epg, entry x % epg of that group.
+__device__ inline void write_entry(const SubtreeArgs &A, int32_t *slot_child, float4 *slot_mbr,
+                                   int64_t g0, int q, int64_t cover, int64_t x, float4 u) {
+    const int64_t epg = A.A2 / cover;
+    const int64_t gg = g0 + x / epg;
+    const int64_t i = (gg * A.a0) / cover + x % epg;  // slot index in level q
+    const int64_t cnt = A.off[q + 1] - A.off[q];
+    if (gg < A.num_groups && i < cnt && (x % epg) * cover < A.a0) {
+        const bool full = i * cover < A.n;
+        slot_child[A.off[q] + i] = full ? static_cast<int32_t>(A.off[q + 1] + i * A.f) : 0;
+        slot_mbr[A.off[q] + i] = full ? u : empty_box();
+    }
+}
+
+// slots of level q past the last group's entries are empty
+__device__ inline void write_empty_tail(const SubtreeArgs &A, int32_t *slot_child, float4 *slot_mbr,
+                                        int q, int64_t cover) {
+    const int64_t cnt = A.off[q + 1] - A.off[q];
+    const int64_t first = ceil_div(A.num_groups * A.a0, cover);
+    for (int64_t i = first + threadIdx.x; i < cnt; i += LS_THREADS) {
+        slot_child[A.off[q] + i] = 0;
+        slot_mbr[A.off[q] + i] = empty_box();
+    }
+}
+
+__global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_td_subtree(SubtreeArgs A) {
     extern __shared__ __align__(16) char st_lds[];
-    float4 *rect = reinterpret_cast<float4 *>(st_lds);
-    uint32_t *ids = reinterpret_cast<uint32_t *>(st_lds + LS_SLOTS * 16);
-    uint64_t *keyx = reinterpret_cast<uint64_t *>(st_lds + LS_SLOTS * 20);
-    uint32_t *ka = reinterpret_cast<uint32_t *>(keyx);  // radix keys alias keyx
-    uint32_t *kb = ka + LS_SLOTS;
-    uint16_t *ord = reinterpret_cast<uint16_t *>(st_lds + LS_SLOTS * 28);
+    uint32_t *kx = reinterpret_cast<uint32_t *>(st_lds);
+    uint32_t *ky = kx + LS_SLOTS;
+    uint32_t *src = ky + LS_SLOTS;
+    uint16_t *ord = reinterpret_cast<uint16_t *>(src + LS_SLOTS);
     uint16_t *ord2 = ord + LS_SLOTS;
-    uint32_t *wcnt = reinterpret_cast<uint32_t *>(st_lds + LS_SLOTS * 32);
-    float4 *box = rect;
-    const int tid = threadIdx.x;
+    uint16_t *wcnt = ord2 + LS_SLOTS;
+    float4 *box = reinterpret_cast<float4 *>(st_lds);  // after the sorts
+    const int tid = threadIdx.x, lane = tid & 63;
     const int spb = LS_SLOTS / A.A2;
     // XCD-aware block order: blocks are dealt to the 8 XCDs round-robin, so
     // block b runs on XCD b % 8; XCD x gets a contiguous range of groups,
@@ -324,10 +322,10 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
     uint32_t *const perm = A.perm[layer];
     const int64_t g0 = blk * spb;  // first group of the block (in its layer)
     const bool last_block = blk == A.blocks_per_layer - 1;
-    // 1. gather the groups' rects and ids into LDS (local index k), in the
-    //    order of the previous level's sort; every index load of the thread
-    //    is issued before the dependent rect and id loads (two memory
-    //    latencies per block, not two per position)
+    // 1. the groups' sort keys and joint positions into LDS (local index k),
+    //    in the order of the previous level's sort; every index load of the
+    //    thread is issued before the dependent rect loads, and the loads are
+    //    unconditional (padding positions read position 0 and keep nothing)
     uint32_t sidx[ST_PER];
     bool live[ST_PER];
 #pragma unroll
@@ -337,47 +335,35 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
         live[m] = gg < A.num_groups && w < A.a0 && pos < A.n;
         sidx[m] = static_cast<uint32_t>(base + pos);
         if (live[m] && A.items) sidx[m] = static_cast<uint32_t>(A.items[base + pos]);
+        if (!live[m]) sidx[m] = 0u;
     }
-    float4 rcs[ST_PER];
-    uint32_t idv[ST_PER];
+    float2 xy[ST_PER];
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
-        rcs[m] = live[m] ? A.rect_cur.at(sidx[m]) : empty_box();
-        idv[m] = live[m] ? (A.id_cur ? A.id_cur[sidx[m]] : sidx[m]) : 0xFFFFFFFFu;
+        const float4 r = A.rect_cur.at(sidx[m]);
+        xy[m] = make_float2(r.x, r.y);
     }
-#pragma unroll
-    for (int m = 0; m < ST_PER; ++m) {
-        rect[tid + m * LS_THREADS] = rcs[m];
-        ids[tid + m * LS_THREADS] = idv[m];
-    }
-    __syncthreads();
-    // ord[pos] = local index of the rect at position pos of the current
-    // order (the previous level's), ST_NONE at padding positions
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
         const int k = tid + m * LS_THREADS;
+        kx[k] = ord32(xy[m].x);
+        ky[k] = ord32(xy[m].y);
+        src[k] = sidx[m];
+        // ord[pos] = local index of the rect at position pos of the current
+        // order (the previous level's), ST_NONE at padding positions
         ord[k] = live[m] ? static_cast<uint16_t>(k) : ST_NONE;
     }
     __syncthreads();
     // 2. one sort per level, each within aligned blocks of B = a_p positions
-    //    by (ord32(x1 or y1), position): an LDS radix sort for blocks of 2048
-    //    positions or more (four passes, ~20 us for any B), the bitonic
-    //    network below (measured at 1M rects: 30 us for B = 4096, 13 us for
-    //    B = 256, where the radix sort takes 20 and 19)
+    //    by (ord32(x1 or y1), position): block_radix for blocks of 512
+    //    positions or more, the bitonic network in registers below
     for (int p = A.p0; p < A.L; ++p) {
-        const int comp = p & 1;
+        const uint32_t *key_of = (p & 1) ? ky : kx;
         int B = 1;
         while (B < A.seg[p]) B <<= 1;
         if (B > A.A2) B = A.A2;
-        if (B >= 2048) {
-#pragma unroll
-            for (int m = 0; m < ST_PER; ++m) {
-                const int k = tid + m * LS_THREADS;
-                const uint16_t o = ord[k];
-                ka[k] = o == ST_NONE ? 0xFFFFFFFFu : ord32(comp ? rect[o].y : rect[o].x);
-            }
-            __syncthreads();
-            block_radix(ord, ord2, ka, kb, wcnt, B);
+        if (B >= ST_RADIX_MIN) {
+            block_radix(ord, ord2, key_of, wcnt, B);
             continue;
         }
         // key of position k = 4t + r: (ord32(coord), position, local index)
@@ -387,33 +373,47 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
             const int k = ST_PER * tid + r;
             const uint16_t o = ord[k];
             kk[r] = o == ST_NONE ? ~0ull
-                                 : (static_cast<uint64_t>(ord32(comp ? rect[o].y : rect[o].x)) << 32) |
+                                 : (static_cast<uint64_t>(key_of[o]) << 32) |
                                        (static_cast<uint64_t>(k) << 12) | o;
         }
-        __syncthreads();
-        block_bitonic(kk, B, keyx);
+        block_bitonic(kk, B);
 #pragma unroll
         for (int r = 0; r < ST_PER; ++r)
             ord[ST_PER * tid + r] = kk[r] == ~0ull ? ST_NONE : static_cast<uint16_t>(kk[r] & 0xFFFu);
         __syncthreads();
     }
-    // 3. leaves in the final order (position pos = tid + m * 1024, coalesced)
+    // 3. leaves in the final order (position k = tid + m * 1024, coalesced);
+    //    each rect and id is read again from global memory (it was read in
+    //    step 1, so from L2) through the joint position of its local index
     const int64_t offL = A.off[A.L - 1], cntL = A.off[A.L] - A.off[A.L - 1];
+    uint32_t s4[ST_PER];
+    bool has[ST_PER];
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        const uint16_t o = ord[tid + m * LS_THREADS];
+        has[m] = o != ST_NONE;
+        s4[m] = has[m] ? src[o] : 0u;
+    }
     float4 mine[ST_PER];
+    uint32_t idv[ST_PER];
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) mine[m] = A.rect_cur.at(s4[m]);
+    if (A.id_cur) {
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) idv[m] = A.id_cur[s4[m]];
+    } else {
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) idv[m] = s4[m];
+    }
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
         const int k = tid + m * LS_THREADS;
-        const uint16_t o = ord[k];
         const int64_t gg = g0 + k / A.A2, w = k % A.A2, dst = gg * A.a0 + w;
-        float4 rc = empty_box();
-        if (o != ST_NONE) {
-            rc = rect[o];
-            perm[dst] = ids[o] - static_cast<uint32_t>(base);
-        }
-        mine[m] = rc;
+        mine[m] = has[m] ? mine[m] : empty_box();
+        if (has[m]) perm[dst] = idv[m] - static_cast<uint32_t>(base);
         if (gg < A.num_groups && w < A.a0 && dst < cntL) {
             slot_child[offL + dst] = dst < A.n ? static_cast<int32_t>(-dst - 1) : 0;
-            slot_mbr[offL + dst] = rc;
+            slot_mbr[offL + dst] = mine[m];
         }
     }
     // leaf slots past the last group are empty
@@ -423,49 +423,48 @@ __global__ __launch_bounds__(LS_THREADS) void k_td_subtree(SubtreeArgs A) {
             slot_mbr[offL + i] = empty_box();
         }
     }
-    __syncthreads();
+    // 4. levels L-2 .. p0: box of an entry of level q = union of the boxes
+    //    of its `cover` leaf positions.  While cover <= 64 those positions
+    //    are lanes of one wave (position k = tid + m * 1024 is lane k % 64):
+    //    butterfly unions across lanes, and the lane of the entry's first
+    //    position writes its slot.  The levels above go through LDS.
+    int64_t cover = 1;
+    int q = A.L - 2;
+    for (; q >= A.p0 && (cover << A.logf) <= 64; --q) {
+        const int64_t prev = cover;
+        cover <<= A.logf;
+        for (int x = static_cast<int>(prev); x < cover; x <<= 1) {
 #pragma unroll
-    for (int m = 0; m < ST_PER; ++m) box[tid + m * LS_THREADS] = mine[m];
+            for (int m = 0; m < ST_PER; ++m) mine[m] = box_union(mine[m], shfl_xor_box(mine[m], x));
+        }
+        if ((lane & (cover - 1)) == 0) {
+#pragma unroll
+            for (int m = 0; m < ST_PER; ++m)
+                write_entry(A, slot_child, slot_mbr, g0, q, cover, (tid + m * LS_THREADS) / cover,
+                            mine[m]);
+        }
+        if (last_block) write_empty_tail(A, slot_child, slot_mbr, q, cover);
+    }
+    if (q < A.p0) return;
+    // the kx/ky region is free since the last sort's barrier
+    if ((lane & (cover - 1)) == 0) {
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) box[(tid + m * LS_THREADS) / cover] = mine[m];
+    }
     __syncthreads();
-    // 4. levels L-2 .. p0: box x of level q = union of boxes x*f .. x*f+f-1
-    //    of level q+1 (in place, compacted to the front); at most two
-    //    boxes per thread (f >= 2)
-    int64_t cover = 1;     // leaves under one entry of level q + 1
-    int width = LS_SLOTS;  // boxes of level q + 1 in LDS
-    for (int q = A.L - 2; q >= A.p0; --q) {
+    int width = static_cast<int>(LS_SLOTS / cover);  // boxes of level q + 1 in LDS
+    for (; q >= A.p0; --q) {
         cover <<= A.logf;
         width >>= A.logf;
-        float4 u0 = empty_box(), u1 = empty_box();
-        const int x0 = tid, x1 = tid + LS_THREADS;
-        if (x0 < width)
-            for (int c = 0; c < A.f; ++c) u0 = box_union(u0, box[x0 * A.f + c]);
-        if (x1 < width)
-            for (int c = 0; c < A.f; ++c) u1 = box_union(u1, box[x1 * A.f + c]);
+        float4 u = empty_box();
+        if (tid < width)
+            for (int c = 0; c < A.f; ++c) u = box_union(u, box[tid * A.f + c]);
         __syncthreads();
-        const int epg = static_cast<int>(A.A2 / cover);
-        const int64_t cnt = A.off[q + 1] - A.off[q];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int x = h ? x1 : x0;
-            if (x >= width) continue;
-            const float4 u = h ? u1 : u0;
-            box[x] = u;
-            const int64_t gg = g0 + x / epg;
-            const int64_t i = (gg * A.a0) / cover + x % epg;  // slot index in level q
-            if (gg < A.num_groups && i < cnt && (x % epg) * cover < A.a0) {
-                const bool full = i * cover < A.n;
-                slot_child[A.off[q] + i] = full ? static_cast<int32_t>(A.off[q + 1] + i * A.f) : 0;
-                slot_mbr[A.off[q] + i] = full ? u : empty_box();
-            }
+        if (tid < width) {
+            box[tid] = u;
+            write_entry(A, slot_child, slot_mbr, g0, q, cover, tid, u);
         }
-        // slots of level q past the last group's entries are empty
-        if (last_block) {
-            const int64_t first = ceil_div(A.num_groups * A.a0, cover);
-            for (int64_t i = first + tid; i < cnt; i += LS_THREADS) {
-                slot_child[A.off[q] + i] = 0;
-                slot_mbr[A.off[q] + i] = empty_box();
-            }
-        }
+        if (last_block) write_empty_tail(A, slot_child, slot_mbr, q, cover);
         __syncthreads();
     }
 }
