@@ -78,6 +78,12 @@ __host__ __device__ inline uint32_t ord32(float v) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// Inverse of ord32 (a zero key gives +0).
+__host__ __device__ inline float unord32(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
 // Centre quantisation of one axis: floor(0.5f*(lo+hi) * 2^15) in [0, 32767].
 __host__ __device__ inline uint32_t quant15(float lo, float hi) {
     float c = 0.5f * (lo + hi);

@@ -115,7 +115,7 @@ struct SubtreeArgs {
     RectLayers rect_cur;     // rects in the order the items index
     const uint32_t *id_cur;  // ids in that order, or null (identity)
     int64_t n, a0;           // n: rects per layer
-    int A2, L, p0, f, logf;
+    int A2, logA2, L, p0, f, logf;
     int64_t seg[PRT_MAX_LEVELS];        // a_p of the sort of level p
     int64_t off[PRT_MAX_LEVELS + 1];    // slot offsets per level
     int32_t *slot_child[PRT_MAX_LAYERS];
@@ -135,8 +135,8 @@ constexpr int ST_PER = LS_SLOTS / LS_THREADS;  // 4 keys per thread
 // dynamic LDS, 72 KB so that two blocks share a CU: kx | ky (ord32 of x1 and
 // y1 per local index, 2 x 16 KB) | src (joint position of each local index,
 // 16 KB) | ord and its radix twin (2 x 8 KB) | per-wave 16-bit digit
-// counters (8 KB).  Once the sorts are done the kx/ky region holds the boxes
-// of the block's upper levels (at most 2048).
+// counters (8 KB).  Once the sorts are done the first 64 KB hold the rects
+// by local index, then the boxes of the block's upper levels.
 constexpr int ST_WAVES = LS_THREADS / 64;
 constexpr size_t ST_LDS = LS_SLOTS * (4 + 4 + 4 + 2 + 2) + ST_WAVES * 256 * 2;
 constexpr uint16_t ST_NONE = 0xFFFFu;  // ord entry of a padding position
@@ -192,7 +192,8 @@ __device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B) {
 __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_of, uint16_t *wcnt,
                             int B) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wps = B / 256;  // waves per segment
+    const int wps = B / 256;  // waves per segment (a power of two)
+    const int lwps = __builtin_ctz(wps);
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     for (int i = tid; i < ST_WAVES * 256; i += LS_THREADS) wcnt[i] = 0;
     __syncthreads();
@@ -231,8 +232,8 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_o
 #pragma unroll
         for (int q = 0; q < ST_PER; ++q) {
             const int e = ST_PER * tid + q;
-            const int seg = e / (256 * wps), rem = e % (256 * wps);
-            const int d = rem / wps, w = seg * wps + rem % wps;
+            const int seg = e >> (8 + lwps), rem = e & ((256 << lwps) - 1);
+            const int d = rem >> lwps, w = (seg << lwps) + (rem & (wps - 1));
             c[q] = wcnt[w * 256 + d];
             run += c[q];
         }
@@ -249,8 +250,8 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_o
 #pragma unroll
         for (int q = 0; q < ST_PER; ++q) {
             const int e = ST_PER * tid + q;
-            const int seg = e / (256 * wps), rem = e % (256 * wps);
-            const int d = rem / wps, w = seg * wps + rem % wps;
+            const int seg = e >> (8 + lwps), rem = e & ((256 << lwps) - 1);
+            const int d = rem >> lwps, w = (seg << lwps) + (rem & (wps - 1));
             wcnt[w * 256 + d] = static_cast<uint16_t>(before);  // first destination of (wave, digit)
             before += c[q];
         }
@@ -271,16 +272,17 @@ __device__ inline float4 shfl_xor_box(float4 v, int x) {
                        __shfl_xor(v.w, x, 64));
 }
 
-// Slot i of level q (entries of `cover` leaves) for box x of the block:
-// group g0 + x / epg, entry x % epg of that group.
+// Slot i of level q (entries of cover = 2^lc leaves) for box x of the
+// block: group g0 + x / epg, entry x % epg of that group (epg = A2 / cover).
+// Powers of two throughout, so shifts and masks instead of divisions.
 __device__ inline void write_entry(const SubtreeArgs &A, int32_t *slot_child, float4 *slot_mbr,
-                                   int64_t g0, int q, int64_t cover, int64_t x, float4 u) {
-    const int64_t epg = A.A2 / cover;
-    const int64_t gg = g0 + x / epg;
-    const int64_t i = (gg * A.a0) / cover + x % epg;  // slot index in level q
+                                   int64_t g0, int q, int lc, int64_t x, float4 u) {
+    const int le = A.logA2 - lc;  // log2(entries per group)
+    const int64_t gg = g0 + (x >> le), e = x & ((int64_t(1) << le) - 1);
+    const int64_t i = ((gg * A.a0) >> lc) + e;  // slot index in level q
     const int64_t cnt = A.off[q + 1] - A.off[q];
-    if (gg < A.num_groups && i < cnt && (x % epg) * cover < A.a0) {
-        const bool full = i * cover < A.n;
+    if (gg < A.num_groups && i < cnt && (e << lc) < A.a0) {
+        const bool full = (i << lc) < A.n;
         slot_child[A.off[q] + i] = full ? static_cast<int32_t>(A.off[q + 1] + i * A.f) : 0;
         slot_mbr[A.off[q] + i] = full ? u : empty_box();
     }
@@ -288,9 +290,9 @@ __device__ inline void write_entry(const SubtreeArgs &A, int32_t *slot_child, fl
 
 // slots of level q past the last group's entries are empty
 __device__ inline void write_empty_tail(const SubtreeArgs &A, int32_t *slot_child, float4 *slot_mbr,
-                                        int q, int64_t cover) {
+                                        int q, int lc) {
     const int64_t cnt = A.off[q + 1] - A.off[q];
-    const int64_t first = ceil_div(A.num_groups * A.a0, cover);
+    const int64_t first = (A.num_groups * A.a0 + (int64_t(1) << lc) - 1) >> lc;
     for (int64_t i = first + threadIdx.x; i < cnt; i += LS_THREADS) {
         slot_child[A.off[q] + i] = 0;
         slot_mbr[A.off[q] + i] = empty_box();
@@ -331,23 +333,22 @@ void k_td_subtree(SubtreeArgs A) {
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
         const int k = tid + m * LS_THREADS;
-        const int64_t gg = g0 + k / A.A2, w = k % A.A2, pos = gg * A.a0 + w;
+        const int64_t gg = g0 + (k >> A.logA2), w = k & (A.A2 - 1), pos = gg * A.a0 + w;
         live[m] = gg < A.num_groups && w < A.a0 && pos < A.n;
         sidx[m] = static_cast<uint32_t>(base + pos);
         if (live[m] && A.items) sidx[m] = static_cast<uint32_t>(A.items[base + pos]);
         if (!live[m]) sidx[m] = 0u;
     }
-    float2 xy[ST_PER];
-#pragma unroll
-    for (int m = 0; m < ST_PER; ++m) {
-        const float4 r = A.rect_cur.at(sidx[m]);
-        xy[m] = make_float2(r.x, r.y);
-    }
+    // x2 and y2 stay in registers until the leaves are written (x1 and y1
+    // come back from their sort keys)
+    float2 zw[ST_PER];
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
         const int k = tid + m * LS_THREADS;
-        kx[k] = ord32(xy[m].x);
-        ky[k] = ord32(xy[m].y);
+        const float4 r = A.rect_cur.at(sidx[m]);
+        zw[m] = make_float2(r.z, r.w);
+        kx[k] = ord32(r.x);
+        ky[k] = ord32(r.y);
         src[k] = sidx[m];
         // ord[pos] = local index of the rect at position pos of the current
         // order (the previous level's), ST_NONE at padding positions
@@ -382,22 +383,50 @@ void k_td_subtree(SubtreeArgs A) {
             ord[ST_PER * tid + r] = kk[r] == ~0ull ? ST_NONE : static_cast<uint16_t>(kk[r] & 0xFFFu);
         __syncthreads();
     }
-    // 3. leaves in the final order (position k = tid + m * 1024, coalesced);
-    //    each rect and id is read again from global memory (it was read in
-    //    step 1, so from L2) through the joint position of its local index
+    // 3. leaves in the final order (position k = tid + m * 1024, coalesced).
+    //    x1 and y1 come back from their sort keys (ord32 is one-to-one but
+    //    for -0 and +0, which share a key: a zero coordinate is read again
+    //    from global memory for its sign); x2 and y2 move from the registers
+    //    of step 1 to the final positions through LDS (the kx and ky arrays,
+    //    free once everything the leaves need from LDS is in registers).
     const int64_t offL = A.off[A.L - 1], cntL = A.off[A.L] - A.off[A.L - 1];
+    uint16_t fo[ST_PER];
     uint32_t s4[ST_PER];
+    float4 mine[ST_PER];
     bool has[ST_PER];
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
-        const uint16_t o = ord[tid + m * LS_THREADS];
-        has[m] = o != ST_NONE;
+        fo[m] = ord[tid + m * LS_THREADS];
+        has[m] = fo[m] != ST_NONE;
+        const int o = has[m] ? fo[m] : 0;
         s4[m] = has[m] ? src[o] : 0u;
+        mine[m].x = unord32(kx[o]);
+        mine[m].y = unord32(ky[o]);
     }
-    float4 mine[ST_PER];
-    uint32_t idv[ST_PER];
+    float *zb = reinterpret_cast<float *>(kx), *wb = reinterpret_cast<float *>(ky);
+    __syncthreads();
 #pragma unroll
-    for (int m = 0; m < ST_PER; ++m) mine[m] = A.rect_cur.at(s4[m]);
+    for (int m = 0; m < ST_PER; ++m) {
+        zb[tid + m * LS_THREADS] = zw[m].x;
+        wb[tid + m * LS_THREADS] = zw[m].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        const int o = has[m] ? fo[m] : 0;
+        mine[m].z = zb[o];
+        mine[m].w = wb[o];
+    }
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        if (has[m] && (mine[m].x == 0.0f || mine[m].y == 0.0f)) {
+            const float4 r = A.rect_cur.at(s4[m]);
+            mine[m].x = r.x;
+            mine[m].y = r.y;
+        }
+    }
+    // the id loads go out after the barriers (which would wait for them)
+    uint32_t idv[ST_PER];
     if (A.id_cur) {
 #pragma unroll
         for (int m = 0; m < ST_PER; ++m) idv[m] = A.id_cur[s4[m]];
@@ -408,13 +437,18 @@ void k_td_subtree(SubtreeArgs A) {
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
         const int k = tid + m * LS_THREADS;
-        const int64_t gg = g0 + k / A.A2, w = k % A.A2, dst = gg * A.a0 + w;
+        const int64_t gg = g0 + (k >> A.logA2), w = k & (A.A2 - 1), dst = gg * A.a0 + w;
         mine[m] = has[m] ? mine[m] : empty_box();
-        if (has[m]) perm[dst] = idv[m] - static_cast<uint32_t>(base);
         if (gg < A.num_groups && w < A.a0 && dst < cntL) {
             slot_child[offL + dst] = dst < A.n ? static_cast<int32_t>(-dst - 1) : 0;
             slot_mbr[offL + dst] = mine[m];
         }
+    }
+#pragma unroll
+    for (int m = 0; m < ST_PER; ++m) {
+        const int k = tid + m * LS_THREADS;
+        const int64_t gg = g0 + (k >> A.logA2), w = k & (A.A2 - 1), dst = gg * A.a0 + w;
+        if (has[m]) perm[dst] = idv[m] - static_cast<uint32_t>(base);
     }
     // leaf slots past the last group are empty
     if (last_block) {
@@ -428,33 +462,32 @@ void k_td_subtree(SubtreeArgs A) {
     //    are lanes of one wave (position k = tid + m * 1024 is lane k % 64):
     //    butterfly unions across lanes, and the lane of the entry's first
     //    position writes its slot.  The levels above go through LDS.
-    int64_t cover = 1;
+    int lc = 0;  // log2(cover)
     int q = A.L - 2;
-    for (; q >= A.p0 && (cover << A.logf) <= 64; --q) {
-        const int64_t prev = cover;
-        cover <<= A.logf;
-        for (int x = static_cast<int>(prev); x < cover; x <<= 1) {
+    for (; q >= A.p0 && lc + A.logf <= 6; --q) {
+        const int prev = 1 << lc;
+        lc += A.logf;
+        for (int x = prev; x < (1 << lc); x <<= 1) {
 #pragma unroll
             for (int m = 0; m < ST_PER; ++m) mine[m] = box_union(mine[m], shfl_xor_box(mine[m], x));
         }
-        if ((lane & (cover - 1)) == 0) {
+        if ((lane & ((1 << lc) - 1)) == 0) {
 #pragma unroll
             for (int m = 0; m < ST_PER; ++m)
-                write_entry(A, slot_child, slot_mbr, g0, q, cover, (tid + m * LS_THREADS) / cover,
-                            mine[m]);
+                write_entry(A, slot_child, slot_mbr, g0, q, lc, (tid + m * LS_THREADS) >> lc, mine[m]);
         }
-        if (last_block) write_empty_tail(A, slot_child, slot_mbr, q, cover);
+        if (last_block) write_empty_tail(A, slot_child, slot_mbr, q, lc);
     }
     if (q < A.p0) return;
-    // the kx/ky region is free since the last sort's barrier
-    if ((lane & (cover - 1)) == 0) {
+    __syncthreads();  // every read of the rects in LDS is done
+    if ((lane & ((1 << lc) - 1)) == 0) {
 #pragma unroll
-        for (int m = 0; m < ST_PER; ++m) box[(tid + m * LS_THREADS) / cover] = mine[m];
+        for (int m = 0; m < ST_PER; ++m) box[(tid + m * LS_THREADS) >> lc] = mine[m];
     }
     __syncthreads();
-    int width = static_cast<int>(LS_SLOTS / cover);  // boxes of level q + 1 in LDS
+    int width = LS_SLOTS >> lc;  // boxes of level q + 1 in LDS
     for (; q >= A.p0; --q) {
-        cover <<= A.logf;
+        lc += A.logf;
         width >>= A.logf;
         float4 u = empty_box();
         if (tid < width)
@@ -462,9 +495,9 @@ void k_td_subtree(SubtreeArgs A) {
         __syncthreads();
         if (tid < width) {
             box[tid] = u;
-            write_entry(A, slot_child, slot_mbr, g0, q, cover, tid, u);
+            write_entry(A, slot_child, slot_mbr, g0, q, lc, tid, u);
         }
-        if (last_block) write_empty_tail(A, slot_child, slot_mbr, q, cover);
+        if (last_block) write_empty_tail(A, slot_child, slot_mbr, q, lc);
         __syncthreads();
     }
 }
@@ -689,6 +722,7 @@ int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *t
     A.n = n;
     A.a0 = td_seg(n, f, L, p0);
     A.A2 = static_cast<int>(next_pow2(A.a0));
+    A.logA2 = __builtin_ctz(static_cast<unsigned>(A.A2));
     A.L = L;
     A.p0 = p0;
     A.f = f;
