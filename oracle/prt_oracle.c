@@ -173,12 +173,26 @@ static void set_empty(float *m)
     m[0] = INFINITY; m[1] = INFINITY; m[2] = -INFINITY; m[3] = -INFINITY;
 }
 
+/* min / max in the total order with -0 below +0 (the GPU's fminf / fmaxf),
+ * so that a union does not depend on the order of its boxes even when
+ * they hold zeros of both signs */
+static float min0(float a, float b)
+{
+    if (a == b) return signbit(a) ? a : b;
+    return b < a ? b : a;
+}
+static float max0(float a, float b)
+{
+    if (a == b) return signbit(a) ? b : a;
+    return b > a ? b : a;
+}
+
 static void merge_into(float *dst, const float *src)
 {
-    if (src[0] < dst[0]) dst[0] = src[0];
-    if (src[1] < dst[1]) dst[1] = src[1];
-    if (src[2] > dst[2]) dst[2] = src[2];
-    if (src[3] > dst[3]) dst[3] = src[3];
+    dst[0] = min0(dst[0], src[0]);
+    dst[1] = min0(dst[1], src[1]);
+    dst[2] = max0(dst[2], src[2]);
+    dst[3] = max0(dst[3], src[3]);
 }
 
 /* Level chain of BU:788-830: ceil(n/f) leaves, then ceil(count/f) per level

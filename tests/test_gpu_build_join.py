@@ -70,7 +70,9 @@ def check_bu(t, o):
     np.testing.assert_array_equal(u32(t.keys), o["keys"])
     for k in ("node_count", "node_start", "node_end", "entry_ref"):
         np.testing.assert_array_equal(getattr(t, k).cpu().numpy(), o[k], err_msg=k)
-    np.testing.assert_array_equal(t.entry_mbr.cpu().numpy(), o["entry_mbr"])
+    # bitwise: the sign of a zero counts
+    np.testing.assert_array_equal(t.entry_mbr.cpu().numpy().view(np.uint32),
+                                  np.asarray(o["entry_mbr"], np.float32).view(np.uint32))
     assert t.levels == o["levels"]
 
 
@@ -103,7 +105,8 @@ def check_td(t, o):
     assert t.level_offset == o["level_offset"]
     np.testing.assert_array_equal(u32(t.perm), o["perm"])
     np.testing.assert_array_equal(t.slot_child.cpu().numpy(), o["slot_child"])
-    np.testing.assert_array_equal(t.slot_mbr.cpu().numpy(), o["slot_mbr"])
+    np.testing.assert_array_equal(t.slot_mbr.cpu().numpy().view(np.uint32),
+                                  np.asarray(o["slot_mbr"], np.float32).view(np.uint32))
 
 
 def test_td_golden(dev):
@@ -171,6 +174,28 @@ def test_kernel_times_of_layered_build(dev):
     assert all(0 < ms < 50 for _, ms in kt.stats.values())
     for t, r in zip(TL.trees, layers):
         check_td(t, O.td_build(r, 16))
+
+
+def signed_zero_rects(n, seed):
+    """Every rect has x1, y1 = a zero of random sign and x2, y2 = a zero of
+    random sign or a positive value, so that every union meets zeros of
+    both signs (min and max take -0 below +0)."""
+    rng = np.random.default_rng(seed)
+    z = lambda: np.where(rng.random(n) < 0.5, np.float32(-0.0), np.float32(0.0))
+    r = np.empty((n, 4), np.float32)
+    r[:, 0], r[:, 1] = z(), z()
+    r[:, 2] = np.where(rng.random(n) < 0.7, z(), rng.random(n).astype(np.float32))
+    r[:, 3] = np.where(rng.random(n) < 0.7, z(), rng.random(n).astype(np.float32))
+    return r
+
+
+@pytest.mark.parametrize("n", [300, 5000, 70000])
+def test_signed_zero_unions(dev, n):
+    r = signed_zero_rects(n, n)
+    check_td(rtree.build_top_down(dev_rects(r), 16), O.td_build(r, 16))
+    check_bu(rtree.build_bottom_up(dev_rects(r), 16), O.bu_build(r, 16))
+    TL = rtree.TopDownLayers([dev_rects(r), dev_rects(r[::-1].copy())], 16)
+    check_td(TL.trees[0], O.td_build(r, 16))
 
 
 def test_td_build_duplicate_coords(dev):

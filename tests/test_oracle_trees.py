@@ -116,6 +116,34 @@ def test_td_matches_numpy_and_invariants(n, f):
                 mbr[s], [blk[:, 0].min(), blk[:, 1].min(), blk[:, 2].max(), blk[:, 3].max()])
 
 
+def test_unions_order_free_with_signed_zeros():
+    """Unions take -0 below +0 whatever the order of the boxes: the root of
+    a tree over zeros of both signs is -0 for the minima and +0 for the
+    maxima, and a permuted input gives the same upper levels."""
+    rng = np.random.default_rng(9)
+    n = 4096
+    r = np.zeros((n, 4), np.float32)
+    r[:, 0] = np.where(rng.random(n) < 0.5, -0.0, 0.0)
+    r[:, 1] = np.where(rng.random(n) < 0.5, -0.0, 0.0)
+    r[:, 2] = np.where(rng.random(n) < 0.5, -0.0, 0.0)
+    r[:, 3] = np.where(rng.random(n) < 0.5, -0.0, 0.0)
+    o = O.bu_build(r, 16)
+    root = np.asarray(o["entry_mbr"], np.float32).reshape(-1, 16, 4)[-1]
+    bits = root[: o["node_count"][-1]].view(np.uint32)
+    # every root entry: min over zeros with at least one -0 -> -0 (0x80000000)
+    assert (bits[:, 0] == 0x80000000).all() and (bits[:, 1] == 0x80000000).all()
+    # maxima over zeros with at least one +0 -> +0
+    assert (bits[:, 2] == 0).all() and (bits[:, 3] == 0).all()
+    t1 = O.td_build(r, 16)
+    t2 = O.td_build(r[::-1].copy(), 16)
+    off = t1["level_offset"]
+    up1 = np.asarray(t1["slot_mbr"], np.float32)[: off[-2]].view(np.uint32)
+    up2 = np.asarray(t2["slot_mbr"], np.float32)[: off[-2]].view(np.uint32)
+    # all keys tie, so both orders keep their input order; the root slot
+    # unions all rects either way
+    np.testing.assert_array_equal(up1[0], up2[0])
+
+
 def test_overlap_golden():
     for c in GOLD["overlap"]:
         a = np.array(c["a"], np.float32)
