@@ -743,32 +743,32 @@ int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *t
     PRT_LAUNCHED();
     int64_t a = 1;
     for (int k = 0; k < L - p0; ++k) a *= f;  // entries of level p0 - 1 cover f^(L-p0)
-    if (p0 > 0 && off[p0] <= 64 * TOP_THREADS) {
-        // levels p0-1 .. 0 hold few slots: one block per layer packs them all
+    // levels above p0: one launch per level while a level has more slots
+    // than a block has threads, then one block per layer packs the rest
+    int p = p0 - 1;
+    for (; p >= 0 && off[p + 1] - off[p] > TOP_THREADS; --p) {
+        const int64_t cnt = off[p + 1] - off[p];
+        for (int l = 0; l < nl; ++l) {
+            PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0, s,
+                       n, f, a, off[p], cnt, off[p + 1], trees[l].slot_child,
+                       reinterpret_cast<float4 *>(trees[l].slot_mbr));
+            PRT_LAUNCHED();
+        }
+        a *= f;
+    }
+    if (p >= 0) {
         TopArgs T{};
         T.n = n;
         T.a = a;
         T.f = f;
-        T.p_top = p0 - 1;
-        for (int p = 0; p <= PRT_MAX_LEVELS; ++p) T.off[p] = off[p];
+        T.p_top = p;
+        for (int q = 0; q <= PRT_MAX_LEVELS; ++q) T.off[q] = off[q];
         for (int l = 0; l < nl; ++l) {
             T.slot_child[l] = trees[l].slot_child;
             T.slot_mbr[l] = reinterpret_cast<float4 *>(trees[l].slot_mbr);
         }
         PRT_LAUNCH(k_td_top, dim3(static_cast<unsigned>(nl)), dim3(TOP_THREADS), 0, s, T);
         PRT_LAUNCHED();
-    } else {
-        for (int l = 0; l < nl; ++l) {
-            float4 *sm = reinterpret_cast<float4 *>(trees[l].slot_mbr);
-            int64_t al = a;
-            for (int p = p0 - 1; p >= 0; --p) {
-                const int64_t cnt = off[p + 1] - off[p];
-                PRT_LAUNCH(k_td_level, dim3(static_cast<unsigned>(ceil_div(cnt, 256))), dim3(256), 0,
-                           s, n, f, al, off[p], cnt, off[p + 1], trees[l].slot_child, sm);
-                PRT_LAUNCHED();
-                al *= f;
-            }
-        }
     }
     return PRT_OK;
 }
