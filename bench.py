@@ -332,19 +332,44 @@ def grad_phase(dev, args, checks):
         hB.zero_()
         gr32["r"] = rtree.area_grad(pd, S32, S32, gA=hA, gB=hB, green=True)
     ms_32 = timed(fb32, 3, 1)
+    # FP64 forward and adjoint in one launch: the area by Green's theorem
+    # over the boundary the adjoint walks (not the sweep's summation order)
+    kA, kB = torch.zeros_like(S64.xy), torch.zeros_like(S64.xy)
+    gr64 = {}
+
+    def fb64():
+        kA.zero_()
+        kB.zero_()
+        gr64["r"] = rtree.area_grad(pd, S64, S64, gA=kA, gB=kB, green=True)
+    ms_fb64 = timed(fb64, 3, 1)
     g64 = (gA + gB).cpu().numpy()
     g32 = (hA + hB).double().cpu().numpy()
     rel32 = np.abs(g32 - g64).max() / np.abs(g64).max()
     a64 = area.cpu().numpy()
+    # Green's area against the sweep's (bit-identical to the oracle): the
+    # relative error with the overlay denominator max(|A|, 1e-12 min(|P|, |Q|))
+    x, y = S.xy[:, 0], S.xy[:, 1]
+    nxt = np.arange(len(x)) + 1
+    nxt[S.off[1:] - 1] = S.off[:-1]
+    shoe = 0.5 * np.abs(np.add.reduceat(x * y[nxt] - x[nxt] * y, S.off[:-1]))
+    ag = gr64["r"][2].cpu().numpy()
+    den = np.maximum(np.abs(a64), 1e-12 * np.minimum(shoe[pairs[:, 0]], shoe[pairs[:, 1]]))
+    den = np.where(den == 0, 1.0, den)
+    green_rel = float((np.abs(ag - a64) / den).max())
+    fused_same_grad = bool(torch.equal(kA, gA) and torch.equal(kB, gB))
     a32 = gr32["r"][2].double().cpu().numpy()
     arel32 = float(np.median(np.abs(a32 - a64) / np.maximum(np.abs(a64), 1e-300)))
     out = {"pairs": npairs, "fp64_forward_ms": round(ms_f, 3), "fp64_adjoint_ms": round(ms_b, 3),
            "fp64_pairs_s": round(npairs / (ms_f + ms_b) * 1e3, 1),
+           "fp64_fused_forward_adjoint_ms": round(ms_fb64, 3),
+           "fp64_fused_pairs_s": round(npairs / ms_fb64 * 1e3, 1),
+           "fp64_fused_area_max_rel_err_vs_sweep": green_rel,
+           "fp64_fused_grad_equals_adjoint": fused_same_grad,
            "fp32_forward_adjoint_ms": round(ms_32, 3),
            "fp32_pairs_s": round(npairs / ms_32 * 1e3, 1),
            "grad_rel_err_fp32_vs_fp64": float(rel32), "area_median_rel_err_fp32": arel32}
-    log(f"grad: fwd {ms_f:.2f} ms, adjoint {ms_b:.2f} ms (fp64), fp32 {ms_32:.2f} ms; "
-        f"fp32 vs fp64 {rel32:.1e}")
+    log(f"grad: fwd {ms_f:.2f} ms, adjoint {ms_b:.2f} ms (fp64), fused fp64 {ms_fb64:.2f} ms "
+        f"(Green area rel err {green_rel:.1e}), fp32 {ms_32:.2f} ms; fp32 vs fp64 {rel32:.1e}")
 
     def check(O):
         # FP64 gradient against the CPU oracle on a sample of pairs
