@@ -15,6 +15,9 @@
 // order and leaf ones are counted (pass 1) or written at
 // offset[q] + running count + rank (pass 2).  An exclusive scan of the
 // counts gives the offsets; a final kernel sorts each query's hits by r.
+// Pass 1 also keeps the first JN_KEEP hits of every query: pass 2 copies
+// them for a query with no more hits than that instead of traversing again
+// (a layer join has one or two hits per query).
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -23,6 +26,7 @@ namespace {
 
 constexpr int JN_THREADS = 256;
 constexpr int JN_WAVES = JN_THREADS / 64;
+constexpr int JN_KEEP = 8;  // hits per query kept by the count pass
 
 struct TreeView {
     int mode;            // 0 = BU, 1 = TD
@@ -40,7 +44,8 @@ __global__ __launch_bounds__(JN_THREADS) void k_join(TreeView tv, const float4 *
                                                      int64_t nq, int stack_cap,
                                                      int64_t *__restrict__ counts,
                                                      const int64_t *__restrict__ offsets,
-                                                     uint32_t *__restrict__ pairs, int64_t cap) {
+                                                     uint32_t *__restrict__ pairs, int64_t cap,
+                                                     uint32_t *__restrict__ kept) {
     extern __shared__ int32_t stk_all[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int G = tv.G, QW = 64 / G;
@@ -53,13 +58,25 @@ __global__ __launch_bounds__(JN_THREADS) void k_join(TreeView tv, const float4 *
 
     float4 qb = valid ? qs[q] : empty_box();
     int top = 0;
-    if (valid) {
+    const int64_t base = (WRITE && valid) ? offsets[q] : 0;
+    // write pass: a query whose hits the count pass kept copies them
+    const int64_t cq = (WRITE && valid) ? counts[q] : 0;
+    const bool copied = WRITE && valid && cq <= JN_KEEP;
+    if (copied) {
+        for (int i = l; i < cq; i += G) {
+            const int64_t o = base + i;
+            if (o < cap) {
+                pairs[2 * o] = static_cast<uint32_t>(q);
+                pairs[2 * o + 1] = kept[q * JN_KEEP + i];
+            }
+        }
+    }
+    if (valid && !copied) {
         if (l == 0) stk[0] = static_cast<int32_t>(tv.root);
         top = 1;
     }
     __builtin_amdgcn_wave_barrier();
     int64_t found = 0;
-    const int64_t base = (WRITE && valid) ? offsets[q] : 0;
 
     while (__any(top > 0)) {
         int64_t node = -1;
@@ -100,6 +117,12 @@ __global__ __launch_bounds__(JN_THREADS) void k_join(TreeView tv, const float4 *
                 pairs[2 * o] = static_cast<uint32_t>(q);
                 pairs[2 * o + 1] = rid;
             }
+        }
+        if (!WRITE && hit && leaf) {
+            const int64_t k = found + __popcll(leafm & below);
+            if (k < JN_KEEP)
+                kept[q * JN_KEEP + k] = (tv.mode == 0) ? static_cast<uint32_t>(ref)
+                                                       : tv.perm[-static_cast<int64_t>(ref) - 1];
         }
         found += __popcll(leafm);
         __builtin_amdgcn_wave_barrier();
@@ -240,6 +263,7 @@ int run_join(const TreeView &tv, int levels, const float *queries, int64_t nq, u
     void *scan_ws = c.take<char>(scan_i64_ws_bytes(nq));
     uint32_t *lists = c.take<uint32_t>(2 * (nq > 0 ? nq : 1));
     int64_t *lcount = c.take<int64_t>(2);
+    uint32_t *kept = c.take<uint32_t>(JN_KEEP * (nq > 0 ? nq : 1));
     PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "join: workspace too small");
     PRT_LAUNCH(k_zero_i64, 1, 64, 0, s, count_d, 2);
     PRT_LAUNCHED();
@@ -252,13 +276,13 @@ int run_join(const TreeView &tv, int levels, const float *queries, int64_t nq, u
     const unsigned grid = static_cast<unsigned>(ceil_div(nq, JN_WAVES * QW));
     const float4 *q4 = reinterpret_cast<const float4 *>(queries);
     PRT_LAUNCH(k_join<false>, grid, JN_THREADS, lds, s, tv, q4, nq, stack_cap, counts, nullptr,
-               nullptr, 0);
+               nullptr, 0, kept);
     PRT_LAUNCHED();
     int rc = scan_i64(counts, offs, nq, count_d, scan_ws, s);
     if (rc) return rc;
     if (cap == 0) return PRT_OK;
     PRT_LAUNCH(k_join<true>, grid, JN_THREADS, lds, s, tv, q4, nq, stack_cap, counts, offs, pairs,
-               cap);
+               cap, kept);
     PRT_LAUNCHED();
     PRT_LAUNCH(k_zero_i64, 1, 64, 0, s, lcount, 2);
     PRT_LAUNCHED();
@@ -294,6 +318,7 @@ extern "C" size_t prt_join_workspace_size(int64_t nq) {
     s.take<char>(scan_i64_ws_bytes(nq));
     s.take<uint32_t>(2 * (nq > 0 ? nq : 1));
     s.take<int64_t>(2);
+    s.take<uint32_t>(JN_KEEP * (nq > 0 ? nq : 1));
     return s.bytes();
 }
 
