@@ -291,7 +291,7 @@ int run_join(const TreeView &tv, int levels, const float *queries, int64_t nq, u
                s, counts, offs, nq, cap, pairs, lists, lc, count_d + 1);
     PRT_LAUNCHED();
     int64_t wgrid = ceil_div(nq, JN_WAVES);
-    if (wgrid > 4096) wgrid = 4096;
+    if (wgrid > 65536) wgrid = 65536;  // one listed query per wave
     PRT_LAUNCH(k_seg_sort_wave, dim3(static_cast<unsigned>(wgrid)), JN_THREADS, 0, s, counts, offs,
                cap, lists, lc, pairs);
     PRT_LAUNCHED();

@@ -230,6 +230,22 @@ def test_join_touching_and_empty(dev):
     assert none.shape[0] == 0
 
 
+def test_join_segment_sizes(dev):
+    """Queries with 0 .. ~1000 hits: every sort path of the hit segments
+    (one thread, a wave with 1-16 registers per lane) against brute force."""
+    r = rects(20000, 17)
+    rng = np.random.default_rng(5)
+    side = np.concatenate([np.full(50, 0.002), np.linspace(0.01, 0.22, 350)]).astype(np.float32)
+    c = rng.random((len(side), 2)).astype(np.float32) * 0.7 + 0.15
+    q = np.stack([c[:, 0], c[:, 1], c[:, 0] + side, c[:, 1] + side], 1).astype(np.float32)
+    brute = O.join_brute(r, q)
+    hits = np.bincount(brute[:, 0], minlength=len(q))
+    assert hits.max() > 700 and (hits <= 1024).all() and (hits == 0).any()
+    for build in (rtree.build_bottom_up, rtree.build_top_down):
+        pairs = rtree.join(build(dev_rects(r), 16), dev_rects(q))
+        np.testing.assert_array_equal(u32(pairs), brute)
+
+
 def test_join_big_segments(dev):
     # one window covering everything: 6000 hits for one query (block sort path)
     r = rects(6000, 9)
