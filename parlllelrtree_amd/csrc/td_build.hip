@@ -141,6 +141,9 @@ constexpr int ST_WAVES = LS_THREADS / 64;
 constexpr size_t ST_LDS = LS_SLOTS * (4 + 4 + 4 + 2 + 2) + ST_WAVES * 256 * 2;
 constexpr uint16_t ST_NONE = 0xFFFFu;  // ord entry of a padding position
 constexpr int ST_RADIX_MIN = 512;       // blocks of this many positions or more: block_radix
+// LDS requested when there are no more blocks than CUs: above half of a CU's
+// 160 KB, so that no CU takes two blocks while another idles
+constexpr size_t ST_LDS_ALONE = 96 * 1024;
 
 // Sorts every aligned block of B positions (B a power of two <= 256)
 // ascending; position of register r of thread t is 4t + r.
@@ -299,7 +302,10 @@ __device__ inline void write_empty_tail(const SubtreeArgs &A, int32_t *slot_chil
     }
 }
 
-__global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+// WPE: waves per SIMD the register budget allows -- 8 when two blocks share
+// a CU (64 VGPRs), 4 when each block has a CU to itself (no spills)
+template <int WPE>
+__global__ __launch_bounds__(LS_THREADS) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 void k_td_subtree(SubtreeArgs A) {
     extern __shared__ __align__(16) char st_lds[];
     uint32_t *kx = reinterpret_cast<uint32_t *>(st_lds);
@@ -582,6 +588,18 @@ int td_levels(int64_t n, int f, int64_t *off) {
     return L;
 }
 
+int td_device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
 int64_t next_pow2(int64_t v) {
     int64_t p = 1;
     while (p < v) p <<= 1;
@@ -738,8 +756,14 @@ int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *t
     }
     A.num_groups = ceil_div(n, A.a0);
     A.blocks_per_layer = ceil_div(A.num_groups, LS_SLOTS / A.A2);
-    PRT_LAUNCH(k_td_subtree, dim3(static_cast<unsigned>(nl * A.blocks_per_layer)),
-               dim3(LS_THREADS), ST_LDS, s, A);
+    const int64_t blocks = nl * A.blocks_per_layer;
+    if (blocks <= td_device_cus()) {
+        PRT_LAUNCH(k_td_subtree<4>, dim3(static_cast<unsigned>(blocks)), dim3(LS_THREADS),
+                   ST_LDS_ALONE, s, A);
+    } else {
+        PRT_LAUNCH(k_td_subtree<8>, dim3(static_cast<unsigned>(blocks)), dim3(LS_THREADS), ST_LDS,
+                   s, A);
+    }
     PRT_LAUNCHED();
     int64_t a = 1;
     for (int k = 0; k < L - p0; ++k) a *= f;  // entries of level p0 - 1 cover f^(L-p0)

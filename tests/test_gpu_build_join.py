@@ -170,7 +170,7 @@ def test_kernel_times_of_layered_build(dev):
         torch.cuda.synchronize()
     assert kt.stats["k_onesweep"][0] == 8          # 4 digit passes x 2 radix levels
     assert kt.stats["k_hist_all<Src>"][0] == 2
-    assert kt.stats["k_td_subtree"][0] == 1 and kt.stats["k_td_top"][0] == 1
+    assert kt.stats["k_td_subtree<8>"][0] == 1 and kt.stats["k_td_top"][0] == 1
     assert all(0 < ms < 50 for _, ms in kt.stats.values())
     for t, r in zip(TL.trees, layers):
         check_td(t, O.td_build(r, 16))
