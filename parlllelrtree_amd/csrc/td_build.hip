@@ -345,14 +345,16 @@ void k_td_subtree(SubtreeArgs A) {
         if (live[m] && A.items) sidx[m] = static_cast<uint32_t>(A.items[base + pos]);
         if (!live[m]) sidx[m] = 0u;
     }
-    // x2 and y2 stay in registers until the leaves are written (x1 and y1
-    // come back from their sort keys)
+    // with the full register budget x2 and y2 stay in registers until the
+    // leaves are written (x1 and y1 come back from their sort keys); with
+    // 64 registers the leaves read them again from global memory instead
+    constexpr bool keep_zw = WPE <= 4;
     float2 zw[ST_PER];
 #pragma unroll
     for (int m = 0; m < ST_PER; ++m) {
         const int k = tid + m * LS_THREADS;
         const float4 r = A.rect_cur.at(sidx[m]);
-        zw[m] = make_float2(r.z, r.w);
+        if (keep_zw) zw[m] = make_float2(r.z, r.w);
         kx[k] = ord32(r.x);
         ky[k] = ord32(r.y);
         src[k] = sidx[m];
@@ -390,11 +392,13 @@ void k_td_subtree(SubtreeArgs A) {
         __syncthreads();
     }
     // 3. leaves in the final order (position k = tid + m * 1024, coalesced).
-    //    x1 and y1 come back from their sort keys (ord32 is one-to-one but
-    //    for -0 and +0, which share a key: a zero coordinate is read again
-    //    from global memory for its sign); x2 and y2 move from the registers
-    //    of step 1 to the final positions through LDS (the kx and ky arrays,
-    //    free once everything the leaves need from LDS is in registers).
+    //    With the full register budget: x1 and y1 come back from their sort
+    //    keys (ord32 is one-to-one but for -0 and +0, which share a key: a
+    //    zero coordinate is read again from global memory for its sign), x2
+    //    and y2 move from the registers of step 1 to the final positions
+    //    through LDS (the kx and ky arrays, free once everything the leaves
+    //    need from LDS is in registers).  With 64 registers the rects are
+    //    read again from global memory (L2) through the joint positions.
     const int64_t offL = A.off[A.L - 1], cntL = A.off[A.L] - A.off[A.L - 1];
     uint16_t fo[ST_PER];
     uint32_t s4[ST_PER];
@@ -406,30 +410,37 @@ void k_td_subtree(SubtreeArgs A) {
         has[m] = fo[m] != ST_NONE;
         const int o = has[m] ? fo[m] : 0;
         s4[m] = has[m] ? src[o] : 0u;
-        mine[m].x = unord32(kx[o]);
-        mine[m].y = unord32(ky[o]);
-    }
-    float *zb = reinterpret_cast<float *>(kx), *wb = reinterpret_cast<float *>(ky);
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < ST_PER; ++m) {
-        zb[tid + m * LS_THREADS] = zw[m].x;
-        wb[tid + m * LS_THREADS] = zw[m].y;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < ST_PER; ++m) {
-        const int o = has[m] ? fo[m] : 0;
-        mine[m].z = zb[o];
-        mine[m].w = wb[o];
-    }
-#pragma unroll
-    for (int m = 0; m < ST_PER; ++m) {
-        if (has[m] && (mine[m].x == 0.0f || mine[m].y == 0.0f)) {
-            const float4 r = A.rect_cur.at(s4[m]);
-            mine[m].x = r.x;
-            mine[m].y = r.y;
+        if (keep_zw) {
+            mine[m].x = unord32(kx[o]);
+            mine[m].y = unord32(ky[o]);
         }
+    }
+    if (keep_zw) {
+        float *zb = reinterpret_cast<float *>(kx), *wb = reinterpret_cast<float *>(ky);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) {
+            zb[tid + m * LS_THREADS] = zw[m].x;
+            wb[tid + m * LS_THREADS] = zw[m].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) {
+            const int o = has[m] ? fo[m] : 0;
+            mine[m].z = zb[o];
+            mine[m].w = wb[o];
+        }
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) {
+            if (has[m] && (mine[m].x == 0.0f || mine[m].y == 0.0f)) {
+                const float4 r = A.rect_cur.at(s4[m]);
+                mine[m].x = r.x;
+                mine[m].y = r.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < ST_PER; ++m) mine[m] = A.rect_cur.at(s4[m]);
     }
     // the id loads go out after the barriers (which would wait for them)
     uint32_t idv[ST_PER];
