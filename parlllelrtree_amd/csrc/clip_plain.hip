@@ -31,7 +31,10 @@ namespace prt {
 namespace {
 
 constexpr int CP_THREADS = 128;
-constexpr int64_t CP_MAX_LANES = 65536;
+// one lane per pair up to 262,144 lanes (four waves per SIMD): with 65,536 (one
+// wave per SIMD) the latency of the lanes' arena accesses went uncovered and
+// 8.4M pairs took 1.6x as long.  The arena is lanes x max_verts x 104 bytes.
+constexpr int64_t CP_MAX_LANES = 262144;
 
 __device__ double sub_beam(const Lane &L, int m, int nP, double ya, double yc, double acc,
                            int op) {

@@ -43,7 +43,7 @@ namespace prt {
 namespace {
 
 constexpr int CT_THREADS = 128;
-constexpr int64_t CT_MAX_LANES = 65536;
+constexpr int64_t CT_MAX_LANES = 262144;  // as the plain kernel's lanes
 
 struct IvArrays {  // one interval list, icap slots per lane
     int32_t *eL, *eR, *sL, *sR, *lp, *rp;
