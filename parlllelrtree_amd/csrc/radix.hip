@@ -77,7 +77,10 @@ __device__ inline void tile_range(const Tiling &t, int64_t tile, int64_t &b, int
 // 2 = inclusive count over the segment's tiles up to and including this one),
 // the low 62 bits the count.  Tiles take their index from an atomic ticket, so
 // every tile a block waits for belongs to a block that is already running and
-// publishes its own count before it looks back: the waits always end.
+// publishes its own count before it looks back: the waits always end.  When
+// the whole grid fits the GPU at once (every block can be resident together)
+// the tile is the block index instead: no block waits for one that cannot
+// start, and a block loads its items at once, without the ticket's round trip.
 constexpr uint64_t LB_AGG = 1ull << 62;
 constexpr uint64_t LB_INC = 2ull << 62;
 constexpr uint64_t LB_VAL = (1ull << 62) - 1;
@@ -239,10 +242,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t mask = (1u << bits) - 1u;
-    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    const bool by_block = ticket == nullptr;  // co-resident grid: tile = block
+    if (tid == 0 && !by_block) s_tile = atomicAdd(ticket, 1u);
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    __syncthreads();
-    const int64_t tile = s_tile;
+    if (!by_block) __syncthreads();
+    const int64_t tile = by_block ? static_cast<int64_t>(blockIdx.x) : static_cast<int64_t>(s_tile);
     const int64_t seg = tile / t.tiles_per_seg, jt = tile % t.tiles_per_seg;
     int64_t b, e;
     tile_range(t, tile, b, e);
@@ -259,6 +263,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
         item[j] = local < count ? in[b + local] : 0ull;
     }
+    if (by_block) __syncthreads();  // the counters are cleared
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
@@ -440,6 +445,22 @@ int64_t radix_status_words(int64_t n, int64_t seg, int64_t layer) {
 
 
 namespace {
+// blocks of k_onesweep the GPU holds at once (CUs x blocks per CU)
+int64_t onesweep_resident_blocks() {
+    static int64_t cap = -1;
+    if (cap < 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_onesweep, RS_THREADS, 0) !=
+                hipSuccess)
+            cap = 0;  // unknown: always take tickets
+        else
+            cap = static_cast<int64_t>(cus) * per_cu;
+    }
+    return cap;
+}
+
 template <class Src>
 int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n,
                     int64_t seg, int bit_lo, int bit_hi, const RadixWs &w, hipStream_t s,
@@ -475,13 +496,15 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
     PRT_LAUNCHED();
     const uint64_t *src = in;
     uint64_t *dst = (in == buf0) ? buf1 : buf0;
+    const bool by_block = t.num_tiles <= onesweep_resident_blocks();
     for (int p = 0; p < passes; ++p) {
         const int shift = bit_lo + 8 * p;
         const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
         const PassGather none{};
         PRT_LAUNCH(k_onesweep, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
                    ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS,
-                   w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS, tickets + p,
+                   w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS,
+                   by_block ? nullptr : tickets + p,
                    p + 1 == passes ? gat : none);
         PRT_LAUNCHED();
         src = dst;
