@@ -246,6 +246,35 @@ def test_join_segment_sizes(dev):
         np.testing.assert_array_equal(u32(pairs), brute)
 
 
+def test_join_truncated_at_cap(dev):
+    """A pair buffer smaller than the hit count: the total still comes back,
+    join() refuses the result, and the queries whose hits all fit below cap
+    hold their complete sorted hit lists (both the copied and the traversed
+    write paths)."""
+    r = rects(20000, 23)
+    rng = np.random.default_rng(8)
+    side = np.where(rng.random(600) < 0.5, 0.004, 0.05).astype(np.float32)
+    c = rng.random((600, 2)).astype(np.float32) * 0.9
+    q = np.stack([c[:, 0], c[:, 1], c[:, 0] + side, c[:, 1] + side], 1).astype(np.float32)
+    brute = O.join_brute(r, q)
+    cap = len(brute) // 2
+    for build in (rtree.build_bottom_up, rtree.build_top_down):
+        t = build(dev_rects(r), 16)
+        with pytest.raises(RuntimeError):
+            rtree.join(t, dev_rects(q), cap=cap)
+        j = rtree.Joiner(t, len(q), cap, torch.device("cuda"))
+        j.run(dev_rects(q))
+        total = int(j.count[0])
+        assert total == len(brute)
+        got = u32(j.pairs[:cap])
+        hits = np.bincount(brute[:, 0], minlength=len(q))
+        ends = np.cumsum(hits)
+        full = int(np.searchsorted(ends, cap, side="right"))   # queries 0..full-1 fit
+        k = int(ends[full - 1]) if full else 0
+        assert k > 1000 and (hits[:full] <= 8).any() and (hits[:full] > 8).any()
+        np.testing.assert_array_equal(got[:k], brute[:k])
+
+
 def test_join_big_segments(dev):
     # one window covering everything: 6000 hits for one query (block sort path)
     r = rects(6000, 9)
