@@ -283,19 +283,32 @@ int prt_poly_mbr_dev(const double *xy, const int64_t *off, int64_t n, float *mbr
  * set to 1 when a pair's output did not match its counts.  The workspace of
  * the count pass is prt_overlay_contours_workspace_size(np, max_verts, 0).
  * The vertex order is this library's own (see the kernel's header), not
- * GPC's. */
+ * GPC's.
+ *
+ * stage (device, optional, NULL for none): the count pass keeps there the
+ * vertices in a pool of P slots, taken 8 at a time by the pairs, P the
+ * largest value (< 2^31) with prt_overlay_contours_stage_size(np, P) <=
+ * stage_bytes (a stage too small for P = 8 is an error).  The write pass,
+ * given the same stage unchanged, copies the kept pairs out and sweeps again
+ * only the pairs that did not fit, so the two passes must see the same
+ * pairs, polygons and op.  The output is the same with or without a stage;
+ * a pass takes as long as its slowest pair, so size P for every pair to
+ * fit (about the total vertex count plus 4 per pair). */
 size_t prt_overlay_contours_workspace_size(int64_t npairs, int32_t max_verts,
                                            int64_t total_vertices);
+size_t prt_overlay_contours_stage_size(int64_t npairs, int64_t pool_verts);
 int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs, const double *xyA,
                                const int64_t *offA, const double *xyB, const int64_t *offB,
                                int32_t max_verts, int32_t op, int64_t *ncont, int64_t *nvert,
-                               void *ws, size_t ws_bytes, void *stream);
+                               void *stage, size_t stage_bytes, void *ws, size_t ws_bytes,
+                               void *stream);
 int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs, const double *xyA,
                                const int64_t *offA, const double *xyB, const int64_t *offB,
                                int32_t max_verts, int32_t op, const int64_t *cont_off,
                                const int64_t *vert_off, int64_t total_vertices,
                                int64_t *cont_start, int32_t *cont_hole, double *xy,
-                               int32_t *status, void *ws, size_t ws_bytes, void *stream);
+                               int32_t *status, const void *stage, size_t stage_bytes, void *ws,
+                               size_t ws_bytes, void *stream);
 
 /* ---- overlay pipeline: MBRs of A and B -> R-tree over B's MBRs -> join
  * with A's MBRs -> batched FP64 clip of every pair -> total area ----------

@@ -130,13 +130,14 @@ SIGNATURES = {
     "prt_overlay_total": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                   c_size_t, c_void_p]),
     "prt_overlay_contours_workspace_size": (c_size_t, [c_int64, c_int32, c_int64]),
+    "prt_overlay_contours_stage_size": (c_size_t, [c_int64, c_int64]),
     "prt_overlay_contours_count": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_int32, c_int32, c_void_p, c_void_p,
-                                           c_void_p, c_size_t, c_void_p]),
+                                           c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]),
     "prt_overlay_contours_write": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_int32, c_int32, c_void_p, c_void_p,
                                            c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                           c_void_p, c_size_t, c_void_p]),
+                                           c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]),
     "prt_poly_mbr_dev": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "prt_pipeline_workspace_size": (c_size_t, [POINTER(PipelineIO)]),
     "prt_pipeline_run": (c_int, [POINTER(PipelineIO), POINTER(TuneStruct), c_void_p, c_size_t,

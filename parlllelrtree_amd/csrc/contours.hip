@@ -35,6 +35,14 @@
 // stored; open contours are tracked for their identity only).  Write pass:
 // every emitted vertex goes to an arena slot at the pair's vertex offset,
 // chained per contour; a closing contour is copied out in path order.
+//
+// With a staging buffer the count pass also keeps the vertices, chained per
+// contour in a vertex pool that the pairs take CT_CHUNK slots at a time, and
+// a record per closed contour (head, length, hole flag, next record of the
+// pair).  The write pass then copies the kept pairs out in path order and
+// sweeps again only the pairs that did not fit (the pools ran out), which
+// the count pass listed.  A pass's time is its slowest wave's, so a stage
+// pays off only when (nearly) every pair fits.
 #include "clip_lane.hpp"
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
@@ -65,6 +73,23 @@ struct ContourOut {  // write pass
     int32_t *chole;
 };
 
+// What the count pass keeps: sc[p] = contours of pair p kept, their records
+// chained from cfirst[p]; -1 when the pair is on the spill list instead (it
+// did not fit, or it failed).  ctr = {spilled pairs, vertex chunks taken,
+// contour records taken}.
+constexpr int CT_CHUNK = 8;
+struct StageWs {
+    unsigned long long *ctr;
+    int64_t *spill;
+    int32_t *sc, *cfirst;
+    int32_t *chead, *clen, *chole, *cnext;  // ccap contour records
+    int32_t *vnext;                         // vcap vertex slots
+    double *vx, *vy;
+    int64_t vcap = 0, ccap = 0;
+};
+
+enum BuildMode { CT_COUNT = 0, CT_STAGE = 1, CT_WRITE = 2 };
+
 struct Pt {
     bool below, right;
     int iv;
@@ -79,10 +104,15 @@ __device__ inline double x_unc(const Lane &L, int e, double y) {
                       fabs(L.at(L.w.dx, e)) * (fabs(y) + fabs(by)));
 }
 
-template <bool W> struct Builder {
+template <int M> struct Builder {
+    static constexpr bool W = M != CT_COUNT;  // vertices go to an arena
     const Lane &L;
     const ContourWs &cw;
     const ContourOut &o;
+    const StageWs *st = nullptr;  // CT_STAGE: the pools
+    bool spill = false;           // CT_STAGE: the pair does not fit the pools
+    int64_t chunk = 0, chunk_end = 0;
+    int32_t first_rec = -1, last_rec = -1;
     int sb = 0;  // iv[sb] holds the intervals below the scanline
     int nb = 0, na = 0, nfree = 0, nnew = 0;
     int64_t nv = 0, nc = 0, written = 0;
@@ -103,6 +133,23 @@ template <bool W> struct Builder {
 
     __device__ int vertex(double x, double y) {
         const int64_t v = nv++;
+        if (M == CT_STAGE) {  // a pool slot; -1 once the pair has spilled
+            if (spill) return -1;
+            if (chunk == chunk_end) {
+                const int64_t c = static_cast<int64_t>(atomicAdd(st->ctr + 1, 1ull)) * CT_CHUNK;
+                if (c + CT_CHUNK > st->vcap) {
+                    spill = true;
+                    return -1;
+                }
+                chunk = c;
+                chunk_end = c + CT_CHUNK;
+            }
+            const int64_t g = chunk++;
+            o.vx[g] = x;
+            o.vy[g] = y;
+            o.vnext[g] = -1;
+            return static_cast<int>(g);
+        }
         if (W) {
             if (v < vlim) {
                 o.vx[vbase + v] = x;
@@ -173,7 +220,23 @@ template <bool W> struct Builder {
         poly_free(q);
     }
     __device__ void close(int p, bool hole) {
-        if (W) {
+        if (M == CT_STAGE) {
+            if (!spill) {
+                const unsigned long long r = atomicAdd(st->ctr + 2, 1ull);
+                if (r >= static_cast<unsigned long long>(st->ccap)) {
+                    spill = true;
+                } else {
+                    const int32_t q = static_cast<int32_t>(r);
+                    st->chead[q] = ph(p);
+                    st->clen[q] = pc(p);
+                    st->chole[q] = hole ? 1 : 0;
+                    st->cnext[q] = -1;
+                    if (last_rec >= 0) st->cnext[last_rec] = q;
+                    else first_rec = q;
+                    last_rec = q;
+                }
+            }
+        } else if (W) {
             if (nc < clim) {
                 o.cstart[cbase + nc] = vbase + written;
                 o.chole[cbase + nc] = hole ? 1 : 0;
@@ -367,8 +430,8 @@ template <bool W> struct Builder {
     }
 };
 
-template <bool W>
-__device__ void contours_one(Builder<W> &cb, const double *__restrict__ P, int nP,
+template <int M>
+__device__ void contours_one(Builder<M> &cb, const double *__restrict__ P, int nP,
                              const double *__restrict__ Q, int nQ, int op) {
     const Lane &L = cb.L;
     double bp[4], bq[4];
@@ -437,43 +500,118 @@ __device__ void contours_one(Builder<W> &cb, const double *__restrict__ P, int n
 }
 
 // Count pass: ncont[p], nvert[p]; -1 in both when the pair does not fit the
-// lane (more than max_verts vertices) or its arena bounds.
+// lane (more than max_verts vertices) or its arena bounds.  CT_STAGE also
+// keeps the pair's contours in st, or lists the pair in st.spill.
+template <int M>
 __global__ __launch_bounds__(CT_THREADS) void k_contours_count(
     const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
     const int64_t *__restrict__ offA, const double *__restrict__ xyB,
-    const int64_t *__restrict__ offB, int op, ContourWs cw, int64_t *__restrict__ ncont,
-    int64_t *__restrict__ nvert) {
+    const int64_t *__restrict__ offB, int op, ContourWs cw, StageWs st,
+    int64_t *__restrict__ ncont, int64_t *__restrict__ nvert) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= cw.base.T) return;
     Lane L(cw.base, t);
-    const ContourOut none{};
+    ContourOut so{};
+    if (M == CT_STAGE) {
+        so.vx = st.vx;
+        so.vy = st.vy;
+        so.vnext = st.vnext;
+    }
     for (int64_t p = t; p < np; p += cw.base.T) {
         const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
         const int64_t oa = offA[a], ob = offB[b];
         const int nA = static_cast<int>(offA[a + 1] - oa), nB = static_cast<int>(offB[b + 1] - ob);
-        Builder<false> cb(L, cw, none);
+        Builder<M> cb(L, cw, so);
+        if (M == CT_STAGE) {  // pool slots are global: no base, the pool's bound
+            cb.st = &st;
+            cb.vlim = st.vcap;
+        }
         if (nA + nB > cw.base.cap && !(nA < 3 && nB < 3)) cb.bad = true;
         else contours_one(cb, xyA + 2 * oa, nA, xyB + 2 * ob, nB, op);
         ncont[p] = cb.bad ? -1 : cb.nc;
         nvert[p] = cb.bad ? -1 : cb.nv;
+        if (M == CT_STAGE) {
+            const bool kept = !cb.bad && !cb.spill;
+            st.sc[p] = kept ? static_cast<int32_t>(cb.nc) : -1;
+            st.cfirst[p] = cb.first_rec;
+            if (!kept) st.spill[atomicAdd(st.ctr, 1ull)] = p;
+        }
+    }
+}
+
+__global__ void k_stage_clear(unsigned long long *ctr) {
+    if (threadIdx.x < 3) ctr[threadIdx.x] = 0;
+}
+
+// Write pass of the kept pairs: the pair's contour records in closing order,
+// each contour's vertices along its chain in the vertex pool.  Every index
+// read from the stage is checked, so a stage that does not belong to these
+// counts sets *status instead of reading or writing out of bounds.
+__global__ __launch_bounds__(256) void k_contours_copy(int64_t np, StageWs st,
+                                                       const int64_t *__restrict__ cont_off,
+                                                       const int64_t *__restrict__ vert_off,
+                                                       ContourOut out, int32_t *status) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < np;
+         p += stride) {
+        const int32_t n = st.sc[p];
+        if (n < 0) continue;  // swept by the write pass of the spill list
+        const int64_t c0 = cont_off[p], v0 = vert_off[p], vend = vert_off[p + 1];
+        bool ok = cont_off[p + 1] - c0 == n;
+        int64_t w = v0;
+        int32_t r = st.cfirst[p];
+        for (int c = 0; ok && c < n; ++c) {
+            if (r < 0 || r >= st.ccap) {
+                ok = false;
+                break;
+            }
+            out.cstart[c0 + c] = w;
+            out.chole[c0 + c] = st.chole[r];
+            int32_t v = st.chead[r];
+            for (int32_t k = st.clen[r]; k > 0; --k) {
+                if (v < 0 || v >= st.vcap || w >= vend) {
+                    ok = false;
+                    break;
+                }
+                out.xy[2 * w] = st.vx[v];
+                out.xy[2 * w + 1] = st.vy[v];
+                ++w;
+                v = st.vnext[v];
+            }
+            r = st.cnext[r];
+        }
+        if (!ok || w != vend) atomicOr(status, 1);
     }
 }
 
 // Write pass at the scanned offsets; *status (cleared by the launch before)
-// gets 1 when a pair did not produce what its counts promised.
+// gets 1 when a pair did not produce what its counts promised.  With a list
+// (the count pass's spill list) only the listed pairs are swept.
 __global__ __launch_bounds__(CT_THREADS) void k_contours_write(
     const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
     const int64_t *__restrict__ offA, const double *__restrict__ xyB,
     const int64_t *__restrict__ offB, int op, ContourWs cw, const int64_t *__restrict__ cont_off,
-    const int64_t *__restrict__ vert_off, ContourOut out, int32_t *status) {
+    const int64_t *__restrict__ vert_off, ContourOut out, const int64_t *__restrict__ list,
+    const unsigned long long *__restrict__ nlist, int32_t *status) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= cw.base.T) return;
     Lane L(cw.base, t);
-    for (int64_t p = t; p < np; p += cw.base.T) {
+    int64_t cnt = np;
+    if (list) {
+        const unsigned long long k = *nlist;
+        cnt = k > static_cast<unsigned long long>(np) ? 0 : static_cast<int64_t>(k);
+        if (k > static_cast<unsigned long long>(np)) atomicOr(status, 1);
+    }
+    for (int64_t i = t; i < cnt; i += cw.base.T) {
+        const int64_t p = list ? list[i] : i;
+        if (p < 0 || p >= np) {
+            atomicOr(status, 1);
+            continue;
+        }
         const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
         const int64_t oa = offA[a], ob = offB[b];
         const int nA = static_cast<int>(offA[a + 1] - oa), nB = static_cast<int>(offB[b + 1] - ob);
-        Builder<true> cb(L, cw, out);
+        Builder<CT_WRITE> cb(L, cw, out);
         cb.vbase = vert_off[p];
         cb.vlim = vert_off[p + 1] - cb.vbase;
         cb.cbase = cont_off[p];
@@ -519,10 +657,70 @@ int carve_contours(ContourWs &cw, int64_t T, int cap, Carver &c) {
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 
+// pool sizes for a vertex pool of vcap slots: a record per 3 slots (a
+// contour has 3 vertices or more unless it is degenerate; a pair that runs
+// out of records spills like one that runs out of slots)
+constexpr int64_t CT_MAX_POOL = int64_t(1) << 31;
+int64_t stage_records(int64_t vcap) { return vcap / 3 + 1; }
+
+template <class C> void carve_stage(C &c, StageWs &st, int64_t np, int64_t vcap) {
+    st.vcap = vcap;
+    st.ccap = stage_records(vcap);
+    st.ctr = c.template take<unsigned long long>(3);
+    st.spill = c.template take<int64_t>(np);
+    st.sc = c.template take<int32_t>(np);
+    st.cfirst = c.template take<int32_t>(np);
+    int32_t **rec[] = {&st.chead, &st.clen, &st.chole, &st.cnext};
+    for (int32_t **q : rec) *q = c.template take<int32_t>(st.ccap);
+    st.vnext = c.template take<int32_t>(vcap);
+    st.vx = c.template take<double>(vcap);
+    st.vy = c.template take<double>(vcap);
+}
+
+// Sizer twin of carve_stage
+struct SizeTaker {
+    Sizer s;
+    template <class T> T *take(size_t n) {
+        s.take<T>(n);
+        return nullptr;
+    }
+};
+
+size_t stage_bytes_for(int64_t np, int64_t vcap) {
+    SizeTaker z;
+    StageWs st;
+    carve_stage(z, st, np, vcap);
+    return z.s.bytes();
+}
+
+// largest vertex pool (< 2^31 slots) whose stage fits `bytes`, 0 if none
+int64_t stage_pool(int64_t np, size_t bytes) {
+    int64_t lo = 0, hi = CT_MAX_POOL - 1;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (stage_bytes_for(np, mid) <= bytes) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+int carve_stage_checked(StageWs &st, int64_t np, const void *stage, size_t bytes) {
+    const int64_t vcap = stage_pool(np, bytes);
+    if (vcap < CT_CHUNK) return PRT_E_NOMEM;
+    Carver c(const_cast<void *>(stage), bytes);
+    carve_stage(c, st, np, vcap);
+    return c.ok() ? PRT_OK : PRT_E_NOMEM;
+}
+
 }  // namespace
 }  // namespace prt
 
 using namespace prt;
+
+extern "C" size_t prt_overlay_contours_stage_size(int64_t npairs, int64_t pool_verts) {
+    if (npairs < 0 || pool_verts < CT_CHUNK || pool_verts >= CT_MAX_POOL) return 0;
+    return stage_bytes_for(npairs, pool_verts);
+}
 
 extern "C" size_t prt_overlay_contours_workspace_size(int64_t npairs, int32_t max_verts,
                                                       int64_t total_vertices) {
@@ -552,8 +750,9 @@ int contours_args(const uint32_t *pairs, int64_t npairs, const double *xyA, cons
 extern "C" int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs, const double *xyA,
                                           const int64_t *offA, const double *xyB,
                                           const int64_t *offB, int32_t max_verts, int32_t op,
-                                          int64_t *ncont, int64_t *nvert, void *ws,
-                                          size_t ws_bytes, void *stream) {
+                                          int64_t *ncont, int64_t *nvert, void *stage,
+                                          size_t stage_bytes, void *ws, size_t ws_bytes,
+                                          void *stream) {
     int rc = contours_args(pairs, npairs, xyA, offA, xyB, offB, max_verts, op);
     if (rc) return rc;
     if (npairs == 0) return PRT_OK;
@@ -566,8 +765,19 @@ extern "C" int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs,
     PRT_REQUIRE(carve_contours(cw, contour_lanes(npairs), contour_cap(max_verts), c) == PRT_OK,
                 PRT_E_NOMEM, "overlay_contours_count: workspace too small");
     const unsigned grid = static_cast<unsigned>(ceil_div(cw.base.T, CT_THREADS));
-    PRT_LAUNCH(k_contours_count, grid, CT_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA,
-               xyB, offB, static_cast<int>(op), cw, ncont, nvert);
+    StageWs st;
+    if (stage) {
+        PRT_REQUIRE(carve_stage_checked(st, npairs, stage, stage_bytes) == PRT_OK, PRT_E_NOMEM,
+                    "overlay_contours_count: stage of %zu bytes too small for %lld pairs",
+                    stage_bytes, static_cast<long long>(npairs));
+        PRT_LAUNCH(k_stage_clear, 1, 64, 0, as_stream(stream), st.ctr);
+        PRT_LAUNCHED();
+        PRT_LAUNCH(k_contours_count<CT_STAGE>, grid, CT_THREADS, 0, as_stream(stream), pairs,
+                   npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert);
+    } else {
+        PRT_LAUNCH(k_contours_count<CT_COUNT>, grid, CT_THREADS, 0, as_stream(stream), pairs,
+                   npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert);
+    }
     PRT_LAUNCHED();
     return PRT_OK;
 }
@@ -578,7 +788,8 @@ extern "C" int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs,
                                           const int64_t *cont_off, const int64_t *vert_off,
                                           int64_t total_vertices, int64_t *cont_start,
                                           int32_t *cont_hole, double *xy, int32_t *status,
-                                          void *ws, size_t ws_bytes, void *stream) {
+                                          const void *stage, size_t stage_bytes, void *ws,
+                                          size_t ws_bytes, void *stream) {
     int rc = contours_args(pairs, npairs, xyA, offA, xyB, offB, max_verts, op);
     if (rc) return rc;
     PRT_REQUIRE(total_vertices >= 0, PRT_E_INVAL, "overlay_contours_write: total_vertices < 0");
@@ -601,10 +812,22 @@ extern "C" int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs,
     out.cstart = cont_start;
     out.chole = cont_hole;
     const unsigned grid = static_cast<unsigned>(ceil_div(cw.base.T, CT_THREADS));
+    StageWs st;
+    if (stage)
+        PRT_REQUIRE(carve_stage_checked(st, npairs, stage, stage_bytes) == PRT_OK, PRT_E_NOMEM,
+                    "overlay_contours_write: stage of %zu bytes too small for %lld pairs",
+                    stage_bytes, static_cast<long long>(npairs));
     PRT_LAUNCH(k_status_clear, 1, 64, 0, as_stream(stream), status);
     PRT_LAUNCHED();
+    if (stage) {
+        const int64_t cgrid = ceil_div(npairs, 256) < 65536 ? ceil_div(npairs, 256) : 65536;
+        PRT_LAUNCH(k_contours_copy, static_cast<unsigned>(cgrid), 256, 0, as_stream(stream),
+                   npairs, st, cont_off, vert_off, out, status);
+        PRT_LAUNCHED();
+    }
     PRT_LAUNCH(k_contours_write, grid, CT_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA,
-               xyB, offB, static_cast<int>(op), cw, cont_off, vert_off, out, status);
+               xyB, offB, static_cast<int>(op), cw, cont_off, vert_off, out,
+               stage ? st.spill : nullptr, stage ? st.ctr : nullptr, status);
     PRT_LAUNCHED();
     return PRT_OK;
 }

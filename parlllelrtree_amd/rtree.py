@@ -420,18 +420,30 @@ class Contours:
         return [(xy[starts[i]:starts[i + 1]], bool(hole[c0 + i])) for i in range(c1 - c0)]
 
 
-def overlay_contours(pairs: torch.Tensor, A: PolyDev, B: PolyDev, op="int") -> Contours:
+def overlay_contours(pairs: torch.Tensor, A: PolyDev, B: PolyDev, op="int",
+                     pool_per_pair: int = 32, stage_limit: int = 1 << 31) -> Contours:
     """Contours of A[pairs[t,0]] op B[pairs[t,1]]: the count pass, a scan of
-    the counts, then the write pass (one lane per pair)."""
+    the counts, then the write pass (one lane per pair).  The count pass
+    keeps the vertices in a pool of pool_per_pair * npairs slots (halved
+    until the stage is at most stage_limit bytes; 0 keeps none), so the
+    write pass sweeps again only the pairs that did not fit."""
     pairs = pairs.contiguous()
     npairs, dev = pairs.shape[0], pairs.device
     mv = A.max_verts + B.max_verts
     ncont = torch.zeros(npairs, dtype=torch.int64, device=dev)
     nvert = torch.zeros(npairs, dtype=torch.int64, device=dev)
+    pool, stage_bytes = min(pool_per_pair * npairs, (1 << 31) - 1), 0
+    while npairs and pool >= 8:
+        stage_bytes = lib().prt_overlay_contours_stage_size(npairs, pool)
+        if stage_bytes <= stage_limit:
+            break
+        pool //= 2
+    stage = _ws(stage_bytes, dev) if npairs and pool >= 8 else None
+    sp, sb = (_ptr(stage), stage.numel()) if stage is not None else (None, 0)
     ws = _ws(lib().prt_overlay_contours_workspace_size(npairs, mv, 0), dev)
     check(lib().prt_overlay_contours_count(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off),
                                            _ptr(B.xy), _ptr(B.off), mv, _op(op), _ptr(ncont),
-                                           _ptr(nvert), _ptr(ws), ws.numel(), _stream()))
+                                           _ptr(nvert), sp, sb, _ptr(ws), ws.numel(), _stream()))
     if npairs and int(torch.minimum(ncont.min(), nvert.min())) < 0:
         raise PrtError(-5, "overlay_contours: a pair exceeds max_verts")
     zero = torch.zeros(1, dtype=torch.int64, device=dev)
@@ -446,8 +458,8 @@ def overlay_contours(pairs: torch.Tensor, A: PolyDev, B: PolyDev, op="int") -> C
     check(lib().prt_overlay_contours_write(_ptr(pairs), npairs, _ptr(A.xy), _ptr(A.off),
                                            _ptr(B.xy), _ptr(B.off), mv, _op(op), _ptr(cont_off),
                                            _ptr(vert_off), nv, _ptr(cont_start), _ptr(hole),
-                                           _ptr(xy), _ptr(status), _ptr(ws), ws.numel(),
-                                           _stream()))
+                                           _ptr(xy), _ptr(status), sp, sb, _ptr(ws),
+                                           ws.numel(), _stream()))
     if int(status.item()) != 0:
         raise PrtError(-4, "overlay_contours: write pass disagrees with the counts")
     return Contours(cont_off, vert_off, cont_start, hole, xy)

@@ -76,6 +76,23 @@ def test_td_layers_workspace_and_argument_checks():
     assert b"null device pointer" in lib.prt_last_error()
 
 
+def test_contour_stage_sizes():
+    lib = _lib.lib()
+    for npairs in (1, 1000, 100_000):
+        sizes = [lib.prt_overlay_contours_stage_size(npairs, k)
+                 for k in (8, 9, 1000, 1 << 20, (1 << 31) - 1)]
+        assert all(b > a for a, b in zip(sizes, sizes[1:]))
+        # 20 bytes per vertex slot, 16 per contour record (one per 3 slots)
+        # and 16 per pair at least
+        assert sizes[3] >= (1 << 20) * 20 + (1 << 20) // 3 * 16 + npairs * 16
+    assert lib.prt_overlay_contours_stage_size(10, 7) == 0
+    assert lib.prt_overlay_contours_stage_size(10, 1 << 31) == 0
+    assert lib.prt_overlay_contours_stage_size(-1, 8) == 0
+    # the stage is optional and checked on the host before any launch
+    assert lib.prt_overlay_contours_count(None, 1, None, None, None, None, 8, 1, None, None,
+                                          None, 0, None, 0, None) == -1
+
+
 def test_known_layout_sizes():
     lib = _lib.lib()
     st = _lib.BUTreeStruct()

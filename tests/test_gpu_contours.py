@@ -15,9 +15,10 @@ from parlllelrtree_amd._lib import PrtError
 pytestmark = pytest.mark.gpu
 
 
-def check_all(pairs, A, B, op):
+def check_all(pairs, A, B, op, **kw):
     dA, dB = rtree.to_device_polys(A), rtree.to_device_polys(B)
-    got = rtree.overlay_contours(torch.from_numpy(pairs.view(np.int32)).cuda(), dA, dB, op)
+    got = rtree.overlay_contours(torch.from_numpy(pairs.view(np.int32)).cuda(), dA, dB, op,
+                                 **kw)
     assert got.cont_off.shape[0] == len(pairs) + 1
     for t, (a, b) in enumerate(pairs):
         ref, holes = O.clip_contours(A.poly(int(a)), B.poly(int(b)), op)
@@ -84,11 +85,80 @@ def test_rejects_bad_arguments(dev):
     args = (pairs.data_ptr(), 1, A.xy.data_ptr(), A.off.data_ptr(), A.xy.data_ptr(),
             A.off.data_ptr())
     s = torch.cuda.current_stream().cuda_stream
-    assert lib().prt_overlay_contours_count(*args, 8, 7, n.data_ptr(), n.data_ptr(),
+    assert lib().prt_overlay_contours_count(*args, 8, 7, n.data_ptr(), n.data_ptr(), None, 0,
                                             ws.data_ptr(), ws.numel(), s) == -1
-    assert lib().prt_overlay_contours_count(*args, 8, 1, n.data_ptr(), n.data_ptr(),
+    assert lib().prt_overlay_contours_count(*args, 8, 1, n.data_ptr(), n.data_ptr(), None, 0,
                                             ws.data_ptr(), 16, s) == -2
+    # a stage too small for one vertex per pair is refused
+    assert lib().prt_overlay_contours_count(*args, 8, 1, n.data_ptr(), n.data_ptr(),
+                                            ws.data_ptr(), 300, ws.data_ptr(), ws.numel(),
+                                            s) == -2
     # a pair larger than max_verts is reported, not clipped
-    assert lib().prt_overlay_contours_count(*args, 3, 1, n.data_ptr(), n.data_ptr(),
+    assert lib().prt_overlay_contours_count(*args, 3, 1, n.data_ptr(), n.data_ptr(), None, 0,
                                             ws.data_ptr(), ws.numel(), s) == 0
     assert int(n.item()) == -1
+
+
+def test_stage_sizes_give_the_same_contours(dev):
+    """No stage; a vertex pool of 4 slots per pair (most pairs spill and are
+    swept again, a few fit); half the vertices needed (some fit); 32 per pair
+    (all fit); a stage limit that halves the pool: the same CSR output,
+    equal to the oracle."""
+    A = gen.stars_grid(gen.seed(2), 3000, 55)
+    B = gen.stars_grid(gen.seed(2, True), 3000, 55)
+    pairs = np.ascontiguousarray(O.join_brute(gen.poly_mbr(B), gen.poly_mbr(A))[:3000])
+    dA, dB = rtree.to_device_polys(A), rtree.to_device_polys(B)
+    pd = torch.from_numpy(pairs.view(np.int32)).cuda()
+    for op in ("int", "xor"):
+        ref = check_all(pairs, A, B, op, pool_per_pair=0)  # no stage, against the oracle
+        total = int(ref.vert_off[-1])
+        for kw in ({"pool_per_pair": 4}, {"pool_per_pair": total // (2 * len(pairs))},
+                   {"pool_per_pair": 32}, {"stage_limit": 300_000}):
+            got = rtree.overlay_contours(pd, dA, dB, op, **kw)
+            for f in ("cont_off", "vert_off", "cont_start", "hole", "xy"):
+                assert torch.equal(getattr(got, f), getattr(ref, f)), (op, kw, f)
+
+
+def test_write_with_a_stage_the_count_pass_did_not_fill(dev):
+    """The write pass checks every index it reads from the stage: a zeroed
+    stage (no pair listed for a new sweep, every pair claiming 0 kept
+    contours from record 0) sets the status instead of writing out of
+    bounds."""
+    from parlllelrtree_amd._lib import lib
+    A = gen.stars_grid(gen.seed(2), 500, 23)
+    B = gen.stars_grid(gen.seed(2, True), 500, 23)
+    pairs = np.ascontiguousarray(O.join_brute(gen.poly_mbr(B), gen.poly_mbr(A))[:400])
+    dA, dB = rtree.to_device_polys(A), rtree.to_device_polys(B)
+    pd = torch.from_numpy(pairs.view(np.int32)).cuda()
+    np_, mv = len(pairs), dA.max_verts + dB.max_verts
+    s = torch.cuda.current_stream().cuda_stream
+    sb = lib().prt_overlay_contours_stage_size(np_, 64 * np_)
+    assert sb > 0
+    stage = torch.empty(sb, dtype=torch.uint8, device="cuda")
+    ncont = torch.empty(np_, dtype=torch.int64, device="cuda")
+    nvert = torch.empty(np_, dtype=torch.int64, device="cuda")
+    ws = torch.empty(lib().prt_overlay_contours_workspace_size(np_, mv, 0), dtype=torch.uint8,
+                     device="cuda")
+    common = (pd.data_ptr(), np_, dA.xy.data_ptr(), dA.off.data_ptr(), dB.xy.data_ptr(),
+              dB.off.data_ptr(), mv, 1)
+    assert lib().prt_overlay_contours_count(*common, ncont.data_ptr(), nvert.data_ptr(),
+                                            stage.data_ptr(), sb, ws.data_ptr(), ws.numel(),
+                                            s) == 0
+    zero = torch.zeros(1, dtype=torch.int64, device="cuda")
+    cont_off = torch.cat([zero, torch.cumsum(ncont, 0)])
+    vert_off = torch.cat([zero, torch.cumsum(nvert, 0)])
+    nc, nv = int(cont_off[-1]), int(vert_off[-1])
+    assert nc > 0 and nv > 0
+    cs = torch.empty(nc, dtype=torch.int64, device="cuda")
+    hole = torch.empty(nc, dtype=torch.int32, device="cuda")
+    xy = torch.empty((nv, 2), dtype=torch.float64, device="cuda")
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws2 = torch.empty(lib().prt_overlay_contours_workspace_size(np_, mv, nv),
+                      dtype=torch.uint8, device="cuda")
+    for st, want in ((stage, 0), (torch.zeros_like(stage), 1)):
+        assert lib().prt_overlay_contours_write(*common, cont_off.data_ptr(),
+                                                vert_off.data_ptr(), nv, cs.data_ptr(),
+                                                hole.data_ptr(), xy.data_ptr(),
+                                                status.data_ptr(), st.data_ptr(), sb,
+                                                ws2.data_ptr(), ws2.numel(), s) == 0
+        assert int(status.item()) == want
