@@ -147,6 +147,32 @@ __host__ __device__ inline float4 box_union(float4 a, float4 b) {
     return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
 }
 
+// ---- wave64 digit match (stable radix ranking) --------------------------
+// Lanes of the wave that do NOT share this lane's 8-bit digit d, or are not
+// valid (inactive lanes count as not valid).  Per digit bit: the bit
+// sign-extended to 0 / -1, one compare for the ballot, and the ballot XORed
+// with it into the mismatch mask (a lane with the bit set mismatches the
+// lanes whose ballot bit is clear, and the other way round).  The lanes
+// equal to this one are the complement; their number below this lane is
+// lane - digit_lanes_below(mismatch), their count 64 - popc(mismatch).
+__device__ inline uint64_t digit_mismatch(uint32_t d, bool valid) {
+    const uint64_t inval = ~__ballot(valid);
+    uint32_t lo = static_cast<uint32_t>(inval), hi = static_cast<uint32_t>(inval >> 32);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int32_t m = static_cast<int32_t>(d << (31 - b)) >> 31;
+        const uint64_t bal = __ballot(m != 0);
+        lo |= static_cast<uint32_t>(bal) ^ static_cast<uint32_t>(m);
+        hi |= static_cast<uint32_t>(bal >> 32) ^ static_cast<uint32_t>(m);
+    }
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+// set bits of `mask` in the lanes below this one
+__device__ inline uint32_t digit_lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
 // ---- polygon set operations (gpc_op order: DIFF, INT, XOR, UNION) ------
 // A gap between neighbouring edges belongs to the result set by the even-odd
 // parities of A (subject, pp) and B (clip, pq).

@@ -94,19 +94,6 @@ __device__ inline void lb_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Lanes of the wave holding the same digit as this lane (valid lanes only).
-// Always 8 ballots: the bits of a narrower digit above its width are zero in
-// every lane and leave the mask unchanged.
-__device__ inline uint64_t digit_peers(uint32_t d, bool valid) {
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint64_t bal = __ballot((d >> k) & 1u);
-        peers &= ((d >> k) & 1u) ? bal : ~bal;
-    }
-    return peers;
-}
-
 __global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -256,7 +243,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
 
     uint64_t item[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // issue every load of the thread before ranking the first item
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
@@ -270,12 +256,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         const bool valid = local < count;
         const uint64_t v = item[j];
         const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
-        const uint64_t peers = digit_peers(d, valid);
+        const uint64_t mism = digit_mismatch(d, valid);
         uint32_t base = 0;
         if (valid) base = wcnt[wave][d];
-        const uint32_t r = static_cast<uint32_t>(__popcll(peers & lt));
+        const uint32_t r = static_cast<uint32_t>(lane) - digit_lanes_below(mism);
         __builtin_amdgcn_wave_barrier();
-        if (valid && r == 0) wcnt[wave][d] = base + static_cast<uint32_t>(__popcll(peers));
+        if (valid && r == 0) wcnt[wave][d] = base + 64u - static_cast<uint32_t>(__popcll(mism));
         __builtin_amdgcn_wave_barrier();
         rank[j] = valid ? base + r : 0xFFFFFFFFu;
     }

@@ -197,7 +197,6 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_o
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wps = B / 256;  // waves per segment (a power of two)
     const int lwps = __builtin_ctz(wps);
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     for (int i = tid; i < ST_WAVES * 256; i += LS_THREADS) wcnt[i] = 0;
     __syncthreads();
     for (int pass = 0; pass < 4; ++pass) {
@@ -214,17 +213,13 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_o
 #pragma unroll
         for (int r = 0; r < ST_PER; ++r) {
             const uint32_t d = dig[r];
-            uint64_t peers = ~0ull;
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const uint64_t bal = __ballot((d >> b) & 1u);
-                peers &= ((d >> b) & 1u) ? bal : ~bal;
-            }
+            const uint64_t mism = digit_mismatch(d, true);
             const uint32_t base = wcnt[wave * 256 + d];
-            const uint32_t before = static_cast<uint32_t>(__popcll(peers & lt));
+            const uint32_t before = static_cast<uint32_t>(lane) - digit_lanes_below(mism);
             __builtin_amdgcn_wave_barrier();
             if (before == 0)
-                wcnt[wave * 256 + d] = static_cast<uint16_t>(base + static_cast<uint32_t>(__popcll(peers)));
+                wcnt[wave * 256 + d] =
+                    static_cast<uint16_t>(base + 64u - static_cast<uint32_t>(__popcll(mism)));
             __builtin_amdgcn_wave_barrier();
             rank[r] = base + before;
         }
