@@ -54,7 +54,7 @@ def cpu_model() -> str:
 # every input read once, every output written once)
 BYTES_PER_ITEM = {
     "k_onesweep": 16,       # one digit pass: 8-B item read + 8-B item written (+ GATHER_BYTES)
-    "k_hist_all<Src>": 24,  # 16-B rect read, 8-B item written (items made in the histogram launch)
+    "k_hist_all<Src>": 24,  # first radix level: 16-B rect read, 8-B item written (8-B read after)
     "k_td_gather": 48,      # 8-B item, 16-B rect + 4-B id read; 16 + 4 written
     "k_td_subtree<8>": 52,  # 8-B item, 16-B rect, 4-B id read; 16-B box + 4-B child + 4-B perm written
     "k_td_subtree<4>": 52,  # the same kernel when each block has a CU to itself

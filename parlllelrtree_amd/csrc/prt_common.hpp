@@ -229,12 +229,16 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
 // buf0 there): (ord32(x1 or y1) << 32 | position), segmented, bits 32..63;
 // or (morton_center30 << 32 | position), bits 32..61.  With zeroed = true the
 // caller has already cleared the sort's counter words (radix_ghist_words).
-// With rects_out set, the last digit pass also writes, at every item's
-// sorted position, its rect and id (ids[i], or i when ids is null).
-int radix_sort_td_coord(const RectLayers &rects, int comp, uint64_t *buf0, uint64_t *buf1,
-                        int64_t n, int64_t seg, int64_t layer, const RadixWs &w, hipStream_t s,
-                        uint64_t **result, bool zeroed, const uint32_t *ids = nullptr,
-                        float4 *rects_out = nullptr, uint32_t *ids_out = nullptr);
+// With items set (buf0 or buf1), the items are there already, written by the
+// last digit pass of the level before.  With rects_out set, the last digit
+// pass writes, at every item's sorted position, its rect and id (ids[i], or
+// i when ids is null) and, in place of the sorted item, the item of the next
+// level: (ord32 of the other coordinate of the rect << 32 | position).
+int radix_sort_td_coord(const RectLayers &rects, int comp, const uint64_t *items, uint64_t *buf0,
+                        uint64_t *buf1, int64_t n, int64_t seg, int64_t layer, const RadixWs &w,
+                        hipStream_t s, uint64_t **result, bool zeroed,
+                        const uint32_t *ids = nullptr, float4 *rects_out = nullptr,
+                        uint32_t *ids_out = nullptr);
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
                       const RadixWs &w, hipStream_t s, uint64_t **result);
 // Counter words (cleared before a sort unless it clears them itself) and

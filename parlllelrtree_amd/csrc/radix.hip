@@ -200,12 +200,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
 
 // Optional gather of the last digit pass: with rin set, the rect (and id)
 // of every item's low word is copied to its sorted position as well, so the
-// rects of the next level come in sorted order without a pass of their own.
+// rects of the next level come in sorted order without a pass of their own;
+// the item stored there is the next level's (ord32 of coordinate next_comp
+// of the rect << 32 | sorted position), which its sort then reads as is.
 struct PassGather {
     RectLayers rin;       // rin.count == 0: no gather
     const uint32_t *iin;  // null: the id is the low word itself
     float4 *rout;
     uint32_t *iout;
+    int next_comp;
 };
 
 // Stable scatter of one tile for one digit pass.  Wave w ranks the items
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         if (i < count) {
             const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
             const uint32_t pos = goff[d] + (i - dstart[d]);
-            out[pos] = item[j];
+            out[pos] = (static_cast<uint64_t>(ord32(gat.next_comp ? rc.y : rc.x)) << 32) | pos;
             gat.rout[pos] = rc;
             gat.iout[pos] = idv[j];
         }
@@ -507,12 +510,18 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
                            false);
 }
 
-int radix_sort_td_coord(const RectLayers &rects, int comp, uint64_t *buf0, uint64_t *buf1,
-                        int64_t n, int64_t seg, int64_t layer, const RadixWs &w, hipStream_t s,
-                        uint64_t **result, bool zeroed, const uint32_t *ids, float4 *rects_out,
-                        uint32_t *ids_out) {
+int radix_sort_td_coord(const RectLayers &rects, int comp, const uint64_t *items, uint64_t *buf0,
+                        uint64_t *buf1, int64_t n, int64_t seg, int64_t layer, const RadixWs &w,
+                        hipStream_t s, uint64_t **result, bool zeroed, const uint32_t *ids,
+                        float4 *rects_out, uint32_t *ids_out) {
     PassGather g{};
-    if (rects_out) g = PassGather{rects, ids, rects_out, ids_out};
+    if (rects_out) g = PassGather{rects, ids, rects_out, ids_out, comp ^ 1};
+    if (items) {
+        PRT_REQUIRE(items == buf0 || items == buf1, PRT_E_INVAL,
+                    "radix sort: items outside the sort's buffers");
+        return radix_sort_impl(SrcItems{items}, items, buf0, buf1, n, seg, 32, 64, w, s, result,
+                               zeroed, g, layer);
+    }
     return radix_sort_impl(SrcTdCoord{rects, buf0, comp}, buf0, buf0, buf1, n, seg, 32, 64, w, s,
                            result, zeroed, g, layer);
 }

@@ -716,6 +716,7 @@ int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *t
         PRT_LAUNCHED();
     }
     const uint64_t *items = nullptr;
+    uint64_t *made = nullptr;  // this level's items, written by the level before
     uint32_t *lg = ghist;
     for (int p = 0; p < p0; ++p) {
         const int64_t a = td_seg(n, f, L, p);
@@ -725,16 +726,17 @@ int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *t
         int rc;
         if (p + 1 < p0) {
             // the last digit pass also moves the rects and ids into this
-            // level's order for the next level
+            // level's order, and writes the next level's items
             float4 *nxt_r = (cur_r.p[0] == ra) ? rb : ra;
             uint32_t *nxt_i = (cur_i == da) ? db : da;
-            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, N, a, n, lw, s, &sorted, true, cur_i,
-                                     nxt_r, nxt_i);
+            rc = radix_sort_td_coord(cur_r, p & 1, made, ia, ib, N, a, n, lw, s, &sorted, true,
+                                     cur_i, nxt_r, nxt_i);
             if (rc) return rc;
             cur_r = one_layer(nxt_r, N);
             cur_i = nxt_i;
+            made = sorted;
         } else {
-            rc = radix_sort_td_coord(cur_r, p & 1, ia, ib, N, a, n, lw, s, &sorted, true);
+            rc = radix_sort_td_coord(cur_r, p & 1, made, ia, ib, N, a, n, lw, s, &sorted, true);
             if (rc) return rc;
             items = sorted;  // the subtree kernel gathers through them
         }
