@@ -162,3 +162,28 @@ def test_write_with_a_stage_the_count_pass_did_not_fill(dev):
                                                 status.data_ptr(), st.data_ptr(), sb,
                                                 ws2.data_ptr(), ws2.numel(), s) == 0
         assert int(status.item()) == want
+
+
+def test_more_pairs_than_lanes_with_and_without_stage(dev):
+    """300,000 pairs: lanes take two pairs each (262,144 lanes at most), so
+    the stage's per-pair records and the spill list span several pairs per
+    lane.  No stage, the default pool and a pool of 2 slots per pair (most
+    pairs spill) give the same output; a sample across the lanes' first and
+    second pairs equals the oracle."""
+    n = 300_000
+    S = gen.star_pairs(gen.seed(3) + 7, n, kmin=3, kmax=12)
+    pairs = np.stack([np.arange(0, 2 * n, 2), np.arange(1, 2 * n, 2)], 1).astype(np.uint32)
+    dS = rtree.to_device_polys(S)
+    pd = torch.from_numpy(pairs.view(np.int32)).cuda()
+    ref = rtree.overlay_contours(pd, dS, dS, "int", pool_per_pair=0)
+    assert int(ref.cont_off[-1]) > n // 2
+    for kw in ({}, {"pool_per_pair": 2}):
+        got = rtree.overlay_contours(pd, dS, dS, "int", **kw)
+        for f in ("cont_off", "vert_off", "cont_start", "hole", "xy"):
+            assert torch.equal(getattr(got, f), getattr(ref, f)), (kw, f)
+    for t in list(range(0, 100)) + list(range(262_100, 262_200)) + list(range(n - 100, n)):
+        want, holes = O.clip_contours(S.poly(2 * t), S.poly(2 * t + 1), "int")
+        mine = ref.pair(t)
+        assert len(mine) == len(want), t
+        for (c, h), r, rh in zip(mine, want, holes):
+            assert h == bool(rh) and np.array_equal(c, r), t
