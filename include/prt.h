@@ -291,7 +291,10 @@ int prt_poly_mbr_dev(const double *xy, const int64_t *off, int64_t n, float *mbr
  * stage_bytes (a stage too small for P = 8 is an error).  The write pass,
  * given the same stage unchanged, copies the kept pairs out and sweeps again
  * only the pairs that did not fit, so the two passes must see the same
- * pairs, polygons and op.  The output is the same with or without a stage;
+ * pairs, polygons and op (the count pass stamps the stage with its
+ * arguments; a write pass with other arguments, or given a stage the count
+ * pass did not fill, sets *status).  The output is the same with or without
+ * a stage;
  * a pass takes as long as its slowest pair, so size P for every pair to
  * fit (about the total vertex count plus 4 per pair). */
 size_t prt_overlay_contours_workspace_size(int64_t npairs, int32_t max_verts,

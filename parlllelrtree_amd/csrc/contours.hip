@@ -76,7 +76,7 @@ struct ContourOut {  // write pass
 // What the count pass keeps: sc[p] = contours of pair p kept, their records
 // chained from cfirst[p]; -1 when the pair is on the spill list instead (it
 // did not fit, or it failed).  ctr = {spilled pairs, vertex chunks taken,
-// contour records taken}.
+// contour records taken, tag of the count pass's arguments}.
 constexpr int CT_CHUNK = 8;
 struct StageWs {
     unsigned long long *ctr;
@@ -539,18 +539,26 @@ __global__ __launch_bounds__(CT_THREADS) void k_contours_count(
     }
 }
 
-__global__ void k_stage_clear(unsigned long long *ctr) {
+__global__ void k_stage_clear(unsigned long long *ctr, unsigned long long tag) {
     if (threadIdx.x < 3) ctr[threadIdx.x] = 0;
+    if (threadIdx.x == 3) ctr[3] = tag;
 }
 
 // Write pass of the kept pairs: the pair's contour records in closing order,
-// each contour's vertices along its chain in the vertex pool.  Every index
-// read from the stage is checked, so a stage that does not belong to these
-// counts sets *status instead of reading or writing out of bounds.
+// each contour's vertices along its chain in the vertex pool.  A stage whose
+// tag is not this call's (the count pass ran with other arguments, or not
+// into this stage) sets *status and is not read; every index read from the
+// stage is checked as well, so a stage that does not belong to these counts
+// sets *status instead of reading or writing out of bounds.
 __global__ __launch_bounds__(256) void k_contours_copy(int64_t np, StageWs st,
+                                                       unsigned long long tag,
                                                        const int64_t *__restrict__ cont_off,
                                                        const int64_t *__restrict__ vert_off,
                                                        ContourOut out, int32_t *status) {
+    if (st.ctr[3] != tag) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(status, 1);
+        return;
+    }
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < np;
          p += stride) {
@@ -592,15 +600,16 @@ __global__ __launch_bounds__(CT_THREADS) void k_contours_write(
     const int64_t *__restrict__ offA, const double *__restrict__ xyB,
     const int64_t *__restrict__ offB, int op, ContourWs cw, const int64_t *__restrict__ cont_off,
     const int64_t *__restrict__ vert_off, ContourOut out, const int64_t *__restrict__ list,
-    const unsigned long long *__restrict__ nlist, int32_t *status) {
+    const unsigned long long *__restrict__ ctr, unsigned long long tag, int32_t *status) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= cw.base.T) return;
     Lane L(cw.base, t);
     int64_t cnt = np;
-    if (list) {
-        const unsigned long long k = *nlist;
-        cnt = k > static_cast<unsigned long long>(np) ? 0 : static_cast<int64_t>(k);
-        if (k > static_cast<unsigned long long>(np)) atomicOr(status, 1);
+    if (list) {  // ctr[0] = listed pairs; the tag as in k_contours_copy
+        const unsigned long long k = ctr[0];
+        const bool ok = ctr[3] == tag && k <= static_cast<unsigned long long>(np);
+        cnt = ok ? static_cast<int64_t>(k) : 0;
+        if (!ok && t == 0) atomicOr(status, 1);
     }
     for (int64_t i = t; i < cnt; i += cw.base.T) {
         const int64_t p = list ? list[i] : i;
@@ -666,7 +675,7 @@ int64_t stage_records(int64_t vcap) { return vcap / 3 + 1; }
 template <class C> void carve_stage(C &c, StageWs &st, int64_t np, int64_t vcap) {
     st.vcap = vcap;
     st.ccap = stage_records(vcap);
-    st.ctr = c.template take<unsigned long long>(3);
+    st.ctr = c.template take<unsigned long long>(4);
     st.spill = c.template take<int64_t>(np);
     st.sc = c.template take<int32_t>(np);
     st.cfirst = c.template take<int32_t>(np);
@@ -702,6 +711,23 @@ int64_t stage_pool(int64_t np, size_t bytes) {
         else hi = mid - 1;
     }
     return lo;
+}
+
+// tag of a contour call's arguments, stamped into the stage by the count
+// pass and checked by the write pass (FNV-1a over the words)
+unsigned long long stage_tag(const void *pairs, int64_t np, const void *xyA, const void *offA,
+                             const void *xyB, const void *offB, int op) {
+    const unsigned long long w[] = {reinterpret_cast<uintptr_t>(pairs),
+                                    static_cast<unsigned long long>(np),
+                                    reinterpret_cast<uintptr_t>(xyA),
+                                    reinterpret_cast<uintptr_t>(offA),
+                                    reinterpret_cast<uintptr_t>(xyB),
+                                    reinterpret_cast<uintptr_t>(offB),
+                                    static_cast<unsigned long long>(op)};
+    unsigned long long h = 0xcbf29ce484222325ull;
+    for (unsigned long long v : w)
+        for (int b = 0; b < 8; ++b) h = (h ^ ((v >> (8 * b)) & 0xFFu)) * 0x100000001b3ull;
+    return h;
 }
 
 int carve_stage_checked(StageWs &st, int64_t np, const void *stage, size_t bytes) {
@@ -770,7 +796,8 @@ extern "C" int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs,
         PRT_REQUIRE(carve_stage_checked(st, npairs, stage, stage_bytes) == PRT_OK, PRT_E_NOMEM,
                     "overlay_contours_count: stage of %zu bytes too small for %lld pairs",
                     stage_bytes, static_cast<long long>(npairs));
-        PRT_LAUNCH(k_stage_clear, 1, 64, 0, as_stream(stream), st.ctr);
+        PRT_LAUNCH(k_stage_clear, 1, 64, 0, as_stream(stream), st.ctr,
+                   stage_tag(pairs, npairs, xyA, offA, xyB, offB, op));
         PRT_LAUNCHED();
         PRT_LAUNCH(k_contours_count<CT_STAGE>, grid, CT_THREADS, 0, as_stream(stream), pairs,
                    npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert);
@@ -817,17 +844,18 @@ extern "C" int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs,
         PRT_REQUIRE(carve_stage_checked(st, npairs, stage, stage_bytes) == PRT_OK, PRT_E_NOMEM,
                     "overlay_contours_write: stage of %zu bytes too small for %lld pairs",
                     stage_bytes, static_cast<long long>(npairs));
+    const unsigned long long tag = stage_tag(pairs, npairs, xyA, offA, xyB, offB, op);
     PRT_LAUNCH(k_status_clear, 1, 64, 0, as_stream(stream), status);
     PRT_LAUNCHED();
     if (stage) {
         const int64_t cgrid = ceil_div(npairs, 256) < 65536 ? ceil_div(npairs, 256) : 65536;
         PRT_LAUNCH(k_contours_copy, static_cast<unsigned>(cgrid), 256, 0, as_stream(stream),
-                   npairs, st, cont_off, vert_off, out, status);
+                   npairs, st, tag, cont_off, vert_off, out, status);
         PRT_LAUNCHED();
     }
     PRT_LAUNCH(k_contours_write, grid, CT_THREADS, 0, as_stream(stream), pairs, npairs, xyA, offA,
                xyB, offB, static_cast<int>(op), cw, cont_off, vert_off, out,
-               stage ? st.spill : nullptr, stage ? st.ctr : nullptr, status);
+               stage ? st.spill : nullptr, stage ? st.ctr : nullptr, tag, status);
     PRT_LAUNCHED();
     return PRT_OK;
 }

@@ -120,10 +120,11 @@ def test_stage_sizes_give_the_same_contours(dev):
 
 
 def test_write_with_a_stage_the_count_pass_did_not_fill(dev):
-    """The write pass checks every index it reads from the stage: a zeroed
-    stage (no pair listed for a new sweep, every pair claiming 0 kept
-    contours from record 0) sets the status instead of writing out of
-    bounds."""
+    """The write pass trusts a stage only when it carries the tag of the
+    count pass with the same arguments, and checks every index it reads from
+    it: a zeroed stage, or the count pass's stage given to a write pass with
+    another op, sets the status instead of writing stale or out-of-bounds
+    output."""
     from parlllelrtree_amd._lib import lib
     A = gen.stars_grid(gen.seed(2), 500, 23)
     B = gen.stars_grid(gen.seed(2, True), 500, 23)
@@ -155,13 +156,16 @@ def test_write_with_a_stage_the_count_pass_did_not_fill(dev):
     status = torch.zeros(1, dtype=torch.int32, device="cuda")
     ws2 = torch.empty(lib().prt_overlay_contours_workspace_size(np_, mv, nv),
                       dtype=torch.uint8, device="cuda")
-    for st, want in ((stage, 0), (torch.zeros_like(stage), 1)):
-        assert lib().prt_overlay_contours_write(*common, cont_off.data_ptr(),
+    # the count pass's stage; a zeroed one; the count pass's stage given to a
+    # write pass with another op (its tag differs)
+    for st, op, want in ((stage, 1, 0), (torch.zeros_like(stage), 1, 1), (stage, 2, 1)):
+        args = common[:-1] + (op,)
+        assert lib().prt_overlay_contours_write(*args, cont_off.data_ptr(),
                                                 vert_off.data_ptr(), nv, cs.data_ptr(),
                                                 hole.data_ptr(), xy.data_ptr(),
                                                 status.data_ptr(), st.data_ptr(), sb,
                                                 ws2.data_ptr(), ws2.numel(), s) == 0
-        assert int(status.item()) == want
+        assert int(status.item()) == want, (op, want)
 
 
 def test_more_pairs_than_lanes_with_and_without_stage(dev):
