@@ -330,3 +330,30 @@ def test_bu_build_ref4d_bit_exact(dev, n, f):
         np.testing.assert_array_equal(getattr(t, k).cpu().numpy(), ot[k], err_msg=k)
     np.testing.assert_array_equal(t.entry_mbr.cpu().numpy().view(np.uint32),
                                   ot["entry_mbr"].view(np.uint32))
+
+
+def test_builds_and_joins_on_signed_points_and_wide_magnitudes(dev):
+    """Negative coordinates, points (zero-size rects), subnormal and huge
+    magnitudes: the top-down keys order them by ord32, the bottom-up
+    CENTER30 key clamps centres outside [0, 1] (long runs of equal keys kept
+    in input order); trees bit-exact, joins equal to brute force."""
+    rng = np.random.default_rng(31)
+    n = 40000
+    lo = rng.choice([-1e3, -1.0, 0.0, 1e-40, 1e-3, 1.0, 1e30], size=(n, 2)).astype(np.float32)
+    lo *= rng.random((n, 2)).astype(np.float32)
+    w = np.where(rng.random((n, 2)) < 0.3, np.float32(0),
+                 rng.random((n, 2)).astype(np.float32) * np.float32(0.01))
+    r = np.concatenate([lo, lo + w], 1).astype(np.float32)
+    check_td(rtree.build_top_down(dev_rects(r), 16), O.td_build(r, 16))
+    obu = O.bu_build(r, 16)
+    check_bu(rtree.build_bottom_up(dev_rects(r), 16), obu)
+    TL = rtree.TopDownLayers([dev_rects(r), dev_rects(r[::-1].copy())], 16)
+    check_td(TL.trees[0], O.td_build(r, 16))
+    check_td(TL.trees[1], O.td_build(r[::-1].copy(), 16))
+    q = r[rng.integers(0, n, 3000)].copy()
+    q[:, 2:] += np.float32(0.005)
+    brute = O.join_brute(r, q)
+    assert len(brute) > 3000
+    np.testing.assert_array_equal(u32(rtree.join(rtree.build_bottom_up(dev_rects(r), 16),
+                                                 dev_rects(q))), brute)
+    np.testing.assert_array_equal(u32(rtree.join(TL.trees[0], dev_rects(q))), brute)
