@@ -173,6 +173,27 @@ __device__ inline uint32_t digit_lanes_below(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (every lane active), by
+// data-parallel-primitive moves instead of LDS permutes: within each row of
+// 16 lanes add the lanes 1, 2 and 3 below (a 4-lane prefix), then 4 below
+// for the upper three banks and 8 below for the upper two, then broadcast
+// lane 15 into rows 1 and 3 and lane 31 into rows 2 and 3.
+template <int CTRL, int ROW, int BANK>
+__device__ inline uint32_t dpp_src(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROW, BANK, true));
+}
+__device__ inline uint32_t wave_incl_scan(uint32_t x) {
+    uint32_t v = x + dpp_src<0x111, 0xF, 0xF>(x);   // row_shr:1
+    v += dpp_src<0x112, 0xF, 0xF>(x);               // row_shr:2
+    v += dpp_src<0x113, 0xF, 0xF>(x);               // row_shr:3
+    v += dpp_src<0x114, 0xF, 0xE>(v);               // row_shr:4, banks 1-3
+    v += dpp_src<0x118, 0xF, 0xC>(v);               // row_shr:8, banks 2-3
+    v += dpp_src<0x142, 0xA, 0xF>(v);               // row_bcast:15 into rows 1, 3
+    v += dpp_src<0x143, 0xC, 0xF>(v);               // row_bcast:31 into rows 2, 3
+    return v;
+}
+
 // ---- polygon set operations (gpc_op order: DIFF, INT, XOR, UNION) ------
 // A gap between neighbouring edges belongs to the result set by the even-odd
 // parities of A (subject, pp) and B (clip, pq).

@@ -281,14 +281,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
         }
         lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
     }
-    uint32_t x = run, g = tot;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64), h = __shfl_up(g, o, 64);
-        if (lane >= o) {
-            x += y;
-            g += h;
-        }
-    }
+    const uint32_t x = wave_incl_scan(run), g = wave_incl_scan(tot);
     if (lane == 63) {
         red[0][wave] = x;
         red[1][wave] = g;

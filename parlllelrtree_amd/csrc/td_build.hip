@@ -235,11 +235,7 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_o
             c[q] = wcnt[w * 256 + d];
             run += c[q];
         }
-        uint32_t x = run;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
+        const uint32_t x = wave_incl_scan(run);
         __shared__ uint32_t wsum[ST_WAVES];
         if (lane == 63) wsum[wave] = x;
         __syncthreads();
