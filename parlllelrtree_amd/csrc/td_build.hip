@@ -261,9 +261,20 @@ __device__ void block_radix(uint16_t *ord, uint16_t *ord2, const uint32_t *key_o
     }
 }
 
-__device__ inline float4 shfl_xor_box(float4 v, int x) {
-    return make_float4(__shfl_xor(v.x, x, 64), __shfl_xor(v.y, x, 64), __shfl_xor(v.z, x, 64),
-                       __shfl_xor(v.w, x, 64));
+// box of lane + x (lanes past the wave's end read their own)
+__device__ inline float4 box_from_above(float4 v, int x) {
+    return make_float4(__shfl_down(v.x, x, 64), __shfl_down(v.y, x, 64), __shfl_down(v.z, x, 64),
+                       __shfl_down(v.w, x, 64));
+}
+// the same for x = CTRL - 0x100 < 16 within a row of 16 lanes, by a DPP
+// row shift (lanes past the row's end read their own)
+template <int CTRL> __device__ inline float dpp_shl(float v) {
+    const int b = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ inline float4 box_from_above_row(float4 v) {
+    return make_float4(dpp_shl<CTRL>(v.x), dpp_shl<CTRL>(v.y), dpp_shl<CTRL>(v.z),
+                       dpp_shl<CTRL>(v.w));
 }
 
 // Slot i of level q (entries of cover = 2^lc leaves) for box x of the
@@ -468,16 +479,29 @@ void k_td_subtree(SubtreeArgs A) {
     // 4. levels L-2 .. p0: box of an entry of level q = union of the boxes
     //    of its `cover` leaf positions.  While cover <= 64 those positions
     //    are lanes of one wave (position k = tid + m * 1024 is lane k % 64):
-    //    butterfly unions across lanes, and the lane of the entry's first
-    //    position writes its slot.  The levels above go through LDS.
+    //    a tree reduction toward the group's first lane (it adds the lane x
+    //    above for x = 1, 2, 4, ...; within a row of 16 lanes by DPP row
+    //    shifts), and that lane writes the entry's slot (the union is order
+    //    free: min and max with -0 below +0).  The levels above go through
+    //    LDS.
     int lc = 0;  // log2(cover)
     int q = A.L - 2;
     for (; q >= A.p0 && lc + A.logf <= 6; --q) {
-        const int prev = 1 << lc;
+        int x = 1 << lc;
         lc += A.logf;
-        for (int x = prev; x < (1 << lc); x <<= 1) {
+        if (x == 1 && lc >= 4) {
 #pragma unroll
-            for (int m = 0; m < ST_PER; ++m) mine[m] = box_union(mine[m], shfl_xor_box(mine[m], x));
+            for (int m = 0; m < ST_PER; ++m) {
+                mine[m] = box_union(mine[m], box_from_above_row<0x101>(mine[m]));
+                mine[m] = box_union(mine[m], box_from_above_row<0x102>(mine[m]));
+                mine[m] = box_union(mine[m], box_from_above_row<0x104>(mine[m]));
+                mine[m] = box_union(mine[m], box_from_above_row<0x108>(mine[m]));
+            }
+            x = 16;
+        }
+        for (; x < (1 << lc); x <<= 1) {
+#pragma unroll
+            for (int m = 0; m < ST_PER; ++m) mine[m] = box_union(mine[m], box_from_above(mine[m], x));
         }
         if ((lane & ((1 << lc) - 1)) == 0) {
 #pragma unroll
