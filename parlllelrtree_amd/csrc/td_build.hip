@@ -147,7 +147,63 @@ constexpr size_t ST_LDS_ALONE = 96 * 1024;
 
 // Sorts every aligned block of B positions (B a power of two <= 256)
 // ascending; position of register r of thread t is 4t + r.
+// v of lane (this lane ^ lx) for a power of two lx below 64, every lane
+// active: within a quad and across half a row (xor 8, a rotation of the row
+// by 8) a DPP move, within 32 lanes a swizzle, across the wave's halves a
+// permute.  Called with constant lx, the choice folds away.
+__device__ inline uint32_t lane_xor(uint32_t v, int lx) {
+    const int x = static_cast<int>(v);
+    switch (lx) {
+    case 1: return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    case 2: return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    case 4: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x101F));
+    case 8: return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
+    case 16: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x401F));
+    default: return static_cast<uint32_t>(__shfl_xor(x, lx, 64));
+    }
+}
+__device__ inline uint64_t lane_xor64(uint64_t v, int lx) {
+    return (static_cast<uint64_t>(lane_xor(static_cast<uint32_t>(v >> 32), lx)) << 32) |
+           lane_xor(static_cast<uint32_t>(v), lx);
+}
+
+// The network for a block size fixed at compile time: fully unrolled, every
+// exchange distance a constant.
+template <int B> __device__ inline void block_bitonic_fixed(uint64_t (&kk)[ST_PER]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 2; k <= B; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j >= ST_PER; j >>= 1) {
+            const int i0 = ST_PER * t;
+            const bool want_max = ((i0 & j) == 0) != ((k == B) || ((i0 & k) == 0));
+#pragma unroll
+            for (int r = 0; r < ST_PER; ++r) {
+                const uint64_t o = lane_xor64(kk[r], j / ST_PER);
+                const bool take = (o < kk[r]) != want_max;
+                kk[r] = take ? o : kk[r];
+            }
+        }
+#pragma unroll
+        for (int j = (k >> 1) < ST_PER ? (k >> 1) : ST_PER / 2; j > 0; j >>= 1) {
+#pragma unroll
+            for (int r = 0; r < ST_PER; ++r) {
+                if (r & j) continue;
+                const int i = ST_PER * t + r;
+                const bool up = (k == B) || ((i & k) == 0);
+                const uint64_t x = kk[r], y = kk[r | j];
+                if ((x > y) == up) {
+                    kk[r] = y;
+                    kk[r | j] = x;
+                }
+            }
+        }
+    }
+}
+
 __device__ inline void block_bitonic(uint64_t (&kk)[ST_PER], int B) {
+    if (B == 256) return block_bitonic_fixed<256>(kk);
+    if (B == 16) return block_bitonic_fixed<16>(kk);
     const int t = threadIdx.x;
     for (int k = 2; k <= B; k <<= 1) {
         int j = k >> 1;
