@@ -8,7 +8,9 @@
  * reference-shaped shim prt_run(), which synchronises like the reference.
  * No entry point allocates device memory except prt_run() and prt_read_data();
  * prt_pipeline_capture allocates the host-side graph objects it returns;
- * the batched overlay creates, once per device, two helper streams.
+ * the batched overlay creates, once per device, two helper streams (shared
+ * by every call on that device: capture one batched overlay at a time per
+ * device).
  * Return value: 0 (PRT_OK) or a negative PRT_E_* code; prt_last_error()
  * returns a message for the last failure on the calling thread.
  *
@@ -59,6 +61,17 @@ extern "C" {
 #define PRT_E_HIP (-3)
 #define PRT_E_OVERFLOW (-4)
 #define PRT_E_UNSUPPORTED (-5)
+#define PRT_E_DEVICE (-6)     /* a kernel raised a PRT_STATUS_* flag (prt_device_status) */
+
+/* Device status flags (prt_device_status).  Kernels raise them when a
+ * result they wrote cannot be trusted; the entry points that launched them
+ * have already returned, so the flags are read by prt_device_status. */
+#define PRT_STATUS_LOOKBACK 1 /* a radix sort's look-back gave up waiting for an
+                                 earlier tile: the sorted order is invalid   */
+#define PRT_STATUS_INEXACT 2  /* prt_bu_build_ref4d rounded int64 coordinates
+                                 that fp32 cannot hold outward: joins over
+                                 that tree return a superset of the exact
+                                 int64 Overlap hit set (BU:496-506)          */
 
 /* Polygon set operations, in the order of GPC's gpc_op (OV:264-306,
  * 492-525): the area of A - B, A & B, A ^ B, A | B. */
@@ -119,6 +132,21 @@ typedef struct prt_tune {
 const char *prt_version(void);
 const char *prt_last_error(void);
 
+/* Waits for `stream`, then reads this device's status flags (PRT_STATUS_*
+ * bits, OR-ed by every kernel that raised one since the last clear) into
+ * *flags_h (optional) and clears them if `clear`.  Returns PRT_OK when no
+ * flag is raised, PRT_E_DEVICE (message in prt_last_error) otherwise.  No
+ * reference counterpart: the reference checks nothing on the device
+ * (checkErrors after run(), OV:78, sees only launch errors). */
+int prt_device_status(int32_t *flags_h, int clear, void *stream);
+
+/* Test hook of the radix sort's look-back: polls before it gives up
+ * (spin_limit < 0: the default, about a second), and one tile (mute_tile >=
+ * 0) that publishes no look-back word in any pass, so that the tiles after
+ * it give up and raise PRT_STATUS_LOOKBACK while every kernel still ends.
+ * (-1, -1) restores the defaults.  No reference counterpart. */
+int prt_debug_lookback(int32_t spin_limit, int64_t mute_tile);
+
 /* ---- optional per-kernel timing (HIP events around every launch) ----
  * No reference counterpart (the reference times whole phases with
  * cudaEvent, BU:836-874).  prt_profile_read aggregates per kernel name:
@@ -149,8 +177,9 @@ int prt_bu_build(const float *rects, int64_t n, int fanout, prt_bu_tree *tree,
  * CalcuateZOrderValue/Interleave BU:73-123, sorted BU:841-849): rects_i64 =
  * int64 (x1, y1, x2, y2) per rect (device), keys = Interleave of rect/scale,
  * leaf order stable by (64-bit key, id).  Node boxes are the rects in fp32,
- * rounded outward (exact for |v| <= 2^24).  tree->keys gets the high word of
- * each sorted key; keys_sorted (optional, device, n) the full keys. */
+ * rounded outward (exact for |v| <= 2^24); a coordinate fp32 cannot hold
+ * raises PRT_STATUS_INEXACT (prt_device_status).  tree->keys gets the high
+ * word of each sorted key; keys_sorted (optional, device, n) the full keys. */
 size_t prt_bu_ref4d_workspace_size(int64_t n, int fanout);
 int prt_bu_build_ref4d(const int64_t *rects_i64, int64_t n, int fanout, int64_t scale,
                        prt_bu_tree *tree, uint64_t *keys_sorted, void *ws, size_t ws_bytes,

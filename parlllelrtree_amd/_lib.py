@@ -18,7 +18,10 @@ PRT_MAX_LAYERS = 4
 PRT_TUNE_BUCKETS = 6
 
 ERRORS = {0: "OK", -1: "PRT_E_INVAL", -2: "PRT_E_NOMEM", -3: "PRT_E_HIP",
-          -4: "PRT_E_OVERFLOW", -5: "PRT_E_UNSUPPORTED"}
+          -4: "PRT_E_OVERFLOW", -5: "PRT_E_UNSUPPORTED", -6: "PRT_E_DEVICE"}
+PRT_E_DEVICE = -6
+PRT_STATUS_LOOKBACK = 1
+PRT_STATUS_INEXACT = 2
 
 
 class PrtError(RuntimeError):
@@ -69,6 +72,8 @@ class PipelineIO(ctypes.Structure):
 SIGNATURES = {
     "prt_version": (c_char_p, []),
     "prt_last_error": (c_char_p, []),
+    "prt_device_status": (c_int, [c_void_p, c_int, c_void_p]),
+    "prt_debug_lookback": (c_int, [c_int32, c_int64]),
     "prt_profile_enable": (None, [c_int]),
     "prt_profile_read": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     "prt_morton_center30": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),

@@ -295,30 +295,52 @@ extern "C" int prt_bu_build(const float *rects, int64_t n, int fanout, prt_bu_tr
 }
 
 namespace {
+// Sign of f - v for an fp32 value f rounded from the int64 v, compared
+// exactly (as integers: every fp32 of magnitude below 2^63 is an integer
+// there once it exceeds 2^24, and the rounding of v is within a unit of it).
+__device__ inline int f32_cmp(float f, int64_t v) {
+    if (f >= 0x1p63f) return 1;
+    if (f < -0x1p63f) return -1;
+    const int64_t fi = static_cast<int64_t>(f);
+    return fi > v ? 1 : (fi < v ? -1 : 0);
+}
 __device__ inline float f32_down(int64_t v) {
     const float f = static_cast<float>(v);
-    return static_cast<double>(f) > static_cast<double>(v) ? nextafterf(f, -__builtin_huge_valf()) : f;
+    return f32_cmp(f, v) > 0 ? nextafterf(f, -__builtin_huge_valf()) : f;
 }
 __device__ inline float f32_up(int64_t v) {
     const float f = static_cast<float>(v);
-    return static_cast<double>(f) < static_cast<double>(v) ? nextafterf(f, __builtin_huge_valf()) : f;
+    return f32_cmp(f, v) < 0 ? nextafterf(f, __builtin_huge_valf()) : f;
 }
 
 // Reference keys (as k_morton_ref4d) and the fp32 boxes of the integer rects
-// (rounded outward: exact for |v| <= 2^24, containing otherwise).
+// (rounded outward: exact for |v| <= 2^24, containing otherwise).  A
+// coordinate that fp32 cannot hold raises PRT_STATUS_INEXACT: the tree's
+// boxes then contain the rects without equalling them, and joins over it
+// return a superset of the int64 Overlap hit set (BU:496-506).
 __global__ void k_ref4d_prepare(const int64_t *__restrict__ rects, int64_t n, int64_t scale,
-                                uint64_t *__restrict__ keys, float4 *__restrict__ r4) {
+                                uint64_t *__restrict__ keys, float4 *__restrict__ r4,
+                                uint32_t *flags) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k = 0;
+    bool inexact = false;
+    if (i < n) {
+        uint64_t k = 0;
+        int64_t v[4];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        const uint64_t a = static_cast<uint64_t>(rects[4 * i + w] / scale);
-        k |= spread_ref4(a) << (6 - 2 * w);
+        for (int w = 0; w < 4; ++w) {
+            v[w] = rects[4 * i + w];
+            const uint64_t a = static_cast<uint64_t>(v[w] / scale);
+            k |= spread_ref4(a) << (6 - 2 * w);
+        }
+        keys[i] = k;
+        const float4 b = make_float4(f32_down(v[0]), f32_down(v[1]), f32_up(v[2]), f32_up(v[3]));
+        r4[i] = b;
+        inexact = f32_cmp(b.x, v[0]) != 0 || f32_cmp(b.y, v[1]) != 0 || f32_cmp(b.z, v[2]) != 0 ||
+                  f32_cmp(b.w, v[3]) != 0;
     }
-    keys[i] = k;
-    r4[i] = make_float4(f32_down(rects[4 * i]), f32_down(rects[4 * i + 1]), f32_up(rects[4 * i + 2]),
-                        f32_up(rects[4 * i + 3]));
+    // one flag write per wave that holds an inexact coordinate
+    if (__ballot(inexact) != 0ull && (threadIdx.x & 63) == 0)
+        atomicOr(flags, static_cast<uint32_t>(PRT_STATUS_INEXACT));
 }
 }  // namespace
 
@@ -354,8 +376,10 @@ extern "C" int prt_bu_build_ref4d(const int64_t *rects, int64_t n, int fanout, i
     RadixWs rw;
     PRT_REQUIRE(radix_carve(c, n, 0, rw) == PRT_OK && c.ok(), PRT_E_NOMEM,
                 "bu_build_ref4d: workspace too small");
+    uint32_t *flags = device_status_flags();
+    PRT_REQUIRE(flags != nullptr, PRT_E_HIP, "bu_build_ref4d: no device status word");
     PRT_LAUNCH(k_ref4d_prepare, dim3(static_cast<unsigned>(ceil_div(n, 256))), dim3(256), 0, s,
-               rects, n, scale, keys, r4);
+               rects, n, scale, keys, r4, flags);
     PRT_LAUNCHED();
     rc = radix_sort_keys64(keys, n, b0, b1, b2, b3, rw, s, items, keys_sorted);
     if (rc) return rc;

@@ -804,12 +804,50 @@ int aux_streams(hipStream_t *out) {
     return PRT_OK;
 }
 
-struct EventSet {  // the fork event and one join event per helper stream
-    hipEvent_t fork = nullptr, join[AUX_STREAMS] = {};
-    ~EventSet() {
-        if (fork) (void)hipEventDestroy(fork);
-        for (hipEvent_t e : join)
+// The helper streams forked from the caller's stream `s` by one event, and
+// joined back into `s` when this goes out of scope, on every return path: an
+// error after the fork must not leave helper-stream work running on the
+// caller's buffers after the call returns, nor leave a capture with work that
+// never joins its origin stream.  The helper streams are shared by every
+// call on the device: capture one batched overlay at a time per device.
+struct StreamFork {
+    hipStream_t s;
+    hipStream_t aux[AUX_STREAMS] = {};
+    hipEvent_t fork = nullptr;
+    bool forked[AUX_STREAMS] = {};
+    explicit StreamFork(hipStream_t origin) : s(origin) {}
+    // helper stream i, made to wait for everything enqueued on s so far
+    int take(int i, hipStream_t *out) {
+        if (!fork) {
+            PRT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            PRT_HIP(hipEventRecord(fork, s));
+        }
+        if (!forked[i]) {
+            PRT_HIP(hipStreamWaitEvent(aux[i], fork, 0));
+            forked[i] = true;
+        }
+        *out = aux[i];
+        return PRT_OK;
+    }
+    // s waits for every helper stream that was forked; the first error wins
+    int join() {
+        int rc = PRT_OK;
+        for (int i = 0; i < AUX_STREAMS; ++i) {
+            if (!forked[i]) continue;
+            forked[i] = false;
+            hipEvent_t e = nullptr;
+            hipError_t er = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+            if (er == hipSuccess) er = hipEventRecord(e, aux[i]);
+            if (er == hipSuccess) er = hipStreamWaitEvent(s, e, 0);
+            if (er != hipSuccess && rc == PRT_OK) rc = hip_fail(er, "join of a helper stream",
+                                                                 __FILE__, __LINE__);
             if (e) (void)hipEventDestroy(e);
+        }
+        return rc;
+    }
+    ~StreamFork() {
+        (void)join();
+        if (fork) (void)hipEventDestroy(fork);
     }
 };
 
@@ -945,13 +983,9 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
     int max_lds = 0, dev = 0;
     PRT_HIP(hipGetDevice(&dev));
     PRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    hipStream_t aux[AUX_STREAMS];
-    rc = aux_streams(aux);
+    StreamFork fk(s);
+    rc = aux_streams(fk.aux);
     if (rc) return rc;
-    EventSet ev;
-    PRT_HIP(hipEventCreateWithFlags(&ev.fork, hipEventDisableTiming));
-    PRT_HIP(hipEventRecord(ev.fork, s));
-    bool forked[AUX_STREAMS] = {};
     for (int b = NBUCKET - 1; b >= 0; --b) {
         const int lo = b == 0 ? 0 : tune->bucket_max_verts[b - 1];
         if (max_verts <= lo) continue;  // no pair can fall in this bucket
@@ -974,11 +1008,8 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         const int slot = NBUCKET - 1 - b;  // 0, 1 for the two largest buckets
         hipStream_t bs = s;
         if (slot < AUX_STREAMS) {
-            bs = aux[slot];
-            if (!forked[slot]) {
-                PRT_HIP(hipStreamWaitEvent(bs, ev.fork, 0));
-                forked[slot] = true;
-            }
+            rc = fk.take(slot, &bs);
+            if (rc) return rc;
         }
         const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
         const size_t lds = static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
@@ -996,12 +1027,8 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         }
         if (rc) return rc;
     }
-    for (int i = 0; i < AUX_STREAMS; ++i) {
-        if (!forked[i]) continue;
-        PRT_HIP(hipEventCreateWithFlags(&ev.join[i], hipEventDisableTiming));
-        PRT_HIP(hipEventRecord(ev.join[i], aux[i]));
-        PRT_HIP(hipStreamWaitEvent(s, ev.join[i], 0));
-    }
+    rc = fk.join();
+    if (rc) return rc;
     return plain_list_launch(pairs, w.ovf_list, reinterpret_cast<const int64_t *>(w.ovf_count),
                              npairs, xyA, offA, xyB, offB, max_verts, op, area, w.plain_ws,
                              w.plain_bytes, s);

@@ -35,6 +35,10 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Address of the current device's status flag word (PRT_STATUS_* bits),
+// passed to the kernels that raise them; null if it cannot be resolved.
+uint32_t *device_status_flags();
+
 // ---- workspace carving: 256-byte aligned sub-buffers --------------------
 struct Carver {
     char *base;

@@ -87,6 +87,11 @@ constexpr uint64_t LB_VAL = (1ull << 62) - 1;
 constexpr int LB_SPIN_LIMIT = 1 << 22;  // safety net only (seconds of polling)
 constexpr int LB_WIN = 8;                // look-back words loaded per step
 
+// Look-back test hook (prt_debug_lookback): polls before giving up, and a
+// tile that publishes nothing.  Read on the host at every launch.
+int g_spin_limit = LB_SPIN_LIMIT;
+int64_t g_mute_tile = -1;
+
 __device__ inline uint64_t lb_load(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -216,13 +221,18 @@ struct PassGather {
 // w*256 + j*64 + l.  Output position of an item with digit d = segment start
 // + (items of the segment with a smaller digit: scan of ghist) + (items with
 // digit d in earlier tiles of the segment: look-back) + rank in the tile.
+// A look-back that polls an unpublished word more than spin_limit times
+// gives up and raises PRT_STATUS_LOOKBACK in *flags (the order it writes is
+// then wrong, and the flag says so); mute_tile (test hook, -1 = none)
+// publishes no look-back word.
 __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restrict__ in,
                                                          uint64_t *__restrict__ out, Tiling t,
                                                          int shift, int bits,
                                                          const uint32_t *__restrict__ ghist_p,
                                                          uint64_t *status_p,
                                                          uint32_t *__restrict__ ticket,
-                                                         PassGather gat) {
+                                                         PassGather gat, uint32_t *flags,
+                                                         int spin_limit, int64_t mute_tile) {
     __shared__ uint64_t stage[RS_TILE];
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
     __shared__ uint32_t dstart[RS_BINS];
@@ -279,7 +289,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
             wcnt[w][tid] = run;
             run += c;
         }
-        lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
+        if (tile != mute_tile)
+            lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
     }
     const uint32_t x = wave_incl_scan(run), g = wave_incl_scan(tot);
     if (lane == 63) {
@@ -321,13 +332,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
                 }
                 if (done) break;
                 if (used == 0) {
-                    if (++spins > LB_SPIN_LIMIT) break;
+                    if (++spins > spin_limit) {
+                        if (flags) atomicOr(flags, static_cast<uint32_t>(PRT_STATUS_LOOKBACK));
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
                 k -= used;
             }
-            lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
+            if (tile != mute_tile) lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
         }
         int64_t sb, se;
         seg_range(t, seg, sb, se);
@@ -479,6 +493,8 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
     const uint64_t *src = in;
     uint64_t *dst = (in == buf0) ? buf1 : buf0;
     const bool by_block = t.num_tiles <= onesweep_resident_blocks();
+    uint32_t *flags = device_status_flags();
+    PRT_REQUIRE(flags != nullptr, PRT_E_HIP, "radix sort: no device status word");
     for (int p = 0; p < passes; ++p) {
         const int shift = bit_lo + 8 * p;
         const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
@@ -487,7 +503,7 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
                    ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS,
                    w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS,
                    by_block ? nullptr : tickets + p,
-                   p + 1 == passes ? gat : none);
+                   p + 1 == passes ? gat : none, flags, g_spin_limit, g_mute_tile);
         PRT_LAUNCHED();
         src = dst;
         dst = (dst == buf0) ? buf1 : buf0;
@@ -660,3 +676,9 @@ int scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *total, void *w
 }
 
 }  // namespace prt
+
+extern "C" int prt_debug_lookback(int32_t spin_limit, int64_t mute_tile) {
+    prt::g_spin_limit = spin_limit < 0 ? prt::LB_SPIN_LIMIT : spin_limit;
+    prt::g_mute_tile = mute_tile < 0 ? -1 : mute_tile;
+    return PRT_OK;
+}

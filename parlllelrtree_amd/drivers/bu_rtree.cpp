@@ -66,6 +66,10 @@ int main(int argc, char **argv) {
     DRV_CHECK(prt_sibling_overlap_bu(&t, st + 6, nullptr));
     int64_t hs[7];
     DRV_HIP(hipMemcpy(hs, st, sizeof hs, hipMemcpyDeviceToHost));
-    print_stats(hs, n, hs[6], static_cast<double>(hs[6]) / t.num_nodes / f, false);
+    // ave3 as BU:1026 prints it: sum / num_nodes / m, with num_nodes the
+    // reference's node budget 3 * num_contours / m (integer division, BU:674)
+    const int64_t ref_nodes = 3 * n / f;
+    print_stats(hs, n, hs[6], static_cast<double>(hs[6]) / static_cast<double>(ref_nodes) / f,
+                false);
     return 0;
 }

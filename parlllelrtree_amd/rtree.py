@@ -13,6 +13,7 @@ Errors raise PrtError (the reference prints and exits, OV:25-29).
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass, field
 
 import torch
@@ -29,6 +30,22 @@ def _ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
 
 def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def device_status(clear: bool = True) -> int:
+    """Waits for the current stream and returns this device's PRT_STATUS_*
+    flags (0: no kernel raised one since the last clear); clears them when
+    `clear`.  Raises nothing: callers decide (see check_device_status)."""
+    flags = ctypes.c_int32(0)
+    lib().prt_device_status(ctypes.byref(flags), 1 if clear else 0, _stream())
+    return int(flags.value)
+
+
+def check_device_status() -> None:
+    """Raises PrtError(PRT_E_DEVICE) when a kernel raised a status flag (a
+    radix look-back that gave up, int64 coordinates fp32 cannot hold); the
+    flags are cleared either way."""
+    check(lib().prt_device_status(None, 1, _stream()))
 
 
 def _ws(nbytes: int, device) -> torch.Tensor:
@@ -283,18 +300,22 @@ def search_stats(tree: BottomUpTree | TopDownTree, queries: torch.Tensor) -> dic
 
 def sibling_overlap(tree: BottomUpTree | TopDownTree) -> dict:
     """sum3: pairs of overlapping entries that share a node (RtreeSearch1,
-    BotomUp-rtree.cu:574-590, TopDown-rtree.cu:78-93); ave3 divides by the
-    node slots as the reference prints it (BU:1026: num_nodes * m, TD:797:
-    all slots)."""
+    BotomUp-rtree.cu:574-590, TopDown-rtree.cu:78-93); ave3 as the reference
+    prints it: bottom-up sum3 / num_nodes / m with the reference's node
+    budget num_nodes = 3 * n // m (BU:674, 1026), top-down sum3 / all slots
+    (TD:797).  A zero budget gives inf (or nan for sum3 = 0), as the C
+    division does."""
     out = torch.zeros(1, dtype=torch.int64, device=tree.perm.device)
     if isinstance(tree, BottomUpTree):
         check(lib().prt_sibling_overlap_bu(ctypes.byref(tree.st), _ptr(out), _stream()))
-        slots = tree.st.num_nodes * tree.st.fanout
+        f = tree.st.fanout
+        den, m = 3 * tree.st.num_rects // f, f
     else:
         check(lib().prt_sibling_overlap_td(ctypes.byref(tree.st), _ptr(out), _stream()))
-        slots = tree.st.num_slots
+        den, m = tree.st.num_slots, 1
     s3 = int(out.item())
-    return {"sum3": s3, "ave3": s3 / slots}
+    ave3 = s3 / den / m if den else (math.inf if s3 else math.nan)
+    return {"sum3": s3, "ave3": ave3}
 
 
 @dataclass

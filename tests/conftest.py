@@ -36,5 +36,18 @@ def dev():
     return torch.device("cuda:0")
 
 
+@pytest.fixture(autouse=True)
+def _device_status_clean(request):
+    """Every GPU test ends with no device status flag raised (a radix
+    look-back that gave up, inexact int64 coordinates), unless it raises
+    them on purpose and clears them itself."""
+    yield
+    if "gpu" not in request.keywords or not _has_gpu():
+        return
+    from parlllelrtree_amd import rtree
+    flags = rtree.device_status(clear=True)
+    assert flags == 0, f"device status flags 0x{flags:x} raised during {request.node.name}"
+
+
 def pytest_report_header(config):
     return f"prt tests, GPU available: {_has_gpu()}, cwd {os.getcwd()}"

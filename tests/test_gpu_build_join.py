@@ -357,3 +357,56 @@ def test_builds_and_joins_on_signed_points_and_wide_magnitudes(dev):
     np.testing.assert_array_equal(u32(rtree.join(rtree.build_bottom_up(dev_rects(r), 16),
                                                  dev_rects(q))), brute)
     np.testing.assert_array_equal(u32(rtree.join(TL.trees[0], dev_rects(q))), brute)
+
+
+def test_lookback_giveup_is_reported(dev):
+    """The radix look-back's safety net, forced from the host: tile 1 of
+    every pass publishes nothing and the others poll it 16 times before they
+    give up.  Every kernel still ends; the sort's order is then wrong and
+    prt_device_status reports PRT_STATUS_LOOKBACK as PRT_E_DEVICE.  With the
+    defaults restored the same sort is exact and raises nothing."""
+    from parlllelrtree_amd._lib import PRT_E_DEVICE, PRT_STATUS_LOOKBACK, PrtError, check, lib
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 1 << 32, size=100_000, dtype=np.uint64).astype(np.uint32)
+    kd = torch.from_numpy(keys.view(np.int32)).to(dev)
+    assert rtree.device_status() == 0
+    check(lib().prt_debug_lookback(16, 1))
+    try:
+        rtree.sort_pairs_u32(kd, 32)
+        assert rtree.device_status(clear=False) & PRT_STATUS_LOOKBACK
+        with pytest.raises(PrtError) as e:
+            rtree.check_device_status()
+        assert e.value.code == PRT_E_DEVICE and "look-back" in str(e.value)
+        assert rtree.device_status() == 0      # check_device_status cleared them
+    finally:
+        check(lib().prt_debug_lookback(-1, -1))
+    sk, si = rtree.sort_pairs_u32(kd, 32)
+    order = O.stable_order(keys)
+    np.testing.assert_array_equal(u32(si), order)
+    np.testing.assert_array_equal(u32(sk), keys[order])
+    assert rtree.device_status() == 0
+
+
+def test_bu_build_ref4d_inexact_coordinates_flagged(dev):
+    """int64 coordinates just above 2^24 that fp32 cannot hold: the boxes
+    are rounded outward (they contain the rects) and PRT_STATUS_INEXACT is
+    raised; coordinates above 2^24 that fp32 holds exactly raise nothing."""
+    from parlllelrtree_amd._lib import PRT_STATUS_INEXACT
+    base = 1 << 24
+    exact = np.array([[base, 0, base + 2, 6], [base + 4, 2, base + 8, 10],
+                      [0, 0, 1 << 30, 1 << 40]], np.int64)
+    rtree.build_bottom_up_ref4d(torch.from_numpy(exact).cuda(), 1, 2)
+    assert rtree.device_status() == 0
+    odd = np.array([[base + 1, 0, base + 3, 5], [base + 5, 1, base + 9, 7],
+                    [-(base + 1), -3, 3, base + 3]], np.int64)
+    t, _ = rtree.build_bottom_up_ref4d(torch.from_numpy(odd).cuda(), 1, 2)
+    assert rtree.device_status() & PRT_STATUS_INEXACT
+    mbr = t.entry_mbr.cpu().numpy().reshape(-1, 4)
+    ref = t.entry_ref.cpu().numpy()
+    for slot in range(t.st.level_count[0] * t.st.fanout):
+        if ref[slot] < 0:
+            continue
+        lo_x, lo_y, hi_x, hi_y = (int(v) for v in mbr[slot])
+        r = odd[ref[slot]]
+        assert lo_x <= r[0] and lo_y <= r[1] and hi_x >= r[2] and hi_y >= r[3]
+        assert (lo_x, lo_y, hi_x, hi_y) != tuple(int(v) for v in r)

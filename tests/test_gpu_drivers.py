@@ -50,6 +50,18 @@ def stat_lines(num, cnt, num1, sum3, ave3, td):
     return out + ["sum3 = %d  ave3 = %0.2f" % (sum3, ave3)]
 
 
+def same_stat_lines(got, want):
+    """Line-for-line equality; an ave3 of a zero node budget (inf or nan,
+    whose sign the C library prints its own way) only has to be one."""
+    if len(got) != len(want):
+        return False
+    for g, w in zip(got, want):
+        if g != w and not (w.endswith(("nan", "inf")) and g.split("ave3 = ")[0] ==
+                           w.split("ave3 = ")[0] and g.endswith(w[-3:])):
+            return False
+    return True
+
+
 @pytest.mark.parametrize("n,f", [(1, 10), (37, 4), (20000, 10), (100003, 16)])
 def test_bu_driver(dev, tmp_path, n, f):
     r = int_rects(n, n)
@@ -63,9 +75,13 @@ def test_bu_driver(dev, tmp_path, n, f):
     q = refio.rects_to_f32(rn)
     pairs, _ = O.join_bu(t, q)
     s3 = O.sibling_overlap(t)
+    # ave3 = sum3 / num_nodes / m with the reference's node budget
+    # num_nodes = 3 * num_contours / m (integer division, BU:674, 1026)
+    budget = 3 * n // f
+    ave3 = s3 / budget / f if budget else (float("inf") if s3 else float("nan"))
     want = head + ["pairs=%d" % len(pairs)] + \
-        stat_lines(*O.search_stats(t, q), s3, s3 / t["num_nodes"] / f, td=False)
-    assert got == want
+        stat_lines(*O.search_stats(t, q), s3, ave3, td=False)
+    assert same_stat_lines(got, want), (got, want)
 
 
 @pytest.mark.parametrize("n,f", [(1, 10), (37, 4), (20000, 10), (100003, 16)])
