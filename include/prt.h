@@ -211,6 +211,16 @@ int prt_join_td(const prt_td_tree *tree, const float *queries, int64_t nq,
                 uint32_t *pairs, int64_t cap, int64_t *count_d, void *ws,
                 size_t ws_bytes, void *stream);
 
+/* prt_join_bu with no allocation and no workspace argument: for callers that
+ * cannot allocate (code under graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The workspace is a
+ * fixed device array of the library (15 MiB per device, held until the
+ * process ends), so nq is at most 262,144 (PRT_E_UNSUPPORTED beyond).  Every
+ * call on a device shares that array: calls that overlap on one device
+ * (two streams, two host threads) race. */
+int prt_join_bu_noalloc(const prt_bu_tree *tree, const float *queries, int64_t nq,
+                        uint32_t *pairs, int64_t cap, int64_t *count_d, void *stream);
+
 /* ---- the reference's search statistics --------------------------------
  * Counters of the reference's CPU search over every query, as its main()
  * prints them (BU:990-1027, TD:760-798): stats[0..5] = sum and max of

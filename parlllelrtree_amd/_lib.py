@@ -100,6 +100,8 @@ SIGNATURES = {
                             c_void_p, c_void_p, c_size_t, c_void_p]),
     "prt_join_td": (c_int, [POINTER(TDTreeStruct), c_void_p, c_int64, c_void_p, c_int64,
                             c_void_p, c_void_p, c_size_t, c_void_p]),
+    "prt_join_bu_noalloc": (c_int, [POINTER(BUTreeStruct), c_void_p, c_int64, c_void_p, c_int64,
+                                    c_void_p, c_void_p]),
     "prt_search_stats_bu": (c_int, [POINTER(BUTreeStruct), c_void_p, c_int64, c_void_p,
                                     c_void_p]),
     "prt_search_stats_td": (c_int, [POINTER(TDTreeStruct), c_void_p, c_int64, c_void_p,

@@ -18,6 +18,8 @@
 // Pass 1 also keeps the first JN_KEEP hits of every query: pass 2 copies
 // them for a query with no more hits than that instead of traversing again
 // (a layer join has one or two hits per query).
+#include <mutex>
+
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -360,4 +362,55 @@ extern "C" int prt_join_td(const prt_td_tree *tree, const float *queries, int64_
     tv.perm = tree->perm;
     return run_join(tv, tree->num_levels, queries, nq, pairs, cap, count_d, ws, ws_bytes,
                     as_stream(stream));
+}
+
+// ---- the join with no allocation: workspace in a file-scope device array --
+// For callers that cannot allocate (graph capture, serving loops): the
+// workspace of prt_join_bu for up to JN_NOALLOC_MAX_QUERIES queries lives in
+// `scratch`, one copy per device, held until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): its address is resolved once per device with
+// hipGetSymbolAddress, with that device current, and only that is used.
+namespace prt {
+namespace {
+constexpr int64_t JN_NOALLOC_MAX_QUERIES = 262144;
+constexpr size_t JN_SCRATCH_BYTES = 15u << 20;  // >= prt_join_workspace_size(262144)
+constexpr int JN_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[JN_SCRATCH_BYTES];
+
+void *g_scratch_ptr[JN_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_device(void **out) {
+    int dev = 0;
+    PRT_HIP(hipGetDevice(&dev));
+    PRT_REQUIRE(dev >= 0 && dev < JN_MAX_DEVICES, PRT_E_UNSUPPORTED,
+                "join_bu_noalloc: device %d beyond %d", dev, JN_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        PRT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PRT_REQUIRE(p != nullptr, PRT_E_HIP, "join_bu_noalloc: no scratch address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PRT_OK;
+}
+}  // namespace
+}  // namespace prt
+
+extern "C" int prt_join_bu_noalloc(const prt_bu_tree *tree, const float *queries, int64_t nq,
+                                   uint32_t *pairs, int64_t cap, int64_t *count_d, void *stream) {
+    PRT_REQUIRE(nq >= 0 && nq <= JN_NOALLOC_MAX_QUERIES, PRT_E_UNSUPPORTED,
+                "join_bu_noalloc: %lld queries, at most %lld per call (the scratch holds the "
+                "workspace of that many)",
+                (long long)nq, (long long)JN_NOALLOC_MAX_QUERIES);
+    const size_t need = prt_join_workspace_size(nq);
+    PRT_REQUIRE(need <= JN_SCRATCH_BYTES, PRT_E_UNSUPPORTED,
+                "join_bu_noalloc: workspace %zu > scratch %zu bytes", need, JN_SCRATCH_BYTES);
+    void *ws = nullptr;
+    int rc = scratch_for_device(&ws);
+    if (rc) return rc;
+    return prt_join_bu(tree, queries, nq, pairs, cap, count_d, ws, JN_SCRATCH_BYTES, stream);
 }
