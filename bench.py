@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
-"""bench.py -- R-tree build throughput (and the join / overlay phases) on one
-MI355X, printed as one JSON line.
+"""bench.py -- R-tree build throughput on one MI355X (and the join / overlay
+phases), printed as one JSON line.
 
-Workload (the step): top-down R-tree build, fanout 16, of both layers of
-1,048,576 fp32 rects (centres U[0,1]^2, sides U[1e-4, 1e-3]), inputs
-resident in HBM, both layers in one shared launch sequence
-(prt_td_build_layers); value = rects built per second (Mrect/s).  The same
-line carries, measured separately: the same builds one layer per call, the
-bottom-up build of 1,048,576 rects, the 65,536-window join, the layer-layer
-join, and the plain FP64 overlay of the first 100,000 sorted layer pairs.
-roofline: the dominant kernel of the step, its average duration from HIP
-events that hipExtLaunchKernel stamps with each launch's own start and end.
-cpu_baseline: the oracle's serial top-down build on the host; the same CPU
-leg then compares the phases' GPU results (copied to the host when each
-phase ends) with the oracle and adds the error fields to each phase.
+Workload (the step): the largest single-GPU configuration, 16,777,216 fp32
+rects from a 64-component Gaussian mixture (means U[0,1]^2, sigma 0.01,
+sides U[1e-5, 1e-4]), inputs resident in HBM; one step builds BOTH trees of
+that configuration over the rects, fanout 16: the bottom-up tree (CENTER30
+Morton keys, stable LSD radix sort, level packing) and the top-down tree
+(per-level segmented sorts, slot layout).  value = rects entered into a tree
+per second = 2 x 16,777,216 / step time (Mrect/s); each build's own rate is
+in phases.w4_builds.
+roofline: the dominant kernel of the step (k_onesweep, one digit pass of the
+radix sort), its average duration from HIP events that hipExtLaunchKernel
+stamps with each launch's own start and end over 3 steps, against its
+algorithmic bytes per launch (every item read once and written once; the
+passes that also move the rects count those bytes).  traffic: the HBM bytes
+per launch that rocprofv3 FETCH_SIZE / WRITE_SIZE recorded (profiles/), when
+recorded for these exact kernel sources.
+cpu_baseline: the oracle's serial bottom-up and top-down builds of the same
+16,777,216 rects on one host core.  Every phase carries its own CPU figures
+(the oracle's builds, DFS joins and FP64 clips, on 1 core and on all cores
+of the host share), and the CPU leg then compares the phases' GPU results
+with the oracle.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -23,16 +31,20 @@ import argparse
 import json
 import os
 import platform
+import statistics
 import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "R-tree build Mrect/s; overlay pairs/s batched vs plain at ≤1e-12 area err, 1 GPU"
+W4_N = 1 << 24
 
 
 def log(msg: str) -> None:
@@ -50,6 +62,23 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def host_cores() -> int:
+    """Host cores the CPU baselines use: this process's CPU set, at most 16
+    (a GPU box's share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def nproc() -> int:
+    try:
+        return int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        return os.cpu_count() or 1
+
+
 # algorithmic HBM bytes per processed item, per kernel (minimal traffic:
 # every input read once, every output written once)
 BYTES_PER_ITEM = {
@@ -58,33 +87,49 @@ BYTES_PER_ITEM = {
     "k_td_gather": 48,      # 8-B item, 16-B rect + 4-B id read; 16 + 4 written
     "k_td_subtree<8>": 52,  # 8-B item, 16-B rect, 4-B id read; 16-B box + 4-B child + 4-B perm written
     "k_td_subtree<4>": 52,  # the same kernel when each block has a CU to itself
-    "k_td_lds_sort": 40,    # (fanouts that are not powers of two) 16 + 4 read, 16 + 4 written
-    "k_td_items": 24,       # (fanouts that are not powers of two) 16-B rect read, 8-B item written
-    "k_td_leaves": 20,      # (fanouts that are not powers of two) 16-B rect read, 4-B child written
     "k_bu_leaves": 52,      # 8-B item, 16-B rect read; 4+4 key/id, 16+4 entry written
 }
-
-
 # the last digit pass of a top-down radix level that feeds another radix
 # level also moves the rects and ids into the sorted order: 16-B rect + 4-B
 # id read, 16 + 4 written per item
 GATHER_BYTES = 40
 TD_LDS_SLOTS = 4096  # segments up to this size are sorted in LDS by the subtree kernel
+BU_KEY_BITS = 30     # CENTER30 keys: 4 digit passes of at most 8 bits
+TD_KEY_BITS = 32     # ord32 coordinates
 
 
-def td_gather_passes(n: int, f: int, levels: int) -> int:
-    """Digit passes per top-down build that also gather (one per radix level
-    followed by another radix level)."""
-    radix_levels = 0
-    for p in range(levels):
+def td_radix_levels(n: int, f: int) -> int:
+    """Levels of a top-down build sorted by the segmented radix sort (their
+    segments hold more than TD_LDS_SLOTS rects)."""
+    from parlllelrtree_amd import rtree
+    L = rtree.td_levels(n, f)
+    count = 0
+    for p in range(L):
         a = 1
-        for _ in range(levels - p):
+        for _ in range(L - p):
             a *= f
             if a >= n:
                 a = n
                 break
-        radix_levels += a > TD_LDS_SLOTS
-    return max(0, radix_levels - 1)
+        count += a > TD_LDS_SLOTS
+    return count
+
+
+def onesweep_mix(n: int, builds: list[str], f: int = 16) -> tuple[int, int]:
+    """(digit-pass launches, algorithmic bytes) of k_onesweep for one
+    bottom-up ("bu") / top-down ("td") build of n rects each."""
+    launches, nbytes = 0, 0
+    for b in builds:
+        if b == "bu":
+            p = (BU_KEY_BITS + 7) // 8
+            launches += p
+            nbytes += p * 16 * n
+        else:
+            lv = td_radix_levels(n, f)
+            p = lv * ((TD_KEY_BITS + 7) // 8)
+            launches += p
+            nbytes += p * 16 * n + max(0, lv - 1) * GATHER_BYTES * n
+    return launches, nbytes
 
 
 def timed(fn, steps: int, warmup: int) -> float:
@@ -104,10 +149,267 @@ def timed(fn, steps: int, warmup: int) -> float:
     return a.elapsed_time(b) / steps
 
 
-def overlay_phase(cp, A, B, dev, args, checks, check_sample=None, plain_reps=(2, 1)):
+def cpu_median(fn, reps: int, warm: bool = True) -> float:
+    """Median wall time (s) of fn over reps runs, after one warm-up run
+    (when `warm`)."""
+    if warm:
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def cpu_rate(fn, units: float, unit: str, reps: int, cores: int, sample: str,
+             fn_n=None) -> dict:
+    """CPU baseline of one phase: units / median time on 1 core (fn) and on
+    `cores` cores (fn_n, when given)."""
+    t1 = cpu_median(fn, reps)
+    out = {"kind": "port", "unit": unit, "1_core": round(units / t1 / 1e6, 4),
+           "sample": sample, "median_of": reps}
+    if fn_n is not None and cores > 1:
+        tn = cpu_median(fn_n, reps)
+        out[f"{cores}_cores"] = round(units / tn / 1e6, 4)
+    return out
+
+
+def recorded(name: str, key: str):
+    """A recorded counter figure from profiles/<name>, only while the kernel
+    sources are the ones it was recorded for (else None)."""
+    from parlllelrtree_amd.profiling import source_digest
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    rec = json.loads(p.read_text())
+    if rec.get("source_digest") != source_digest():
+        return None
+    return rec.get(key)
+
+
+# ---------------------------------------------------------------- headline
+def headline(dev, args, cpu_jobs):
+    """The step: bottom-up + top-down build of the 16,777,216 mixture rects."""
+    import torch
+
+    from parlllelrtree_amd import rtree, workloads
+    from parlllelrtree_amd.profiling import KernelTimes
+
+    n = W4_N
+    r = workloads.w4_rects(n)
+    d = torch.from_numpy(r).to(dev)
+    tb = rtree.build_bottom_up(d, 16)
+    tt = rtree.build_top_down(d, 16)
+    torch.cuda.synchronize()
+
+    def step():
+        rtree.rebuild(tb, d)
+        rtree.rebuild(tt, d)
+
+    log(f"timing bottom-up + top-down build of {n} mixture rects: {args.warmup} warmup + "
+        f"{args.steps} steps")
+    ms = timed(step, args.steps, args.warmup)
+    rtree.check_device_status()
+    value = 2 * n / (ms * 1e-3) / 1e6
+    log(f"step {ms:.3f} ms -> {value:.1f} Mrect/s (each rect into both trees)")
+
+    with KernelTimes() as kt:
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+    dom = max((k for k in kt.stats if k in BYTES_PER_ITEM), key=lambda k: kt.stats[k][1])
+    cnt, tot = kt.stats[dom]
+    avg_ms = tot / cnt
+    if dom == "k_onesweep":
+        launches, nbytes = onesweep_mix(n, ["bu", "td"])
+        assert cnt == 3 * launches, (cnt, launches)
+        alg_bytes = nbytes // launches
+    else:
+        alg_bytes = BYTES_PER_ITEM[dom] * n
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = recorded("hbm_traffic.json", dom)
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["traffic_bytes"] if traffic else None,
+                "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": alg_bytes,
+                "launches_per_step": cnt // 3,
+                "kernels_ms_per_step": {k: round(v[1] / 3, 4) for k, v in
+                                        sorted(kt.stats.items(), key=lambda kv: -kv[1][1])}}
+    if traffic:
+        roofline["traffic_ratio"] = round(traffic["traffic_bytes"] / alg_bytes, 3)
+    log(f"dominant kernel {dom}: {avg_ms * 1e3:.1f} us avg over {cnt // 3} launches a step, "
+        f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
+
+    # each build alone
+    ms_bu = timed(lambda: rtree.rebuild(tb, d), args.steps, args.warmup)
+    ms_td = timed(lambda: rtree.rebuild(tt, d), args.steps, args.warmup)
+    with KernelTimes() as kb:
+        rtree.rebuild(tb, d)
+        torch.cuda.synchronize()
+    with KernelTimes() as ktd:
+        rtree.rebuild(tt, d)
+        torch.cuda.synchronize()
+    rtree.check_device_status()
+    phase = {"rects": n, "bu_ms": round(ms_bu, 4), "bu_Mrect_s": round(n / ms_bu / 1e3, 1),
+             "td_ms": round(ms_td, 4), "td_Mrect_s": round(n / ms_td / 1e3, 1),
+             "bu_kernels_ms": {k: round(v[1], 4) for k, v in
+                               sorted(kb.stats.items(), key=lambda kv: -kv[1][1])},
+             "td_kernels_ms": {k: round(v[1], 4) for k, v in
+                               sorted(ktd.stats.items(), key=lambda kv: -kv[1][1])}}
+    log(f"w4 builds alone: bu {ms_bu:.3f} ms ({n / ms_bu / 1e3:.0f} Mrect/s), "
+        f"td {ms_td:.3f} ms ({n / ms_td / 1e3:.0f} Mrect/s)")
+    del tb, tt, d
+    torch.cuda.empty_cache()
+
+    cpu = {}
+
+    def cpu_job(O):
+        reps = args.cpu_reps_large
+        t_bu = cpu_median(lambda: O.bu_build(r, 16), reps, warm=False)
+        t_td = cpu_median(lambda: O.td_build(r, 16), reps, warm=False)
+        cpu.update({"value": round(2 * n / (t_bu + t_td) / 1e6, 3), "unit": "Mrect/s",
+                    "cores": 1, "kind": "port",
+                    "sample": f"the whole workload: the oracle's serial bottom-up build "
+                              f"(Morton keys, stable sort, packing) {t_bu:.2f} s and serial "
+                              f"top-down build {t_td:.2f} s of the 16,777,216 rects, median of "
+                              f"{reps} each; host {cpu_model()}, nproc {nproc()}"})
+        phase["cpu_baseline"] = {"kind": "port", "unit": "Mrect/s",
+                                 "bu_1_core": round(n / t_bu / 1e6, 3),
+                                 "td_1_core": round(n / t_td / 1e6, 3), "median_of": reps}
+        log(f"cpu baseline: bu {t_bu:.2f} s, td {t_td:.2f} s -> {cpu['value']} Mrect/s")
+    cpu_jobs.append(cpu_job)
+    line = {"value": value, "ms": ms, "roofline": roofline, "cpu": cpu}
+    return line, phase
+
+
+# ---------------------------------------------------------------- phases
+def layers_phase(dev, args, cpu_jobs, checks):
+    """Top-down builds of two 1,048,576-rect uniform layers in one launch
+    sequence and one layer per call; the layer-layer join; plain vs batched
+    overlay of the first 100,000 sorted pairs; their contours."""
+    import numpy as np
+    import torch
+
+    from parlllelrtree_amd import rtree, workloads
+    from parlllelrtree_amd.profiling import KernelTimes
+
+    n = 1 << 20
+    ra, rb = workloads.w1(n)
+    da, db = torch.from_numpy(ra).to(dev), torch.from_numpy(rb).to(dev)
+    TL = rtree.TopDownLayers([da, db], 16)
+    ta = TL.trees[0]
+    ms = timed(lambda: TL.rebuild([da, db]), args.steps, args.warmup)
+    with KernelTimes() as kt:
+        for _ in range(3):
+            TL.rebuild([da, db])
+        torch.cuda.synchronize()
+    c, t = kt.stats["k_onesweep"]
+    # both layers per launch: a top-down build of 2n items, segmented per layer
+    lv = td_radix_levels(n, 16)
+    per_step = lv * 4
+    alg = (per_step * 16 * 2 * n + max(0, lv - 1) * GATHER_BYTES * 2 * n) // per_step
+    t1a, t1b = rtree.build_top_down(da, 16), rtree.build_top_down(db, 16)
+
+    def per_layer():
+        rtree.rebuild(t1a, da)
+        rtree.rebuild(t1b, db)
+    ms1 = timed(per_layer, args.steps, args.warmup)
+    del t1a, t1b
+    out = {"td_build_2x1M": {
+        "ms": round(ms, 4), "Mrect_s": round(2 * n / ms / 1e3, 1),
+        "one_layer_per_call_ms": round(ms1, 4),
+        "one_layer_per_call_Mrect_s": round(2 * n / ms1 / 1e3, 1),
+        "k_onesweep_avg_us": round(t / c * 1e3, 2),
+        "k_onesweep_GBs": round(alg / (t / c * 1e-3) / 1e9, 1),
+        "note": "working set inside the 256 MB Infinity Cache: latency-bound, not an HBM figure"}}
+    log(f"td 2 x 1M layers: {ms:.3f} ms ({2 * n / ms / 1e3:.0f} Mrect/s), one layer per call "
+        f"{ms1:.3f} ms")
+
+    def cpu_td(O):
+        out["td_build_2x1M"]["cpu_baseline"] = cpu_rate(
+            lambda: O.td_build(ra, 16), n, "Mrect/s", args.cpu_reps, 1,
+            "the oracle's serial top-down build of one 1,048,576-rect layer")
+    cpu_jobs.append(cpu_td)
+
+    # layer-layer join (top-down tree of A, queries = B)
+    cap1 = rtree.join(ta, db).shape[0]
+    j1 = rtree.Joiner(ta, n, cap1, dev)
+    ms_j1 = timed(lambda: j1.run(db), max(args.steps // 2, 3), 1)
+    pairs_qr = j1.result().cpu().numpy().view(np.uint32)
+    out["join_layers_1M_x_1M"] = {"ms": round(ms_j1, 4), "pairs": cap1,
+                                  "Mpairs_s": round(cap1 / ms_j1 / 1e3, 1)}
+    log(f"layer join {ms_j1:.3f} ms ({cap1} pairs)")
+
+    def cpu_join(O):
+        ot = O.td_build(ra, 16)
+        out["join_layers_1M_x_1M"]["cpu_baseline"] = cpu_join_rate(
+            O, ot, rb, "td", cap1, args.cpu_reps,
+            "the oracle's DFS search (RtreeSearch semantics) of all 1,048,576 B rects "
+            "in the top-down tree of A")
+    cpu_jobs.append(cpu_join)
+
+    cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
+    out["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, checks, cpu_jobs)
+    out["overlay_layers_100k"].pop("_tune")
+    out["contours_layers_100k"] = contours_phase(cp, A, B, dev, checks, cpu_jobs)
+    return out
+
+
+def cpu_join_rate(O, tree, q, kind, pairs, reps, sample):
+    """The oracle's DFS join on 1 core and on all cores (queries split in
+    contiguous chunks over threads; the C search releases the GIL)."""
+    import numpy as np
+    fn = O.join_bu if kind == "bu" else O.join_td
+    cores = host_cores()
+    chunks = np.array_split(np.ascontiguousarray(q), cores)
+
+    def par():
+        with ThreadPoolExecutor(cores) as ex:
+            return sum(len(p) for p, _ in ex.map(lambda c: fn(tree, c), chunks))
+    got = par()
+    assert got == pairs, (got, pairs)
+    return cpu_rate(lambda: fn(tree, q), pairs, "Mpairs/s", reps, cores, sample, par)
+
+
+def bu_phase(dev, args, cpu_jobs):
+    """Bottom-up build of 1,048,576 uniform rects + 65,536 windows."""
+    import torch
+
+    from parlllelrtree_amd import rtree, workloads
+
+    n = 1 << 20
+    r0, q0 = workloads.w0()
+    d0, dq0 = torch.from_numpy(r0).to(dev), torch.from_numpy(q0).to(dev)
+    t0 = rtree.build_bottom_up(d0, 16)
+    ms_bu = timed(lambda: rtree.rebuild(t0, d0), args.steps, args.warmup)
+    out = {"bu_build_1M": {"ms": round(ms_bu, 4), "Mrect_s": round(n / ms_bu / 1e3, 1)}}
+    cap0 = rtree.join(t0, dq0).shape[0]
+    j0 = rtree.Joiner(t0, q0.shape[0], cap0, dev)
+    ms_j0 = timed(lambda: j0.run(dq0), max(args.steps // 2, 3), 1)
+    out["join_windows_65536"] = {"ms": round(ms_j0, 4), "pairs": cap0,
+                                 "Mpairs_s": round(cap0 / ms_j0 / 1e3, 1)}
+    log(f"bu build {ms_bu:.3f} ms, window join {ms_j0:.3f} ms ({cap0} pairs)")
+
+    def cpu(O):
+        out["bu_build_1M"]["cpu_baseline"] = cpu_rate(
+            lambda: O.bu_build(r0, 16), n, "Mrect/s", args.cpu_reps, 1,
+            "the oracle's serial bottom-up build of the 1,048,576 rects")
+        ot = O.bu_build(r0, 16)
+        out["join_windows_65536"]["cpu_baseline"] = cpu_join_rate(
+            O, ot, q0, "bu", cap0, args.cpu_reps,
+            "the oracle's DFS search of the 65,536 windows in the bottom-up tree")
+    cpu_jobs.append(cpu)
+    return out
+
+
+def overlay_phase(cp, A, B, dev, args, checks, cpu_jobs, check_sample=None,
+                  plain_reps=(2, 1), cpu_sample=20_000):
     """Plain vs batched FP64 overlay of pairs cp over CSR sets A, B.  Appends
     to `checks` the comparison of the areas with the oracle (on
-    `check_sample` pairs if given), run after every GPU phase."""
+    `check_sample` pairs if given) and the oracle's FP64 flop count, and to
+    `cpu_jobs` the oracle's clip rate on `cpu_sample` pairs (1 core, all
+    cores)."""
     import numpy as np
     import torch
 
@@ -145,32 +447,45 @@ def overlay_phase(cp, A, B, dev, args, checks, check_sample=None, plain_reps=(2,
     def check(O):
         sel = np.arange(npairs) if check_sample is None or check_sample >= npairs else \
             np.sort(np.random.default_rng(0).choice(npairs, check_sample, replace=False))
-        ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16),
-                               with_flops=True)
+        ref, flops = O.overlay(cp[sel], A, B, nthreads=host_cores(), with_flops=True)
         pa = np.abs([O.poly_area(A.poly(i)) for i in cp[sel, 0]])
         pb = np.abs([O.poly_area(B.poly(i)) for i in cp[sel, 1]])
         den = np.maximum(np.abs(ref), 1e-12 * np.minimum(pa, pb))
         den = np.where(den == 0, 1.0, den)
         rel_p = np.abs(got_p[sel] - ref) / den
         rel_b = np.abs(got_b[sel] - ref) / den
-        flops_all = flops * npairs / len(sel)
+        fpp = flops / len(sel)
+        tflops = fpp * npairs / (ms_b * 1e-3) / 1e12
         out.update({
             "checked_pairs": int(len(sel)),
             "max_rel_err_plain": float(rel_p.max()), "max_rel_err_batched": float(rel_b.max()),
             "bit_identical": bool(np.array_equal(got_p[sel], ref)
                                   and np.array_equal(got_b[sel], ref)),
-            "oracle_fp64_flops_per_pair": round(flops / len(sel), 1),
-            "batched_fp64_tflops": round(flops_all / (ms_b * 1e-3) / 1e12, 4)})
+            "fp64_valu": {"oracle_flops_per_pair": round(fpp, 1),
+                          "batched_tflops": round(tflops, 4), "peak_tflops": FP64_PEAK_TFLOPS,
+                          "frac_of_peak": round(tflops / FP64_PEAK_TFLOPS, 6)}})
         log(f"overlay {npairs} pairs against the oracle: max rel err "
             f"{max(rel_p.max(), rel_b.max()):.1e} on {len(sel)} pairs")
     checks.append(check)
+
+    def cpu(O):
+        k = min(cpu_sample, npairs)
+        sel = np.sort(np.random.default_rng(1).choice(npairs, k, replace=False))
+        cps = np.ascontiguousarray(cp[sel])
+        cores = host_cores()
+        out["cpu_baseline"] = cpu_rate(
+            lambda: O.overlay(cps, A, B, nthreads=1), k, "Mpairs/s", args.cpu_reps, cores,
+            f"the oracle's FP64 scanbeam clip (the area of gpc_polygon_clip's GPC_INT sweep) "
+            f"of {k} random pairs of the {npairs}",
+            lambda: O.overlay(cps, A, B, nthreads=cores))
+    cpu_jobs.append(cpu)
     return out
 
 
-def contours_phase(cp, A, B, dev, checks, sample=2000):
+def contours_phase(cp, A, B, dev, checks, cpu_jobs, sample=2000):
     """Result contours (intersection) of the pairs cp: count pass, scan,
     write pass.  Appends to `checks` the vertex-for-vertex comparison of the
-    first `sample` pairs with the oracle."""
+    first `sample` pairs with the oracle, timed (the CPU figure)."""
     import numpy as np
     import torch
 
@@ -189,20 +504,29 @@ def contours_phase(cp, A, B, dev, checks, sample=2000):
 
     def check(O):
         same = True
+        polys = [(A.poly(int(cp[t, 0])), B.poly(int(cp[t, 1]))) for t in range(k)]
+        t0 = time.perf_counter()
+        refs = [O.clip_contours(P, Q, "int") for P, Q in polys]
+        t_cpu = time.perf_counter() - t0
         for t in range(k):
-            ref, holes = O.clip_contours(A.poly(int(cp[t, 0])), B.poly(int(cp[t, 1])), "int")
+            ref, holes = refs[t]
             same &= len(mine[t]) == len(ref) and all(
                 h == bool(rh) and np.array_equal(c, r)
                 for (c, h), r, rh in zip(mine[t], ref, holes))
-        out.update({"checked_pairs": int(k), "equal_to_oracle": bool(same)})
+        out.update({"checked_pairs": int(k), "equal_to_oracle": bool(same),
+                    "cpu_baseline": {"kind": "port", "unit": "Mpairs/s",
+                                     "1_core": round(k / t_cpu / 1e6, 4),
+                                     "sample": f"the oracle's contour assembly of the first {k} "
+                                               f"pairs, one run (the comparison's own pass)"}})
         log(f"contours against the oracle on {k} pairs: equal {same}")
     checks.append(check)
     return out
 
 
-def grid_phase(dev, args, checks):
+def grid_phase(dev, args, checks, cpu_jobs):
     """2 x 500,000 stars on the jittered 708x708 grid: bottom-up tree of
-    layer A's MBRs, join with layer B's MBRs, overlay of every pair."""
+    layer A's MBRs, join with layer B's MBRs, overlay of every pair, and the
+    whole pipeline as one graph."""
     import numpy as np
     import torch
 
@@ -218,9 +542,13 @@ def grid_phase(dev, args, checks):
     pqr = j.result().cpu().numpy().view(np.uint32)
     cp = np.ascontiguousarray(pqr[:, ::-1])      # (A index, B index)
     log(f"grid join {ms_j:.3f} ms ({cap} pairs)")
-    out = overlay_phase(cp, A, B, dev, args, checks, check_sample=300_000, plain_reps=(1, 0))
+    out = overlay_phase(cp, A, B, dev, args, checks, cpu_jobs, check_sample=300_000,
+                        plain_reps=(1, 0))
     out["join_ms"] = round(ms_j, 3)
     out["pipeline"] = pipeline_phase(A, B, cap, out.pop("_tune"), out, dev)
+    valu = recorded("clip_fp64_valu.json", "grid_batched")
+    if valu:
+        out["fp64_valu_counters"] = valu
     return out
 
 
@@ -257,8 +585,7 @@ def pipeline_phase(A, B, cap, tune, ov, dev):
 def grid_batched_phase(dev, args, checks):
     """Batched FP64 overlay alone (default team shapes, no plain run, no
     autotune): every candidate pair of the 2 x 500,000-star grid.  The target
-    of the FP64 VALU counter runs.  Appends to `checks` the comparison with
-    the oracle and its FP64 flops per pair on a 50,000-pair sample."""
+    of the FP64 VALU counter runs."""
     import numpy as np
     import torch
 
@@ -286,8 +613,7 @@ def grid_batched_phase(dev, args, checks):
     out = {"pairs": npairs, "batched_ms": round(ms, 3)}
 
     def check(O):
-        ref, flops = O.overlay(cp[sel], A, B, nthreads=min(os.cpu_count() or 1, 16),
-                               with_flops=True)
+        ref, flops = O.overlay(cp[sel], A, B, nthreads=host_cores(), with_flops=True)
         fpp = flops / len(sel)
         out.update({"bit_identical_sample": bool(np.array_equal(got, ref)),
                     "oracle_fp64_flops_per_pair": round(fpp, 1),
@@ -298,14 +624,15 @@ def grid_batched_phase(dev, args, checks):
     return out
 
 
-def grad_phase(dev, args, checks):
+def grad_phase(dev, args, checks, cpu_jobs):
     """262,144 independent star pairs (16..128 vertices): FP64 forward area
-    (batched), FP64 adjoint, FP32 adjoint (+ Green's area), errors against the
-    FP64 CPU oracle gradient and FP32 against FP64."""
+    (batched), FP64 adjoint, FP32 adjoint (+ Green's area), errors against
+    central finite differences (h = 1e-6) of the FP64 area, the FP64 CPU
+    oracle gradient, and FP32 against FP64."""
     import numpy as np
     import torch
 
-    from parlllelrtree_amd import rtree, workloads
+    from parlllelrtree_amd import autograd, rtree, workloads
     from parlllelrtree_amd._lib import lib
 
     pairs, S = workloads.w3()
@@ -360,6 +687,16 @@ def grad_phase(dev, args, checks):
     fused_same_grad = bool(torch.equal(kA, gA) and torch.equal(kB, gB))
     a32 = gr32["r"][2].double().cpu().numpy()
     arel32 = float(np.median(np.abs(a32 - a64) / np.maximum(np.abs(a64), 1e-300)))
+    # central finite differences (h = 1e-6) of the FP64 area on a sample of
+    # pairs: every perturbed polygon is one more pair of one batched call
+    fsel = np.sort(np.random.default_rng(7).choice(npairs, 128, replace=False))
+    fd = autograd.finite_difference_grad(pairs[fsel], S, 1e-6, dev)
+    fd_err = []
+    for i, (a, b) in enumerate(pairs[fsel]):
+        for F, idx in zip(fd[i], (a, b)):
+            G = g64[S.off[idx]:S.off[idx + 1]]
+            sc = max(np.abs(G).max(), 1e-300)
+            fd_err.append(np.abs(F - G).max() / sc)
     out = {"pairs": npairs, "fp64_forward_ms": round(ms_f, 3), "fp64_adjoint_ms": round(ms_b, 3),
            "fp64_pairs_s": round(npairs / (ms_f + ms_b) * 1e3, 1),
            "fp64_fused_forward_adjoint_ms": round(ms_fb64, 3),
@@ -368,17 +705,25 @@ def grad_phase(dev, args, checks):
            "fp64_fused_grad_equals_adjoint": fused_same_grad,
            "fp32_forward_adjoint_ms": round(ms_32, 3),
            "fp32_pairs_s": round(npairs / ms_32 * 1e3, 1),
-           "grad_rel_err_fp32_vs_fp64": float(rel32), "area_median_rel_err_fp32": arel32}
+           "grad_rel_err_fp32_vs_fp64": float(rel32), "area_median_rel_err_fp32": arel32,
+           "grad_rel_err_vs_central_fd_h1e-6": {"max": float(np.max(fd_err)),
+                                                "median": float(np.median(fd_err)),
+                                                "pairs": int(len(fsel)),
+                                                "scale": "per polygon: max |g - fd| / max |g|"}}
     log(f"grad: fwd {ms_f:.2f} ms, adjoint {ms_b:.2f} ms (fp64), fused fp64 {ms_fb64:.2f} ms "
-        f"(Green area rel err {green_rel:.1e}), fp32 {ms_32:.2f} ms; fp32 vs fp64 {rel32:.1e}")
+        f"(Green area rel err {green_rel:.1e}), fp32 {ms_32:.2f} ms; fp32 vs fp64 {rel32:.1e}; "
+        f"vs central FD max {np.max(fd_err):.1e} on {len(fsel)} pairs")
+
+    csel = np.sort(np.random.default_rng(0).choice(npairs, 2000, replace=False))
 
     def check(O):
-        # FP64 gradient against the CPU oracle on a sample of pairs
-        sel = np.sort(np.random.default_rng(0).choice(npairs, 2000, replace=False))
+        # FP64 gradient against the CPU oracle on a sample of pairs (timed:
+        # the CPU figure of the adjoint)
         worst, exact, tot = 0.0, 0, 0
-        for t in sel:
-            a, b = pairs[t]
-            gP, gQ = O.area_grad(S.poly(a), S.poly(b))
+        t0 = time.perf_counter()
+        refs = [O.area_grad(S.poly(a), S.poly(b)) for a, b in pairs[csel]]
+        t_cpu = time.perf_counter() - t0
+        for (a, b), (gP, gQ) in zip(pairs[csel], refs):
             for G, idx in ((gP, a), (gQ, b)):
                 mine = g64[S.off[idx]:S.off[idx + 1]]
                 sc = max(np.abs(G).max(), 1e-300)
@@ -386,47 +731,25 @@ def grad_phase(dev, args, checks):
                 exact += int((mine == G).sum())
                 tot += G.size
         out.update({"grad_rel_err_vs_oracle_fp64": worst,
-                    "grad_bit_identical_fraction": exact / tot, "checked_pairs": int(len(sel))})
-        log(f"grad against the oracle on {len(sel)} pairs: rel err {worst:.1e}")
+                    "grad_bit_identical_fraction": exact / tot, "checked_pairs": int(len(csel)),
+                    "cpu_baseline": {"kind": "port", "unit": "Mpairs/s",
+                                     "1_core": round(len(csel) / t_cpu / 1e6, 4),
+                                     "sample": f"the oracle's FP64 adjoint (area + gradient) of "
+                                               f"{len(csel)} random pairs, one run (the "
+                                               f"comparison's own pass)"}})
+        log(f"grad against the oracle on {len(csel)} pairs: rel err {worst:.1e}")
     checks.append(check)
     return out
 
 
-def w4_phase(dev, args, checks):
-    """16,777,216 Gaussian-mixture rects: bottom-up and top-down build
-    throughput and the dominant build kernel against HBM (the working set
-    exceeds the 256 MB Infinity Cache); overlay of two 4,000,000-star layers
-    (join + plain vs batched)."""
+def w4_overlay_phase(dev, args, checks, cpu_jobs):
+    """Overlay of two 4,000,000-star mixture layers: join on the MBRs, then
+    plain vs batched clip of every candidate pair."""
     import numpy as np
     import torch
 
     from parlllelrtree_amd import gen, rtree, workloads
-    from parlllelrtree_amd.profiling import KernelTimes
 
-    n = 1 << 24
-    r = workloads.w4_rects(n)
-    d = torch.from_numpy(r).to(dev)
-    tb = rtree.build_bottom_up(d, 16)
-    ms_bu = timed(lambda: rtree.rebuild(tb, d), 5, 1)
-    tt = rtree.build_top_down(d, 16)
-    ms_td = timed(lambda: rtree.rebuild(tt, d), 3, 1)
-    with KernelTimes() as kt:
-        rtree.rebuild(tb, d)
-        rtree.rebuild(tt, d)
-        torch.cuda.synchronize()
-    # k_scatter over all 16.7M items: 16 B per item algorithmic
-    sc = [v for k, v in kt.stats.items() if k == "k_scatter"]
-    out = {"rects": n, "bu_ms": round(ms_bu, 3), "bu_Mrect_s": round(n / ms_bu / 1e3, 1),
-           "td_ms": round(ms_td, 3), "td_Mrect_s": round(n / ms_td / 1e3, 1),
-           "kernels_ms": {k: round(v[1], 3) for k, v in
-                          sorted(kt.stats.items(), key=lambda kv: -kv[1][1])[:8]}}
-    if sc:
-        cnt, tot = sc[0]
-        # the unsegmented passes (BU and TD level 0) process all n items per launch
-        out["k_scatter_avg_us"] = round(tot / cnt * 1e3, 2)
-    log(f"w4 builds: bu {ms_bu:.3f} ms ({n / ms_bu / 1e3:.0f} Mrect/s), td {ms_td:.3f} ms")
-    del tb, tt, d
-    torch.cuda.empty_cache()
     A, B = workloads.w4_polys()
     ma, mb = gen.poly_mbr(A), gen.poly_mbr(B)
     da, db = torch.from_numpy(ma).to(dev), torch.from_numpy(mb).to(dev)
@@ -438,11 +761,11 @@ def w4_phase(dev, args, checks):
     cp = np.ascontiguousarray(pqr[:, ::-1])
     del j, t
     log(f"w4 join {ms_j:.3f} ms ({cap} pairs)")
-    ov = overlay_phase(cp, A, B, dev, args, checks, check_sample=200_000, plain_reps=(1, 0))
+    ov = overlay_phase(cp, A, B, dev, args, checks, cpu_jobs, check_sample=200_000,
+                       plain_reps=(1, 0))
     ov["join_ms"] = round(ms_j, 3)
     ov.pop("_tune")
-    out["overlay_2x4M"] = ov
-    return out
+    return ov
 
 
 def main() -> int:
@@ -451,8 +774,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true",
-                    help="skip the CPU leg: the oracle's serial build and the oracle "
-                         "comparisons of the phases' results")
+                    help="skip the CPU leg: the oracle's timings and the oracle comparisons "
+                         "of the phases' results")
+    ap.add_argument("--cpu-reps", type=int, default=5,
+                    help="median of this many runs for the phases' CPU figures")
+    ap.add_argument("--cpu-reps-large", type=int, default=3,
+                    help="median of this many runs for the serial 16.7M-rect builds")
     ap.add_argument("--no-phases", action="store_true")
     ap.add_argument("--phases", default="bu,layers,grid,grad,w4",
                     help="comma list of the phases to run (bu,layers,grid,grad,w4; "
@@ -470,147 +797,58 @@ def main() -> int:
         print(json.dumps({"copy_bytes": src.numel() * 4}))
         return 0
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); this build has no "
-              "multi-GPU path", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): every box here has one "
+              "GPU and this build has no multi-GPU path (its sharding is a design, not code)",
+              file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
-    from parlllelrtree_amd import rtree, workloads
-    from parlllelrtree_amd.profiling import KernelTimes
+    from parlllelrtree_amd import rtree
 
     dev = torch.device("cuda:0")
     log(f"device {torch.cuda.get_device_name(0)}")
-    n = 1 << 20
-    ra, rb = workloads.w1(n)
-    da, db = torch.from_numpy(ra).to(dev), torch.from_numpy(rb).to(dev)
-    # both layers in one shared launch sequence (every launch covers both)
-    TL = rtree.TopDownLayers([da, db], 16)
-    ta = TL.trees[0]
-    torch.cuda.synchronize()
-
-    def step():
-        TL.rebuild([da, db])
-
-    log(f"timing top-down build of 2 x {n} rects: {args.warmup} warmup + {args.steps} steps")
-    ms = timed(step, args.steps, args.warmup)
-    value = 2 * n / (ms * 1e-3) / 1e6
-    log(f"step {ms:.3f} ms -> {value:.1f} Mrect/s")
-
-    # dominant kernel of the step; durations from HIP events that
-    # hipExtLaunchKernel stamps with each launch's own start and end
-    with KernelTimes() as kt:
-        for _ in range(3):
-            step()
-        torch.cuda.synchronize()
-    dom = max((k for k in kt.stats if k in BYTES_PER_ITEM), key=lambda k: kt.stats[k][1])
-    cnt, tot = kt.stats[dom]
-    avg_ms = tot / cnt
-    items = 2 * n                      # every launch of the step covers both layers
-    alg_bytes = BYTES_PER_ITEM[dom] * items
-    if dom == "k_onesweep":
-        # mean over the step's launches, gathering passes included
-        per_step = cnt / 3
-        gathers = td_gather_passes(n, 16, ta.st.num_levels)
-        alg_bytes = int(items * (16 * per_step + GATHER_BYTES * gathers) / per_step)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # (scaled by a copy-kernel calibration), when recorded for this kernel
-    traffic = None
-    tpath = ROOT / "profiles" / "hbm_traffic.json"
-    if tpath.exists():
-        rec = json.loads(tpath.read_text()).get(dom)
-        if rec and rec.get("items_per_launch") == items:
-            traffic = rec["traffic_bytes"]
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "avg_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": alg_bytes,
-                "kernels_ms_per_step": {k: round(v[1] / 3, 4) for k, v in
-                                        sorted(kt.stats.items(), key=lambda kv: -kv[1][1])}}
-    log(f"dominant kernel {dom}: {avg_ms * 1e3:.1f} us avg, {achieved:.0f} GB/s algorithmic")
-
-    phases = {}
-    checks = []     # oracle comparisons of the phases' results, run with the CPU baseline
-    # the same two builds, one prt_td_build call per layer
-    t1a, t1b = rtree.build_top_down(da, 16), rtree.build_top_down(db, 16)
-
-    def step_per_layer():
-        rtree.rebuild(t1a, da)
-        rtree.rebuild(t1b, db)
-    ms1 = timed(step_per_layer, args.steps, args.warmup)
-    phases["td_build_one_layer_per_call"] = {"ms": round(ms1, 4),
-                                             "Mrect_s": round(2 * n / ms1 / 1e3, 1)}
-    log(f"one layer per call: {ms1:.3f} ms -> {2 * n / ms1 / 1e3:.1f} Mrect/s")
-    del t1a, t1b
+    cpu_jobs, checks = [], []     # CPU figures and oracle comparisons, run after the GPU phases
+    head, w4_builds = headline(dev, args, cpu_jobs)
+    phases = {"w4_builds": w4_builds}
     want = set() if args.no_phases else set(args.phases.split(","))
     if "bu" in want:
-        # bottom-up build of 1,048,576 rects + 65,536 windows
-        r0, q0 = workloads.w0()
-        d0, dq0 = torch.from_numpy(r0).to(dev), torch.from_numpy(q0).to(dev)
-        t0 = rtree.build_bottom_up(d0, 16)
-        ms_bu = timed(lambda: rtree.rebuild(t0, d0), args.steps, args.warmup)
-        phases["bu_build_1M"] = {"ms": round(ms_bu, 4), "Mrect_s": round(n / ms_bu / 1e3, 1)}
-        cap0 = rtree.join(t0, dq0).shape[0]
-        j0 = rtree.Joiner(t0, q0.shape[0], cap0, dev)
-        ms_j0 = timed(lambda: j0.run(dq0), max(args.steps // 2, 3), 1)
-        phases["join_windows_65536"] = {"ms": round(ms_j0, 4), "pairs": cap0,
-                                        "Mpairs_s": round(cap0 / ms_j0 / 1e3, 1)}
-        log(f"bu build {ms_bu:.3f} ms, window join {ms_j0:.3f} ms ({cap0} pairs)")
+        phases.update(bu_phase(dev, args, cpu_jobs))
     if "layers" in want:
-        # layer-layer join (top-down tree of A, queries = B)
-        cap1 = rtree.join(ta, db).shape[0]
-        j1 = rtree.Joiner(ta, n, cap1, dev)
-        ms_j1 = timed(lambda: j1.run(db), max(args.steps // 2, 3), 1)
-        pairs_qr = j1.result().cpu().numpy().view(np.uint32)
-        phases["join_layers_1M_x_1M"] = {"ms": round(ms_j1, 4), "pairs": cap1,
-                                         "Mpairs_s": round(cap1 / ms_j1 / 1e3, 1)}
-        log(f"layer join {ms_j1:.3f} ms ({cap1} pairs)")
-        # plain and batched FP64 overlay of the first 100,000 sorted pairs
-        cp, A, B, _, _ = workloads.clip_inputs_from_pairs(pairs_qr, ra, rb, 100_000)
-        phases["overlay_layers_100k"] = overlay_phase(cp, A, B, dev, args, checks)
-        phases["contours_layers_100k"] = contours_phase(cp, A, B, dev, checks)
-        phases["overlay_layers_100k"].pop("_tune")
+        phases.update(layers_phase(dev, args, cpu_jobs, checks))
     if "grid" in want:
-        # jittered-grid overlay: 2 x 500,000 lognormal stars, every candidate
-        # pair, stage by stage and as one captured pipeline
-        phases["overlay_grid_2x500k"] = grid_phase(dev, args, checks)
+        phases["overlay_grid_2x500k"] = grid_phase(dev, args, checks, cpu_jobs)
     if "grid_batched" in want:
         phases["overlay_grid_batched"] = grid_batched_phase(dev, args, checks)
     if "grad" in want:
-        # differentiable overlay: 262,144 independent pairs, area + adjoint
-        phases["overlay_grad_262144"] = grad_phase(dev, args, checks)
+        phases["overlay_grad_262144"] = grad_phase(dev, args, checks, cpu_jobs)
     if "w4" in want:
-        # largest workload: 16.7M-rect builds, 2 x 4M-star overlay
-        phases["w4_gmm"] = w4_phase(dev, args, checks)
+        phases["overlay_w4_2x4M"] = w4_overlay_phase(dev, args, checks, cpu_jobs)
+    rtree.check_device_status()
 
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
-        t = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            O.td_build(ra, 16)
-            t.append(time.perf_counter() - t0)
-        t_med = sorted(t)[1]
-        cpu = {"value": round(n / t_med / 1e6, 3), "unit": "Mrect/s", "cores": 1, "kind": "port",
-               "sample": f"serial top-down build of one 1,048,576-rect layer (oracle, C), "
-                         f"median of 3 = {t_med:.3f} s; host {cpu_model()}"}
-        log(f"cpu baseline {cpu['value']} Mrect/s")
+        log(f"CPU leg: host {cpu_model()}, nproc {nproc()}, using up to {host_cores()} cores")
+        for job in cpu_jobs:
+            job(O)
+        cpu = head["cpu"]
         for chk in checks:
             chk(O)
 
     line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "Mrect/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "metric": METRIC, "value": round(head["value"], 1), "unit": "Mrect/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 boxes, u64 sort keys", "data": "synthetic",
-        "config": {"workload": "top-down R-tree build, fanout 16, two layers of 1,048,576 "
-                               "uniform fp32 rects (sides U[1e-4,1e-3]), both layers in one "
-                               "launch sequence",
-                   "rects_per_step": 2 * n, "fanout": 16, "parallelism": "1 GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "phases": phases,
+        "config": {"workload": "bottom-up + top-down R-tree builds, fanout 16, of 16,777,216 fp32 "
+                               "rects from a 64-component Gaussian mixture (sigma 0.01, sides "
+                               "U[1e-5,1e-4]); value counts each rect once per tree",
+                   "rects": W4_N, "rects_per_step": 2 * W4_N, "fanout": 16,
+                   "parallelism": "1 GPU"},
+        "roofline": head["roofline"], "cpu_baseline": cpu,
+        "host": {"cpu": cpu_model(), "nproc": nproc(), "cores_used": host_cores()},
+        "phases": phases,
     }
     print(json.dumps(line), flush=True)
     return 0

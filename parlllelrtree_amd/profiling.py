@@ -1,16 +1,46 @@
-"""Per-kernel durations measured with HIP events inside libprt.
+"""Per-kernel durations measured with HIP events inside libprt, and the
+summary of rocprofv3 counter runs that bench.py reports beside them.
 
     with KernelTimes() as kt:
         ... calls into libprt ...
     kt.stats  # {kernel name: (launches, total ms)}
+
+    python -m parlllelrtree_amd.profiling counters --run NAME=DIR ... [--calib-bytes B]
+        per-kernel averages per dispatch of rocprofv3 --pmc CSV output
+        (markdown table on stdout, JSON with --json OUT)
+    python -m parlllelrtree_amd.profiling trace KERNEL_TRACE.csv
+        per-kernel durations of a rocprofv3 --kernel-trace CSV
+
+source_digest() names the kernel sources a recorded counter figure belongs
+to: bench.py reports a recorded figure only while the sources are unchanged.
 """
 from __future__ import annotations
 
+import argparse
+import csv
 import ctypes
+import hashlib
+import json
+import sys
+from collections import defaultdict
+from pathlib import Path
 
 import numpy as np
 
 from ._lib import lib
+
+CSRC = Path(__file__).resolve().parent / "csrc"
+
+
+def source_digest() -> str:
+    """sha256 (first 16 hex digits) of every kernel and host source of
+    libprt (csrc/*.hip, *.hpp, *.cpp, in name order)."""
+    h = hashlib.sha256()
+    for p in sorted(CSRC.iterdir()):
+        if p.suffix in (".hip", ".hpp", ".cpp"):
+            h.update(p.name.encode())
+            h.update(p.read_bytes())
+    return h.hexdigest()[:16]
 
 
 class KernelTimes:
@@ -42,3 +72,126 @@ class KernelTimes:
 
     def dominant(self) -> str:
         return max(self.stats, key=lambda k: self.stats[k][1])
+
+
+# ---------------------------------------------------------------- counters
+def read_counter_csv(run_dir: str | Path) -> dict:
+    """{(kernel, grid): {counter: [value per dispatch]}} from the
+    *counter_collection.csv files rocprofv3 --pmc --output-format csv wrote
+    under run_dir (one row per dispatch and counter)."""
+    out: dict = defaultdict(lambda: defaultdict(list))
+    files = sorted(Path(run_dir).rglob("*counter_collection.csv"))
+    if not files:
+        raise FileNotFoundError(f"no *counter_collection.csv under {run_dir}")
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name", "").strip()
+                grid = int(float(row.get("Grid_Size", 0) or 0))
+                out[(name, grid)][row["Counter_Name"].strip()].append(
+                    float(row["Counter_Value"]))
+    return out
+
+
+def short_name(kernel: str) -> str:
+    """Kernel name without namespace, argument list and return type."""
+    k = kernel.replace("prt::(anonymous namespace)::", "").replace("prt::", "")
+    k = k.split("(")[0].strip()
+    for pre in ("void ", "__global__ "):
+        if k.startswith(pre):
+            k = k[len(pre):]
+    return k
+
+
+def counters_main(argv: list[str]) -> int:
+    ap = argparse.ArgumentParser(prog="profiling counters")
+    ap.add_argument("--run", action="append", required=True, metavar="NAME=DIR",
+                    help="a rocprofv3 --pmc output directory (one counter per run)")
+    ap.add_argument("--calib", action="append", default=[], metavar="COUNTER=DIR",
+                    help="the same counter's run over a device copy of --calib-bytes")
+    ap.add_argument("--calib-bytes", type=float, default=0.0)
+    ap.add_argument("--json", default=None)
+    ap.add_argument("--traffic-json", default=None,
+                    help="also write, per kernel name, the scaled bytes per dispatch averaged "
+                         "over all its dispatches (the form bench.py reads)")
+    ap.add_argument("--items", type=int, default=0, help="items per launch, for --traffic-json")
+    args = ap.parse_args(argv)
+    per = {}
+    for spec in args.run:
+        name, d = spec.split("=", 1)
+        per[name] = read_counter_csv(d)
+    scale = {}
+    for spec in args.calib:
+        cname, d = spec.split("=", 1)
+        data = read_counter_csv(d)
+        # the copy kernel: the dispatch with the largest value of the counter
+        best = max((max(v.get(cname, [0.0])) for v in data.values()), default=0.0)
+        if best > 0 and args.calib_bytes > 0:
+            scale[cname] = args.calib_bytes / (best * 1024.0)
+    rows = {}
+    for run, data in per.items():
+        for (kern, grid), cnts in data.items():
+            for cname, vals in cnts.items():
+                key = (short_name(kern), grid)
+                r = rows.setdefault(key, {"dispatches": len(vals)})
+                r[cname] = float(np.mean(vals))
+    print("| kernel | grid | dispatches | " + " | ".join(
+        f"{c} KiB/dispatch" for c in sorted({c for r in rows.values() for c in r
+                                               if c != "dispatches"})) + " | scaled bytes |")
+    cols = sorted({c for r in rows.values() for c in r if c != "dispatches"})
+    print("|" + "---|" * (4 + len(cols)))
+    res = {}
+    for (k, g), r in sorted(rows.items(), key=lambda kv: -sum(v for c, v in kv[1].items()
+                                                               if c != "dispatches")):
+        scaled = sum(r.get(c, 0.0) * 1024.0 * scale.get(c, 1.0) for c in cols)
+        print(f"| {k} | {g} | {r['dispatches']} | " +
+              " | ".join(f"{r.get(c, float('nan')):.1f}" for c in cols) + f" | {scaled:.0f} |")
+        res[f"{k}@{g}"] = {"kernel": k, "grid": g, "dispatches": r["dispatches"],
+                           "kib_per_dispatch": {c: r.get(c) for c in cols},
+                           "scaled_bytes_per_dispatch": scaled}
+    if args.json:
+        Path(args.json).write_text(json.dumps({"scale": scale, "kernels": res,
+                                               "source_digest": source_digest()}, indent=1))
+    if args.traffic_json:
+        per_kernel: dict = {}
+        for r in res.values():
+            d = per_kernel.setdefault(r["kernel"], [0, 0.0])
+            d[0] += r["dispatches"]
+            d[1] += r["dispatches"] * r["scaled_bytes_per_dispatch"]
+        Path(args.traffic_json).write_text(json.dumps(
+            {"source_digest": source_digest(), "scale": scale,
+             **{k: {"traffic_bytes": int(round(t / c)), "dispatches": c,
+                    "items_per_launch": args.items} for k, (c, t) in per_kernel.items()}},
+            indent=1))
+    print(f"\nscale factors: {scale}; source digest {source_digest()}")
+    return 0
+
+
+def trace_main(argv: list[str]) -> int:
+    """Per (kernel, grid): dispatches, average / median / total duration of a
+    rocprofv3 --kernel-trace CSV (markdown on stdout)."""
+    ap = argparse.ArgumentParser(prog="profiling trace")
+    ap.add_argument("trace_csv")
+    args = ap.parse_args(argv)
+    groups: dict = defaultdict(list)
+    with open(args.trace_csv, newline="") as fh:
+        for row in csv.DictReader(fh):
+            k = short_name(row["Kernel_Name"])
+            g = int(row.get("Grid_Size_X") or row.get("Grid_Size") or 0)
+            blk = int(row.get("Workgroup_Size_X") or row.get("Workgroup_Size") or 0)
+            groups[(k, g, blk)].append((int(row["End_Timestamp"]) - int(row["Start_Timestamp"]))
+                                       / 1000.0)
+    print("| kernel | grid | block | dispatches | avg us | median us | total ms |")
+    print("|---|---|---|---|---|---|---|")
+    for (k, g, b), v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| {k} | {g} | {b} | {len(v)} | {np.mean(v):.2f} | {np.median(v):.2f} | "
+              f"{sum(v) / 1e3:.2f} |")
+    return 0
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "counters":
+        sys.exit(counters_main(sys.argv[2:]))
+    if len(sys.argv) > 1 and sys.argv[1] == "trace":
+        sys.exit(trace_main(sys.argv[2:]))
+    print(__doc__)

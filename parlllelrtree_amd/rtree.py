@@ -48,6 +48,13 @@ def check_device_status() -> None:
     check(lib().prt_device_status(None, 1, _stream()))
 
 
+def td_levels(n: int, fanout: int = 16) -> int:
+    """Levels of the top-down slot layout of n rects (TopDown-rtree.cu:504-524)."""
+    st = TDTreeStruct()
+    check(lib().prt_td_plan(n, fanout, ctypes.byref(st)))
+    return int(st.num_levels)
+
+
 def _ws(nbytes: int, device) -> torch.Tensor:
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
 

@@ -42,11 +42,27 @@ def test_fp64_gradient_matches_oracle(dev):
 
 
 def test_fp64_gradient_matches_finite_differences(dev):
-    """Central differences with h = 1e-6, all on the GPU: every perturbed
-    polygon is one more pair of a single batched overlay call."""
+    """Central differences with h = 1e-6 of the GPU forward area, on 160
+    random pairs of the workload (every perturbed polygon is one more pair of
+    a single batched overlay call), and on 12 pairs with the perturbed
+    polygons built here one by one (no shared helper)."""
+    pairs, S = _pairs(4096)
+    sel = np.sort(np.random.default_rng(11).choice(len(pairs), 160, replace=False))
+    Sd = rtree.to_device_polys(S)
+    g1, g2 = rtree.area_grad(torch.from_numpy(pairs.view(np.int32)).cuda(), Sd, Sd)
+    g = (g1 + g2).cpu().numpy()
+    fd = autograd.finite_difference_grad(pairs[sel], S, 1e-6)
+    errs = []
+    for (a, b), (fP, fQ) in zip(pairs[sel], fd):
+        for F, idx in ((fP, a), (fQ, b)):
+            G = g[S.off[idx]:S.off[idx + 1]]
+            errs.append(np.abs(F - G).max() / np.abs(G).max())
+    assert max(errs) < 1e-5, max(errs)
+    assert np.median(errs) < 1e-7, np.median(errs)
+
     pairs, S = _pairs(12)
     h = 1e-6
-    rows, meta = [], []
+    meta = []
     polys = [S.poly(i) for i in range(S.n)]
     extra = []
     for t, (a, b) in enumerate(pairs):
@@ -71,9 +87,9 @@ def test_fp64_gradient_matches_finite_differences(dev):
     prs = np.array(prs, np.uint32)
     Cd = rtree.to_device_polys(C)
     areas = rtree.overlay_batched(torch.from_numpy(prs.view(np.int32)).cuda(), Cd, Cd).cpu().numpy()
-    fd = {}
+    fdd = {}
     for k, (t, side, v, c, sgn) in enumerate(meta):
-        fd[(t, side, v, c)] = fd.get((t, side, v, c), 0.0) + sgn * areas[k] / (2 * h)
+        fdd[(t, side, v, c)] = fdd.get((t, side, v, c), 0.0) + sgn * areas[k] / (2 * h)
     Sd = rtree.to_device_polys(S)
     g1, g2 = rtree.area_grad(torch.from_numpy(pairs.view(np.int32)).cuda(), Sd, Sd)
     g = (g1 + g2).cpu().numpy()
@@ -81,7 +97,7 @@ def test_fp64_gradient_matches_finite_differences(dev):
     for t, (a, b) in enumerate(pairs):
         for side, idx in ((0, a), (1, b)):
             G = g[S.off[idx]:S.off[idx + 1]]
-            F = np.array([[fd[(t, side, v, c)] for c in range(2)] for v in range(len(G))])
+            F = np.array([[fdd[(t, side, v, c)] for c in range(2)] for v in range(len(G))])
             errs.append(np.abs(F - G).max() / np.abs(G).max())
     assert max(errs) < 1e-5, max(errs)
 

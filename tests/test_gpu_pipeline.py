@@ -149,3 +149,37 @@ def test_empty_layer(dev, which):
     pairs, area, total, nonempty = p.result()
     assert pairs.shape[0] == 0 and total == 0.0 and nonempty == 0
     p.close()
+
+
+@pytest.mark.parametrize("tree", ["bu", "td"])
+def test_graph_on_lognormal_grid_subset(dev, tree):
+    """The captured pipeline on the grid overlay's own polygons: the first
+    60,000 stars of both 500,000-star layers (rows 0-84 of the jittered
+    708x708 grid, lognormal vertex counts up to 512, the same polygons as the
+    full workload's).  Three replays with eager work between them, each equal
+    to the eager run; pairs equal to the oracle's join of the MBRs, every
+    area bit-identical to the oracle."""
+    A, B = grid_layers(60000, 708)
+    assert A.max_verts() > 128 and B.max_verts() > 128
+    ot = O.td_build(gen.poly_mbr(B), 16)
+    ref_pairs, _ = O.join_td(ot, gen.poly_mbr(A))           # (A id, B id)
+    ref_area = O.overlay(ref_pairs, A, B, nthreads=16)
+    dA, dB = rtree.to_device_polys(A), rtree.to_device_polys(B)
+    p = OverlayPipeline(dA, dB, cap=len(ref_pairs) + 4096, tree=tree)
+    p.run()
+    eager = snapshot(p)
+    np.testing.assert_array_equal(eager[0].cpu().numpy().view(np.uint32), ref_pairs)
+    np.testing.assert_array_equal(eager[1].cpu().numpy(), ref_area)
+    assert eager[3] == int((ref_area > 0).sum())
+    p.capture()
+    for rep in range(3):
+        x = torch.randn(1 << 20, device="cuda", dtype=torch.float64)
+        _ = float((x * x).sum())
+        p.pairs.fill_(-1)
+        p.area.fill_(float("nan"))
+        p.count.fill_(-7)
+        p.total.fill_(float("nan"))
+        p.nonempty.fill_(-7)
+        p.replay()
+        assert_same(snapshot(p), eager)
+    p.close()
