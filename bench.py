@@ -89,10 +89,12 @@ BYTES_PER_ITEM = {
     "k_td_subtree<4>": 52,  # the same kernel when each block has a CU to itself
     "k_bu_leaves": 52,      # 8-B item, 16-B rect read; 4+4 key/id, 16+4 entry written
 }
-# the last digit pass of a top-down radix level that feeds another radix
-# level also moves the rects and ids into the sorted order: 16-B rect + 4-B
-# id read, 16 + 4 written per item
-GATHER_BYTES = 40
+# the last digit pass of the top-down radix level before the last one also
+# moves the rects and ids into the sorted order (16-B rect read, 16 + 4 B
+# written per item; the id is the item's payload); the last digit pass of a
+# level before that reads each item's rect for the next level's key (16 B)
+GATHER_BYTES = 36
+NEXT_KEY_BYTES = 16
 TD_LDS_SLOTS = 4096  # segments up to this size are sorted in LDS by the subtree kernel
 BU_KEY_BITS = 30     # CENTER30 keys: 4 digit passes of at most 8 bits
 TD_KEY_BITS = 32     # ord32 coordinates
@@ -125,11 +127,29 @@ def onesweep_mix(n: int, builds: list[str], f: int = 16) -> tuple[int, int]:
             launches += p
             nbytes += p * 16 * n
         else:
-            lv = td_radix_levels(n, f)
-            p = lv * ((TD_KEY_BITS + 7) // 8)
+            p = td_radix_levels(n, f) * ((TD_KEY_BITS + 7) // 8)
             launches += p
-            nbytes += p * 16 * n + max(0, lv - 1) * GATHER_BYTES * n
+            nbytes += p * 16 * n + td_extra_bytes(n, f)
     return launches, nbytes
+
+
+def td_extra_bytes(n: int, f: int = 16) -> int:
+    """Bytes the last digit passes of a top-down build's radix levels move
+    besides the items: NEXT_KEY_BYTES per item for each level followed by two
+    or more radix levels, GATHER_BYTES for the level before the last."""
+    lv = td_radix_levels(n, f)
+    return (max(0, lv - 2) * NEXT_KEY_BYTES + (1 if lv >= 2 else 0) * GATHER_BYTES) * n
+
+
+def digit_pass_bytes(stats: dict, runs: int, n: int, builds: list[str], f: int = 16) -> dict:
+    """Algorithmic bytes per launch of k_onesweep over `runs` runs of the
+    builds (n rects each), the mean over the digit passes: every pass moves
+    16 B per item, the last passes of the top-down radix levels more
+    (td_extra_bytes)."""
+    launches, nbytes = onesweep_mix(n, builds, f)
+    c = stats.get("k_onesweep", (0, 0.0))[0] // runs
+    assert c == launches, (c, launches)
+    return {"k_onesweep": nbytes // launches}
 
 
 def timed(fn, steps: int, warmup: int) -> float:
@@ -221,19 +241,26 @@ def headline(dev, args, cpu_jobs):
     dom = max((k for k in kt.stats if k in BYTES_PER_ITEM), key=lambda k: kt.stats[k][1])
     cnt, tot = kt.stats[dom]
     avg_ms = tot / cnt
-    if dom == "k_onesweep":
-        launches, nbytes = onesweep_mix(n, ["bu", "td"])
-        assert cnt == 3 * launches, (cnt, launches)
-        alg_bytes = nbytes // launches
-    else:
-        alg_bytes = BYTES_PER_ITEM[dom] * n
+    passes = digit_pass_bytes(kt.stats, 3, n, ["bu", "td"])
+    alg_bytes = passes[dom] if dom in passes else BYTES_PER_ITEM[dom] * n
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    # every digit pass of the step (both kernels): bytes / time
+    p_launch, p_bytes = onesweep_mix(n, ["bu", "td"])
+    p_ms = sum(kt.stats[k][1] for k in passes) / 3
+    digit = {"launches": p_launch, "ms": round(p_ms, 4),
+             "achieved": round(p_bytes / (p_ms * 1e-3) / 1e9, 1),
+             "frac": round(p_bytes / (p_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+             "per_kernel": {k: {"launches": kt.stats[k][0] // 3,
+                                "avg_us": round(kt.stats[k][1] / kt.stats[k][0] * 1e3, 2),
+                                "bytes_per_launch": b,
+                                "GBs": round(b / (kt.stats[k][1] / kt.stats[k][0] * 1e-3) / 1e9, 1)}
+                            for k, b in passes.items()}}
     traffic = recorded("hbm_traffic.json", dom)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic["traffic_bytes"] if traffic else None,
                 "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": alg_bytes,
-                "launches_per_step": cnt // 3,
+                "launches_per_step": cnt // 3, "digit_passes": digit,
                 "kernels_ms_per_step": {k: round(v[1] / 3, 4) for k, v in
                                         sorted(kt.stats.items(), key=lambda kv: -kv[1][1])}}
     if traffic:
@@ -306,9 +333,8 @@ def layers_phase(dev, args, cpu_jobs, checks):
         torch.cuda.synchronize()
     c, t = kt.stats["k_onesweep"]
     # both layers per launch: a top-down build of 2n items, segmented per layer
-    lv = td_radix_levels(n, 16)
-    per_step = lv * 4
-    alg = (per_step * 16 * 2 * n + max(0, lv - 1) * GATHER_BYTES * 2 * n) // per_step
+    per_step = td_radix_levels(n, 16) * 4
+    alg = (per_step * 16 * 2 * n + 2 * td_extra_bytes(n, 16)) // per_step
     t1a, t1b = rtree.build_top_down(da, 16), rtree.build_top_down(db, 16)
 
     def per_layer():

@@ -257,13 +257,16 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
 // With items set (buf0 or buf1), the items are there already, written by the
 // last digit pass of the level before.  With rects_out set, the last digit
 // pass writes, at every item's sorted position, its rect and id (ids[i], or
-// i when ids is null) and, in place of the sorted item, the item of the next
-// level: (ord32 of the other coordinate of the rect << 32 | position).
+// i when ids is null; i is the item's low word, the rect's index in `rects`)
+// and, in place of the sorted item, the item of the next level: (ord32 of
+// the other coordinate of the rect << 32 | position).  With next_items_only
+// instead, it writes only the next level's items, whose low word stays the
+// rect's index in `rects`.
 int radix_sort_td_coord(const RectLayers &rects, int comp, const uint64_t *items, uint64_t *buf0,
                         uint64_t *buf1, int64_t n, int64_t seg, int64_t layer, const RadixWs &w,
                         hipStream_t s, uint64_t **result, bool zeroed,
                         const uint32_t *ids = nullptr, float4 *rects_out = nullptr,
-                        uint32_t *ids_out = nullptr);
+                        uint32_t *ids_out = nullptr, bool next_items_only = false);
 int radix_sort_morton(const float4 *rects, uint64_t *buf0, uint64_t *buf1, int64_t n,
                       const RadixWs &w, hipStream_t s, uint64_t **result);
 // Counter words (cleared before a sort unless it clears them itself) and

@@ -40,10 +40,14 @@ class ProfScope {
 }  // namespace prt
 
 // Launch `kernel<<<grid, block, lds, stream>>>(args...)`; while profiling,
-// through hipExtLaunchKernelGGL with the scope's events.
+// through hipExtLaunchKernelGGL with the scope's events, recorded under the
+// kernel's name (PRT_LAUNCH) or under `name` (PRT_LAUNCH_AS: template
+// instances of one kernel recorded together).
 #define PRT_LAUNCH(kernel, grid, block, lds, stream, ...)                              \
+    PRT_LAUNCH_AS(#kernel, kernel, grid, block, lds, stream, __VA_ARGS__)
+#define PRT_LAUNCH_AS(name, kernel, grid, block, lds, stream, ...)                     \
     do {                                                                               \
-        ::prt::ProfScope _ps(#kernel);                                                 \
+        ::prt::ProfScope _ps(name);                                                    \
         if (_ps.active())                                                              \
             hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block),                     \
                                   static_cast<std::uint32_t>(lds), (stream), _ps.start(), \

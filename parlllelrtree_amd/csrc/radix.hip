@@ -21,6 +21,8 @@
 //     stay inside a segment, so one launch sequence sorts many independent
 //     segments (the top-down build's per-level segmented sort,
 //     TopDown-rtree.cu:577-588).
+#include <mutex>
+
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -29,8 +31,16 @@ namespace {
 
 constexpr int RS_THREADS = 1024;
 constexpr int RS_WAVES = RS_THREADS / 64;
-constexpr int RS_ITEMS = 4;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 items per tile
+// Items per thread: 4 (tiles of 4096) for sorts of fewer than
+// RS_BIG_ITEMS items, 8 (tiles of 8192) from there on.  Larger tiles halve
+// the tiles a pass walks through and the look-back words each item pays for,
+// and keep twice the items in flight per block; below RS_BIG_ITEMS they would
+// leave resident blocks idle (measured: 16.7M-rect bottom-up build 1.064 ->
+// 0.870 ms, top-down 2.929 -> 2.568 ms; two 1M-rect top-down layers
+// 0.267 -> 0.281 ms).  LDS per block stays under half a CU's (two blocks of
+// 16 waves per CU).
+constexpr int RS_BIG_ITEMS = 1 << 22;
+inline int tile_items_for(int64_t n) { return n >= RS_BIG_ITEMS ? 8 : 4; }
 constexpr int RS_BINS = 256;
 constexpr int RS_MAX_PASSES = 4;  // keys of up to 32 bits
 
@@ -39,14 +49,16 @@ constexpr int RS_MAX_PASSES = 4;  // keys of up to 32 bits
 struct Tiling {
     int64_t n, seg, layer;
     int64_t tiles_per_seg, segs_per_layer, num_segs, num_tiles;
+    int64_t tile;  // items per tile: RS_THREADS * (4 or 8)
 };
 
 Tiling make_tiling(int64_t n, int64_t seg, int64_t layer = 0) {
     Tiling t;
     t.n = n;
+    t.tile = static_cast<int64_t>(RS_THREADS) * tile_items_for(n);
     t.layer = (layer <= 0 || layer > n) ? n : layer;
     t.seg = (seg <= 0 || seg > t.layer) ? t.layer : seg;
-    t.tiles_per_seg = ceil_div(t.seg, RS_TILE);
+    t.tiles_per_seg = ceil_div(t.seg, t.tile);
     t.segs_per_layer = ceil_div(t.layer, t.seg);
     t.num_segs = t.segs_per_layer * (n / t.layer);
     t.num_tiles = t.tiles_per_seg * t.num_segs;
@@ -64,8 +76,8 @@ __device__ inline void tile_range(const Tiling &t, int64_t tile, int64_t &b, int
     const int64_t s = tile / t.tiles_per_seg, j = tile % t.tiles_per_seg;
     int64_t sb, se;
     seg_range(t, s, sb, se);
-    b = sb + j * RS_TILE;
-    e = b + RS_TILE < se ? b + RS_TILE : se;
+    b = sb + j * t.tile;
+    e = b + t.tile < se ? b + t.tile : se;
     if (b > e) b = e;
 }
 
@@ -159,7 +171,7 @@ struct SrcKeyHi {  // (high word of the key of the rect at position j of the
 // ghist[(pass * num_segs + s) * 256 + d] += items of tile blockIdx.x (in
 // segment s) whose digit of `pass` is d, for every pass; also clears the
 // tile's look-back words of every pass.
-template <class Src>
+template <class Src, int IT>
 __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int bit_lo, int bit_hi,
                                                          int passes, uint32_t *__restrict__ ghist,
                                                          uint64_t *__restrict__ status) {
@@ -177,15 +189,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
     // latency per tile, not one per item); then plain LDS atomics: a digit
     // shared by many lanes serialises only its own additions, which costs
     // less than ranking every item by ballots
-    uint64_t vals[RS_ITEMS];
+    uint64_t vals[IT];
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
+    for (int j = 0; j < IT; ++j) {
+        const int local = wave * (IT * 64) + j * 64 + lane;
         vals[j] = local < count ? src(b + local) : 0ull;
     }
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
+    for (int j = 0; j < IT; ++j) {
+        const int local = wave * (IT * 64) + j * 64 + lane;
         if (local >= count) continue;
         const uint64_t v = vals[j];
         for (int p = 0; p < passes; ++p) {
@@ -208,45 +220,207 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
 // rects of the next level come in sorted order without a pass of their own;
 // the item stored there is the next level's (ord32 of coordinate next_comp
 // of the rect << 32 | sorted position), which its sort then reads as is.
+// With rout null the pass writes only the next level's items, keeping the
+// low word (the rect's index in rin) as their payload.
 struct PassGather {
     RectLayers rin;       // rin.count == 0: no gather
     const uint32_t *iin;  // null: the id is the low word itself
-    float4 *rout;
+    float4 *rout;         // null: next items only
     uint32_t *iout;
     int next_comp;
 };
 
-// Stable scatter of one tile for one digit pass.  Wave w ranks the items
-// [w*256, (w+1)*256) of the tile in input order; lane l of round j holds item
-// w*256 + j*64 + l.  Output position of an item with digit d = segment start
-// + (items of the segment with a smaller digit: scan of ghist) + (items with
-// digit d in earlier tiles of the segment: look-back) + rank in the tile.
-// A look-back that polls an unpublished word more than spin_limit times
-// gives up and raises PRT_STATUS_LOOKBACK in *flags (the order it writes is
-// then wrong, and the flag says so); mute_tile (test hook, -1 = none)
-// publishes no look-back word.
-__global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restrict__ in,
-                                                         uint64_t *__restrict__ out, Tiling t,
-                                                         int shift, int bits,
-                                                         const uint32_t *__restrict__ ghist_p,
-                                                         uint64_t *status_p,
-                                                         uint32_t *__restrict__ ticket,
-                                                         PassGather gat, uint32_t *flags,
-                                                         int spin_limit, int64_t mute_tile) {
-    __shared__ uint64_t stage[RS_TILE];
-    __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
-    __shared__ uint32_t dstart[RS_BINS];
-    __shared__ uint32_t goff[RS_BINS];
-    __shared__ uint32_t red[2][RS_WAVES];
-    __shared__ uint32_t s_tile;
+// ---- one tile of one digit pass (k_onesweep), in phases.  Wave w ranks
+// the items [w*64*IT, (w+1)*64*IT) of the tile in input order; lane l of
+// round j holds item w*64*IT + j*64 + l.  Output position of an item with
+// digit d = segment start + (items of the segment with a smaller digit: scan
+// of ghist) + (items with digit d in earlier tiles of the segment: look-back)
+// + rank in the tile.  Per-wave digit counters are 16-bit (a tile holds at
+// most 8192 items).
+template <int IT> struct OsShared {
+    uint64_t stage[RS_THREADS * IT];
+    uint16_t wcnt[RS_WAVES][RS_BINS];
+    uint32_t dstart[RS_BINS];
+    uint32_t goff[RS_BINS];
+    uint32_t red[2][RS_WAVES];
+    uint32_t s_tile;
+};
 
+// the tile's items [b, b + count) into the thread's registers; every load
+// is issued before the first use (one memory latency per tile)
+template <int IT>
+__device__ inline void os_load(const uint64_t *__restrict__ in, int64_t b, int count,
+                               uint64_t (&item)[IT]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+        const int local = wave * (IT * 64) + j * 64 + lane;
+        item[j] = local < count ? in[b + local] : 0ull;
+    }
+}
+
+// stable rank of each item among the wave's earlier items of its digit
+// (wcnt cleared and a barrier passed); 0xFFFFFFFF for positions past count
+template <int IT>
+__device__ inline void os_rank(OsShared<IT> &S, const uint64_t (&item)[IT], int count, int shift,
+                               uint32_t mask, uint32_t (&rank)[IT]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+        const int local = wave * (IT * 64) + j * 64 + lane;
+        const bool valid = local < count;
+        const uint64_t v = item[j];
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
+        const uint64_t mism = digit_mismatch(d, valid);
+        uint32_t base = 0;
+        if (valid) base = S.wcnt[wave][d];
+        const uint32_t r = static_cast<uint32_t>(lane) - digit_lanes_below(mism);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && r == 0)
+            S.wcnt[wave][d] = static_cast<uint16_t>(base + 64u - static_cast<uint32_t>(__popcll(mism)));
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = valid ? base + r : 0xFFFFFFFFu;
+    }
+}
+
+// After the ranking's barrier.  Threads 0..255 own digit tid: the tile's
+// count of it (published at once: the aggregate, or the inclusive count for
+// the first tile of a segment) and the prefix over waves.  Every thread takes
+// part in the wave scans of the tile counts (x) and of the segment's digit
+// totals (g), whose wave sums go to red.
+template <class Sh>
+__device__ inline void os_publish(Sh &S, int64_t tile, int64_t jt, uint32_t tot,
+                                  uint64_t *status_p, int64_t mute_tile, uint32_t &run,
+                                  uint32_t &x, uint32_t &g) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    run = 0;
+    if (tid < RS_BINS) {
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const uint32_t c = S.wcnt[w][tid];
+            S.wcnt[w][tid] = static_cast<uint16_t>(run);
+            run += c;
+        }
+        if (tile != mute_tile)
+            lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
+    }
+    x = wave_incl_scan(run);
+    g = wave_incl_scan(tot);
+    if (lane == 63) {
+        S.red[0][wave] = x;
+        S.red[1][wave] = g;
+    }
+}
+
+// After the barrier that follows os_publish, threads 0..255: the tile's
+// digit starts (dstart); returns the digit's start in the segment (the scan
+// of the segment's digit totals).
+template <class Sh>
+__device__ inline uint32_t os_digit_starts(Sh &S, uint32_t run, uint32_t x, uint32_t g,
+                                           uint32_t tot) {
+    const int tid = threadIdx.x, wave = tid >> 6;
+    uint32_t before = 0, gbefore = 0;
+    for (int w = 0; w < wave; ++w) {
+        before += S.red[0][w];
+        gbefore += S.red[1][w];
+    }
+    S.dstart[tid] = before + x - run;
+    return gbefore + g - tot;
+}
+
+// Threads 0..255: the segment's output offset of digit tid (goff), from a
+// look-back over the earlier tiles of the segment, LB_WIN words in flight
+// per step: counts are added from the nearest tile outwards up to the first
+// inclusive word; a word not yet published ends the step there (the first
+// tile of a segment publishes its inclusive count at once, so every walk
+// ends inside the segment).  A walk that polls an unpublished word more than
+// spin_limit times gives up and raises PRT_STATUS_LOOKBACK in *flags (the
+// order written is then wrong, and the flag says so); mute_tile (test hook,
+// -1 = none) publishes no look-back word.
+template <class Sh>
+__device__ inline void os_lookback(Sh &S, const Tiling &t, int64_t tile, int64_t seg,
+                                   int64_t jt, uint32_t run, uint32_t gbase, uint64_t *status_p,
+                                   uint32_t *flags, int spin_limit, int64_t mute_tile) {
+    const int tid = threadIdx.x;
+    uint64_t excl = 0;
+    if (jt > 0) {
+        int64_t k = tile - 1;
+        const int64_t first = tile - jt;
+        int spins = 0;
+        for (;;) {
+            uint64_t v[LB_WIN];
+#pragma unroll
+            for (int w = 0; w < LB_WIN; ++w)
+                v[w] = k - w >= first ? lb_load(&status_p[(k - w) * RS_BINS + tid]) : LB_INC;
+            int used = 0;
+            bool done = false;
+#pragma unroll
+            for (int w = 0; w < LB_WIN; ++w) {
+                if (done || used != w || (v[w] >> 62) == 0) continue;
+                excl += v[w] & LB_VAL;
+                used = w + 1;
+                done = (v[w] >> 62) == 2;
+            }
+            if (done) break;
+            if (used == 0) {
+                if (++spins > spin_limit) {
+                    if (flags) atomicOr(flags, static_cast<uint32_t>(PRT_STATUS_LOOKBACK));
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            k -= used;
+        }
+        if (tile != mute_tile) lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
+    }
+    int64_t sb, se;
+    seg_range(t, seg, sb, se);
+    S.goff[tid] = static_cast<uint32_t>(sb) + gbase + static_cast<uint32_t>(excl);
+}
+
+// After the barrier that follows os_digit_starts: the items in digit order
+template <int IT>
+__device__ inline void os_stage(OsShared<IT> &S, const uint64_t (&item)[IT],
+                                const uint32_t (&rank)[IT], int shift, uint32_t mask) {
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < IT; ++j) {
+        if (rank[j] == 0xFFFFFFFFu) continue;
+        const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
+        S.stage[S.dstart[d] + S.wcnt[wave][d] + rank[j]] = item[j];
+    }
+}
+
+// After os_stage and os_lookback and a barrier: each digit's run written to
+// its place, consecutive threads on consecutive items (coalesced)
+template <class Sh>
+__device__ inline void os_write(Sh &S, uint64_t *__restrict__ out, int count, int shift,
+                                uint32_t mask) {
+    for (int i = threadIdx.x; i < count; i += RS_THREADS) {
+        const uint64_t v = S.stage[i];
+        const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
+        out[S.goff[d] + (i - S.dstart[d])] = v;
+    }
+}
+
+// One tile per block: the tile is the block index when the whole grid fits
+// the GPU at once, else taken from an atomic ticket (so every tile a block
+// waits for belongs to a block that is already running).  With gat set
+// (the last pass of a top-down level), the pass also gathers the rects.
+template <int IT>
+__global__ __launch_bounds__(RS_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_onesweep(
+    const uint64_t *__restrict__ in, uint64_t *__restrict__ out, Tiling t, int shift, int bits,
+    const uint32_t *__restrict__ ghist_p, uint64_t *status_p, uint32_t *__restrict__ ticket,
+    PassGather gat, uint32_t *flags, int spin_limit, int64_t mute_tile) {
+    __shared__ OsShared<IT> S;
+    const int tid = threadIdx.x;
     const uint32_t mask = (1u << bits) - 1u;
     const bool by_block = ticket == nullptr;  // co-resident grid: tile = block
-    if (tid == 0 && !by_block) s_tile = atomicAdd(ticket, 1u);
-    for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
+    if (tid == 0 && !by_block) S.s_tile = atomicAdd(ticket, 1u);
+    for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&S.wcnt[0][0])[i] = 0;
     if (!by_block) __syncthreads();
-    const int64_t tile = by_block ? static_cast<int64_t>(blockIdx.x) : static_cast<int64_t>(s_tile);
+    const int64_t tile = by_block ? static_cast<int64_t>(blockIdx.x)
+                                  : static_cast<int64_t>(S.s_tile);
     const int64_t seg = tile / t.tiles_per_seg, jt = tile % t.tiles_per_seg;
     int64_t b, e;
     tile_range(t, tile, b, e);
@@ -254,150 +428,72 @@ __global__ __launch_bounds__(RS_THREADS) void k_onesweep(const uint64_t *__restr
     // the segment's digit total is loaded now and used after the ranking
     const uint32_t tot = tid < RS_BINS ? ghist_p[seg * RS_BINS + tid] : 0u;
 
-    uint64_t item[RS_ITEMS];
-    uint32_t rank[RS_ITEMS];
-    // issue every load of the thread before ranking the first item
-#pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
-        item[j] = local < count ? in[b + local] : 0ull;
-    }
+    uint64_t item[IT];
+    uint32_t rank[IT];
+    os_load<IT>(in, b, count, item);
     if (by_block) __syncthreads();  // the counters are cleared
-#pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int local = wave * (RS_ITEMS * 64) + j * 64 + lane;
-        const bool valid = local < count;
-        const uint64_t v = item[j];
-        const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
-        const uint64_t mism = digit_mismatch(d, valid);
-        uint32_t base = 0;
-        if (valid) base = wcnt[wave][d];
-        const uint32_t r = static_cast<uint32_t>(lane) - digit_lanes_below(mism);
-        __builtin_amdgcn_wave_barrier();
-        if (valid && r == 0) wcnt[wave][d] = base + 64u - static_cast<uint32_t>(__popcll(mism));
-        __builtin_amdgcn_wave_barrier();
-        rank[j] = valid ? base + r : 0xFFFFFFFFu;
-    }
+    os_rank<IT>(S, item, count, shift, mask, rank);
     __syncthreads();
-    // threads 0..255 own digit tid: tile count (published at once), prefix
-    // over waves; block-wide exclusive scans of the tile counts (dstart) and
-    // of the segment's digit totals (gbase)
-    uint32_t run = 0;
-    if (tid < RS_BINS) {
-        for (int w = 0; w < RS_WAVES; ++w) {
-            const uint32_t c = wcnt[w][tid];
-            wcnt[w][tid] = run;
-            run += c;
-        }
-        if (tile != mute_tile)
-            lb_store(&status_p[tile * RS_BINS + tid], (jt == 0 ? LB_INC : LB_AGG) | run);
-    }
-    const uint32_t x = wave_incl_scan(run), g = wave_incl_scan(tot);
-    if (lane == 63) {
-        red[0][wave] = x;
-        red[1][wave] = g;
-    }
+    uint32_t run, x, g;
+    os_publish(S, tile, jt, tot, status_p, mute_tile, run, x, g);
     __syncthreads();
-    if (tid < RS_BINS) {
-        uint32_t before = 0, gbefore = 0;
-        for (int w = 0; w < wave; ++w) {
-            before += red[0][w];
-            gbefore += red[1][w];
-        }
-        dstart[tid] = before + x - run;
-        const uint32_t gbase = gbefore + g - tot;
-        // look-back over the earlier tiles of this segment, LB_WIN words in
-        // flight per step: counts are added from the nearest tile outwards up
-        // to the first inclusive word; a word not yet published ends the step
-        // there (the first tile of a segment publishes its inclusive count at
-        // once, so every walk ends inside the segment)
-        uint64_t excl = 0;
-        if (jt > 0) {
-            int64_t k = tile - 1;
-            const int64_t first = tile - jt;
-            int spins = 0;
-            for (;;) {
-                uint64_t v[LB_WIN];
-#pragma unroll
-                for (int w = 0; w < LB_WIN; ++w)
-                    v[w] = k - w >= first ? lb_load(&status_p[(k - w) * RS_BINS + tid]) : LB_INC;
-                int used = 0;
-                bool done = false;
-#pragma unroll
-                for (int w = 0; w < LB_WIN; ++w) {
-                    if (done || used != w || (v[w] >> 62) == 0) continue;
-                    excl += v[w] & LB_VAL;
-                    used = w + 1;
-                    done = (v[w] >> 62) == 2;
-                }
-                if (done) break;
-                if (used == 0) {
-                    if (++spins > spin_limit) {
-                        if (flags) atomicOr(flags, static_cast<uint32_t>(PRT_STATUS_LOOKBACK));
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                k -= used;
-            }
-            if (tile != mute_tile) lb_store(&status_p[tile * RS_BINS + tid], LB_INC | (excl + run));
-        }
-        int64_t sb, se;
-        seg_range(t, seg, sb, se);
-        goff[tid] = static_cast<uint32_t>(sb) + gbase + static_cast<uint32_t>(excl);
-    }
+    uint32_t gbase = 0;
+    if (tid < RS_BINS) gbase = os_digit_starts(S, run, x, g, tot);
     __syncthreads();
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        if (rank[j] == 0xFFFFFFFFu) continue;
-        const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
-        stage[dstart[d] + wcnt[wave][d] + rank[j]] = item[j];
-    }
+    // the tile goes to LDS in digit order before the look-back, so that the
+    // items and ranks hold no registers during the walk
+    os_stage<IT>(S, item, rank, shift, mask);
+    if (tid < RS_BINS)
+        os_lookback(S, t, tile, seg, jt, run, gbase, status_p, flags, spin_limit, mute_tile);
     __syncthreads();
     if (gat.rin.count == 0) {
-        for (int i = tid; i < count; i += RS_THREADS) {
-            const uint64_t v = stage[i];
-            const uint32_t d = static_cast<uint32_t>(v >> shift) & mask;
-            out[goff[d] + (i - dstart[d])] = v;
-        }
+        os_write(S, out, count, shift, mask);
         return;
     }
-    // the items' rects and ids are read (at random) before the first store,
-    // unconditionally (positions past the tile's count read entry 0, which
-    // exists since n >= 1, and store nothing) and into named registers:
-    // loads under a branch or into an array of rects were issued one at a
-    // time and parked in scratch memory
-    uint32_t src[RS_ITEMS], idv[RS_ITEMS];
+    // the items' rects and ids, four per thread at a time, are read (at
+    // random) before the first store, unconditionally (positions past the
+    // tile's count read entry 0, which exists since n >= 1, and store
+    // nothing) and into named registers: loads under a branch or into an
+    // array of rects were issued one at a time and parked in scratch memory
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-        const int i = tid + j * RS_THREADS;
-        item[j] = i < count ? stage[i] : 0ull;
-        src[j] = static_cast<uint32_t>(item[j]);
-    }
-    static_assert(RS_ITEMS == 4, "the gather names four rects per thread");
-    const float4 rc0 = gat.rin.at(src[0]), rc1 = gat.rin.at(src[1]), rc2 = gat.rin.at(src[2]),
-                 rc3 = gat.rin.at(src[3]);
-    if (gat.iin) {
+    for (int h = 0; h < IT; h += 4) {
+        uint64_t it4[4];
+        uint32_t src[4], idv[4];
 #pragma unroll
-        for (int j = 0; j < RS_ITEMS; ++j) idv[j] = gat.iin[src[j]];
-    } else {
-#pragma unroll
-        for (int j = 0; j < RS_ITEMS; ++j) idv[j] = src[j];
-    }
-    auto put = [&](int j, const float4 &rc) {
-        const int i = tid + j * RS_THREADS;
-        if (i < count) {
-            const uint32_t d = static_cast<uint32_t>(item[j] >> shift) & mask;
-            const uint32_t pos = goff[d] + (i - dstart[d]);
-            out[pos] = (static_cast<uint64_t>(ord32(gat.next_comp ? rc.y : rc.x)) << 32) | pos;
-            gat.rout[pos] = rc;
-            gat.iout[pos] = idv[j];
+        for (int j = 0; j < 4; ++j) {
+            const int i = tid + (h + j) * RS_THREADS;
+            it4[j] = i < count ? S.stage[i] : 0ull;
+            src[j] = static_cast<uint32_t>(it4[j]);
         }
-    };
-    put(0, rc0);
-    put(1, rc1);
-    put(2, rc2);
-    put(3, rc3);
+        const float4 rc0 = gat.rin.at(src[0]), rc1 = gat.rin.at(src[1]),
+                     rc2 = gat.rin.at(src[2]), rc3 = gat.rin.at(src[3]);
+        if (gat.iin) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) idv[j] = gat.iin[src[j]];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) idv[j] = src[j];
+        }
+        auto put = [&](int j, const float4 &rc) {
+            const int i = tid + (h + j) * RS_THREADS;
+            if (i < count) {
+                const uint32_t d = static_cast<uint32_t>(it4[j] >> shift) & mask;
+                const uint32_t pos = S.goff[d] + (i - S.dstart[d]);
+                const uint64_t key = static_cast<uint64_t>(ord32(gat.next_comp ? rc.y : rc.x)) << 32;
+                if (gat.rout) {
+                    out[pos] = key | pos;
+                    gat.rout[pos] = rc;
+                    gat.iout[pos] = idv[j];
+                } else {
+                    out[pos] = key | src[j];
+                }
+            }
+        };
+        put(0, rc0);
+        put(1, rc1);
+        put(2, rc2);
+        put(3, rc3);
+    }
 }
 
 struct SortLayout {
@@ -441,20 +537,24 @@ int64_t radix_status_words(int64_t n, int64_t seg, int64_t layer) {
 
 
 namespace {
-// blocks of k_onesweep the GPU holds at once (CUs x blocks per CU)
-int64_t onesweep_resident_blocks() {
-    static int64_t cap = -1;
-    if (cap < 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_onesweep, RS_THREADS, 0) !=
+// blocks of k_onesweep<IT> the GPU holds at once (CUs x blocks per CU), per
+// device; 0 when unknown (then every pass takes tickets)
+template <int IT> int64_t onesweep_resident_blocks() {
+    static int64_t cache[64] = {};
+    static std::mutex mu;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_onesweep<IT>, RS_THREADS, 0) !=
                 hipSuccess)
-            cap = 0;  // unknown: always take tickets
+            cache[dev] = -1;
         else
-            cap = static_cast<int64_t>(cus) * per_cu;
+            cache[dev] = static_cast<int64_t>(cus) * per_cu;
     }
-    return cap;
+    return cache[dev] > 0 ? cache[dev] : 0;
 }
 
 template <class Src>
@@ -487,23 +587,40 @@ int radix_sort_impl(const Src &prod, const uint64_t *in, uint64_t *buf0, uint64_
         PRT_LAUNCHED();
     }
     const dim3 grid(static_cast<unsigned>(t.num_tiles));
-    PRT_LAUNCH(k_hist_all<Src>, grid, dim3(RS_THREADS), 0, s, prod, t, bit_lo, bit_hi, passes, ghist,
-               w.status);
+    const bool big = t.tile > RS_THREADS * 4;
+    if (big) {
+        auto kh = k_hist_all<Src, 8>;
+        PRT_LAUNCH_AS("k_hist_all<Src>", kh, grid, dim3(RS_THREADS), 0, s, prod, t, bit_lo, bit_hi,
+                      passes, ghist, w.status);
+    } else {
+        auto kh = k_hist_all<Src, 4>;
+        PRT_LAUNCH_AS("k_hist_all<Src>", kh, grid, dim3(RS_THREADS), 0, s, prod, t, bit_lo, bit_hi,
+                      passes, ghist, w.status);
+    }
     PRT_LAUNCHED();
     const uint64_t *src = in;
     uint64_t *dst = (in == buf0) ? buf1 : buf0;
-    const bool by_block = t.num_tiles <= onesweep_resident_blocks();
+    const bool by_block =
+        t.num_tiles <= (big ? onesweep_resident_blocks<8>() : onesweep_resident_blocks<4>());
     uint32_t *flags = device_status_flags();
     PRT_REQUIRE(flags != nullptr, PRT_E_HIP, "radix sort: no device status word");
     for (int p = 0; p < passes; ++p) {
         const int shift = bit_lo + 8 * p;
         const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
         const PassGather none{};
-        PRT_LAUNCH(k_onesweep, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
-                   ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS,
-                   w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS,
-                   by_block ? nullptr : tickets + p,
-                   p + 1 == passes ? gat : none, flags, g_spin_limit, g_mute_tile);
+        const PassGather &pg = p + 1 == passes ? gat : none;
+        const uint32_t *gh = ghist + static_cast<int64_t>(p) * t.num_segs * RS_BINS;
+        uint64_t *st = w.status + static_cast<int64_t>(p) * t.num_tiles * RS_BINS;
+        uint32_t *tk = by_block ? nullptr : tickets + p;
+        if (big) {
+            auto ko = k_onesweep<8>;
+            PRT_LAUNCH_AS("k_onesweep", ko, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
+                          gh, st, tk, pg, flags, g_spin_limit, g_mute_tile);
+        } else {
+            auto ko = k_onesweep<4>;
+            PRT_LAUNCH_AS("k_onesweep", ko, grid, dim3(RS_THREADS), 0, s, src, dst, t, shift, bits,
+                          gh, st, tk, pg, flags, g_spin_limit, g_mute_tile);
+        }
         PRT_LAUNCHED();
         src = dst;
         dst = (dst == buf0) ? buf1 : buf0;
@@ -522,9 +639,10 @@ int radix_sort_u64(const uint64_t *in, uint64_t *buf0, uint64_t *buf1, int64_t n
 int radix_sort_td_coord(const RectLayers &rects, int comp, const uint64_t *items, uint64_t *buf0,
                         uint64_t *buf1, int64_t n, int64_t seg, int64_t layer, const RadixWs &w,
                         hipStream_t s, uint64_t **result, bool zeroed, const uint32_t *ids,
-                        float4 *rects_out, uint32_t *ids_out) {
+                        float4 *rects_out, uint32_t *ids_out, bool next_items_only) {
     PassGather g{};
     if (rects_out) g = PassGather{rects, ids, rects_out, ids_out, comp ^ 1};
+    else if (next_items_only) g = PassGather{rects, nullptr, nullptr, nullptr, comp ^ 1};
     if (items) {
         PRT_REQUIRE(items == buf0 || items == buf1, PRT_E_INVAL,
                     "radix sort: items outside the sort's buffers");

@@ -800,7 +800,17 @@ int td_build_pow2(int nl, const RectLayers &in, int64_t n, int f, prt_td_tree *t
         lg += lw.ghist_cap;
         uint64_t *sorted = nullptr;
         int rc;
-        if (p + 1 < p0) {
+        if (p + 2 < p0) {
+            // two or more radix levels follow: the last digit pass writes
+            // only the next level's items, and their payload stays the
+            // original index (the rects are not moved: the next level that
+            // moves them reads them from the input, one random read per rect,
+            // instead of from a moved copy plus its id array)
+            rc = radix_sort_td_coord(cur_r, p & 1, made, ia, ib, N, a, n, lw, s, &sorted, true,
+                                     nullptr, nullptr, nullptr, true);
+            if (rc) return rc;
+            made = sorted;
+        } else if (p + 1 < p0) {
             // the last digit pass also moves the rects and ids into this
             // level's order, and writes the next level's items
             float4 *nxt_r = (cur_r.p[0] == ra) ? rb : ra;

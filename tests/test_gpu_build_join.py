@@ -53,7 +53,7 @@ def test_morton_ref4d(dev):
     np.testing.assert_array_equal(out.cpu().numpy().view(np.uint64), O.morton_ref4d(r, scale))
 
 
-@pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 100003, 1 << 20])
+@pytest.mark.parametrize("n", [1, 2, 4095, 4096, 4097, 100003, 1 << 20, (1 << 22) + 4099])
 @pytest.mark.parametrize("bits,hi", [(30, 1 << 30), (32, 1 << 32), (8, 7)])
 def test_sort_pairs_stable(dev, n, bits, hi):
     rng = np.random.default_rng(n + bits)
@@ -118,7 +118,8 @@ def test_td_golden(dev):
 
 
 @pytest.mark.parametrize("n,f", [(1, 16), (2, 16), (16, 16), (17, 16), (300, 4), (1000, 10),
-                                 (5000, 16), (70000, 16), (200000, 10), (1 << 20, 16)])
+                                 (5000, 16), (70000, 16), (200000, 10), (300000, 4), (1 << 20, 16),
+                                 ((1 << 22) + 17, 16)])
 def test_td_build_parity(dev, n, f):
     r = rects(n, 7 * n)
     t = rtree.build_top_down(dev_rects(r), f)
