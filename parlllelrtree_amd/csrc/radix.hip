@@ -188,7 +188,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
     // all loads of the thread are issued before the first use (one memory
     // latency per tile, not one per item); then plain LDS atomics: a digit
     // shared by many lanes serialises only its own additions, which costs
-    // less than ranking every item by ballots
+    // less than ranking every item by ballots.  A digit shared by every lane
+    // of the wave (the high digits inside a top-down segment, whose keys span
+    // a narrow range) is added once, by the first lane, instead of 64 times
+    // to one counter.
     uint64_t vals[IT];
 #pragma unroll
     for (int j = 0; j < IT; ++j) {
@@ -198,12 +201,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_hist_all(Src src, Tiling t, int 
 #pragma unroll
     for (int j = 0; j < IT; ++j) {
         const int local = wave * (IT * 64) + j * 64 + lane;
-        if (local >= count) continue;
+        const bool valid = local < count;
+        const uint64_t vmask = __ballot(valid);
         const uint64_t v = vals[j];
         for (int p = 0; p < passes; ++p) {
             const int shift = bit_lo + 8 * p;
             const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
-            atomicAdd(&cnt[p][static_cast<uint32_t>(v >> shift) & ((1u << bits) - 1u)], 1u);
+            const uint32_t d = static_cast<uint32_t>(v >> shift) & ((1u << bits) - 1u);
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            if (vmask == ~0ull && __ballot(d != d0) == 0ull) {
+                if (lane == 0) atomicAdd(&cnt[p][d0], 64u);
+            } else if (valid) {
+                atomicAdd(&cnt[p][d], 1u);
+            }
         }
     }
     __syncthreads();
