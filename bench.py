@@ -195,17 +195,22 @@ def cpu_rate(fn, units: float, unit: str, reps: int, cores: int, sample: str,
     return out
 
 
-def recorded(name: str, key: str):
+def recorded(name: str, key: str, which: str = "build"):
     """A recorded counter figure from profiles/<name>, only while the kernel
-    sources are the ones it was recorded for (else None)."""
+    sources it depends on are the ones it was recorded for (else None)."""
     from parlllelrtree_amd.profiling import source_digest
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     rec = json.loads(p.read_text())
-    if rec.get("source_digest") != source_digest():
+    if rec.get("source_digest") != source_digest(which):
         return None
-    return rec.get(key)
+    if key in rec:
+        return rec[key]
+    # the profiler names template instances (k_onesweep<8>); the HIP-event
+    # record names the kernel: take its one instance
+    inst = [k for k in rec if k.startswith(key + "<")]
+    return rec[inst[0]] if len(inst) == 1 else None
 
 
 # ---------------------------------------------------------------- headline
@@ -572,7 +577,7 @@ def grid_phase(dev, args, checks, cpu_jobs):
                         plain_reps=(1, 0))
     out["join_ms"] = round(ms_j, 3)
     out["pipeline"] = pipeline_phase(A, B, cap, out.pop("_tune"), out, dev)
-    valu = recorded("clip_fp64_valu.json", "grid_batched")
+    valu = recorded("clip_fp64_valu.json", "grid_batched", "clip")
     if valu:
         out["fp64_valu_counters"] = valu
     return out

@@ -32,14 +32,20 @@ from ._lib import lib
 CSRC = Path(__file__).resolve().parent / "csrc"
 
 
-def source_digest() -> str:
-    """sha256 (first 16 hex digits) of every kernel and host source of
-    libprt (csrc/*.hip, *.hpp, *.cpp, in name order)."""
+# the sources each recorded counter set depends on
+DIGEST_SETS = {
+    "build": ("prt_common.hpp", "radix.hip", "bu_build.hip", "td_build.hip"),
+    "clip": ("prt_common.hpp", "clip_lane.hpp", "clip_plain.hip", "clip_batched.hip"),
+}
+
+
+def source_digest(which: str = "build") -> str:
+    """sha256 (first 16 hex digits) of the libprt sources a recorded counter
+    figure depends on (DIGEST_SETS[which], in that order)."""
     h = hashlib.sha256()
-    for p in sorted(CSRC.iterdir()):
-        if p.suffix in (".hip", ".hpp", ".cpp"):
-            h.update(p.name.encode())
-            h.update(p.read_bytes())
+    for name in DIGEST_SETS[which]:
+        h.update(name.encode())
+        h.update((CSRC / name).read_bytes())
     return h.hexdigest()[:16]
 
 
@@ -115,6 +121,8 @@ def counters_main(argv: list[str]) -> int:
                     help="also write, per kernel name, the scaled bytes per dispatch averaged "
                          "over all its dispatches (the form bench.py reads)")
     ap.add_argument("--items", type=int, default=0, help="items per launch, for --traffic-json")
+    ap.add_argument("--digest", default="build", choices=sorted(DIGEST_SETS),
+                    help="the source set the figures are stamped with")
     args = ap.parse_args(argv)
     per = {}
     for spec in args.run:
@@ -151,7 +159,8 @@ def counters_main(argv: list[str]) -> int:
                            "scaled_bytes_per_dispatch": scaled}
     if args.json:
         Path(args.json).write_text(json.dumps({"scale": scale, "kernels": res,
-                                               "source_digest": source_digest()}, indent=1))
+                                               "source_digest": source_digest(args.digest)},
+                                              indent=1))
     if args.traffic_json:
         per_kernel: dict = {}
         for r in res.values():
@@ -159,11 +168,11 @@ def counters_main(argv: list[str]) -> int:
             d[0] += r["dispatches"]
             d[1] += r["dispatches"] * r["scaled_bytes_per_dispatch"]
         Path(args.traffic_json).write_text(json.dumps(
-            {"source_digest": source_digest(), "scale": scale,
+            {"source_digest": source_digest(args.digest), "scale": scale,
              **{k: {"traffic_bytes": int(round(t / c)), "dispatches": c,
                     "items_per_launch": args.items} for k, (c, t) in per_kernel.items()}},
             indent=1))
-    print(f"\nscale factors: {scale}; source digest {source_digest()}")
+    print(f"\nscale factors: {scale}; source digest ({args.digest}) {source_digest(args.digest)}")
     return 0
 
 
