@@ -72,11 +72,15 @@ __host__ __device__ inline int pow2_ge(int v) {
 
 // entries (and crossing heights) of one beam the team sweeps cooperatively;
 // none for the smallest pairs, whose serial sweep is short and whose teams
-// are limited by LDS
+// are limited by LDS.  At most 64: a beam holds few active edges (jittered
+// grid of lognormal stars, 82,241 beams of 3,000 pairs: 7.1 on average, 24 at
+// the 99th percentile, 42 at most), and a beam with more entries or crossing
+// heights takes the serial sweep (same sums), so the LDS a larger scratch
+// would cost every team goes to more teams per CU instead.
 __host__ __device__ inline int gen_cap(int NV) {
     if (NV <= 32) return 0;
     const int p = pow2_ge(NV);
-    return p < 512 ? p : 512;
+    return p < 64 ? p : 64;
 }
 
 __host__ __device__ inline size_t team_bytes(int NV, int SCAP) {
