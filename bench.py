@@ -812,6 +812,9 @@ def main() -> int:
     ap.add_argument("--cpu-reps-large", type=int, default=3,
                     help="median of this many runs for the serial 16.7M-rect builds")
     ap.add_argument("--no-phases", action="store_true")
+    ap.add_argument("--phases-only", action="store_true",
+                    help="skip the headline step (profiling runs of single phases); the line "
+                         "then carries no value")
     ap.add_argument("--phases", default="bu,layers,grid,grad,w4",
                     help="comma list of the phases to run (bu,layers,grid,grad,w4; "
                          "also grid_batched: the batched overlay alone)")
@@ -840,8 +843,12 @@ def main() -> int:
     dev = torch.device("cuda:0")
     log(f"device {torch.cuda.get_device_name(0)}")
     cpu_jobs, checks = [], []     # CPU figures and oracle comparisons, run after the GPU phases
-    head, w4_builds = headline(dev, args, cpu_jobs)
-    phases = {"w4_builds": w4_builds}
+    if args.phases_only:
+        head = {"value": float("nan"), "ms": float("nan"), "roofline": None, "cpu": None}
+        phases = {}
+    else:
+        head, w4_builds = headline(dev, args, cpu_jobs)
+        phases = {"w4_builds": w4_builds}
     want = set() if args.no_phases else set(args.phases.split(","))
     if "bu" in want:
         phases.update(bu_phase(dev, args, cpu_jobs))
@@ -863,7 +870,7 @@ def main() -> int:
         log(f"CPU leg: host {cpu_model()}, nproc {nproc()}, using up to {host_cores()} cores")
         for job in cpu_jobs:
             job(O)
-        cpu = head["cpu"]
+        cpu = head["cpu"] or None
         for chk in checks:
             chk(O)
 

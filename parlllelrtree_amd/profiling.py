@@ -176,6 +176,70 @@ def counters_main(argv: list[str]) -> int:
     return 0
 
 
+def valu_main(argv: list[str]) -> int:
+    """FP64 VALU of the batched overlay kernels from two rocprofv3 --pmc runs
+    (the FP64 instruction counters; the VALU activity counters), per bucket
+    launch and per call (the sum over one call's launches); markdown on
+    stdout, JSON with --json (stamped with the clip sources' digest).
+    FP64 counters count wave instructions: SQ_INSTS_VALU_FLOPS_FP64 = ADD +
+    MUL + 2 FMA + TRANS per wave instruction, so x64 lanes bounds the flops.
+    VALUBusy = SQ_ACTIVE_INST_VALU / (CUs x GRBM_GUI_ACTIVE / XCDs);
+    VALUUtilization = SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU)."""
+    ap = argparse.ArgumentParser(prog="profiling valu")
+    ap.add_argument("--fp64", required=True, help="run directory of the FP64 counters")
+    ap.add_argument("--active", required=True, help="run directory of the activity counters")
+    ap.add_argument("--kernel", default="k_clip_batched")
+    ap.add_argument("--calls", type=int, required=True, help="batched calls in each run")
+    ap.add_argument("--call-ms", type=float, required=True, help="wall time of one call (ms)")
+    ap.add_argument("--pairs", type=int, required=True)
+    ap.add_argument("--cus", type=int, default=256)
+    ap.add_argument("--xcds", type=int, default=8)
+    ap.add_argument("--peak-tflops", type=float, default=78.6)
+    ap.add_argument("--json", default=None)
+    args = ap.parse_args(argv)
+    data = {}
+    for d in (args.fp64, args.active):
+        for key, cnts in read_counter_csv(d).items():
+            if short_name(key[0]).startswith(args.kernel):
+                rec = data.setdefault((short_name(key[0]), key[1]), {})
+                for c, v in cnts.items():
+                    rec[c] = (float(np.sum(v)), len(v))
+    rows, tot_flops, tot_valu = [], 0.0, 0.0
+    for (k, g), r in sorted(data.items()):
+        get = lambda c: r[c][0] / args.calls if c in r else 0.0  # noqa: E731
+        flops = get("SQ_INSTS_VALU_FLOPS_FP64") * 64
+        valu = get("SQ_INSTS_VALU")
+        act = get("SQ_ACTIVE_INST_VALU")
+        thr = get("SQ_THREAD_CYCLES_VALU")
+        grbm = get("GRBM_GUI_ACTIVE") / args.xcds
+        busy = 100.0 * act / (args.cus * grbm) if grbm else 0.0
+        util = 100.0 * thr / (64 * act) if act else 0.0
+        fp64_share = 100.0 * get("SQ_INSTS_VALU_FLOPS_FP64") / valu if valu else 0.0
+        rows.append((k, g, flops, valu, fp64_share, busy, util))
+        tot_flops += flops
+        tot_valu += valu
+    print("| kernel (bucket launch) | grid | FP64 flops per call (64 lanes x wave instr) | VALU wave-instr per call | FP64 share | VALUBusy % | VALUUtilization % |")
+    print("|---|---|---|---|---|---|---|")
+    for k, g, fl, va, sh, bu, ut in rows:
+        print(f"| {k} | {g} | {fl:.3e} | {va:.3e} | {sh:.1f} % | {bu:.1f} | {ut:.1f} |")
+    tflops = tot_flops / (args.call_ms * 1e-3) / 1e12
+    print(f"\nper call ({args.call_ms} ms, {args.pairs} pairs): {tot_flops:.3e} FP64 flops at most "
+          f"(every lane of every FP64 wave instruction) = {tflops:.3f} TFLOP/s, "
+          f"{100 * tflops / args.peak_tflops:.2f} % of {args.peak_tflops} TFLOP/s")
+    if args.json:
+        Path(args.json).write_text(json.dumps({
+            "source_digest": source_digest("clip"),
+            "grid_batched": {
+                "pairs": args.pairs, "call_ms": args.call_ms,
+                "fp64_flops_upper_per_call": tot_flops, "valu_wave_instr_per_call": tot_valu,
+                "fp64_tflops_upper": round(tflops, 4),
+                "frac_of_peak_upper": round(tflops / args.peak_tflops, 5),
+                "launches": [{"kernel": k, "grid": g, "fp64_share_pct": round(sh, 2),
+                              "valu_busy_pct": round(bu, 1), "valu_utilization_pct": round(ut, 1)}
+                             for k, g, _, _, sh, bu, ut in rows]}}, indent=1))
+    return 0
+
+
 def trace_main(argv: list[str]) -> int:
     """Per (kernel, grid): dispatches, average / median / total duration of a
     rocprofv3 --kernel-trace CSV (markdown on stdout)."""
@@ -203,4 +267,6 @@ if __name__ == "__main__":
         sys.exit(counters_main(sys.argv[2:]))
     if len(sys.argv) > 1 and sys.argv[1] == "trace":
         sys.exit(trace_main(sys.argv[2:]))
+    if len(sys.argv) > 1 and sys.argv[1] == "valu":
+        sys.exit(valu_main(sys.argv[2:]))
     print(__doc__)
