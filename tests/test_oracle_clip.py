@@ -1,7 +1,9 @@
 """Oracle: scanbeam intersection area and its gradient, pinned by closed forms,
-an independent Green's-theorem area, Sutherland-Hodgman on convex inputs,
-symmetry/translation invariants and central finite differences."""
+an independent Green's-theorem area, the exact rational area, Sutherland-
+Hodgman on convex inputs, symmetry/translation invariants and central finite
+differences."""
 import json
+from fractions import Fraction
 from pathlib import Path
 
 import numpy as np
@@ -55,6 +57,9 @@ def test_convex_matches_sutherland_hodgman():
 
 
 def test_stars_match_green_theorem():
+    """Stars of radius 2e-3 around (0.3, 0.7), the scale of the grid
+    workload's polygons: the sweep's area against the independent Green's-
+    theorem area (measured over 3,200 such pairs: at most 1.5e-13)."""
     rng = np.random.default_rng(6)
     rel = []
     for _ in range(400):
@@ -64,8 +69,68 @@ def test_stars_match_green_theorem():
         a, g = O.clip_area(P, Q), O.green_area(P, Q)
         den = max(abs(g), 1e-12 * min(abs(O.poly_area(P)), abs(O.poly_area(Q))))
         rel.append(abs(a - g) / den)
-    assert max(rel) < 1e-9
-    assert np.median(rel) < 1e-13
+    assert max(rel) < 1e-12
+    assert np.median(rel) < 1e-14
+
+
+def _inside_exact(px, py, V):
+    c = False
+    for i in range(len(V)):
+        x0, y0 = V[i]
+        x1, y1 = V[(i + 1) % len(V)]
+        if (y0 > py) != (y1 > py) and px < x0 + (py - y0) * (x1 - x0) / (y1 - y0):
+            c = not c
+    return c
+
+
+def exact_intersection_area(P, Q):
+    """The exact area of P & Q (simple polygons, counter-clockwise, in general
+    position) in rational arithmetic: Green's theorem over the parts of each
+    boundary inside the other polygon, every crossing parameter and midpoint
+    test exact (the float vertices are exact rationals)."""
+    Pf = [(Fraction(x), Fraction(y)) for x, y in P]
+    Qf = [(Fraction(x), Fraction(y)) for x, y in Q]
+    tot = Fraction(0)
+    for A, B in ((Pf, Qf), (Qf, Pf)):
+        for i in range(len(A)):
+            (ax, ay), (bx, by) = A[i], A[(i + 1) % len(A)]
+            dx, dy = bx - ax, by - ay
+            ts = {Fraction(0), Fraction(1)}
+            for j in range(len(B)):
+                (cx, cy), (ex, ey) = B[j], B[(j + 1) % len(B)]
+                fx, fy = ex - cx, ey - cy
+                den = dx * fy - dy * fx
+                if den != 0:
+                    t = ((cx - ax) * fy - (cy - ay) * fx) / den
+                    u = ((cx - ax) * dy - (cy - ay) * dx) / den
+                    if 0 < t < 1 and 0 <= u <= 1:
+                        ts.add(t)
+            ts = sorted(ts)
+            for t0, t1 in zip(ts, ts[1:]):
+                tm = (t0 + t1) / 2
+                if _inside_exact(ax + tm * dx, ay + tm * dy, B):
+                    x0, y0, x1, y1 = ax + t0 * dx, ay + t0 * dy, ax + t1 * dx, ay + t1 * dy
+                    tot += (x0 * y1 - x1 * y0) / 2
+    return tot
+
+
+@pytest.mark.parametrize("R,bound", [(2e-3, 1e-12), (1e-5, 1e-10)])
+def test_sweep_against_exact_area(R, bound):
+    """The sweep's area (GPC's crossing heights and trapezoids, which the
+    GPU follows bit for bit) against the exact rational area: its error grows
+    with the coordinates' magnitude over the polygons' size (measured over
+    9,600 star pairs at 0.3-0.7: at most 1.5e-13 at radius 2e-3, 1.7e-11 at
+    radius 1e-5, median 6e-16).  The FP64 Green's-theorem area stays within
+    1e-13 of the exact one, so it is the sharper independent check."""
+    rng = np.random.default_rng(17 if R > 1e-4 else 18)
+    for _ in range(25):
+        P = star(rng, int(rng.integers(4, 36)), 0.3, 0.7, R)
+        Q = star(rng, int(rng.integers(4, 36)), 0.3 + rng.uniform(-R, R),
+                 0.7 + rng.uniform(-R, R), R)
+        ex = exact_intersection_area(P, Q)
+        den = max(abs(float(ex)), 1e-12 * min(abs(O.poly_area(P)), abs(O.poly_area(Q))))
+        assert abs(Fraction(O.clip_area(P, Q)) - ex) / Fraction(den) <= bound
+        assert abs(Fraction(O.green_area(P, Q)) - ex) / Fraction(den) <= 1e-13
 
 
 def test_self_nested_disjoint_symmetry_translation():
