@@ -147,6 +147,12 @@ int prt_device_status(int32_t *flags_h, int clear, void *stream);
  * (-1, -1) restores the defaults.  No reference counterpart. */
 int prt_debug_lookback(int32_t spin_limit, int64_t mute_tile);
 
+/* Test hook of the contour count pass: one_lane = 1 sends every pair to the
+ * one-lane-per-pair kernel instead of the team kernels; tune (null: the
+ * defaults) sets the team shapes per vertex-count bucket.  No reference
+ * counterpart. */
+int prt_debug_contours(int32_t one_lane, const prt_tune *tune);
+
 /* ---- optional per-kernel timing (HIP events around every launch) ----
  * No reference counterpart (the reference times whole phases with
  * cudaEvent, BU:836-874).  prt_profile_read aggregates per kernel name:

@@ -74,6 +74,7 @@ SIGNATURES = {
     "prt_last_error": (c_char_p, []),
     "prt_device_status": (c_int, [c_void_p, c_int, c_void_p]),
     "prt_debug_lookback": (c_int, [c_int32, c_int64]),
+    "prt_debug_contours": (c_int, [c_int32, POINTER(TuneStruct)]),
     "prt_profile_enable": (None, [c_int]),
     "prt_profile_read": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int]),
     "prt_morton_center30": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),

@@ -1,0 +1,499 @@
+// clip_team.hpp -- the per-pair tables a team of TL lanes (a slice of one
+// wave64) builds in LDS for the batched overlay kernels (the FP64 area in
+// clip_batched.hip, the result contours in contours.hip), and the vertex-
+// count bucketing of the pairs that both launch per bucket.
+//
+// The tables restate GPC's edge and scanbeam tables
+// (vector_map_overlay_gpc_kernel.cu:1019-1085, 1162-1307) with the plain
+// kernels' FP64 expressions: edges bottom to top with dx = (top.x - bot.x) /
+// (top.y - bot.y), horizontal edges dropped, the distinct vertex heights of
+// the op's y range, ascending; every edge registers itself in the lists of
+// the beams it spans, with its x at the beam bottom and top.
+#pragma once
+
+#include "prt_common.hpp"
+
+namespace prt {
+namespace {
+
+constexpr int BK_THREADS_MAX = 512;
+constexpr int NBUCKET = PRT_TUNE_BUCKETS;
+
+__device__ inline void team_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+struct TeamLds {
+    double *bx, *by, *tx, *ty, *dx;  // [NV] edge slot = vertex index (P: i, Q: nP + j)
+    int32_t *ekb, *ekt;              // [NV] beams [ekb, ekt) spanned by the edge
+    double *ys;                      // [P2] scanbeam table (sorted, then unique)
+    double *bsum;                    // [NV] beam sums (also the unique-compaction buffer)
+    int32_t *cnt;                    // [NV + 1]
+    int32_t *boff;                   // [NV + 1]
+    uint16_t *list;                  // [SCAP] active edges of the beams of a window
+    double *lxb, *lxt;               // [SCAP] their x at the beam bottom and top
+    // team-cooperative sweep of beams with crossings (GS entries, GC heights)
+    double *gxa, *gxc;               // [GS] x of the entries at the sub-beam ends
+    uint16_t *gidx;                  // [GS] their edge ids
+    double *giy;                     // [GC] crossing heights of the beam
+    int32_t *gmisc;                  // [4] deferred-beam count, crossing count
+    int GS, GC;
+};
+
+__host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
+
+__host__ __device__ inline int pow2_ge(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// entries (and crossing heights) of one beam the team sweeps cooperatively;
+// none for the smallest pairs, whose serial sweep is short and whose teams
+// are limited by LDS.  At most 64: a beam holds few active edges (jittered
+// grid of lognormal stars, 82,241 beams of 3,000 pairs: 7.1 on average, 24 at
+// the 99th percentile, 42 at most), and a beam with more entries or crossing
+// heights takes the serial sweep (same sums), so the LDS a larger scratch
+// would cost every team goes to more teams per CU instead.
+__host__ __device__ inline int gen_cap(int NV) {
+    if (NV <= 32) return 0;
+    const int p = pow2_ge(NV);
+    return p < 64 ? p : 64;
+}
+
+__host__ __device__ inline size_t team_bytes(int NV, int SCAP) {
+    const int P2 = pow2_ge(NV);
+    const int G = gen_cap(NV);
+    return align16(5 * 8 * size_t(NV)) + 2 * align16(4 * size_t(NV)) + align16(8 * size_t(P2)) +
+           align16(8 * size_t(NV)) + 2 * align16(4 * size_t(NV + 1)) + align16(2 * size_t(SCAP)) +
+           2 * align16(8 * size_t(SCAP)) + 2 * align16(8 * size_t(G)) + align16(2 * size_t(G)) +
+           align16(8 * size_t(G)) + align16(4 * 4);
+}
+
+__device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
+    TeamLds t;
+    const int P2 = pow2_ge(NV);
+    t.bx = reinterpret_cast<double *>(p);
+    t.by = t.bx + NV;
+    t.tx = t.by + NV;
+    t.ty = t.tx + NV;
+    t.dx = t.ty + NV;
+    p += align16(5 * 8 * size_t(NV));
+    t.ekb = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV));
+    t.ekt = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV));
+    t.ys = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(P2));
+    t.bsum = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(NV));
+    t.cnt = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV + 1));
+    t.boff = reinterpret_cast<int32_t *>(p);
+    p += align16(4 * size_t(NV + 1));
+    t.list = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(SCAP));
+    t.lxb = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(SCAP));
+    t.lxt = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(SCAP));
+    const int G = gen_cap(NV);
+    t.GS = G;
+    t.GC = G;
+    t.gxa = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(G));
+    t.gxc = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(G));
+    t.gidx = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(G));
+    t.giy = reinterpret_cast<double *>(p);
+    p += align16(8 * size_t(G));
+    t.gmisc = reinterpret_cast<int32_t *>(p);
+    return t;
+}
+
+__device__ inline double ex(const TeamLds &t, int e, double y) {
+    if (y == t.ty[e]) return t.tx[e];
+    return t.bx[e] + t.dx[e] * (y - t.by[e]);
+}
+
+// first index k in [0, n) with ys[k] >= v (n if none)
+__device__ inline int lower_bound(const double *ys, int n, double v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (ys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// team-wide min/max/sum over TL lanes (butterfly inside the team's lanes)
+template <int TL> __device__ inline double team_min(double v) {
+#pragma unroll
+    for (int o = 1; o < TL; o <<= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <int TL> __device__ inline double team_max(double v) {
+#pragma unroll
+    for (int o = 1; o < TL; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// inclusive scan inside the team
+template <int TL> __device__ inline int team_incl_scan(int v, int tl) {
+#pragma unroll
+    for (int o = 1; o < TL; o <<= 1) {
+        int y = __shfl_up(v, o, TL);
+        if (tl >= o) v += y;
+    }
+    return v;
+}
+template <int TL> __device__ inline int team_bcast(int v, int src_lane_in_team, int team_base) {
+    return __shfl(v, team_base + src_lane_in_team, 64);
+}
+
+// Strict order (xa + xc, xa, id) inside a sub-beam.
+__device__ inline bool key_before(double ka, double xa_a, int ea, double kb, double xa_b, int eb) {
+    if (ka != kb) return ka < kb;
+    if (xa_a != xa_b) return xa_a < xa_b;
+    return ea < eb;
+}
+
+// Sorts the list entries [lo, hi) of one beam in place by the sub-beam key
+// (xb + xt, xb, edge id) (shell sort); returns true when xb and xt are then
+// both non-decreasing: no two of the beam's edges cross inside it, and the
+// sorted entries are the order of its only sub-beam.
+__device__ inline bool beam_sort_entries(const TeamLds &t, int lo, int hi) {
+    const int m = hi - lo;
+    const int gaps[4] = {23, 10, 4, 1};
+    for (int gi = 0; gi < 4; ++gi) {
+        const int g = gaps[gi];
+        if (g >= m) continue;
+        for (int i = lo + g; i < hi; ++i) {
+            const int e = t.list[i];
+            const double xb = t.lxb[i], xt = t.lxt[i], k = xb + xt;
+            int j = i;
+            while (j - g >= lo) {
+                const int e2 = t.list[j - g];
+                const double xb2 = t.lxb[j - g], xt2 = t.lxt[j - g];
+                if (!key_before(k, xb, e, xb2 + xt2, xb2, e2)) break;
+                t.list[j] = static_cast<uint16_t>(e2);
+                t.lxb[j] = xb2;
+                t.lxt[j] = xt2;
+                j -= g;
+            }
+            t.list[j] = static_cast<uint16_t>(e);
+            t.lxb[j] = xb;
+            t.lxt[j] = xt;
+        }
+    }
+    bool mono = true;
+    for (int i = lo + 1; i < hi && mono; ++i)
+        mono = !(t.lxb[i] < t.lxb[i - 1]) && !(t.lxt[i] < t.lxt[i - 1]);
+    return mono;
+}
+
+// team-wide bitonic sort of M2 (power of two) entries (gxa, gxc, gidx) by
+// the strict key (xa + xc, xa, id), ascending
+template <int TL> __device__ inline void team_sort_entries(const TeamLds &t, int M2, int tl) {
+    for (int k = 2; k <= M2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tl; q < M2 / 2; q += TL) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i | j;
+                const bool up = (i & k) == 0;
+                const double ai = t.gxa[i], ci = t.gxc[i], al = t.gxa[l], cl = t.gxc[l];
+                const int ei = t.gidx[i], el = t.gidx[l];
+                // swap when the pair is out of order for this direction
+                const bool l_first = key_before(al + cl, al, el, ai + ci, ai, ei);
+                const bool i_first = key_before(ai + ci, ai, ei, al + cl, al, el);
+                if (up ? l_first : i_first) {
+                    t.gxa[i] = al; t.gxc[i] = cl; t.gidx[i] = static_cast<uint16_t>(el);
+                    t.gxa[l] = ai; t.gxc[l] = ci; t.gidx[l] = static_cast<uint16_t>(ei);
+                }
+            }
+            team_sync();
+        }
+    }
+}
+
+// team-wide bitonic sort of giy[0, M2) ascending
+template <int TL> __device__ inline void team_sort_heights(const TeamLds &t, int M2, int tl) {
+    for (int k = 2; k <= M2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tl; q < M2 / 2; q += TL) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i | j;
+                const bool up = (i & k) == 0;
+                const double x = t.giy[i], y = t.giy[l];
+                if ((x > y) == up) {
+                    t.giy[i] = y;
+                    t.giy[l] = x;
+                }
+            }
+            team_sync();
+        }
+    }
+}
+
+// The tables of one pair in the team's LDS: edge table (slot = vertex index,
+// P: i, Q: nP + j; horizontal edges and edges outside the op's y range get
+// a -inf top and are never active), the distinct vertex heights of the op's
+// y range ys[0, NY) (bitonic sort across the team, then unique), the beams
+// [ekb, ekt) each edge spans and the exclusive scan boff[0, NY) of the
+// number of edges spanning each beam.  nP / nQ are 0 for empty operands
+// (eP / eQ).  Returns NY, or 0 when the result of `op` is empty (no beam).
+template <int TL>
+__device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int nP,
+                           const double *__restrict__ Q, int nQ, bool eP, bool eQ, int op, int tl,
+                           int base) {
+    const int n = nP + nQ;
+    // bounding boxes
+    double bp0 = __builtin_huge_val(), bp1 = bp0, bp2 = -bp0, bp3 = -bp0;
+    double bq0 = bp0, bq1 = bp0, bq2 = -bp0, bq3 = -bp0;
+    for (int i = tl; i < nP; i += TL) {
+        const double x = P[2 * i], y = P[2 * i + 1];
+        bp0 = fmin(bp0, x); bp2 = fmax(bp2, x); bp1 = fmin(bp1, y); bp3 = fmax(bp3, y);
+    }
+    for (int i = tl; i < nQ; i += TL) {
+        const double x = Q[2 * i], y = Q[2 * i + 1];
+        bq0 = fmin(bq0, x); bq2 = fmax(bq2, x); bq1 = fmin(bq1, y); bq3 = fmax(bq3, y);
+    }
+    bp0 = team_min<TL>(bp0); bp1 = team_min<TL>(bp1); bp2 = team_max<TL>(bp2); bp3 = team_max<TL>(bp3);
+    bq0 = team_min<TL>(bq0); bq1 = team_min<TL>(bq1); bq2 = team_max<TL>(bq2); bq3 = team_max<TL>(bq3);
+    const double bpa[4] = {bp0, bp1, bp2, bp3}, bqa[4] = {bq0, bq1, bq2, bq3};
+    double ylo = 0.0, yhi = 0.0;
+    if (!op_range(op, eP, eQ, bpa, bqa, ylo, yhi)) return 0;
+
+    // edge table and raw scanbeam table
+    const int P2 = pow2_ge(n);
+    const double INF = __builtin_huge_val();
+    for (int s = tl; s < P2; s += TL) {
+        if (s >= n) {
+            t.ys[s] = INF;
+            continue;
+        }
+        const bool inP = s < nP;
+        const double *V = inP ? P : Q;
+        const int nn = inP ? nP : nQ;
+        const int i = inP ? s : s - nP;
+        const int j = i + 1 == nn ? 0 : i + 1;
+        const double x0 = V[2 * i], y0 = V[2 * i + 1], x1 = V[2 * j], y1 = V[2 * j + 1];
+        t.ys[s] = (y0 >= ylo && y0 <= yhi) ? y0 : INF;
+        double bx, by, tx, ty;
+        if (y0 < y1) { bx = x0; by = y0; tx = x1; ty = y1; }
+        else { bx = x1; by = y1; tx = x0; ty = y0; }
+        const bool valid = y0 != y1 && !(ty <= ylo || by >= yhi);
+        t.bx[s] = bx;
+        t.by[s] = by;
+        t.tx[s] = tx;
+        t.ty[s] = valid ? ty : -INF;  // -inf top: never active
+        t.dx[s] = valid ? (tx - bx) / (ty - by) : 0.0;
+    }
+    team_sync();
+    // bitonic sort of ys[0, P2)
+    for (int k = 2; k <= P2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = tl; q < P2 / 2; q += TL) {
+                const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const int l = i | j;
+                const bool up = (i & k) == 0;
+                const double x = t.ys[i], y = t.ys[l];
+                if ((x > y) == up) {
+                    t.ys[i] = y;
+                    t.ys[l] = x;
+                }
+            }
+            team_sync();
+        }
+    }
+    // unique (finite values) -> compacted into bsum, then back into ys
+    const int per = (P2 + TL - 1) / TL;
+    const int c0 = tl * per, c1 = c0 + per < P2 ? c0 + per : P2;
+    int mine = 0;
+    for (int s = c0; s < c1; ++s) {
+        const double y = t.ys[s];
+        if (y != INF && (s == 0 || y != t.ys[s - 1])) ++mine;
+    }
+    const int incl = team_incl_scan<TL>(mine, tl);
+    const int NY = team_bcast<TL>(incl, TL - 1, base);
+    int w = incl - mine;
+    for (int s = c0; s < c1; ++s) {
+        const double y = t.ys[s];
+        if (y != INF && (s == 0 || y != t.ys[s - 1])) t.bsum[w++] = y;
+    }
+    team_sync();
+    for (int s = tl; s < NY; s += TL) t.ys[s] = t.bsum[s];
+    if (NY < 2) return 0;
+    const int nb = NY - 1;
+    for (int k = tl; k <= nb; k += TL) t.cnt[k] = 0;
+    team_sync();
+    // beam span of every edge; list entries per beam
+    for (int e = tl; e < n; e += TL) {
+        const double ty = t.ty[e];
+        int kb = 0, kt = 0;
+        if (ty != -INF) {
+            kb = lower_bound(t.ys, NY, t.by[e]);
+            kt = lower_bound(t.ys, NY, ty);
+            kt = kt < nb ? kt : nb;
+        }
+        t.ekb[e] = kb;
+        t.ekt[e] = kt;
+        for (int k = kb; k < kt; ++k) atomicAdd(&t.cnt[k], 1);
+    }
+    team_sync();
+    // exclusive scan of cnt[0, nb) -> boff, boff[nb] = S
+    const int pb = (nb + TL - 1) / TL;
+    const int b0 = tl * pb, b1 = b0 + pb < nb ? b0 + pb : nb;
+    int sum = 0;
+    for (int k = b0; k < b1; ++k) sum += t.cnt[k];
+    const int sincl = team_incl_scan<TL>(sum, tl);
+    const int S = team_bcast<TL>(sincl, TL - 1, base);
+    int run = sincl - sum;
+    for (int k = b0; k < b1; ++k) {
+        t.boff[k] = run;
+        run += t.cnt[k];
+    }
+    if (tl == 0) t.boff[nb] = S;
+    team_sync();
+    return NY;
+}
+
+// Active-edge lists of the window of beams [k0, k1): k1 is the largest beam
+// end whose entries fit SCAP list slots (at least k0 + 1).  Entry pos =
+// boff[k] - boff[k0] + i holds an edge spanning beam k and its x at the
+// beam's bottom (lxb) and top (lxt); the order inside a beam is the atomics'
+// (callers sort).  Returns k1, or -1 when one beam alone exceeds SCAP.  The
+// caller syncs the team before reading the lists.
+template <int TL>
+__device__ int team_window(const TeamLds &t, int k0, int nb, int n, int SCAP, int tl) {
+    // k1 = largest beam end with boff[k1] - boff[k0] <= SCAP (at least k0 + 1)
+    int lo = k0 + 1, hi = nb;
+    const int lim = t.boff[k0] + SCAP;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.boff[mid] <= lim) lo = mid;
+        else hi = mid - 1;
+    }
+    const int k1 = lo;
+    const int wbase = t.boff[k0];
+    if (t.boff[k1] - wbase > SCAP) return -1;  // one beam alone exceeds the lists
+    for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
+    team_sync();
+    for (int e = tl; e < n; e += TL) {
+        const int kb = t.ekb[e] > k0 ? t.ekb[e] : k0;
+        const int kt = t.ekt[e] < k1 ? t.ekt[e] : k1;
+        for (int k = kb; k < kt; ++k) {
+            const int pos = t.boff[k] - wbase + atomicAdd(&t.cnt[k], 1);
+            t.list[pos] = static_cast<uint16_t>(e);
+            t.lxb[pos] = ex(t, e, t.ys[k]);
+            t.lxt[pos] = ex(t, e, t.ys[k + 1]);
+        }
+    }
+    return k1;
+}
+
+// ---- bucketing -----------------------------------------------------------
+struct Bounds {
+    int32_t v[NBUCKET];
+};
+
+__device__ inline int bucket_of(int n, const Bounds &bd) {
+    for (int b = 0; b < NBUCKET; ++b)
+        if (n <= bd.v[b]) return b;
+    return NBUCKET - 1;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// per-bucket counts (block-aggregated atomics)
+__global__ __launch_bounds__(256) void k_bucket_count(const uint32_t *__restrict__ pairs,
+                                                      const int64_t *np_d, int64_t cap,
+                                                      const int64_t *__restrict__ offA,
+                                                      const int64_t *__restrict__ offB, Bounds bd,
+                                                      unsigned long long *__restrict__ counts) {
+    const int64_t np = device_count(np_d, cap);
+    __shared__ unsigned int c[NBUCKET];
+    if (threadIdx.x < NBUCKET) c[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < np;
+         p += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int n = static_cast<int>((offA[a + 1] - offA[a]) + (offB[b + 1] - offB[b]));
+        atomicAdd(&c[bucket_of(n, bd)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < NBUCKET && c[threadIdx.x]) atomicAdd(&counts[threadIdx.x], c[threadIdx.x]);
+}
+
+// bucket b's pairs go to order[start[b] + ...]; start = exclusive scan
+__global__ void k_bucket_starts(const unsigned long long *counts, int64_t *starts,
+                                int64_t *counts_i64, unsigned long long *fill) {
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int b = 0; b < NBUCKET; ++b) {
+            starts[b] = run;
+            counts_i64[b] = static_cast<int64_t>(counts[b]);
+            run += static_cast<int64_t>(counts[b]);
+            fill[b] = 0ull;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bucket_scatter(const uint32_t *__restrict__ pairs,
+                                                        const int64_t *np_d, int64_t cap,
+                                                        const int64_t *__restrict__ offA,
+                                                        const int64_t *__restrict__ offB, Bounds bd,
+                                                        const int64_t *__restrict__ starts,
+                                                        unsigned long long *__restrict__ fill,
+                                                        uint32_t *__restrict__ order) {
+    const int64_t np = device_count(np_d, cap);
+    __shared__ unsigned int c[NBUCKET];
+    __shared__ unsigned long long base[NBUCKET];
+    for (int64_t p0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; p0 < np;
+         p0 += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (threadIdx.x < NBUCKET) c[threadIdx.x] = 0;
+        __syncthreads();
+        const int64_t p = p0 + threadIdx.x;
+        int bk = -1;
+        unsigned int r = 0;
+        if (p < np) {
+            const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+            const int n = static_cast<int>((offA[a + 1] - offA[a]) + (offB[b + 1] - offB[b]));
+            bk = bucket_of(n, bd);
+            r = atomicAdd(&c[bk], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < NBUCKET && c[threadIdx.x])
+            base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], static_cast<unsigned long long>(c[threadIdx.x]));
+        __syncthreads();
+        if (bk >= 0) order[starts[bk] + static_cast<int64_t>(base[bk]) + r] = static_cast<uint32_t>(p);
+        __syncthreads();
+    }
+}
+
+// list capacity per team (entries of a window of beams; a pair whose beams
+// need more is processed in several windows)
+int scap_for(int NV, int factor) {
+    const int c = factor * NV;
+    const int cap = NV > 2048 ? NV : 2048;
+    return c < cap ? c : cap;
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+}  // namespace
+}  // namespace prt

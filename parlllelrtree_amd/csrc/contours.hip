@@ -43,7 +43,10 @@
 // sweeps again only the pairs that did not fit (the pools ran out), which
 // the count pass listed.  A pass's time is its slowest wave's, so a stage
 // pays off only when (nearly) every pair fits.
+#include <mutex>
+
 #include "clip_lane.hpp"
+#include "clip_team.hpp"
 #include "prt_common.hpp"
 #include "prt_profile.hpp"
 
@@ -63,6 +66,11 @@ struct ContourWs {
     IvArrays iv[2];
     int32_t *phead, *ptail, *pcnt, *pfree;  // open contours, pcap slots per lane
     int icap, pcap;
+    // team kernels only: x and uncertainty of the intervals above at the
+    // sub-beam top (icap slots), and the x of a sub-beam's entries at its
+    // ends for the beams one lane sweeps (cap slots)
+    double *tL = nullptr, *tR = nullptr, *tuL = nullptr, *tuR = nullptr;
+    double *sxa = nullptr, *sxc = nullptr;
 };
 
 struct ContourOut {  // write pass
@@ -104,7 +112,11 @@ __device__ inline double x_unc(const Lane &L, int e, double y) {
                       fabs(L.at(L.w.dx, e)) * (fabs(y) + fabs(by)));
 }
 
-template <int M> struct Builder {
+// FIXED: the lists below and above the scanline stay in iv[0] and iv[1] and
+// advance() copies the intervals above into iv[0] (the team kernels: a
+// local ContourWs indexed by the runtime side would live in scratch memory);
+// otherwise advance() flips which list is below.
+template <int M, bool FIXED = false> struct Builder {
     static constexpr bool W = M != CT_COUNT;  // vertices go to an arena
     const Lane &L;
     const ContourWs &cw;
@@ -122,13 +134,17 @@ template <int M> struct Builder {
     __device__ Builder(const Lane &l, const ContourWs &w, const ContourOut &out)
         : L(l), cw(w), o(out) {}
 
-    __device__ int32_t &B(int32_t *const IvArrays::*f, int k) const { return L.at(cw.iv[sb].*f, k); }
-    __device__ int32_t &A(int32_t *const IvArrays::*f, int k) const {
-        return L.at(cw.iv[1 - sb].*f, k);
+    __device__ int32_t &B(int32_t *const IvArrays::*f, int k) const {
+        return L.at(cw.iv[FIXED ? 0 : sb].*f, k);
     }
-    __device__ double &Bx(double *const IvArrays::*f, int k) const { return L.at(cw.iv[sb].*f, k); }
+    __device__ int32_t &A(int32_t *const IvArrays::*f, int k) const {
+        return L.at(cw.iv[FIXED ? 1 : 1 - sb].*f, k);
+    }
+    __device__ double &Bx(double *const IvArrays::*f, int k) const {
+        return L.at(cw.iv[FIXED ? 0 : sb].*f, k);
+    }
     __device__ double &Ax(double *const IvArrays::*f, int k) const {
-        return L.at(cw.iv[1 - sb].*f, k);
+        return L.at(cw.iv[FIXED ? 1 : 1 - sb].*f, k);
     }
 
     __device__ int vertex(double x, double y) {
@@ -203,7 +219,7 @@ template <int M> struct Builder {
         }
     }
     // path of p, then path of q; every chain end of q now belongs to p
-    __device__ void join(int p, int q) {
+    __device__ __attribute__((always_inline)) void join(int p, int q) {
         if (W && arena_ok(pt(p))) {
             o.vnext[vbase + pt(p)] = ph(q);
             pt(p) = pt(q);
@@ -343,7 +359,7 @@ template <int M> struct Builder {
         }
     }
 
-    __device__ void scanline(double y) {
+    __device__ __attribute__((always_inline)) void scanline(double y) {
         int i = 0, j = 0;
         bool have = false;
         const int nbp = 2 * nb, nap = 2 * na;
@@ -373,6 +389,69 @@ template <int M> struct Builder {
                 have = false;
             }
         }
+    }
+
+    // Team kernels (FIXED): the sub-beam's entries in order given directly --
+    // edge ids (edge slot = id), x at the sub-beam bottom ya and top yc --
+    // and each interval's values at both ends computed when it is made, so
+    // that advance_direct() only moves them below.  Same intervals, x's and
+    // uncertainties as intervals() and advance().
+    __device__ void intervals_direct(const uint16_t *ids, const double *xa, const double *xc, int m,
+                                     int nP, int op, double ya, double yc) {
+        int pp = 0, pq = 0, start = 0, last = 0;
+        bool open = false;
+        na = 0;
+        for (int t = 0; t + 1 < m; ++t) {
+            if (ids[t] < nP) pp ^= 1;
+            else pq ^= 1;
+            if (xa[t] == xa[t + 1] && xc[t] == xc[t + 1]) continue;
+            if (op_inside(op, pp, pq)) {
+                if (!open) {
+                    open = true;
+                    start = t;
+                }
+                last = t + 1;
+            } else if (open) {
+                add_direct(ids, xa, xc, start, last, ya, yc);
+                open = false;
+            }
+        }
+        if (open) add_direct(ids, xa, xc, start, last, ya, yc);
+    }
+    __device__ void add_direct(const uint16_t *ids, const double *xa, const double *xc, int s,
+                               int e, double ya, double yc) {
+        if (na >= cw.icap) {
+            bad = true;
+            return;
+        }
+        const int es = ids[s], ee = ids[e];
+        A(&IvArrays::eL, na) = es;
+        A(&IvArrays::eR, na) = ee;
+        Ax(&IvArrays::xL, na) = xa[s];
+        Ax(&IvArrays::xR, na) = xa[e];
+        Ax(&IvArrays::uL, na) = x_unc(L, es, ya);
+        Ax(&IvArrays::uR, na) = x_unc(L, ee, ya);
+        cw.tL[na] = xc[s];
+        cw.tR[na] = xc[e];
+        cw.tuL[na] = x_unc(L, es, yc);
+        cw.tuR[na] = x_unc(L, ee, yc);
+        A(&IvArrays::lp, na) = -1;
+        A(&IvArrays::rp, na) = -1;
+        ++na;
+    }
+    __device__ void advance_direct() {
+        for (int k = 0; k < na; ++k) {
+            B(&IvArrays::eL, k) = A(&IvArrays::eL, k);
+            B(&IvArrays::eR, k) = A(&IvArrays::eR, k);
+            B(&IvArrays::lp, k) = A(&IvArrays::lp, k);
+            B(&IvArrays::rp, k) = A(&IvArrays::rp, k);
+            Bx(&IvArrays::xL, k) = cw.tL[k];
+            Bx(&IvArrays::xR, k) = cw.tR[k];
+            Bx(&IvArrays::uL, k) = cw.tuL[k];
+            Bx(&IvArrays::uR, k) = cw.tuR[k];
+        }
+        nb = na;
+        na = 0;
     }
 
     // A <- inside intervals of the sub-beam (ord sorted), x at its bottom
@@ -417,7 +496,18 @@ template <int M> struct Builder {
     }
     // the intervals above become the ones below, x at the sub-beam top yc
     __device__ void advance(double yc) {
-        sb = 1 - sb;
+        if (FIXED) {
+            for (int k = 0; k < na; ++k) {
+                B(&IvArrays::eL, k) = A(&IvArrays::eL, k);
+                B(&IvArrays::eR, k) = A(&IvArrays::eR, k);
+                B(&IvArrays::sL, k) = A(&IvArrays::sL, k);
+                B(&IvArrays::sR, k) = A(&IvArrays::sR, k);
+                B(&IvArrays::lp, k) = A(&IvArrays::lp, k);
+                B(&IvArrays::rp, k) = A(&IvArrays::rp, k);
+            }
+        } else {
+            sb = 1 - sb;
+        }
         nb = na;
         na = 0;
         for (int k = 0; k < nb; ++k) {
@@ -501,13 +591,15 @@ __device__ void contours_one(Builder<M> &cb, const double *__restrict__ P, int n
 
 // Count pass: ncont[p], nvert[p]; -1 in both when the pair does not fit the
 // lane (more than max_verts vertices) or its arena bounds.  CT_STAGE also
-// keeps the pair's contours in st, or lists the pair in st.spill.
+// keeps the pair's contours in st, or lists the pair in st.spill.  With a
+// list (the pairs the team kernels passed on) only the listed pairs.
 template <int M>
 __global__ __launch_bounds__(CT_THREADS) void k_contours_count(
     const uint32_t *__restrict__ pairs, int64_t np, const double *__restrict__ xyA,
     const int64_t *__restrict__ offA, const double *__restrict__ xyB,
     const int64_t *__restrict__ offB, int op, ContourWs cw, StageWs st,
-    int64_t *__restrict__ ncont, int64_t *__restrict__ nvert) {
+    int64_t *__restrict__ ncont, int64_t *__restrict__ nvert, const uint32_t *__restrict__ list,
+    const unsigned long long *__restrict__ list_count) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= cw.base.T) return;
     Lane L(cw.base, t);
@@ -517,7 +609,13 @@ __global__ __launch_bounds__(CT_THREADS) void k_contours_count(
         so.vy = st.vy;
         so.vnext = st.vnext;
     }
-    for (int64_t p = t; p < np; p += cw.base.T) {
+    int64_t cnt = np;
+    if (list) {
+        const unsigned long long k = *list_count;
+        cnt = k < static_cast<unsigned long long>(np) ? static_cast<int64_t>(k) : np;
+    }
+    for (int64_t i = t; i < cnt; i += cw.base.T) {
+        const int64_t p = list ? static_cast<int64_t>(list[i]) : i;
         const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
         const int64_t oa = offA[a], ob = offB[b];
         const int nA = static_cast<int>(offA[a + 1] - oa), nB = static_cast<int>(offB[b + 1] - ob);
@@ -537,6 +635,361 @@ __global__ __launch_bounds__(CT_THREADS) void k_contours_count(
             if (!kept) st.spill[atomicAdd(st.ctr, 1ull)] = p;
         }
     }
+}
+
+// ---- team per pair ---------------------------------------------------------
+//
+// The count pass of pairs that fit a team: a team of TL lanes (a slice of one
+// wave64) takes a pair from its vertex-count bucket's ticket and builds the
+// batched overlay's tables in LDS (edge table, scanbeam table, per-beam
+// active-edge lists; clip_team.hpp).  The lanes sort every beam's entries by
+// the sub-beam key at once; a beam whose sorted entries cross is then swept by
+// the whole team (crossing heights and each sub-beam's order by bitonic sorts
+// across the team; beams with more entries or heights than the team's
+// scratch by lane 0 alone).  Lane 0 assembles the contours beam after beam
+// with the one-lane kernel's Builder, whose interval lists and open-contour
+// table live in the team's LDS: the same sub-beams, orders, x's and
+// uncertainties in the same sequence, so the same contours, vertex for
+// vertex.  A pair that does not fit the team's tables (more vertices than the
+// bucket's bound, a beam longer than the list capacity, more intervals or
+// open contours than the LDS arrays hold) goes to the one-lane kernel
+// through a list.
+
+struct TeamContourArgs {
+    const uint32_t *pairs;
+    const uint32_t *order;  // pair indices grouped by bucket
+    const int64_t *start;   // first index of this bucket in order (device)
+    const int64_t *count;   // number of pairs in the bucket (device)
+    const double *xyA, *xyB;
+    const int64_t *offA, *offB;
+    unsigned long long *ticket;
+    uint32_t *ovf_list;  // pairs left to the one-lane kernel
+    unsigned long long *ovf_count;
+    int64_t *ncont, *nvert;
+    StageWs st;
+    int NV, SCAP, ICAP, PCAP;
+    int op;
+};
+
+// interval slots and open contours a team holds (the one-lane kernel's
+// bounds, capped: pairs that need more take the one-lane kernel)
+__host__ __device__ inline int team_icap(int NV) { return NV / 2 + 1 < 65 ? NV / 2 + 1 : 65; }
+__host__ __device__ inline int team_pcap(int NV) { return NV + 2 < 130 ? NV + 2 : 130; }
+
+// LDS of one team: the overlay tables, then the sub-beam arrays (edge ids,
+// active edges, order, x at the sub-beam ends), two interval lists, the open
+// contours
+__host__ __device__ inline size_t contour_team_bytes(int NV, int SCAP) {
+    const size_t ic = team_icap(NV), pc = team_pcap(NV);
+    return team_bytes(NV, SCAP) + 3 * align16(4 * size_t(NV)) + 2 * align16(8 * size_t(NV)) +
+           2 * (6 * align16(4 * ic) + 4 * align16(8 * ic)) + 4 * align16(4 * pc);
+}
+
+// The Builder's view of a team's LDS: a lane arena of one lane (stride 1)
+// whose edge table is the team's, edge id = edge slot.
+__device__ inline ContourWs carve_contour_team(char *p, const TeamLds &t, int NV, int SCAP) {
+    ContourWs cw;
+    PlainWs &w = cw.base;
+    p += team_bytes(NV, SCAP);
+    auto take_i = [&](size_t n) {
+        int32_t *q = reinterpret_cast<int32_t *>(p);
+        p += align16(4 * n);
+        return q;
+    };
+    auto take_d = [&](size_t n) {
+        double *q = reinterpret_cast<double *>(p);
+        p += align16(8 * n);
+        return q;
+    };
+    w.bx = t.bx;
+    w.by = t.by;
+    w.tx = t.tx;
+    w.ty = t.ty;
+    w.dx = t.dx;
+    w.ys = t.ys;
+    w.xb = w.xt = w.bsum = nullptr;
+    w.eord = nullptr;
+    w.id = take_i(NV);
+    w.act = take_i(NV);
+    w.ord = take_i(NV);
+    w.xa = take_d(NV);
+    w.xc = take_d(NV);
+    w.T = 1;
+    w.cap = NV;
+    cw.icap = team_icap(NV);
+    cw.pcap = team_pcap(NV);
+    for (IvArrays &v : cw.iv) {
+        v.eL = take_i(cw.icap);
+        v.eR = take_i(cw.icap);
+        v.sL = take_i(cw.icap);
+        v.sR = take_i(cw.icap);
+        v.lp = take_i(cw.icap);
+        v.rp = take_i(cw.icap);
+        v.xL = take_d(cw.icap);
+        v.xR = take_d(cw.icap);
+        v.uL = take_d(cw.icap);
+        v.uR = take_d(cw.icap);
+    }
+    cw.phead = take_i(cw.pcap);
+    cw.ptail = take_i(cw.pcap);
+    cw.pcnt = take_i(cw.pcap);
+    cw.pfree = take_i(cw.pcap);
+    return cw;
+}
+
+// Next sub-beam of the general sweep of one beam by lane 0 alone (more
+// entries or crossing heights than the team's scratch): the lowest crossing
+// height above ya (yt when none), and the region [lo, hi) insertion-sorted
+// by the strict key (xa + xc, xa, edge id) at (ya, yc) -- the one-lane
+// kernel's sweep step.
+__device__ double serial_next(const TeamLds &t, int lo, int hi, double yb, double yt, double ya) {
+    const double dy = yt - yb;
+    double yc = yt;
+    for (int i = lo; i < hi; ++i) {
+        const double xbi = t.lxb[i], xti = t.lxt[i];
+        for (int j = i + 1; j < hi; ++j) {
+            const double xbj = t.lxb[j], xtj = t.lxt[j];
+            double xbL, xtL, xbR, xtR;
+            if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+            else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+            else continue;
+            const double den = (xtL - xbL) - (xtR - xbR);
+            if (!(den > 0.0)) continue;
+            const double r = (xbR - xbL) / den;
+            const double iy = r * dy + yb;
+            if (iy > ya && iy < yc) yc = iy;
+        }
+    }
+    auto xat = [&](int i, double y) -> double {
+        if (y == yb) return t.lxb[i];
+        if (y == yt) return t.lxt[i];
+        return ex(t, t.list[i], y);
+    };
+    for (int i = lo + 1; i < hi; ++i) {
+        const int e = t.list[i];
+        const double xa = xat(i, ya), k = xa + xat(i, yc);
+        const double xb = t.lxb[i], xt = t.lxt[i];
+        int j = i;
+        while (j > lo) {
+            const double xa2 = xat(j - 1, ya), k2 = xa2 + xat(j - 1, yc);
+            if (!key_before(k, xa, e, k2, xa2, t.list[j - 1])) break;
+            t.list[j] = t.list[j - 1];
+            t.lxb[j] = t.lxb[j - 1];
+            t.lxt[j] = t.lxt[j - 1];
+            --j;
+        }
+        t.list[j] = static_cast<uint16_t>(e);
+        t.lxb[j] = xb;
+        t.lxt[j] = xt;
+    }
+    return yc;
+}
+
+// Contours of one pair by the team, the Builder's state on lane 0.  Every
+// sub-beam -- of a beam with no entries inside (its intervals are none), of
+// a crossing-free beam (its sorted entries), of a beam with crossings (swept
+// by the team, or by lane 0 when it exceeds the team's scratch), and the
+// final scanline at the top height -- reaches the Builder through one call
+// site.  Returns 1 when the pair was handled (the Builder holds its counts
+// unless it went bad), 0 when it must go to the one-lane kernel.
+template <int TL, int M>
+__device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const TeamContourArgs &A,
+                                 uint32_t p, int tl, int base, Builder<M, true> &cb) {
+    const uint32_t a = A.pairs[2 * p], b = A.pairs[2 * p + 1];
+    const int64_t oa = A.offA[a], ob = A.offB[b];
+    int nP = static_cast<int>(A.offA[a + 1] - oa);
+    int nQ = static_cast<int>(A.offB[b + 1] - ob);
+    const double *P = A.xyA + 2 * oa, *Q = A.xyB + 2 * ob;
+    const bool eP = nP < 3, eQ = nQ < 3;  // empty operands
+    if (eP && eQ) return 1;
+    if (nP + nQ > A.NV) return 0;
+    if (eP) nP = 0;
+    if (eQ) nQ = 0;
+    const int n = nP + nQ;
+    const int NY = team_tables<TL>(t, P, nP, Q, nQ, eP, eQ, A.op, tl, base);
+    if (NY == 0) return 1;
+    const int nb = NY - 1;
+    const double INF = __builtin_huge_val();
+    const PlainWs &w = cw.base;
+    for (int k0 = 0; k0 < nb;) {
+        const int k1 = team_window<TL>(t, k0, nb, n, A.SCAP, tl);
+        if (k1 < 0) return 0;
+        const int wbase = t.boff[k0];
+        team_sync();
+        // every beam's entries sorted at once; cnt[k] = 1 when they cross
+        for (int k = k0 + tl; k < k1; k += TL)
+            t.cnt[k] = beam_sort_entries(t, t.boff[k] - wbase, t.boff[k + 1] - wbase) ? 0 : 1;
+        team_sync();
+        const int kend = k1 == nb ? nb + 1 : k1;  // k = nb: the final scanline
+        for (int k = k0; k < kend; ++k) {
+            int lo = 0, hi = 0;
+            double yb = t.ys[k], yt = yb;
+            if (k < nb) {
+                lo = t.boff[k] - wbase;
+                hi = t.boff[k + 1] - wbase;
+                yt = t.ys[k + 1];
+            }
+            const int m = hi - lo;
+            // 0: one sub-beam; 1: swept by the team; 2: swept by lane 0
+            int mode = (m < 2 || t.cnt[k] == 0) ? 0 : (m <= t.GS ? 1 : 2);
+            int nc = 0;
+            if (mode == 1) {  // the beam's crossing heights
+                const double dy = yt - yb;
+                if (tl == 0) t.gmisc[1] = 0;
+                team_sync();
+                for (int i = lo + tl; i < hi; i += TL) {
+                    const double xbi = t.lxb[i], xti = t.lxt[i];
+                    for (int j = i + 1; j < hi; ++j) {
+                        const double xbj = t.lxb[j], xtj = t.lxt[j];
+                        double xbL, xtL, xbR, xtR;
+                        if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+                        else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+                        else continue;
+                        const double den = (xtL - xbL) - (xtR - xbR);
+                        if (!(den > 0.0)) continue;
+                        const double r = (xbR - xbL) / den;
+                        const double iy = r * dy + yb;
+                        if (iy > yb && iy < yt) {
+                            const int pos = atomicAdd(&t.gmisc[1], 1);
+                            if (pos < t.GC) t.giy[pos] = iy;
+                        }
+                    }
+                }
+                team_sync();
+                nc = t.gmisc[1];
+                if (nc > t.GC) {
+                    mode = 2;  // more heights than the scratch holds
+                } else if (nc > 1) {
+                    const int C2 = pow2_ge(nc);
+                    for (int q = nc + tl; q < C2; q += TL) t.giy[q] = INF;
+                    team_sync();
+                    team_sort_heights<TL>(t, C2, tl);
+                }
+            }
+            double ya = yb;
+            int ci = 0;
+            for (;;) {
+                double yc = yt;
+                if (mode == 0) {
+                    for (int q = tl; q < m; q += TL) {
+                        w.act[q] = t.list[lo + q];
+                        w.ord[q] = q;
+                        w.xa[q] = t.lxb[lo + q];
+                        w.xc[q] = t.lxt[lo + q];
+                    }
+                } else if (mode == 1) {
+                    while (ci < nc && !(t.giy[ci] > ya)) ++ci;
+                    yc = ci < nc ? t.giy[ci] : yt;
+                    const int M2 = pow2_ge(m);
+                    for (int q = tl; q < M2; q += TL) {
+                        if (q < m) {
+                            const int i = lo + q;
+                            t.gxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+                            t.gxc[q] = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
+                            t.gidx[q] = t.list[i];
+                        } else {
+                            t.gxa[q] = INF;
+                            t.gxc[q] = INF;
+                            t.gidx[q] = 0xFFFFu;
+                        }
+                    }
+                    team_sync();
+                    team_sort_entries<TL>(t, M2, tl);
+                    for (int q = tl; q < m; q += TL) {
+                        w.act[q] = t.gidx[q];
+                        w.ord[q] = q;
+                        w.xa[q] = t.gxa[q];
+                        w.xc[q] = t.gxc[q];
+                    }
+                } else {
+                    double v = 0.0;
+                    if (tl == 0) {
+                        v = serial_next(t, lo, hi, yb, yt, ya);
+                        for (int q = 0; q < m; ++q) {
+                            const int i = lo + q;
+                            w.act[q] = t.list[i];
+                            w.ord[q] = q;
+                            w.xa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+                            w.xc[q] = v == yb ? t.lxb[i] : (v == yt ? t.lxt[i] : ex(t, t.list[i], v));
+                        }
+                    }
+                    yc = __shfl(v, base, 64);
+                }
+                team_sync();
+                if (tl == 0) {
+                    cb.intervals(m, nP, A.op, ya);
+                    cb.scanline(ya);
+                    cb.advance(yc);
+                }
+                team_sync();
+                if (yc == yt) break;
+                ya = yc;
+            }
+            if (__shfl(static_cast<int>(cb.bad), base, 64)) return 0;
+        }
+        k0 = k1;
+    }
+    return 1;
+}
+
+template <int TL, int M>
+__global__ __launch_bounds__(BK_THREADS_MAX) void k_contours_team(TeamContourArgs A) {
+    extern __shared__ __align__(16) char lds[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiw = lane / TL, tl = lane % TL;
+    const int base = tiw * TL;
+    const int team = wave * (64 / TL) + tiw;
+    char *mine = lds + team * contour_team_bytes(A.NV, A.SCAP);
+    const TeamLds t = carve_team(mine, A.NV, A.SCAP);
+    const ContourWs cw = carve_contour_team(mine, t, A.NV, A.SCAP);
+    for (int q = tl; q < A.NV; q += TL) cw.base.id[q] = q;  // edge id = edge slot
+    team_sync();
+    const Lane L(cw.base, 0);
+    ContourOut so{};
+    if (M == CT_STAGE) {
+        so.vx = A.st.vx;
+        so.vy = A.st.vy;
+        so.vnext = A.st.vnext;
+    }
+    const int64_t count = *A.count;
+    const uint32_t *order = A.order + *A.start;
+    for (;;) {
+        unsigned long long idx = 0;
+        if (tl == 0) idx = atomicAdd(A.ticket, 1ull);
+        idx = __shfl(idx, base, 64);
+        if (static_cast<int64_t>(idx) >= count) break;
+        const uint32_t p = order[idx];
+        Builder<M, true> cb(L, cw, so);
+        if (M == CT_STAGE) {  // pool slots are global: no base, the pool's bound
+            cb.st = &A.st;
+            cb.vlim = A.st.vcap;
+        }
+        const int ok = team_contour_pair<TL, M>(t, cw, A, p, tl, base, cb);
+        if (tl == 0) {
+            if (!ok || cb.bad) {
+                A.ovf_list[atomicAdd(A.ovf_count, 1ull)] = p;
+            } else {
+                A.ncont[p] = cb.nc;
+                A.nvert[p] = cb.nv;
+                if (M == CT_STAGE) {
+                    A.st.sc[p] = cb.spill ? -1 : static_cast<int32_t>(cb.nc);
+                    A.st.cfirst[p] = cb.first_rec;
+                    if (cb.spill) A.st.spill[atomicAdd(A.st.ctr, 1ull)] = p;
+                }
+            }
+        }
+        team_sync();
+    }
+}
+
+// Appends every pair of a bucket to the one-lane list (bucket too large for
+// the team's LDS).
+__global__ void k_contours_route(TeamContourArgs A) {
+    const int64_t count = *A.count;
+    const uint32_t *order = A.order + *A.start;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        A.ovf_list[atomicAdd(A.ovf_count, 1ull)] = order[i];
 }
 
 __global__ void k_stage_clear(unsigned long long *ctr, unsigned long long tag) {
@@ -647,7 +1100,7 @@ int contour_cap(int32_t max_verts) { return max_verts < 3 ? 3 : max_verts; }
 size_t contour_lane_bytes(int64_t T, int cap) {
     const int64_t icap = cap / 2 + 1, pcap = cap + 2;
     return lane_base_bytes(T, cap) +
-           static_cast<size_t>(T) * (2 * icap * (6 * 4 + 4 * 8) + pcap * 4 * 4) + 32 * 256;
+           static_cast<size_t>(T) * (2 * icap * (6 * 4 + 4 * 8) + pcap * 4 * 4) + 64 * 256;
 }
 
 int carve_contours(ContourWs &cw, int64_t T, int cap, Carver &c) {
@@ -738,6 +1191,157 @@ int carve_stage_checked(StageWs &st, int64_t np, const void *stage, size_t bytes
     return c.ok() ? PRT_OK : PRT_E_NOMEM;
 }
 
+// ---- the team path's host side ----------------------------------------
+
+// bucketing arrays of the team path (after the lane arena in the workspace)
+struct TeamCtWs {
+    unsigned long long *counts, *fill, *tickets, *ovf_count;
+    int64_t *starts, *counts_i64;
+    uint32_t *order, *ovf_list;
+};
+
+template <class C> void team_ct_layout(C &c, int64_t np, TeamCtWs *w) {
+    auto u64 = c.template take<unsigned long long>(4 * NBUCKET + 1);
+    auto i64 = c.template take<int64_t>(2 * NBUCKET);
+    auto ord = c.template take<uint32_t>(np > 0 ? np : 1);
+    auto ovf = c.template take<uint32_t>(np > 0 ? np : 1);
+    if (w) {
+        w->counts = u64;
+        w->fill = u64 + NBUCKET;
+        w->tickets = u64 + 2 * NBUCKET;
+        w->ovf_count = u64 + 4 * NBUCKET;
+        w->starts = i64;
+        w->counts_i64 = i64 + NBUCKET;
+        w->order = ord;
+        w->ovf_list = ovf;
+    }
+}
+
+// Team shapes of the count pass per vertex-count bucket (the batched
+// overlay's bounds), measured on the layer pairs and the grid workload.
+prt_tune contour_tune_default() {
+    prt_tune t;
+    const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
+    const int lanes[NBUCKET] = {16, 32, 64, 64, 64, 64};
+    const int waves[NBUCKET] = {2, 2, 1, 1, 1, 1};
+    for (int b = 0; b < NBUCKET; ++b) {
+        t.bucket_max_verts[b] = bounds[b];
+        t.lanes_per_pair[b] = lanes[b];
+        t.waves_per_block[b] = waves[b];
+        t.list_factor[b] = 2;
+    }
+    return t;
+}
+
+// test hook state: 1 = every pair takes the one-lane kernel; shapes
+int g_ct_one_lane = 0;
+bool g_ct_tune_set = false;
+prt_tune g_ct_tune;
+std::mutex g_ct_mu;
+
+// profiler labels of the team launches, per vertex-count bucket
+const char *const kTeamLabels[NBUCKET] = {"k_contours_team[b0]", "k_contours_team[b1]",
+                                          "k_contours_team[b2]", "k_contours_team[b3]",
+                                          "k_contours_team[b4]", "k_contours_team[b5]"};
+
+template <int TL, int M>
+int launch_contour_team(const TeamContourArgs &A, int waves, size_t lds, int bucket,
+                        hipStream_t s) {
+    auto kern = k_contours_team<TL, M>;
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, waves * 64, lds) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const unsigned grid = static_cast<unsigned>(device_cus() * per_cu);
+    PRT_LAUNCH_AS(kTeamLabels[bucket], kern, grid, waves * 64, lds, s, A);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}
+
+template <int M>
+int launch_contour_team_any(const TeamContourArgs &A, int lanes, int waves, size_t lds,
+                            int bucket, hipStream_t s) {
+    switch (lanes) {
+    case 8: return launch_contour_team<8, M>(A, waves, lds, bucket, s);
+    case 16: return launch_contour_team<16, M>(A, waves, lds, bucket, s);
+    case 32: return launch_contour_team<32, M>(A, waves, lds, bucket, s);
+    default: return launch_contour_team<64, M>(A, waves, lds, bucket, s);
+    }
+}
+
+// The count pass by teams: bucket the pairs by vertex count, one persistent
+// launch per bucket (largest first), then the one-lane kernel over the
+// pairs the teams passed on.
+int contours_count_team(const uint32_t *pairs, int64_t np, const double *xyA,
+                        const int64_t *offA, const double *xyB, const int64_t *offB,
+                        int32_t max_verts, int op, const ContourWs &cw, const TeamCtWs &tw,
+                        bool staged, const StageWs &st, int64_t *ncont, int64_t *nvert,
+                        hipStream_t s) {
+    prt_tune tune;
+    {
+        std::lock_guard<std::mutex> lk(g_ct_mu);
+        tune = g_ct_tune_set ? g_ct_tune : contour_tune_default();
+    }
+    Bounds bd;
+    for (int b = 0; b < NBUCKET; ++b) bd.v[b] = tune.bucket_max_verts[b];
+    PRT_LAUNCH(k_zero_u64, 1, 64, 0, s, tw.counts, 4 * NBUCKET + 1);
+    PRT_LAUNCHED();
+    const unsigned g = static_cast<unsigned>(ceil_div(np, 256) < 2048 ? ceil_div(np, 256) : 2048);
+    PRT_LAUNCH(k_bucket_count, g, 256, 0, s, pairs, nullptr, np, offA, offB, bd, tw.counts);
+    PRT_LAUNCHED();
+    PRT_LAUNCH(k_bucket_starts, 1, 64, 0, s, tw.counts, tw.starts, tw.counts_i64, tw.fill);
+    PRT_LAUNCHED();
+    PRT_LAUNCH(k_bucket_scatter, g, 256, 0, s, pairs, nullptr, np, offA, offB, bd, tw.starts,
+               tw.fill, tw.order);
+    PRT_LAUNCHED();
+    int max_lds = 0, dev = 0;
+    PRT_HIP(hipGetDevice(&dev));
+    PRT_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    for (int b = NBUCKET - 1; b >= 0; --b) {
+        const int lo = b == 0 ? 0 : tune.bucket_max_verts[b - 1];
+        if (max_verts <= lo) continue;  // no pair can fall in this bucket
+        TeamContourArgs A;
+        A.pairs = pairs;
+        A.order = tw.order;
+        A.start = tw.starts + b;
+        A.count = tw.counts_i64 + b;
+        A.xyA = xyA;
+        A.xyB = xyB;
+        A.offA = offA;
+        A.offB = offB;
+        A.ticket = tw.tickets + b;
+        A.ovf_list = tw.ovf_list;
+        A.ovf_count = tw.ovf_count;
+        A.ncont = ncont;
+        A.nvert = nvert;
+        A.st = st;
+        const int nvb = tune.bucket_max_verts[b] < max_verts ? tune.bucket_max_verts[b] : max_verts;
+        A.NV = nvb < 3 ? 3 : nvb;
+        A.SCAP = scap_for(A.NV, tune.list_factor[b]);
+        A.ICAP = team_icap(A.NV);
+        A.PCAP = team_pcap(A.NV);
+        A.op = op;
+        const int L = tune.lanes_per_pair[b], W = tune.waves_per_block[b];
+        const size_t lds = static_cast<size_t>(W) * (64 / L) * contour_team_bytes(A.NV, A.SCAP);
+        int rc;
+        if (lds > static_cast<size_t>(max_lds)) {
+            PRT_LAUNCH(k_contours_route, 256, 256, 0, s, A);
+            PRT_LAUNCHED();
+            continue;
+        }
+        rc = staged ? launch_contour_team_any<CT_STAGE>(A, L, W, lds, b, s)
+                    : launch_contour_team_any<CT_COUNT>(A, L, W, lds, b, s);
+        if (rc) return rc;
+    }
+    const unsigned grid = static_cast<unsigned>(ceil_div(cw.base.T, CT_THREADS));
+    auto one_lane = staged ? k_contours_count<CT_STAGE> : k_contours_count<CT_COUNT>;
+    PRT_LAUNCH_AS("k_contours_count[list]", one_lane, grid, CT_THREADS, 0, s, pairs, np, xyA, offA,
+                  xyB, offB, op, cw, st, ncont, nvert, tw.ovf_list, tw.ovf_count);
+    PRT_LAUNCHED();
+    return PRT_OK;
+}
+
 }  // namespace
 }  // namespace prt
 
@@ -757,6 +1361,9 @@ extern "C" size_t prt_overlay_contours_workspace_size(int64_t npairs, int32_t ma
     s.take<double>(total_vertices);
     s.take<int32_t>(total_vertices);
     s.take<int32_t>(64);  // status
+    SizeTaker z;          // the team path's bucketing (count pass)
+    team_ct_layout(z, npairs, nullptr);
+    s.take<char>(z.s.bytes());
     return s.bytes();
 }
 
@@ -790,6 +1397,9 @@ extern "C" int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs,
     ContourWs cw;
     PRT_REQUIRE(carve_contours(cw, contour_lanes(npairs), contour_cap(max_verts), c) == PRT_OK,
                 PRT_E_NOMEM, "overlay_contours_count: workspace too small");
+    TeamCtWs tw;
+    team_ct_layout(c, npairs, &tw);
+    PRT_REQUIRE(c.ok(), PRT_E_NOMEM, "overlay_contours_count: workspace too small");
     const unsigned grid = static_cast<unsigned>(ceil_div(cw.base.T, CT_THREADS));
     StageWs st;
     if (stage) {
@@ -799,12 +1409,24 @@ extern "C" int prt_overlay_contours_count(const uint32_t *pairs, int64_t npairs,
         PRT_LAUNCH(k_stage_clear, 1, 64, 0, as_stream(stream), st.ctr,
                    stage_tag(pairs, npairs, xyA, offA, xyB, offB, op));
         PRT_LAUNCHED();
-        PRT_LAUNCH(k_contours_count<CT_STAGE>, grid, CT_THREADS, 0, as_stream(stream), pairs,
-                   npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert);
-    } else {
-        PRT_LAUNCH(k_contours_count<CT_COUNT>, grid, CT_THREADS, 0, as_stream(stream), pairs,
-                   npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert);
     }
+    int one_lane;
+    {
+        std::lock_guard<std::mutex> lk(g_ct_mu);
+        one_lane = g_ct_one_lane;
+    }
+    if (!one_lane && npairs < (int64_t(1) << 32))
+        return contours_count_team(pairs, npairs, xyA, offA, xyB, offB, max_verts,
+                                   static_cast<int>(op), cw, tw, stage != nullptr, st, ncont,
+                                   nvert, as_stream(stream));
+    if (stage)
+        PRT_LAUNCH(k_contours_count<CT_STAGE>, grid, CT_THREADS, 0, as_stream(stream), pairs,
+                   npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert,
+                   nullptr, nullptr);
+    else
+        PRT_LAUNCH(k_contours_count<CT_COUNT>, grid, CT_THREADS, 0, as_stream(stream), pairs,
+                   npairs, xyA, offA, xyB, offB, static_cast<int>(op), cw, st, ncont, nvert,
+                   nullptr, nullptr);
     PRT_LAUNCHED();
     return PRT_OK;
 }
@@ -857,5 +1479,29 @@ extern "C" int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs,
                xyB, offB, static_cast<int>(op), cw, cont_off, vert_off, out,
                stage ? st.spill : nullptr, stage ? st.ctr : nullptr, tag, status);
     PRT_LAUNCHED();
+    return PRT_OK;
+}
+
+// Test hook: one_lane = 1 sends every pair of the count pass to the one-lane
+// kernel; tune (null: the defaults) sets the team shapes per bucket.
+extern "C" int prt_debug_contours(int32_t one_lane, const prt_tune *tune) {
+    std::lock_guard<std::mutex> lk(g_ct_mu);
+    if (tune) {
+        for (int b = 0; b < NBUCKET; ++b) {
+            const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
+            PRT_REQUIRE(L == 8 || L == 16 || L == 32 || L == 64, PRT_E_INVAL,
+                        "debug_contours: lanes_per_pair[%d] = %d", b, L);
+            PRT_REQUIRE(W >= 1 && W <= 8, PRT_E_INVAL, "debug_contours: waves_per_block[%d] = %d",
+                        b, W);
+            PRT_REQUIRE(tune->list_factor[b] >= 1 && tune->list_factor[b] <= 8, PRT_E_INVAL,
+                        "debug_contours: list_factor[%d] = %d", b, tune->list_factor[b]);
+            PRT_REQUIRE(tune->bucket_max_verts[b] >= 3 &&
+                            (b == 0 || tune->bucket_max_verts[b] > tune->bucket_max_verts[b - 1]),
+                        PRT_E_INVAL, "debug_contours: bucket bounds must increase");
+        }
+        g_ct_tune = *tune;
+    }
+    g_ct_tune_set = tune != nullptr;
+    g_ct_one_lane = one_lane ? 1 : 0;
     return PRT_OK;
 }

@@ -35,7 +35,7 @@ CSRC = Path(__file__).resolve().parent / "csrc"
 # the sources each recorded counter set depends on
 DIGEST_SETS = {
     "build": ("prt_common.hpp", "radix.hip", "bu_build.hip", "td_build.hip"),
-    "clip": ("prt_common.hpp", "clip_lane.hpp", "clip_plain.hip", "clip_batched.hip"),
+    "clip": ("prt_common.hpp", "clip_lane.hpp", "clip_team.hpp", "clip_plain.hip", "clip_batched.hip"),
 }
 
 
