@@ -671,22 +671,26 @@ struct TeamContourArgs {
     int op;
 };
 
-// interval slots and open contours a team holds (the one-lane kernel's
-// bounds, capped: pairs that need more take the one-lane kernel)
-__host__ __device__ inline int team_icap(int NV) { return NV / 2 + 1 < 65 ? NV / 2 + 1 : 65; }
-__host__ __device__ inline int team_pcap(int NV) { return NV + 2 < 130 ? NV + 2 : 130; }
+// interval slots and open contours a team holds (capped below the one-lane
+// kernel's bounds: pairs that need more take the one-lane kernel)
+__host__ __device__ inline int team_icap(int NV) { return NV / 2 + 1 < 33 ? NV / 2 + 1 : 33; }
+__host__ __device__ inline int team_pcap(int NV) { return NV + 2 < 66 ? NV + 2 : 66; }
 
-// LDS of one team: the overlay tables, then the sub-beam arrays (edge ids,
-// active edges, order, x at the sub-beam ends), two interval lists, the open
+// LDS of one team: the overlay tables, then the x at the sub-beam ends of a
+// beam one lane sweeps, the two interval lists (edge ids, chains, x and
+// uncertainty), the values at the top of the intervals above, the open
 // contours
 __host__ __device__ inline size_t contour_team_bytes(int NV, int SCAP) {
     const size_t ic = team_icap(NV), pc = team_pcap(NV);
-    return team_bytes(NV, SCAP) + 3 * align16(4 * size_t(NV)) + 2 * align16(8 * size_t(NV)) +
-           2 * (6 * align16(4 * ic) + 4 * align16(8 * ic)) + 4 * align16(4 * pc);
+    return team_bytes(NV, SCAP) + 2 * align16(8 * size_t(NV)) +
+           2 * (4 * align16(4 * ic) + 4 * align16(8 * ic)) + 4 * align16(8 * ic) +
+           4 * align16(4 * pc);
 }
 
 // The Builder's view of a team's LDS: a lane arena of one lane (stride 1)
-// whose edge table is the team's, edge id = edge slot.
+// whose edge table is the team's (edge id = edge slot); the sub-beam arrays
+// of the one-lane sweep are not used (the team hands the entries over
+// directly).
 __device__ inline ContourWs carve_contour_team(char *p, const TeamLds &t, int NV, int SCAP) {
     ContourWs cw;
     PlainWs &w = cw.base;
@@ -707,22 +711,18 @@ __device__ inline ContourWs carve_contour_team(char *p, const TeamLds &t, int NV
     w.ty = t.ty;
     w.dx = t.dx;
     w.ys = t.ys;
-    w.xb = w.xt = w.bsum = nullptr;
-    w.eord = nullptr;
-    w.id = take_i(NV);
-    w.act = take_i(NV);
-    w.ord = take_i(NV);
-    w.xa = take_d(NV);
-    w.xc = take_d(NV);
+    w.xb = w.xt = w.bsum = w.xa = w.xc = nullptr;
+    w.eord = w.id = w.act = w.ord = nullptr;
     w.T = 1;
     w.cap = NV;
+    cw.sxa = take_d(NV);
+    cw.sxc = take_d(NV);
     cw.icap = team_icap(NV);
     cw.pcap = team_pcap(NV);
     for (IvArrays &v : cw.iv) {
         v.eL = take_i(cw.icap);
         v.eR = take_i(cw.icap);
-        v.sL = take_i(cw.icap);
-        v.sR = take_i(cw.icap);
+        v.sL = v.sR = nullptr;
         v.lp = take_i(cw.icap);
         v.rp = take_i(cw.icap);
         v.xL = take_d(cw.icap);
@@ -730,6 +730,10 @@ __device__ inline ContourWs carve_contour_team(char *p, const TeamLds &t, int NV
         v.uL = take_d(cw.icap);
         v.uR = take_d(cw.icap);
     }
+    cw.tL = take_d(cw.icap);
+    cw.tR = take_d(cw.icap);
+    cw.tuL = take_d(cw.icap);
+    cw.tuR = take_d(cw.icap);
     cw.phead = take_i(cw.pcap);
     cw.ptail = take_i(cw.pcap);
     cw.pcnt = take_i(cw.pcap);
@@ -810,7 +814,6 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
     if (NY == 0) return 1;
     const int nb = NY - 1;
     const double INF = __builtin_huge_val();
-    const PlainWs &w = cw.base;
     for (int k0 = 0; k0 < nb;) {
         const int k1 = team_window<TL>(t, k0, nb, n, A.SCAP, tl);
         if (k1 < 0) return 0;
@@ -870,14 +873,10 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
             int ci = 0;
             for (;;) {
                 double yc = yt;
-                if (mode == 0) {
-                    for (int q = tl; q < m; q += TL) {
-                        w.act[q] = t.list[lo + q];
-                        w.ord[q] = q;
-                        w.xa[q] = t.lxb[lo + q];
-                        w.xc[q] = t.lxt[lo + q];
-                    }
-                } else if (mode == 1) {
+                // the sub-beam's entries in order: edge ids, x at ya and yc
+                const uint16_t *ids = t.list + lo;
+                const double *xa = t.lxb + lo, *xc = t.lxt + lo;
+                if (mode == 1) {
                     while (ci < nc && !(t.giy[ci] > ya)) ++ci;
                     yc = ci < nc ? t.giy[ci] : yt;
                     const int M2 = pow2_ge(m);
@@ -895,38 +894,35 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
                     }
                     team_sync();
                     team_sort_entries<TL>(t, M2, tl);
-                    for (int q = tl; q < m; q += TL) {
-                        w.act[q] = t.gidx[q];
-                        w.ord[q] = q;
-                        w.xa[q] = t.gxa[q];
-                        w.xc[q] = t.gxc[q];
-                    }
-                } else {
+                    ids = t.gidx;
+                    xa = t.gxa;
+                    xc = t.gxc;
+                } else if (mode == 2) {
                     double v = 0.0;
                     if (tl == 0) {
                         v = serial_next(t, lo, hi, yb, yt, ya);
                         for (int q = 0; q < m; ++q) {
                             const int i = lo + q;
-                            w.act[q] = t.list[i];
-                            w.ord[q] = q;
-                            w.xa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
-                            w.xc[q] = v == yb ? t.lxb[i] : (v == yt ? t.lxt[i] : ex(t, t.list[i], v));
+                            cw.sxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+                            cw.sxc[q] = v == yb ? t.lxb[i] : (v == yt ? t.lxt[i] : ex(t, t.list[i], v));
                         }
                     }
                     yc = __shfl(v, base, 64);
+                    xa = cw.sxa;
+                    xc = cw.sxc;
                 }
-                team_sync();
                 if (tl == 0) {
-                    cb.intervals(m, nP, A.op, ya);
+                    cb.intervals_direct(ids, xa, xc, m, nP, A.op, ya, yc);
                     cb.scanline(ya);
-                    cb.advance(yc);
+                    cb.advance_direct();
                 }
-                team_sync();
+                if (mode != 0) team_sync();  // the team's scratch is rewritten next
                 if (yc == yt) break;
                 ya = yc;
             }
-            if (__shfl(static_cast<int>(cb.bad), base, 64)) return 0;
         }
+        team_sync();  // lane 0's reads of the lists end before the next window's
+        if (__shfl(static_cast<int>(cb.bad), base, 64)) return 0;
         k0 = k1;
     }
     return 1;
@@ -942,8 +938,6 @@ __global__ __launch_bounds__(BK_THREADS_MAX) void k_contours_team(TeamContourArg
     char *mine = lds + team * contour_team_bytes(A.NV, A.SCAP);
     const TeamLds t = carve_team(mine, A.NV, A.SCAP);
     const ContourWs cw = carve_contour_team(mine, t, A.NV, A.SCAP);
-    for (int q = tl; q < A.NV; q += TL) cw.base.id[q] = q;  // edge id = edge slot
-    team_sync();
     const Lane L(cw.base, 0);
     ContourOut so{};
     if (M == CT_STAGE) {
@@ -1222,13 +1216,11 @@ template <class C> void team_ct_layout(C &c, int64_t np, TeamCtWs *w) {
 prt_tune contour_tune_default() {
     prt_tune t;
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
-    const int lanes[NBUCKET] = {16, 32, 64, 64, 64, 64};
-    const int waves[NBUCKET] = {2, 2, 1, 1, 1, 1};
     for (int b = 0; b < NBUCKET; ++b) {
         t.bucket_max_verts[b] = bounds[b];
-        t.lanes_per_pair[b] = lanes[b];
-        t.waves_per_block[b] = waves[b];
-        t.list_factor[b] = 2;
+        t.lanes_per_pair[b] = 64;  // a wave per pair: teams sharing a wave
+        t.waves_per_block[b] = 1;  // diverge in lane 0's assembly
+        t.list_factor[b] = 1;
     }
     return t;
 }

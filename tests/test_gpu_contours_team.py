@@ -137,7 +137,7 @@ def comb(k):
 @pytest.mark.parametrize("k", [30, 80, 140])
 def test_combs_beyond_the_team_arrays(hook, k):
     """A comb against a band across its teeth: k inside intervals in one
-    sub-beam.  Up to 65 the team keeps them; beyond, the pair takes the
+    sub-beam.  Up to 33 the team keeps them; beyond, the pair takes the
     one-lane kernel.  Both give the oracle's contours."""
     band = [(-1.0, 1.0), (2.0 * k, 1.0), (2.0 * k, 2.0), (-1.0, 2.0)]
     A, B = csr([comb(k), comb(k)]), csr([band, [(0.5, -0.5), (2.0 * k, -0.5), (2.0 * k, 2.5)]])
