@@ -1,6 +1,9 @@
 // contours.hip -- result contours of A[pairs[2t]] op B[pairs[2t+1]] (GPC's
 // output polygon, vector_map_overlay_gpc_kernel.cu:527-750 and its vertex
-// copy :720-728), one lane per pair, in two passes: count, then write.
+// copy :720-728), in two passes: count, then write.  The count pass runs a
+// team of lanes per pair (below, "team per pair"); the one-lane-per-pair
+// kernels described first take the pairs a team cannot hold and the write
+// pass of the pairs whose vertices were not kept.
 //
 // The sweep is the one of the plain area kernel (same edge table, beams,
 // crossing heights and sub-beam order).  The contours are assembled from it

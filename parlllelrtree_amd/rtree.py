@@ -450,11 +450,12 @@ class Contours:
 
 def overlay_contours(pairs: torch.Tensor, A: PolyDev, B: PolyDev, op="int",
                      pool_per_pair: int = 32, stage_limit: int = 1 << 31) -> Contours:
-    """Contours of A[pairs[t,0]] op B[pairs[t,1]]: the count pass, a scan of
-    the counts, then the write pass (one lane per pair).  The count pass
-    keeps the vertices in a pool of pool_per_pair * npairs slots (halved
+    """Contours of A[pairs[t,0]] op B[pairs[t,1]]: the count pass (a team of
+    lanes per pair), a scan of the counts, then the write pass.  The count
+    pass keeps the vertices in a pool of pool_per_pair * npairs slots (halved
     until the stage is at most stage_limit bytes; 0 keeps none), so the
-    write pass sweeps again only the pairs that did not fit."""
+    write pass sweeps again (one lane per pair) only the pairs that did not
+    fit."""
     pairs = pairs.contiguous()
     npairs, dev = pairs.shape[0], pairs.device
     mv = A.max_verts + B.max_verts

@@ -1,4 +1,4 @@
-"""GPU: result contours (count pass, scan, write pass; one lane per pair)
+"""GPU: result contours (count pass by teams, scan, write pass)
 equal to the oracle's, vertex for vertex, with the same hole flags and the
 same contour order, for the four operations on grid-overlay pairs, random
 star pairs of uneven sizes, and closed-form cases (rects, nested holes,
