@@ -744,6 +744,86 @@ __device__ inline ContourWs carve_contour_team(char *p, const TeamLds &t, int NV
     return cw;
 }
 
+// The intervals above a scanline built by a whole wave (a team of 64 lanes,
+// m <= 64 entries): lane q holds entry q and the gap between entries q and
+// q + 1.  Parities of the gaps are prefix counts of ballots; a gap with
+// equal x at both ends is skipped; a run of inside gaps starts at an inside
+// gap whose previous non-skipped gap is outside (or none) and ends at one
+// whose next non-skipped gap is outside (or none) -- the runs the Builder's
+// serial loop finds.  The k-th start and the k-th end make interval k,
+// written with its values at both ends (x, uncertainty at ya and yc).
+// Returns the number of intervals (on every lane); at most icap are
+// written.
+__device__ inline int wave_intervals(const ContourWs &cw, const Lane &L, const uint16_t *ids,
+                                     const double *xa, const double *xc, int m, int nP, int op,
+                                     double ya, double yc, int q) {
+    const bool have = q < m, gap = q + 1 < m;
+    const int e = have ? ids[q] : 0;
+    const uint64_t mP = __ballot(have && e < nP), mQ = __ballot(have && !(e < nP));
+    const uint64_t upto = q == 63 ? ~0ull : (2ull << q) - 1;  // lanes <= q
+    const uint64_t below = (1ull << q) - 1;                   // lanes < q
+    const int pp = __popcll(mP & upto) & 1, pq = __popcll(mQ & upto) & 1;
+    double xa0 = 0.0, xc0 = 0.0, xa1 = 0.0, xc1 = 0.0;
+    int e1 = 0;
+    if (have) {
+        xa0 = xa[q];
+        xc0 = xc[q];
+    }
+    if (gap) {
+        xa1 = xa[q + 1];
+        xc1 = xc[q + 1];
+        e1 = ids[q + 1];
+    }
+    const bool nz = gap && !(xa0 == xa1 && xc0 == xc1);
+    const bool in = nz && op_inside(op, pp, pq);
+    const uint64_t mNZ = __ballot(nz), mIN = __ballot(in);
+    const uint64_t nzb = mNZ & below, nza = mNZ & ~upto;
+    const bool prev_in = nzb != 0 && ((mIN >> (63 - __clzll(nzb))) & 1ull);
+    const bool next_in = nza != 0 && ((mIN >> __ffsll(static_cast<long long>(nza)) - 1) & 1ull);
+    const bool start = in && !prev_in, end = in && !next_in;
+    const uint64_t mS = __ballot(start), mE = __ballot(end);
+    const IvArrays &A = cw.iv[1];
+    if (start) {
+        const int k = __popcll(mS & below);
+        if (k < cw.icap) {
+            A.eL[k] = e;
+            A.xL[k] = xa0;
+            A.uL[k] = x_unc(L, e, ya);
+            cw.tL[k] = xc0;
+            cw.tuL[k] = x_unc(L, e, yc);
+            A.lp[k] = -1;
+            A.rp[k] = -1;
+        }
+    }
+    if (end) {
+        const int k = __popcll(mE & below);
+        if (k < cw.icap) {
+            A.eR[k] = e1;
+            A.xR[k] = xa1;
+            A.uR[k] = x_unc(L, e1, ya);
+            cw.tR[k] = xc1;
+            cw.tuR[k] = x_unc(L, e1, yc);
+        }
+    }
+    return __popcll(mS);
+}
+
+// The intervals above become the ones below (x and uncertainty at the
+// sub-beam top), by every lane of the team.
+template <int TL> __device__ inline void team_advance(const ContourWs &cw, int na, int tl) {
+    const IvArrays &B = cw.iv[0], &A = cw.iv[1];
+    for (int k = tl; k < na; k += TL) {
+        B.eL[k] = A.eL[k];
+        B.eR[k] = A.eR[k];
+        B.lp[k] = A.lp[k];
+        B.rp[k] = A.rp[k];
+        B.xL[k] = cw.tL[k];
+        B.xR[k] = cw.tR[k];
+        B.uL[k] = cw.tuL[k];
+        B.uR[k] = cw.tuR[k];
+    }
+}
+
 // Next sub-beam of the general sweep of one beam by lane 0 alone (more
 // entries or crossing heights than the team's scratch): the lowest crossing
 // height above ya (yt when none), and the region [lo, hi) insertion-sorted
@@ -914,12 +994,30 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
                     xa = cw.sxa;
                     xc = cw.sxc;
                 }
-                if (tl == 0) {
-                    cb.intervals_direct(ids, xa, xc, m, nP, A.op, ya, yc);
-                    cb.scanline(ya);
-                    cb.advance_direct();
+                if (TL == 64 && m <= 64) {  // intervals and their move by the wave
+                    const int na = wave_intervals(cw, cb.L, ids, xa, xc, m, nP, A.op, ya, yc, tl);
+                    team_sync();
+                    if (tl == 0) {
+                        cb.na = na < cw.icap ? na : cw.icap;
+                        if (na > cw.icap) cb.bad = true;
+                        cb.scanline(ya);
+                    }
+                    team_sync();
+                    const int nm = na < cw.icap ? na : cw.icap;
+                    team_advance<TL>(cw, nm, tl);
+                    if (tl == 0) {
+                        cb.nb = nm;
+                        cb.na = 0;
+                    }
+                    team_sync();
+                } else {
+                    if (tl == 0) {
+                        cb.intervals_direct(ids, xa, xc, m, nP, A.op, ya, yc);
+                        cb.scanline(ya);
+                        cb.advance_direct();
+                    }
+                    if (mode != 0) team_sync();  // the team's scratch is rewritten next
                 }
-                if (mode != 0) team_sync();  // the team's scratch is rewritten next
                 if (yc == yt) break;
                 ya = yc;
             }
