@@ -363,26 +363,24 @@ template <int M, bool FIXED = false> struct Builder {
     }
 
     __device__ __attribute__((always_inline)) void scanline(double y) {
+        // merge of the points below and above by x (each list in its own
+        // order; on equal x a right end first, then the point below), the
+        // head of each list kept so that every point is read once
         int i = 0, j = 0;
         bool have = false;
         const int nbp = 2 * nb, nap = 2 * na;
         Pt pend{false, false, 0, 0.0, 0.0};
+        Pt pb{true, false, 0, 0.0, 0.0}, pa{false, false, 0, 0.0, 0.0};
+        if (nbp > 0) pb = point(true, 0);
+        if (nap > 0) pa = point(false, 0);
         while (i < nbp || j < nap) {
-            Pt p;
-            if (j >= nap) {
-                p = point(true, i++);
-            } else if (i >= nbp) {
-                p = point(false, j++);
+            const bool take_b =
+                j >= nap || (i < nbp && (pb.x != pa.x ? pb.x < pa.x : (pb.right || !pa.right)));
+            const Pt p = take_b ? pb : pa;
+            if (take_b) {
+                if (++i < nbp) pb = point(true, i);
             } else {
-                const Pt pb = point(true, i), pa = point(false, j);
-                const bool take_b = pb.x != pa.x ? pb.x < pa.x : (pb.right || !pa.right);
-                if (take_b) {
-                    p = pb;
-                    ++i;
-                } else {
-                    p = pa;
-                    ++j;
-                }
+                if (++j < nap) pa = point(false, j);
             }
             if (!have) {
                 pend = p;
