@@ -362,8 +362,7 @@ int prt_overlay_contours_write(const uint32_t *pairs, int64_t npairs, const doub
  * with A's MBRs -> batched FP64 clip of every pair -> total area ----------
  * Pairs come out as (A id, B id) sorted by (A id, B id); count[0] = pairs
  * found (only the first cap are kept and clipped: count[0] > cap means the
- * capacity was too small), count[1] = join status (nonzero: a query had more
- * than 8192 hits and its pairs are unsorted).  area[t] is the FP64 area of
+ * capacity was too small), count[1] = 0 (reserved).  area[t] is the FP64 area of
  * pair t; total / nonempty (optional) get the sum and the count of area > 0.
  * prt_pipeline_run enqueues the sequence; prt_pipeline_capture records it
  * once into a hipGraph on `stream` (a created stream, not the null stream)

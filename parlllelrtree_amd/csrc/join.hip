@@ -133,8 +133,8 @@ __global__ __launch_bounds__(JN_THREADS) void k_join(TreeView tv, const float4 *
 }
 
 // Sort the r's of every query segment: short segments by one thread, up to
-// 1024 hits by one wave (bitonic in LDS), up to 8192 by one block; beyond
-// that a status bit is raised.
+// 1024 hits by one wave (bitonic in LDS), longer ones by one block (up to
+// 8192 in LDS, beyond in place in the pair buffer).
 constexpr int SS_WAVE_MAX = 1024;
 constexpr int SS_BLOCK_MAX = 8192;
 
@@ -161,16 +161,14 @@ constexpr int SS_THREAD_MAX = 16;  // segments sorted by one thread
 
 // Per query: segments of up to SS_THREAD_MAX hits are sorted here by one
 // thread (insertion sort, in place); longer ones are appended to the list
-// of the wave sort (<= 1024) or of the block sort (<= 8192); beyond that the
-// status bit is raised and the segment stays in traversal order.
+// of the wave sort (<= 1024) or of the block sort.
 // lists[0] / lists[nq]: the two lists; lcount[0..1]: their lengths.
 __global__ __launch_bounds__(JN_THREADS) void k_seg_classify(const int64_t *__restrict__ counts,
                                                              const int64_t *__restrict__ offsets,
                                                              int64_t nq, int64_t cap,
                                                              uint32_t *__restrict__ pairs,
                                                              uint32_t *__restrict__ lists,
-                                                             unsigned long long *lcount,
-                                                             int64_t *__restrict__ status) {
+                                                             unsigned long long *lcount) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * JN_THREADS + threadIdx.x;
     if (q >= nq) return;
     const int64_t o = offsets[q];
@@ -190,10 +188,8 @@ __global__ __launch_bounds__(JN_THREADS) void k_seg_classify(const int64_t *__re
         }
     } else if (len <= SS_WAVE_MAX) {
         lists[atomicAdd(&lcount[0], 1ull)] = static_cast<uint32_t>(q);
-    } else if (len <= SS_BLOCK_MAX) {
-        lists[nq + atomicAdd(&lcount[1], 1ull)] = static_cast<uint32_t>(q);
     } else {
-        atomicOr(reinterpret_cast<unsigned long long *>(status), 1ull);
+        lists[nq + atomicAdd(&lcount[1], 1ull)] = static_cast<uint32_t>(q);
     }
 }
 
@@ -224,6 +220,65 @@ __global__ __launch_bounds__(JN_THREADS) void k_seg_sort_wave(const int64_t *__r
     }
 }
 
+// A segment of more than SS_BLOCK_MAX hits, sorted in place by one block:
+// r[2 * i] for i < len, positions from len to the next power of two are
+// +inf and never stored.  Bitonic sort in its ascending form (the first
+// compare of each merge pairs i with its mirror i ^ (k - 1), then
+// half-cleaners i, i ^ j): every compare puts the smaller value at the
+// lower position, so a virtual +inf never moves.  Chunks of SS_BLOCK_MAX
+// are sorted in LDS, merge steps with j >= SS_BLOCK_MAX run over the pair
+// buffer, the rest of each merge chunk by chunk in LDS.
+__device__ void sort_big_segment(uint32_t *r, int64_t len, uint32_t *v) {
+    constexpr int C = SS_BLOCK_MAX;
+    int64_t n2 = C;
+    while (n2 < len) n2 <<= 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto load = [&](int64_t c0) {
+        for (int i = tid; i < C; i += nt) v[i] = c0 + i < len ? r[2 * (c0 + i)] : 0xFFFFFFFFu;
+        __syncthreads();
+    };
+    auto store = [&](int64_t c0) {
+        for (int i = tid; i < C && c0 + i < len; i += nt) r[2 * (c0 + i)] = v[i];
+        __syncthreads();
+    };
+    for (int64_t c0 = 0; c0 < len; c0 += C) {  // every chunk ascending
+        load(c0);
+        bitonic_lds_u32(v, C, tid, nt, true);
+        store(c0);
+    }
+    for (int64_t k = 2 * C; k <= n2; k <<= 1) {
+        for (int64_t j = k >> 1; j >= C; j >>= 1) {  // over the pair buffer
+            for (int64_t p = tid; p < n2 / 2; p += nt) {
+                const int64_t i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+                const int64_t m = j == (k >> 1) ? (i ^ (k - 1)) : (i | j);
+                if (i >= len || m >= len) continue;  // m >= len: +inf stays above
+                const uint32_t a = r[2 * i], b = r[2 * m];
+                if (a > b) {
+                    r[2 * i] = b;
+                    r[2 * m] = a;
+                }
+            }
+            __syncthreads();
+        }
+        for (int64_t c0 = 0; c0 < len; c0 += C) {  // half-cleaners j < C
+            load(c0);
+            for (int j = C >> 1; j > 0; j >>= 1) {
+                for (int p = tid; p < C / 2; p += nt) {
+                    const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+                    const int m = i | j;
+                    const uint32_t a = v[i], b = v[m];
+                    if (a > b) {
+                        v[i] = b;
+                        v[m] = a;
+                    }
+                }
+                __syncthreads();
+            }
+            store(c0);
+        }
+    }
+}
+
 // One block per listed query (grid-stride over the list).
 __global__ __launch_bounds__(1024) void k_seg_sort_block(const int64_t *__restrict__ counts,
                                                          const int64_t *__restrict__ offsets,
@@ -238,6 +293,10 @@ __global__ __launch_bounds__(1024) void k_seg_sort_block(const int64_t *__restri
         const int64_t o = offsets[q];
         int64_t len = counts[q];
         if (o + len > cap) len = cap - o;
+        if (len > SS_BLOCK_MAX) {
+            sort_big_segment(pairs + 2 * o + 1, len, v);
+            continue;
+        }
         int n2 = 1;
         while (n2 < len) n2 <<= 1;
         for (int i = threadIdx.x; i < n2; i += blockDim.x)
@@ -290,7 +349,7 @@ int run_join(const TreeView &tv, int levels, const float *queries, int64_t nq, u
     PRT_LAUNCHED();
     unsigned long long *lc = reinterpret_cast<unsigned long long *>(lcount);
     PRT_LAUNCH(k_seg_classify, dim3(static_cast<unsigned>(ceil_div(nq, JN_THREADS))), JN_THREADS, 0,
-               s, counts, offs, nq, cap, pairs, lists, lc, count_d + 1);
+               s, counts, offs, nq, cap, pairs, lists, lc);
     PRT_LAUNCHED();
     int64_t wgrid = ceil_div(nq, JN_WAVES);
     if (wgrid > 65536) wgrid = 65536;  // one listed query per wave

@@ -284,6 +284,21 @@ def test_join_big_segments(dev):
     np.testing.assert_array_equal(u32(pairs), O.join_brute(r, q))
 
 
+@pytest.mark.parametrize("n", [8193, 20000, 100003])
+def test_join_segments_beyond_the_block_sort(dev, n):
+    """Queries with more hits than one block sorts in LDS (8192): sorted in
+    place in the pair buffer (chunks in LDS, the merges over the buffer),
+    for both trees, beside queries with a few hits; the join status stays 0."""
+    r = rects(n, 31)
+    q = np.array([[-1, -1, 2, 2], [0.4, 0.4, 0.45, 0.45], [0, 0, 0.7, 1.5], [5, 5, 6, 6]],
+                 np.float32)
+    brute = O.join_brute(r, q)
+    assert np.bincount(brute[:, 0], minlength=4)[0] == n
+    for build in (rtree.build_bottom_up, rtree.build_top_down):
+        pairs = rtree.join(build(dev_rects(r), 16), dev_rects(q))
+        np.testing.assert_array_equal(u32(pairs), brute)
+
+
 def test_workload0_windows_full(dev):
     """1,048,576 rects, BU fanout 16, 65,536 windows of side 1e-2."""
     r = gen.rects_uniform(gen.seed(0), 1 << 20)
