@@ -440,6 +440,160 @@ template <int M, bool FIXED = false> struct Builder {
         A(&IvArrays::rp, na) = -1;
         ++na;
     }
+    // Team kernels with a team of one wave: a scanline whose merge pairs
+    // every interval below with the interval of the same index above --
+    // left end with left end, right end with right end, as the serial merge
+    // would (checked per interval by replaying its comparisons) -- is one
+    // chain continuation per pair, each on its own chain end (an open
+    // contour has one head and one tail), so the wave does them at once:
+    // lane q the two pairs of interval q, vertex slots by a prefix sum over
+    // the wave (pool chunks taken in one atomic), the contour's vertex count
+    // by an LDS atomic.  Called by all 64 lanes with n = nb = na; lane 0
+    // holds the Builder's state.  Returns false (nothing done) when the
+    // pairing is any other.
+    __device__ bool wave_continuations(double y, int n, int q) {
+        const bool mine = q < n;
+        Pt bl{true, false, q, 0.0, 0.0}, br{true, true, q, 0.0, 0.0};
+        Pt al{false, false, q, 0.0, 0.0}, ar{false, true, q, 0.0, 0.0};
+        bool ok = true;
+        auto take_b = [](double bx, bool bright, double ax, bool aright) {
+            return bx != ax ? bx < ax : (bright || !aright);
+        };
+        if (mine) {
+            bl.x = Bx(&IvArrays::xL, q);
+            bl.unc = Bx(&IvArrays::uL, q);
+            br.x = Bx(&IvArrays::xR, q);
+            br.unc = Bx(&IvArrays::uR, q);
+            al.x = Ax(&IvArrays::xL, q);
+            al.unc = Ax(&IvArrays::uL, q);
+            ar.x = Ax(&IvArrays::xR, q);
+            ar.unc = Ax(&IvArrays::uR, q);
+            // first two points taken: bl and al (either first)
+            ok = take_b(bl.x, false, al.x, false) ? !take_b(br.x, true, al.x, false)
+                                                   : take_b(bl.x, false, ar.x, true);
+            // next two: br and ar, then the next interval's left ends
+            if (q + 1 < n) {
+                if (take_b(br.x, true, ar.x, true))
+                    ok = ok && !take_b(Bx(&IvArrays::xL, q + 1), false, ar.x, true);
+                else
+                    ok = ok && take_b(br.x, true, Ax(&IvArrays::xL, q + 1), false);
+            }
+        }
+        if (__ballot(mine && !ok)) return false;
+        int pL = 0, pR = 0, nL = 0, nR = 0;
+        bool twoL = false, twoR = false;
+        double x1L = 0.0, x1R = 0.0;
+        if (mine) {
+            pL = B(&IvArrays::lp, q);
+            pR = B(&IvArrays::rp, q);
+            if (B(&IvArrays::eL, q) != A(&IvArrays::eL, q)) {
+                twoL = !near(bl, al);
+                x1L = (twoL || !(al.unc < bl.unc)) ? bl.x : al.x;
+                nL = twoL ? 2 : 1;
+            }
+            if (B(&IvArrays::eR, q) != A(&IvArrays::eR, q)) {
+                twoR = !near(br, ar);
+                x1R = (twoR || !(ar.unc < br.unc)) ? br.x : ar.x;
+                nR = twoR ? 2 : 1;
+            }
+        }
+        const bool dead = mine && !(pL >= 0 && pL < cw.pcap && pR >= 0 && pR < cw.pcap);
+        if (__ballot(dead)) {  // as live() in pair(): the pair is discarded
+            if (q == 0) bad = true;
+            return true;
+        }
+        int incl = nL + nR;  // vertices of this lane, inclusive prefix over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (q >= o) incl += v;
+        }
+        const int total = __shfl(incl, 63, 64);
+        int g = incl - nL - nR;
+        int64_t base = 0, rem = 0, fresh = 0;
+        bool sp = false;
+        if (M == CT_STAGE) {
+            int64_t ch = __shfl(chunk, 0, 64), che = __shfl(chunk_end, 0, 64);
+            sp = __shfl(static_cast<int>(spill), 0, 64) != 0;
+            rem = che - ch;
+            base = ch;
+            if (!sp && total > rem) {
+                const int64_t nch = (total - rem + CT_CHUNK - 1) / CT_CHUNK;
+                int64_t c = 0;
+                if (q == 0) c = static_cast<int64_t>(atomicAdd(st->ctr + 1, static_cast<unsigned long long>(nch))) * CT_CHUNK;
+                c = __shfl(c, 0, 64);
+                if (c + nch * CT_CHUNK > st->vcap) {
+                    sp = true;
+                } else {
+                    fresh = c;
+                    if (q == 0) {
+                        chunk = c + (total - rem);
+                        chunk_end = c + nch * CT_CHUNK;
+                    }
+                }
+            } else if (!sp && q == 0) {
+                chunk = ch + total;
+            }
+            if (q == 0) spill = sp;
+        }
+        auto slot = [&](int k) -> int {
+            if (M != CT_STAGE) return k;
+            if (sp) return -1;
+            return static_cast<int>(k < rem ? base + k : fresh + (k - rem));
+        };
+        auto put = [&](int v, double x) {
+            if (M == CT_STAGE && v >= 0) {
+                o.vx[v] = x;
+                o.vy[v] = y;
+                o.vnext[v] = -1;
+            }
+        };
+        if (W && mine) {
+            if (nL) {
+                const int v1 = slot(g++);
+                put(v1, x1L);
+                if (arena_ok(v1)) {
+                    o.vnext[vbase + v1] = ph(pL);
+                    ph(pL) = v1;
+                    atomicAdd(&pc(pL), 1);
+                }
+                if (twoL) {
+                    const int v2 = slot(g++);
+                    put(v2, al.x);
+                    if (arena_ok(v2)) {
+                        o.vnext[vbase + v2] = ph(pL);
+                        ph(pL) = v2;
+                        atomicAdd(&pc(pL), 1);
+                    }
+                }
+            }
+            if (nR) {
+                const int v1 = slot(g++);
+                put(v1, x1R);
+                if (arena_ok(v1) && arena_ok(pt(pR))) {
+                    o.vnext[vbase + pt(pR)] = v1;
+                    pt(pR) = v1;
+                    atomicAdd(&pc(pR), 1);
+                }
+                if (twoR) {
+                    const int v2 = slot(g++);
+                    put(v2, ar.x);
+                    if (arena_ok(v2) && arena_ok(pt(pR))) {
+                        o.vnext[vbase + pt(pR)] = v2;
+                        pt(pR) = v2;
+                        atomicAdd(&pc(pR), 1);
+                    }
+                }
+            }
+        }
+        if (mine) {
+            A(&IvArrays::lp, q) = pL;
+            A(&IvArrays::rp, q) = pR;
+        }
+        if (q == 0) nv += total;
+        return true;
+    }
+
     __device__ void advance_direct() {
         for (int k = 0; k < na; ++k) {
             B(&IvArrays::eL, k) = A(&IvArrays::eL, k);
@@ -995,13 +1149,17 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
                 if (TL == 64 && m <= 64) {  // intervals and their move by the wave
                     const int na = wave_intervals(cw, cb.L, ids, xa, xc, m, nP, A.op, ya, yc, tl);
                     team_sync();
-                    if (tl == 0) {
-                        cb.na = na < cw.icap ? na : cw.icap;
-                        if (na > cw.icap) cb.bad = true;
-                        cb.scanline(ya);
-                    }
-                    team_sync();
                     const int nm = na < cw.icap ? na : cw.icap;
+                    if (tl == 0) {
+                        cb.na = nm;
+                        if (na > cw.icap) cb.bad = true;
+                    }
+                    const int nbu = __shfl(cb.nb, base, 64);
+                    bool done = nbu == 0 && nm == 0;  // no point on this scanline
+                    if (!done && nbu == nm) done = cb.wave_continuations(ya, nm, tl);
+                    team_sync();
+                    if (!done && tl == 0) cb.scanline(ya);
+                    team_sync();
                     team_advance<TL>(cw, nm, tl);
                     if (tl == 0) {
                         cb.nb = nm;
