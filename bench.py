@@ -472,6 +472,8 @@ def overlay_phase(cp, A, B, dev, args, checks, cpu_jobs, check_sample=None,
         "plain_equals_batched": bool(np.array_equal(got_p, got_b)),
         "total_area": float(got_b.sum()),
         "tune": tune_to_json(tune, tstats)["buckets"], "_tune": tune})
+    if "whole" in tstats:  # the tuned shapes and the defaults timed on every pair
+        out["tune_check"] = tstats["whole"]
     log(f"overlay {npairs} pairs: plain {ms_p:.2f} ms, batched {ms_b:.2f} ms "
         f"({ms_p / ms_b:.2f}x)")
 

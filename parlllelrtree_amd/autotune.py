@@ -67,9 +67,16 @@ def tune_from_json(d: dict) -> TuneStruct:
 
 
 def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000,
-             cache: str | Path | None = None, reps: int = 3) -> tuple[TuneStruct, dict]:
+             cache: str | Path | None = None, reps: int = 3,
+             whole_check: bool = True) -> tuple[TuneStruct, dict]:
     """pairs: (np, 2) u32 host array; A, B: host PolyCSR; Ad, Bd: device
-    PolyDev of the same polygons.  Returns (tune, per-bucket stats)."""
+    PolyDev of the same polygons.  Returns (tune, per-bucket stats).
+
+    Buckets are timed alone on samples, but in a call they share the GPU
+    (the largest two run on helper streams beside the others), so with
+    whole_check the picked shapes are timed against the default shapes on
+    every pair, and the defaults are kept when they are faster (stats
+    ["whole"] records both times)."""
     tune = rtree.default_tune()
     n = np.diff(A.off)[pairs[:, 0]] + np.diff(B.off)[pairs[:, 1]]
     mv = Ad.max_verts + Bd.max_verts
@@ -116,6 +123,16 @@ def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000
             tune.waves_per_block[b] = best[2]
             tune.list_factor[b] = best[3]
             stats[b] = {"pairs": int(len(idx)), "ms": round(best[0], 4)}
+    if whole_check and len(pairs):
+        allp = torch.from_numpy(np.ascontiguousarray(pairs).view(np.int32)).cuda()
+        area = torch.empty(len(pairs), dtype=torch.float64, device="cuda")
+        default = rtree.default_tune()
+        t_tuned = _time(lambda: rtree.overlay_batched(allp, Ad, Bd, tune, area), 2)
+        t_def = _time(lambda: rtree.overlay_batched(allp, Ad, Bd, default, area), 2)
+        stats["whole"] = {"tuned_ms": round(t_tuned, 3), "default_ms": round(t_def, 3)}
+        if t_def < t_tuned:
+            tune = default
+        del allp, area
     if cache is not None:
         Path(cache).write_text(json.dumps(tune_to_json(tune, stats), indent=1) + "\n")
     return tune, stats

@@ -185,10 +185,15 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
     }
     const double INF = __builtin_huge_val();
     if (nc > 1) {
-        const int C2 = pow2_ge(nc);
-        for (int q = nc + tl; q < C2; q += TL) t.giy[q] = INF;
-        team_sync();
-        team_sort_heights<TL>(t, C2, tl);
+        if (TL == 64) {
+            wave_sort_heights(t, nc, tl);
+            team_sync();
+        } else {
+            const int C2 = pow2_ge(nc);
+            for (int q = nc + tl; q < C2; q += TL) t.giy[q] = INF;
+            team_sync();
+            team_sort_heights<TL>(t, C2, tl);
+        }
     }
     const int M2 = pow2_ge(m);
     double bs = 0.0;  // meaningful on lane 0 of the team
@@ -198,20 +203,25 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
         while (ci < nc && !(t.giy[ci] > ya)) ++ci;
         const double yc = ci < nc ? t.giy[ci] : yt;
         const double h = yc - ya;
-        for (int q = tl; q < M2; q += TL) {
-            if (q < m) {
-                const int i = lo + q;
-                t.gxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
-                t.gxc[q] = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
-                t.gidx[q] = t.list[i];
-            } else {
-                t.gxa[q] = INF;
-                t.gxc[q] = INF;
-                t.gidx[q] = 0xFFFFu;
+        if (TL == 64) {
+            wave_sort_sub_beam(t, lo, m, yb, yt, ya, yc, tl);
+            team_sync();
+        } else {
+            for (int q = tl; q < M2; q += TL) {
+                if (q < m) {
+                    const int i = lo + q;
+                    t.gxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+                    t.gxc[q] = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
+                    t.gidx[q] = t.list[i];
+                } else {
+                    t.gxa[q] = INF;
+                    t.gxc[q] = INF;
+                    t.gidx[q] = 0xFFFFu;
+                }
             }
+            team_sync();
+            team_sort_entries<TL>(t, M2, tl);
         }
-        team_sync();
-        team_sort_entries<TL>(t, M2, tl);
         if (tl == 0) {
             int pp = 0, pq = 0;
             for (int q = 0; q + 1 < m; ++q) {

@@ -194,6 +194,58 @@ __device__ inline bool beam_sort_entries(const TeamLds &t, int lo, int hi) {
     return mono;
 }
 
+// A double of lane `lane` (uniform) on every lane.
+__device__ inline double readlane_d(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(u), lane);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(u >> 32), lane);
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Teams of one wave: the entries [lo, lo + m) of a beam's list (m <= 64)
+// with their x at the sub-beam ends ya and yc (the expressions of the serial
+// sweep), sorted by the strict key (xa + xc, xa, edge id) into gxa / gxc /
+// gidx[0, m): lane q holds entry q and counts the entries before it (read
+// from the other lanes); the key is a strict order, so the ranks are a
+// permutation.  The order the bitonic sort and the serial insertion sort
+// give.  The caller syncs the team before reading the result.
+__device__ inline void wave_sort_sub_beam(const TeamLds &t, int lo, int m, double yb, double yt,
+                                          double ya, double yc, int q) {
+    double a = 0.0, c = 0.0;
+    int e = 0;
+    if (q < m) {
+        const int i = lo + q;
+        a = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+        c = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
+        e = t.list[i];
+    }
+    const double k = a + c;
+    int rank = 0;
+    for (int p = 0; p < m; ++p) {
+        const double kp = readlane_d(k, p), ap = readlane_d(a, p);
+        const int ep = __builtin_amdgcn_readlane(e, p);
+        rank += key_before(kp, ap, ep, k, a, e) ? 1 : 0;
+    }
+    if (q < m) {
+        t.gxa[rank] = a;
+        t.gxc[rank] = c;
+        t.gidx[rank] = static_cast<uint16_t>(e);
+    }
+}
+
+// Teams of one wave: giy[0, n) ascending (n <= 64), by ranks as above
+// (equal heights keep their order).  The caller syncs the team afterwards.
+__device__ inline void wave_sort_heights(const TeamLds &t, int n, int q) {
+    const double h = q < n ? t.giy[q] : 0.0;
+    int rank = 0;
+    for (int p = 0; p < n; ++p) {
+        const double hp = readlane_d(h, p);
+        rank += (hp < h || (hp == h && p < q)) ? 1 : 0;
+    }
+    team_sync();  // every lane has read its height
+    if (q < n) t.giy[rank] = h;
+}
+
 // team-wide bitonic sort of M2 (power of two) entries (gxa, gxc, gidx) by
 // the strict key (xa + xc, xa, id), ascending
 template <int TL> __device__ inline void team_sort_entries(const TeamLds &t, int M2, int tl) {

@@ -1098,10 +1098,15 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
                 if (nc > t.GC) {
                     mode = 2;  // more heights than the scratch holds
                 } else if (nc > 1) {
-                    const int C2 = pow2_ge(nc);
-                    for (int q = nc + tl; q < C2; q += TL) t.giy[q] = INF;
-                    team_sync();
-                    team_sort_heights<TL>(t, C2, tl);
+                    if (TL == 64) {
+                        wave_sort_heights(t, nc, tl);
+                        team_sync();
+                    } else {
+                        const int C2 = pow2_ge(nc);
+                        for (int q = nc + tl; q < C2; q += TL) t.giy[q] = INF;
+                        team_sync();
+                        team_sort_heights<TL>(t, C2, tl);
+                    }
                 }
             }
             double ya = yb;
@@ -1114,21 +1119,26 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
                 if (mode == 1) {
                     while (ci < nc && !(t.giy[ci] > ya)) ++ci;
                     yc = ci < nc ? t.giy[ci] : yt;
-                    const int M2 = pow2_ge(m);
-                    for (int q = tl; q < M2; q += TL) {
-                        if (q < m) {
-                            const int i = lo + q;
-                            t.gxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
-                            t.gxc[q] = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
-                            t.gidx[q] = t.list[i];
-                        } else {
-                            t.gxa[q] = INF;
-                            t.gxc[q] = INF;
-                            t.gidx[q] = 0xFFFFu;
+                    if (TL == 64) {
+                        wave_sort_sub_beam(t, lo, m, yb, yt, ya, yc, tl);
+                        team_sync();
+                    } else {
+                        const int M2 = pow2_ge(m);
+                        for (int q = tl; q < M2; q += TL) {
+                            if (q < m) {
+                                const int i = lo + q;
+                                t.gxa[q] = ya == yb ? t.lxb[i] : (ya == yt ? t.lxt[i] : ex(t, t.list[i], ya));
+                                t.gxc[q] = yc == yb ? t.lxb[i] : (yc == yt ? t.lxt[i] : ex(t, t.list[i], yc));
+                                t.gidx[q] = t.list[i];
+                            } else {
+                                t.gxa[q] = INF;
+                                t.gxc[q] = INF;
+                                t.gidx[q] = 0xFFFFu;
+                            }
                         }
+                        team_sync();
+                        team_sort_entries<TL>(t, M2, tl);
                     }
-                    team_sync();
-                    team_sort_entries<TL>(t, M2, tl);
                     ids = t.gidx;
                     xa = t.gxa;
                     xc = t.gxc;
