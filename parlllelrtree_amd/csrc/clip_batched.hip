@@ -439,9 +439,9 @@ extern "C" int prt_tune_default(prt_tune *t) {
     // the shapes the autotuner picks on the grid, the 2 x 4M-star and the
     // layer-pair workloads (MI355X)
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
-    const int lanes[NBUCKET] = {16, 32, 64, 64, 64, 64};
-    const int waves[NBUCKET] = {4, 2, 4, 1, 2, 1};
-    const int factor[NBUCKET] = {4, 2, 1, 2, 4, 2};
+    const int lanes[NBUCKET] = {16, 64, 64, 64, 64, 64};
+    const int waves[NBUCKET] = {4, 1, 4, 1, 2, 1};
+    const int factor[NBUCKET] = {4, 4, 1, 2, 4, 4};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];
         t->lanes_per_pair[b] = lanes[b];
