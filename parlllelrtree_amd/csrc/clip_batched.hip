@@ -107,28 +107,12 @@ __device__ double beam_sum_serial(const TeamLds &t, int lo, int hi, double yb, d
     return bs;
 }
 
-// Doubled area of one beam from its list entries [lo, hi).  The entries are
-// sorted in place by the sub-beam key (xb + xt, xb, id) (shell sort); when
-// both xb and xt are then non-decreasing no two edges cross inside the beam
-// and the gaps are summed left to right directly -- the same terms, in the
-// same order, as the general sweep.  Otherwise the general sweep runs.
-// Sorts the beam's entries by the sub-beam key and sums a crossing-free
-// beam in one linear pass.  A beam with crossings is summed here by the
-// serial sweep when it has more than GS entries; otherwise the function
-// returns false and the team sweeps it (team_general).
-__device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
-                             int op, double &out) {
-    const int m = hi - lo;
-    out = 0.0;
-    if (m < 2) return true;
-    const bool mono = beam_sort_entries(t, lo, hi);
-    if (!mono) {
-        if (m <= t.GS) return false;  // the team sweeps it (measured: sweeping the
-                                      // small ones in their lanes, 16 or 32
-                                      // entries at most, is 8-20 % slower)
-        out = beam_sum_serial(t, lo, hi, yb, yt, nP, op);
-        return true;
-    }
+// Doubled area of one beam whose list entries [lo, hi) are sorted by the
+// sub-beam key and do not cross (cnt[k] = 0 after team_sort_window): the
+// gaps summed left to right -- the same terms, in the same order, as the
+// general sweep.
+__device__ inline double beam_sum_sorted(const TeamLds &t, int lo, int hi, double yb, double yt,
+                                         int nP, int op) {
     const double h = yt - yb;
     double bs = 0.0;
     int pp = 0, pq = 0;
@@ -137,8 +121,7 @@ __device__ bool beam_sum_try(const TeamLds &t, int lo, int hi, double yb, double
         else pq ^= 1;
         if (op_inside(op, pp, pq)) bs += ((t.lxb[i + 1] - t.lxb[i]) + (t.lxt[i + 1] - t.lxt[i])) * h;
     }
-    out = bs;
-    return true;
+    return bs;
 }
 
 // The general sweep of one beam (list entries [lo, hi), m <= GS) by the
@@ -279,21 +262,27 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         if (k1 < 0) return 0;
         const int wbase = t.boff[k0];
         if (tl == 0) t.gmisc[0] = 0;
-        team_sync();
-        // lane-parallel beams; beams with crossings are queued (cnt is free
-        // once the lists are built) and swept by the whole team afterwards
+        team_sort_window<TL>(t, k0, k1, tl);  // cnt[k]: the beam's edges cross
+        // lane-parallel beams; beams with crossings of up to GS entries are
+        // queued (in beam_of, free once the window is sorted) and swept by
+        // the whole team afterwards (measured: sweeping the small ones in
+        // their lanes, 16 or 32 entries at most, is 8-20 % slower), longer
+        // ones by their lane
         for (int k = k0 + tl; k < k1; k += TL) {
-            double bsk;
-            if (beam_sum_try(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k], t.ys[k + 1], nP,
-                             A.op, bsk))
-                t.bsum[k] = bsk;
+            const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase, m = hi - lo;
+            if (m < 2)
+                t.bsum[k] = 0.0;
+            else if (t.cnt[k] == 0)
+                t.bsum[k] = beam_sum_sorted(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op);
+            else if (m <= t.GS)
+                t.beam_of[atomicAdd(&t.gmisc[0], 1)] = static_cast<uint16_t>(k);
             else
-                t.cnt[atomicAdd(&t.gmisc[0], 1)] = k;
+                t.bsum[k] = beam_sum_serial(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op);
         }
         team_sync();
         const int ndef = t.gmisc[0];
         for (int q = 0; q < ndef; ++q) {
-            const int k = t.cnt[q];
+            const int k = t.beam_of[q];
             const double bsk = team_general<TL>(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k],
                                                 t.ys[k + 1], nP, A.op, tl, base);
             if (tl == 0) t.bsum[k] = bsk;
@@ -440,7 +429,7 @@ extern "C" int prt_tune_default(prt_tune *t) {
     // layer-pair workloads (MI355X)
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
     const int lanes[NBUCKET] = {16, 64, 64, 64, 64, 64};
-    const int waves[NBUCKET] = {4, 1, 4, 1, 2, 1};
+    const int waves[NBUCKET] = {2, 1, 4, 1, 2, 1};
     const int factor[NBUCKET] = {4, 4, 1, 2, 4, 4};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];

@@ -26,13 +26,15 @@ __device__ inline void team_sync() {
 
 struct TeamLds {
     double *bx, *by, *tx, *ty, *dx;  // [NV] edge slot = vertex index (P: i, Q: nP + j)
-    int32_t *ekb, *ekt;              // [NV] beams [ekb, ekt) spanned by the edge
+    uint16_t *ekb, *ekt;             // [NV] beams [ekb, ekt) spanned by the edge
     double *ys;                      // [P2] scanbeam table (sorted, then unique)
     double *bsum;                    // [NV] beam sums (also the unique-compaction buffer)
     int32_t *cnt;                    // [NV + 1]
     int32_t *boff;                   // [NV + 1]
     uint16_t *list;                  // [SCAP] active edges of the beams of a window
     double *lxb, *lxt;               // [SCAP] their x at the beam bottom and top
+    uint16_t *beam_of;               // [SCAP] beam of each list entry (then a beam queue)
+    uint16_t *tlist;                 // [SCAP] the entries' edge ids in sorted order
     // team-cooperative sweep of beams with crossings (GS entries, GC heights)
     double *gxa, *gxc;               // [GS] x of the entries at the sub-beam ends
     uint16_t *gidx;                  // [GS] their edge ids
@@ -65,10 +67,11 @@ __host__ __device__ inline int gen_cap(int NV) {
 __host__ __device__ inline size_t team_bytes(int NV, int SCAP) {
     const int P2 = pow2_ge(NV);
     const int G = gen_cap(NV);
-    return align16(5 * 8 * size_t(NV)) + 2 * align16(4 * size_t(NV)) + align16(8 * size_t(P2)) +
+    return align16(5 * 8 * size_t(NV)) + 2 * align16(2 * size_t(NV)) + align16(8 * size_t(P2)) +
            align16(8 * size_t(NV)) + 2 * align16(4 * size_t(NV + 1)) + align16(2 * size_t(SCAP)) +
-           2 * align16(8 * size_t(SCAP)) + 2 * align16(8 * size_t(G)) + align16(2 * size_t(G)) +
-           align16(8 * size_t(G)) + align16(4 * 4);
+           2 * align16(8 * size_t(SCAP)) + 2 * align16(2 * size_t(SCAP)) +
+           2 * align16(8 * size_t(G)) + align16(2 * size_t(G)) + align16(8 * size_t(G)) +
+           align16(4 * 4);
 }
 
 __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
@@ -80,10 +83,10 @@ __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
     t.ty = t.tx + NV;
     t.dx = t.ty + NV;
     p += align16(5 * 8 * size_t(NV));
-    t.ekb = reinterpret_cast<int32_t *>(p);
-    p += align16(4 * size_t(NV));
-    t.ekt = reinterpret_cast<int32_t *>(p);
-    p += align16(4 * size_t(NV));
+    t.ekb = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(NV));
+    t.ekt = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(NV));
     t.ys = reinterpret_cast<double *>(p);
     p += align16(8 * size_t(P2));
     t.bsum = reinterpret_cast<double *>(p);
@@ -98,6 +101,10 @@ __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
     p += align16(8 * size_t(SCAP));
     t.lxt = reinterpret_cast<double *>(p);
     p += align16(8 * size_t(SCAP));
+    t.beam_of = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(SCAP));
+    t.tlist = reinterpret_cast<uint16_t *>(p);
+    p += align16(2 * size_t(SCAP));
     const int G = gen_cap(NV);
     t.GS = G;
     t.GC = G;
@@ -160,38 +167,44 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
     return ea < eb;
 }
 
-// Sorts the list entries [lo, hi) of one beam in place by the sub-beam key
-// (xb + xt, xb, edge id) (shell sort); returns true when xb and xt are then
-// both non-decreasing: no two of the beam's edges cross inside it, and the
-// sorted entries are the order of its only sub-beam.
-__device__ inline bool beam_sort_entries(const TeamLds &t, int lo, int hi) {
-    const int m = hi - lo;
-    const int gaps[4] = {23, 10, 4, 1};
-    for (int gi = 0; gi < 4; ++gi) {
-        const int g = gaps[gi];
-        if (g >= m) continue;
-        for (int i = lo + g; i < hi; ++i) {
-            const int e = t.list[i];
-            const double xb = t.lxb[i], xt = t.lxt[i], k = xb + xt;
-            int j = i;
-            while (j - g >= lo) {
-                const int e2 = t.list[j - g];
-                const double xb2 = t.lxb[j - g], xt2 = t.lxt[j - g];
-                if (!key_before(k, xb, e, xb2 + xt2, xb2, e2)) break;
-                t.list[j] = static_cast<uint16_t>(e2);
-                t.lxb[j] = xb2;
-                t.lxt[j] = xt2;
-                j -= g;
-            }
-            t.list[j] = static_cast<uint16_t>(e);
-            t.lxb[j] = xb;
-            t.lxt[j] = xt;
+// The list entries of every beam of the window [k0, k1) sorted by the
+// sub-beam key (xb + xt, xb, edge id), by the whole team at once: every
+// entry counts the entries of its beam that come before it (the key is a
+// strict order, so the counts are a permutation) and its edge id goes to
+// that place; the x's at the beam ends are then computed again for the new
+// order (the expressions of the list fill, so the same values).  Then
+// cnt[k] = 1 for a beam whose sorted xb or xt decreases somewhere (edges
+// crossing inside the beam), else 0.  Syncs the team before and after.
+template <int TL> __device__ void team_sort_window(const TeamLds &t, int k0, int k1, int tl) {
+    const int wbase = t.boff[k0], S = t.boff[k1] - wbase;
+    team_sync();
+    for (int i = tl; i < S; i += TL) {
+        const int k = t.beam_of[i];
+        const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
+        const double xb = t.lxb[i], xt = t.lxt[i], key = xb + xt;
+        const int e = t.list[i];
+        int r = 0;
+        for (int j = lo; j < hi; ++j) {
+            const double xbj = t.lxb[j];
+            r += key_before(xbj + t.lxt[j], xbj, t.list[j], key, xb, e) ? 1 : 0;
         }
+        t.tlist[lo + r] = static_cast<uint16_t>(e);
     }
-    bool mono = true;
-    for (int i = lo + 1; i < hi && mono; ++i)
-        mono = !(t.lxb[i] < t.lxb[i - 1]) && !(t.lxt[i] < t.lxt[i - 1]);
-    return mono;
+    for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
+    team_sync();
+    for (int i = tl; i < S; i += TL) {
+        const int k = t.beam_of[i], e = t.tlist[i];
+        t.list[i] = static_cast<uint16_t>(e);
+        t.lxb[i] = ex(t, e, t.ys[k]);
+        t.lxt[i] = ex(t, e, t.ys[k + 1]);
+    }
+    team_sync();
+    for (int i = tl; i < S; i += TL) {
+        const int k = t.beam_of[i];
+        if (i > t.boff[k] - wbase && (t.lxb[i] < t.lxb[i - 1] || t.lxt[i] < t.lxt[i - 1]))
+            t.cnt[k] = 1;
+    }
+    team_sync();
 }
 
 // A double of lane `lane` (uniform) on every lane.
@@ -390,8 +403,8 @@ __device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int n
             kt = lower_bound(t.ys, NY, ty);
             kt = kt < nb ? kt : nb;
         }
-        t.ekb[e] = kb;
-        t.ekt[e] = kt;
+        t.ekb[e] = static_cast<uint16_t>(kb);
+        t.ekt[e] = static_cast<uint16_t>(kt);
         for (int k = kb; k < kt; ++k) atomicAdd(&t.cnt[k], 1);
     }
     team_sync();
@@ -439,6 +452,7 @@ __device__ int team_window(const TeamLds &t, int k0, int nb, int n, int SCAP, in
         for (int k = kb; k < kt; ++k) {
             const int pos = t.boff[k] - wbase + atomicAdd(&t.cnt[k], 1);
             t.list[pos] = static_cast<uint16_t>(e);
+            t.beam_of[pos] = static_cast<uint16_t>(k);
             t.lxb[pos] = ex(t, e, t.ys[k]);
             t.lxt[pos] = ex(t, e, t.ys[k + 1]);
         }

@@ -1053,11 +1053,7 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
         const int k1 = team_window<TL>(t, k0, nb, n, A.SCAP, tl);
         if (k1 < 0) return 0;
         const int wbase = t.boff[k0];
-        team_sync();
-        // every beam's entries sorted at once; cnt[k] = 1 when they cross
-        for (int k = k0 + tl; k < k1; k += TL)
-            t.cnt[k] = beam_sort_entries(t, t.boff[k] - wbase, t.boff[k + 1] - wbase) ? 0 : 1;
-        team_sync();
+        team_sort_window<TL>(t, k0, k1, tl);  // cnt[k] = 1 when the beam's edges cross
         const int kend = k1 == nb ? nb + 1 : k1;  // k = nb: the final scanline
         for (int k = k0; k < kend; ++k) {
             int lo = 0, hi = 0;
