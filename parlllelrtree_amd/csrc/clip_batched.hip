@@ -221,6 +221,118 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
     return __shfl(bs, base, 64);
 }
 
+// A double pushed to lane dst (every lane active, dst a permutation of the
+// lanes): lane dst receives this lane's v.
+__device__ inline double permute_d(double v, int dst) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_ds_permute(dst << 2, static_cast<int>(static_cast<uint32_t>(u))));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_ds_permute(dst << 2, static_cast<int>(static_cast<uint32_t>(u >> 32))));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Teams of one wave: team_general with the beam's entries in registers
+// instead of LDS.  Lane q holds entry lo + q (m = hi - lo <= 64) with its
+// x at the beam ends and its edge's bottom, top and dx.  The crossing
+// heights are the same pairs (q < p) tested with the same expressions (p's
+// x's read across lanes); per sub-beam every lane evaluates its entry at
+// the sub-beam ends (the expressions of ex() and the serial sweep), counts
+// the entries before it in the strict key order, and moves to that lane by
+// a cross-lane permute; lane q then holds the q-th entry of the sub-beam,
+// its parities are prefix counts of ballots, and its gap term is the
+// serial sweep's ((xa1 - xa0) + (xc1 - xc0)) * h.  The terms of the inside
+// gaps are added to the beam sum one at a time in gap order (read across
+// lanes), so the sum is the serial sweep's bit for bit.  Returns the beam
+// sum on every lane.
+__device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                               int op, int q) {
+    const int m = hi - lo;
+    const double dy = yt - yb;
+    const bool live = q < m;
+    int e = 0;
+    double xb = 0.0, xt = 0.0, ebx = 0.0, eby = 0.0, edx = 0.0, ety = 0.0, etx = 0.0;
+    if (live) {
+        e = t.list[lo + q];
+        xb = t.lxb[lo + q];
+        xt = t.lxt[lo + q];
+        ebx = t.bx[e];
+        eby = t.by[e];
+        edx = t.dx[e];
+        ety = t.ty[e];
+        etx = t.tx[e];
+    }
+    if (q == 0) t.gmisc[1] = 0;
+    team_sync();
+    for (int p = 1; p < m; ++p) {
+        const double xbj = readlane_d(xb, p), xtj = readlane_d(xt, p);
+        if (q >= p) continue;
+        double xbL, xtL, xbR, xtR;
+        if (xb < xbj && xt > xtj) { xbL = xb; xtL = xt; xbR = xbj; xtR = xtj; }
+        else if (xbj < xb && xtj > xt) { xbL = xbj; xtL = xtj; xbR = xb; xtR = xt; }
+        else continue;
+        const double den = (xtL - xbL) - (xtR - xbR);
+        if (!(den > 0.0)) continue;
+        const double r = (xbR - xbL) / den;
+        const double iy = r * dy + yb;
+        if (iy > yb && iy < yt) {
+            const int pos = atomicAdd(&t.gmisc[1], 1);
+            if (pos < t.GC) t.giy[pos] = iy;
+        }
+    }
+    team_sync();
+    const int nc = t.gmisc[1];
+    if (nc > t.GC) {
+        // more crossing heights than the scratch holds: the serial sweep
+        double r = 0.0;
+        if (q == 0) r = beam_sum_serial(t, lo, hi, yb, yt, nP, op);
+        const double v = readlane_d(r, 0);
+        team_sync();
+        return v;
+    }
+    if (nc > 1) {
+        wave_sort_heights(t, nc, q);
+        team_sync();
+    }
+    double bs = 0.0, ya = yb;
+    int ci = 0;
+    for (;;) {
+        while (ci < nc && !(t.giy[ci] > ya)) ++ci;
+        const double yc = ci < nc ? t.giy[ci] : yt;
+        const double h = yc - ya;
+        double a = 0.0, c = 0.0;
+        if (live) {
+            a = ya == yb ? xb : (ya == yt ? xt : (ya == ety ? etx : ebx + edx * (ya - eby)));
+            c = yc == yb ? xb : (yc == yt ? xt : (yc == ety ? etx : ebx + edx * (yc - eby)));
+        }
+        const double k = a + c;
+        int rank = 0;
+        for (int p = 0; p < m; ++p) {
+            const double kp = readlane_d(k, p), ap = readlane_d(a, p);
+            const int ep = __builtin_amdgcn_readlane(e, p);
+            rank += key_before(kp, ap, ep, k, a, e) ? 1 : 0;
+        }
+        if (!live) rank = q;  // lanes past the entries keep their place
+        const double sa = permute_d(a, rank), sc = permute_d(c, rank);
+        const int se = __builtin_amdgcn_ds_permute(rank << 2, e);
+        const bool in_p = q < m && se < nP, in_q = q < m && !(se < nP);
+        const uint64_t upto = (2ull << q) - 1ull;  // lanes 0..q
+        const int pp = __popcll(__ballot(in_p) & upto) & 1;
+        const int pq = __popcll(__ballot(in_q) & upto) & 1;
+        const double an = __shfl_down(sa, 1, 64), cn = __shfl_down(sc, 1, 64);
+        const double term = ((an - sa) + (cn - sc)) * h;
+        uint64_t inside = __ballot(q + 1 < m && op_inside(op, pp, pq));
+        while (inside) {
+            const int g = __builtin_ctzll(inside);
+            inside &= inside - 1ull;
+            bs += readlane_d(term, g);
+        }
+        if (yc == yt) break;
+        ya = yc;
+    }
+    return bs;
+}
+
 struct BatchArgs {
     const uint32_t *pairs;
     const uint32_t *order;   // pair indices grouped by bucket
@@ -283,8 +395,10 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         const int ndef = t.gmisc[0];
         for (int q = 0; q < ndef; ++q) {
             const int k = t.beam_of[q];
-            const double bsk = team_general<TL>(t, t.boff[k] - wbase, t.boff[k + 1] - wbase, t.ys[k],
-                                                t.ys[k + 1], nP, A.op, tl, base);
+            const int blo = t.boff[k] - wbase, bhi = t.boff[k + 1] - wbase;
+            const double bsk =
+                TL == 64 ? wave_general(t, blo, bhi, t.ys[k], t.ys[k + 1], nP, A.op, tl)
+                         : team_general<TL>(t, blo, bhi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
             if (tl == 0) t.bsum[k] = bsk;
         }
         team_sync();
