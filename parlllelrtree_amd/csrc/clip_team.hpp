@@ -183,11 +183,21 @@ template <int TL> __device__ void team_sort_window(const TeamLds &t, int k0, int
         const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
         const double xb = t.lxb[i], xt = t.lxt[i], key = xb + xt;
         const int e = t.list[i];
-        int r = 0;
-        for (int j = lo; j < hi; ++j) {
-            const double xbj = t.lxb[j];
-            r += key_before(xbj + t.lxt[j], xbj, t.list[j], key, xb, e) ? 1 : 0;
+        // entries before this one: every operand loaded unconditionally and
+        // compared without branches, four entries per step, so that the
+        // loads of a step are in flight together
+        auto before = [&](double xbj, double xtj, int ej) -> int {
+            const double kj = xbj + xtj;
+            return static_cast<int>((kj < key) | ((kj == key) & ((xbj < xb) | ((xbj == xb) & (ej < e)))));
+        };
+        int r = 0, j = lo;
+        for (; j + 4 <= hi; j += 4) {
+            const double b0 = t.lxb[j], b1 = t.lxb[j + 1], b2 = t.lxb[j + 2], b3 = t.lxb[j + 3];
+            const double c0 = t.lxt[j], c1 = t.lxt[j + 1], c2 = t.lxt[j + 2], c3 = t.lxt[j + 3];
+            const int e0 = t.list[j], e1 = t.list[j + 1], e2 = t.list[j + 2], e3 = t.list[j + 3];
+            r += before(b0, c0, e0) + before(b1, c1, e1) + before(b2, c2, e2) + before(b3, c3, e3);
         }
+        for (; j < hi; ++j) r += before(t.lxb[j], t.lxt[j], t.list[j]);
         t.tlist[lo + r] = static_cast<uint16_t>(e);
     }
     for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
@@ -449,12 +459,21 @@ __device__ int team_window(const TeamLds &t, int k0, int nb, int n, int SCAP, in
     for (int e = tl; e < n; e += TL) {
         const int kb = t.ekb[e] > k0 ? t.ekb[e] : k0;
         const int kt = t.ekt[e] < k1 ? t.ekt[e] : k1;
+        if (kb >= kt) continue;
+        // the edge's entries in registers (the stores below would otherwise
+        // make every ex() read them again); the same values ex() reads
+        const double ebx = t.bx[e], eby = t.by[e], edx = t.dx[e], ety = t.ty[e], etx = t.tx[e];
+        double y0 = t.ys[kb];
+        double x0 = y0 == ety ? etx : ebx + edx * (y0 - eby);
         for (int k = kb; k < kt; ++k) {
+            const double y1 = t.ys[k + 1];
+            const double x1 = y1 == ety ? etx : ebx + edx * (y1 - eby);
             const int pos = t.boff[k] - wbase + atomicAdd(&t.cnt[k], 1);
             t.list[pos] = static_cast<uint16_t>(e);
             t.beam_of[pos] = static_cast<uint16_t>(k);
-            t.lxb[pos] = ex(t, e, t.ys[k]);
-            t.lxt[pos] = ex(t, e, t.ys[k + 1]);
+            t.lxb[pos] = x0;
+            t.lxt[pos] = x1;
+            x0 = x1;
         }
     }
     return k1;
