@@ -221,28 +221,70 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
     return __shfl(bs, base, 64);
 }
 
-// A double pushed to lane dst (every lane active, dst a permutation of the
-// lanes): lane dst receives this lane's v.
-__device__ inline double permute_d(double v, int dst) {
+// v of lane (this lane ^ J), every lane active: within a quad and across
+// half a row a DPP move, within 32 lanes a swizzle, across the wave's
+// halves a permute
+template <int J> __device__ inline uint32_t xor_lane(uint32_t v) {
+    const int x = static_cast<int>(v);
+    if constexpr (J == 1) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    else if constexpr (J == 2) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    else if constexpr (J == 4) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x101F));
+    else if constexpr (J == 8) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
+    else if constexpr (J == 16) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x401F));
+    else return static_cast<uint32_t>(__shfl_xor(x, 32, 64));
+}
+template <int J> __device__ inline double xor_lane_d(double v) {
     const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = static_cast<uint32_t>(
-        __builtin_amdgcn_ds_permute(dst << 2, static_cast<int>(static_cast<uint32_t>(u))));
-    const uint32_t hi = static_cast<uint32_t>(
-        __builtin_amdgcn_ds_permute(dst << 2, static_cast<int>(static_cast<uint32_t>(u >> 32))));
+    const uint32_t lo = xor_lane<J>(static_cast<uint32_t>(u)), hi = xor_lane<J>(static_cast<uint32_t>(u >> 32));
     return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// One compare-exchange of the bitonic network at distance J inside blocks
+// of K lanes (ascending where q & K == 0): the lane keeps the first of its
+// entry and its partner's in the strict key order (xa + xc, xa, edge id)
+// when it is the lower lane of an ascending pair or the upper lane of a
+// descending one, the other entry otherwise.
+template <int J> __device__ inline void bitonic_xchg(int q, int K, double &a, double &c, int &e) {
+    const double oa = xor_lane_d<J>(a), oc = xor_lane_d<J>(c);
+    const int oe = static_cast<int>(xor_lane<J>(static_cast<uint32_t>(e)));
+    const bool o_first = key_before(oa + oc, oa, oe, a + c, a, e);
+    const bool keep_first = ((q & J) == 0) == ((q & K) == 0);
+    if (o_first == keep_first) {
+        a = oa;
+        c = oc;
+        e = oe;
+    }
+}
+
+// Teams of one wave: the entries of lanes [0, M2) (M2 a power of two <= 64)
+// sorted ascending by the strict key (xa + xc, xa, edge id) across the
+// lanes, lane q ending with the q-th entry.  Every lane active.
+__device__ inline void wave_bitonic(int q, int M2, double &a, double &c, int &e) {
+    for (int K = 2; K <= M2; K <<= 1) {
+        for (int J = K >> 1; J > 0; J >>= 1) {
+            switch (J) {
+            case 1: bitonic_xchg<1>(q, K, a, c, e); break;
+            case 2: bitonic_xchg<2>(q, K, a, c, e); break;
+            case 4: bitonic_xchg<4>(q, K, a, c, e); break;
+            case 8: bitonic_xchg<8>(q, K, a, c, e); break;
+            case 16: bitonic_xchg<16>(q, K, a, c, e); break;
+            default: bitonic_xchg<32>(q, K, a, c, e); break;
+            }
+        }
+    }
 }
 
 // Teams of one wave: team_general with the beam's entries in registers
 // instead of LDS.  Lane q holds entry lo + q (m = hi - lo <= 64) with its
 // x at the beam ends and its edge's bottom, top and dx.  The crossing
-// heights are the same pairs (q < p) tested with the same expressions (p's
-// x's read across lanes); per sub-beam every lane evaluates its entry at
-// the sub-beam ends (the expressions of ex() and the serial sweep), counts
-// the entries before it in the strict key order, and moves to that lane by
-// a cross-lane permute; lane q then holds the q-th entry of the sub-beam,
-// its parities are prefix counts of ballots, and its gap term is the
-// serial sweep's ((xa1 - xa0) + (xc1 - xc0)) * h.  The terms of the inside
-// gaps are added to the beam sum one at a time in gap order (read across
+// heights come from the same pairs (i < j), tested with the same
+// expressions, one pair per lane per step; per sub-beam every lane
+// evaluates its entry at the sub-beam ends (the expressions of ex() and the
+// serial sweep) and a bitonic network across the lanes sorts the entries
+// by the strict key, so lane q holds the q-th entry of the sub-beam; its
+// parities are prefix counts of ballots, and its gap term is the serial
+// sweep's ((xa1 - xa0) + (xc1 - xc0)) * h.  The terms of the inside gaps
+// are added to the beam sum one at a time in gap order (read across
 // lanes), so the sum is the serial sweep's bit for bit.  Returns the beam
 // sum on every lane.
 __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
@@ -264,12 +306,21 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
     }
     if (q == 0) t.gmisc[1] = 0;
     team_sync();
-    for (int p = 1; p < m; ++p) {
-        const double xbj = readlane_d(xb, p), xtj = readlane_d(xt, p);
-        if (q >= p) continue;
+    // the pairs (i, j), i < j, spread over the lanes: pair index
+    // j (j - 1) / 2 + i
+    const int npairs = m * (m - 1) / 2;
+    for (int t0 = 0; t0 < npairs; t0 += 64) {
+        const int tt = t0 + q;
+        if (tt >= npairs) continue;
+        int j = static_cast<int>((1.0f + __builtin_sqrtf(1.0f + 8.0f * static_cast<float>(tt))) * 0.5f);
+        while (j * (j - 1) / 2 > tt) --j;
+        while ((j + 1) * j / 2 <= tt) ++j;
+        const int i = tt - j * (j - 1) / 2;
+        const double xbi = t.lxb[lo + i], xti = t.lxt[lo + i];
+        const double xbj = t.lxb[lo + j], xtj = t.lxt[lo + j];
         double xbL, xtL, xbR, xtR;
-        if (xb < xbj && xt > xtj) { xbL = xb; xtL = xt; xbR = xbj; xtR = xtj; }
-        else if (xbj < xb && xtj > xt) { xbL = xbj; xtL = xtj; xbR = xb; xtR = xt; }
+        if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+        else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
         else continue;
         const double den = (xtL - xbL) - (xtR - xbR);
         if (!(den > 0.0)) continue;
@@ -294,27 +345,22 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
         wave_sort_heights(t, nc, q);
         team_sync();
     }
+    const int M2 = pow2_ge(m);
     double bs = 0.0, ya = yb;
     int ci = 0;
     for (;;) {
         while (ci < nc && !(t.giy[ci] > ya)) ++ci;
         const double yc = ci < nc ? t.giy[ci] : yt;
         const double h = yc - ya;
-        double a = 0.0, c = 0.0;
+        // lanes past the entries hold +inf keys, which sort last
+        double sa = __builtin_huge_val(), sc = __builtin_huge_val();
+        int se = 0xFFFF;
         if (live) {
-            a = ya == yb ? xb : (ya == yt ? xt : (ya == ety ? etx : ebx + edx * (ya - eby)));
-            c = yc == yb ? xb : (yc == yt ? xt : (yc == ety ? etx : ebx + edx * (yc - eby)));
+            sa = ya == yb ? xb : (ya == yt ? xt : (ya == ety ? etx : ebx + edx * (ya - eby)));
+            sc = yc == yb ? xb : (yc == yt ? xt : (yc == ety ? etx : ebx + edx * (yc - eby)));
+            se = e;
         }
-        const double k = a + c;
-        int rank = 0;
-        for (int p = 0; p < m; ++p) {
-            const double kp = readlane_d(k, p), ap = readlane_d(a, p);
-            const int ep = __builtin_amdgcn_readlane(e, p);
-            rank += key_before(kp, ap, ep, k, a, e) ? 1 : 0;
-        }
-        if (!live) rank = q;  // lanes past the entries keep their place
-        const double sa = permute_d(a, rank), sc = permute_d(c, rank);
-        const int se = __builtin_amdgcn_ds_permute(rank << 2, e);
+        wave_bitonic(q, M2, sa, sc, se);
         const bool in_p = q < m && se < nP, in_q = q < m && !(se < nP);
         const uint64_t upto = (2ull << q) - 1ull;  // lanes 0..q
         const int pp = __popcll(__ballot(in_p) & upto) & 1;
