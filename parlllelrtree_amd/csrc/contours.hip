@@ -931,7 +931,7 @@ __device__ inline int wave_intervals(const ContourWs &cw, const Lane &L, const u
     const uint64_t mNZ = __ballot(nz), mIN = __ballot(in);
     const uint64_t nzb = mNZ & below, nza = mNZ & ~upto;
     const bool prev_in = nzb != 0 && ((mIN >> (63 - __clzll(nzb))) & 1ull);
-    const bool next_in = nza != 0 && ((mIN >> __ffsll(static_cast<long long>(nza)) - 1) & 1ull);
+    const bool next_in = nza != 0 && ((mIN >> (__ffsll(static_cast<long long>(nza)) - 1)) & 1ull);
     const bool start = in && !prev_in, end = in && !next_in;
     const uint64_t mS = __ballot(start), mE = __ballot(end);
     const IvArrays &A = cw.iv[1];
