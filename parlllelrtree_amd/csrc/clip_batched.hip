@@ -239,6 +239,28 @@ template <int J> __device__ inline double xor_lane_d(double v) {
     return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+// A double pushed to lane dst (every lane active, dst a permutation of the
+// lanes): lane dst receives this lane's v.
+__device__ inline double permute_d(double v, int dst) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_ds_permute(dst << 2, static_cast<int>(static_cast<uint32_t>(u))));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_ds_permute(dst << 2, static_cast<int>(static_cast<uint32_t>(u >> 32))));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// v of lane q + 1 (every lane active; lane 63 reads 0): a DPP shift of the
+// whole wave by one lane
+__device__ inline double next_lane_d(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), 0x130, 0xF, 0xF, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), 0x130, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 // One compare-exchange of the bitonic network at distance J inside blocks
 // of K lanes (ascending where q & K == 0): the lane keeps the first of its
 // entry and its partner's in the strict key order (xa + xc, xa, edge id)
@@ -304,35 +326,42 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
         ety = t.ty[e];
         etx = t.tx[e];
     }
-    if (q == 0) t.gmisc[1] = 0;
-    team_sync();
-    // the pairs (i, j), i < j, spread over the lanes: pair index
-    // j (j - 1) / 2 + i
+    // the pairs (i, j), i < j, spread over the lanes (pair index
+    // j (j - 1) / 2 + i); the heights in (yb, yt) go to giy in lane order
     const int npairs = m * (m - 1) / 2;
+    int nc = 0;
     for (int t0 = 0; t0 < npairs; t0 += 64) {
         const int tt = t0 + q;
-        if (tt >= npairs) continue;
-        int j = static_cast<int>((1.0f + __builtin_sqrtf(1.0f + 8.0f * static_cast<float>(tt))) * 0.5f);
-        while (j * (j - 1) / 2 > tt) --j;
-        while ((j + 1) * j / 2 <= tt) ++j;
-        const int i = tt - j * (j - 1) / 2;
-        const double xbi = t.lxb[lo + i], xti = t.lxt[lo + i];
-        const double xbj = t.lxb[lo + j], xtj = t.lxt[lo + j];
-        double xbL, xtL, xbR, xtR;
-        if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
-        else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
-        else continue;
-        const double den = (xtL - xbL) - (xtR - xbR);
-        if (!(den > 0.0)) continue;
-        const double r = (xbR - xbL) / den;
-        const double iy = r * dy + yb;
-        if (iy > yb && iy < yt) {
-            const int pos = atomicAdd(&t.gmisc[1], 1);
+        bool has = false;
+        double iy = 0.0;
+        if (tt < npairs) {
+            int j = static_cast<int>((1.0f + __builtin_sqrtf(1.0f + 8.0f * static_cast<float>(tt))) * 0.5f);
+            while (j * (j - 1) / 2 > tt) --j;
+            while ((j + 1) * j / 2 <= tt) ++j;
+            const int i = tt - j * (j - 1) / 2;
+            const double xbi = t.lxb[lo + i], xti = t.lxt[lo + i];
+            const double xbj = t.lxb[lo + j], xtj = t.lxt[lo + j];
+            double xbL = 0.0, xtL = 0.0, xbR = 0.0, xtR = 0.0;
+            bool cr = true;
+            if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+            else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+            else cr = false;
+            if (cr) {
+                const double den = (xtL - xbL) - (xtR - xbR);
+                if (den > 0.0) {
+                    const double r = (xbR - xbL) / den;
+                    iy = r * dy + yb;
+                    has = iy > yb && iy < yt;
+                }
+            }
+        }
+        const uint64_t hm = __ballot(has);
+        if (has) {
+            const int pos = nc + __popcll(hm & ((1ull << q) - 1ull));
             if (pos < t.GC) t.giy[pos] = iy;
         }
+        nc += __popcll(hm);
     }
-    team_sync();
-    const int nc = t.gmisc[1];
     if (nc > t.GC) {
         // more crossing heights than the scratch holds: the serial sweep
         double r = 0.0;
@@ -341,16 +370,29 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
         team_sync();
         return v;
     }
-    if (nc > 1) {
-        wave_sort_heights(t, nc, q);
+    // the heights ascending across the lanes: lane i holds the i-th (ranks
+    // read across the wave, equal heights in lane order)
+    double sh = 0.0;
+    if (nc > 0) {
         team_sync();
+        const double h = q < nc ? t.giy[q] : 0.0;
+        int rank = q;
+        if (nc > 1) {
+            int r = 0;
+            for (int p = 0; p < nc; ++p) {
+                const double hp = readlane_d(h, p);
+                r += (hp < h || (hp == h && p < q)) ? 1 : 0;
+            }
+            if (q < nc) rank = r;
+        }
+        sh = permute_d(h, rank);
     }
     const int M2 = pow2_ge(m);
     double bs = 0.0, ya = yb;
     int ci = 0;
     for (;;) {
-        while (ci < nc && !(t.giy[ci] > ya)) ++ci;
-        const double yc = ci < nc ? t.giy[ci] : yt;
+        while (ci < nc && !(readlane_d(sh, ci) > ya)) ++ci;
+        const double yc = ci < nc ? readlane_d(sh, ci) : yt;
         const double h = yc - ya;
         // lanes past the entries hold +inf keys, which sort last
         double sa = __builtin_huge_val(), sc = __builtin_huge_val();
@@ -365,7 +407,7 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
         const uint64_t upto = (2ull << q) - 1ull;  // lanes 0..q
         const int pp = __popcll(__ballot(in_p) & upto) & 1;
         const int pq = __popcll(__ballot(in_q) & upto) & 1;
-        const double an = __shfl_down(sa, 1, 64), cn = __shfl_down(sc, 1, 64);
+        const double an = next_lane_d(sa), cn = next_lane_d(sc);
         const double term = ((an - sa) + (cn - sc)) * h;
         uint64_t inside = __ballot(q + 1 < m && op_inside(op, pp, pq));
         while (inside) {
@@ -439,13 +481,35 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         }
         team_sync();
         const int ndef = t.gmisc[0];
-        for (int q = 0; q < ndef; ++q) {
-            const int k = t.beam_of[q];
-            const int blo = t.boff[k] - wbase, bhi = t.boff[k + 1] - wbase;
-            const double bsk =
-                TL == 64 ? wave_general(t, blo, bhi, t.ys[k], t.ys[k + 1], nP, A.op, tl)
-                         : team_general<TL>(t, blo, bhi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
-            if (tl == 0) t.bsum[k] = bsk;
+        if constexpr (TL == 64) {
+            // the queued beams' ranges and heights loaded by the lanes at once,
+            // then read across the wave beam after beam
+            for (int q0 = 0; q0 < ndef; q0 += 64) {
+                int kq = 0, loq = 0, hiq = 0;
+                double ybq = 0.0, ytq = 0.0;
+                if (q0 + tl < ndef) {
+                    kq = t.beam_of[q0 + tl];
+                    loq = t.boff[kq] - wbase;
+                    hiq = t.boff[kq + 1] - wbase;
+                    ybq = t.ys[kq];
+                    ytq = t.ys[kq + 1];
+                }
+                const int cnt = ndef - q0 < 64 ? ndef - q0 : 64;
+                for (int j = 0; j < cnt; ++j) {
+                    const int k = __builtin_amdgcn_readlane(kq, j);
+                    const double bsk = wave_general(t, __builtin_amdgcn_readlane(loq, j),
+                                                    __builtin_amdgcn_readlane(hiq, j), readlane_d(ybq, j),
+                                                    readlane_d(ytq, j), nP, A.op, tl);
+                    if (tl == 0) t.bsum[k] = bsk;
+                }
+            }
+        } else {
+            for (int q = 0; q < ndef; ++q) {
+                const int k = t.beam_of[q];
+                const double bsk = team_general<TL>(t, t.boff[k] - wbase, t.boff[k + 1] - wbase,
+                                                    t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
+                if (tl == 0) t.bsum[k] = bsk;
+            }
         }
         team_sync();
         k0 = k1;
