@@ -122,8 +122,10 @@ typedef struct prt_td_tree {
 #define PRT_TUNE_BUCKETS 6
 typedef struct prt_tune {
     int32_t bucket_max_verts[PRT_TUNE_BUCKETS]; /* upper bound of n1+n2 per bucket */
-    int32_t lanes_per_pair[PRT_TUNE_BUCKETS];   /* 8, 16, 32 or 64                  */
-    int32_t waves_per_block[PRT_TUNE_BUCKETS];  /* 1..8                             */
+    int32_t lanes_per_pair[PRT_TUNE_BUCKETS];   /* 8, 16, 32, 64 (a slice of one
+                                                   wave), or 128 / 256 (a block of 2
+                                                   or 4 whole waves per pair)       */
+    int32_t waves_per_block[PRT_TUNE_BUCKETS];  /* 1..8; 1 when lanes_per_pair > 64 */
     int32_t list_factor[PRT_TUNE_BUCKETS];      /* beam-list capacity per team, in
                                                    units of the bucket's vertex
                                                    bound: 1..8                      */

@@ -22,7 +22,7 @@ import torch
 from . import rtree
 from ._lib import PRT_TUNE_BUCKETS, TuneStruct, lib
 
-LANES = (8, 16, 32, 64)
+LANES = (8, 16, 32, 64, 128, 256)   # 128, 256: a block of whole waves per pair
 WAVES = (1, 2, 4, 8)
 FACTORS = (1, 2, 4)
 LDS_BYTES = 160 * 1024
@@ -96,7 +96,10 @@ def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000
         nv = min(hi, mv)
 
         def trial(lanes, waves, factor):
-            if waves * (64 // lanes) * _team_bytes(nv, factor) > LDS_BYTES:
+            if lanes > 64:
+                if waves != 1 or _team_bytes(nv, factor) > LDS_BYTES:
+                    return None
+            elif waves * (64 // lanes) * _team_bytes(nv, factor) > LDS_BYTES:
                 return None
             cand = rtree.default_tune()
             for k in range(PRT_TUNE_BUCKETS):

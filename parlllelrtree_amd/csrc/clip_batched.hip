@@ -5,9 +5,9 @@
 //
 // Same scanbeam sweep and the same FP64 expressions as the plain kernel
 // (clip_plain.hip) and the CPU oracle, so areas are bit-identical, but:
-//   * a TEAM of TL lanes (8..64, a slice of one wave64) owns a pair; the
-//     pair's edges, its scanbeam table and its per-beam active-edge lists
-//     live in LDS;
+//   * a TEAM of TL lanes (8..64, a slice of one wave64, or 128 / 256: two
+//     or four whole waves making up a block) owns a pair; the pair's edges,
+//     its scanbeam table and its per-beam active-edge lists live in LDS;
 //   * the scanbeam table is sorted with a bitonic network across the team;
 //   * every edge registers itself in the lists of the beams it spans (LDS
 //     atomics + a team scan), with its x at the beam bottom and top, so a
@@ -479,16 +479,21 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
             else
                 t.bsum[k] = beam_sum_serial(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op);
         }
-        team_sync();
+        tsync<TL>();
         const int ndef = t.gmisc[0];
-        if constexpr (TL == 64) {
-            // the queued beams' ranges and heights loaded by the lanes at once,
-            // then read across the wave beam after beam
-            for (int q0 = 0; q0 < ndef; q0 += 64) {
+        if constexpr (TL >= 64) {
+            // one wave per queued beam (teams of several waves take turns);
+            // a wave's queued beams' ranges and heights are loaded by its
+            // lanes at once, then read across the wave beam after beam
+            constexpr int W = TL / 64;
+            const int w = tl >> 6, lane = tl & 63;
+            TeamLds tw = t;
+            if constexpr (W > 1) tw.giy = t.wgiy + 64 * w;
+            for (int q0 = 64 * w; q0 < ndef; q0 += 64 * W) {
                 int kq = 0, loq = 0, hiq = 0;
                 double ybq = 0.0, ytq = 0.0;
-                if (q0 + tl < ndef) {
-                    kq = t.beam_of[q0 + tl];
+                if (q0 + lane < ndef) {
+                    kq = t.beam_of[q0 + lane];
                     loq = t.boff[kq] - wbase;
                     hiq = t.boff[kq + 1] - wbase;
                     ybq = t.ys[kq];
@@ -497,10 +502,10 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
                 const int cnt = ndef - q0 < 64 ? ndef - q0 : 64;
                 for (int j = 0; j < cnt; ++j) {
                     const int k = __builtin_amdgcn_readlane(kq, j);
-                    const double bsk = wave_general(t, __builtin_amdgcn_readlane(loq, j),
+                    const double bsk = wave_general(tw, __builtin_amdgcn_readlane(loq, j),
                                                     __builtin_amdgcn_readlane(hiq, j), readlane_d(ybq, j),
-                                                    readlane_d(ytq, j), nP, A.op, tl);
-                    if (tl == 0) t.bsum[k] = bsk;
+                                                    readlane_d(ytq, j), nP, A.op, lane);
+                    if (lane == 0) t.bsum[k] = bsk;
                 }
             }
         } else {
@@ -511,12 +516,12 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
                 if (tl == 0) t.bsum[k] = bsk;
             }
         }
-        team_sync();
+        tsync<TL>();
         k0 = k1;
     }
     for (int wdt = 1; wdt < nb; wdt *= 2) {
         for (int i = tl * 2 * wdt; i + wdt < nb; i += TL * 2 * wdt) t.bsum[i] += t.bsum[i + wdt];
-        team_sync();
+        tsync<TL>();
     }
     result = 0.5 * t.bsum[0];
     return 1;
@@ -548,6 +553,42 @@ __global__ __launch_bounds__(BK_THREADS_MAX) void k_clip_batched(BatchArgs A) {
     }
 }
 
+// Teams of TL / 64 whole waves (TL = 128 or 256), one team per block: the
+// pair's tables are shared by the waves, which split the lanes' loops and
+// take turns at the queued crossing beams, so a large pair is done by more
+// lanes without more LDS.
+template <int TL>
+__global__ __launch_bounds__(256) void k_clip_batched_team(BatchArgs A) {
+    extern __shared__ __align__(16) char lds[];
+    constexpr int W = TL / 64;
+    __shared__ double red[8 * W];
+    __shared__ int32_t ired[W + 1];
+    __shared__ double wgiy[64 * W];
+    __shared__ unsigned long long s_idx;
+    TeamLds t = carve_team(lds, A.NV, A.SCAP);
+    t.red = red;
+    t.ired = ired;
+    t.wgiy = wgiy;
+    const int tl = threadIdx.x;
+    const int64_t count = *A.count;
+    const uint32_t *order = A.order + *A.start;
+    for (;;) {
+        if (tl == 0) s_idx = atomicAdd(A.ticket, 1ull);
+        __syncthreads();
+        const unsigned long long idx = s_idx;
+        __syncthreads();
+        if (static_cast<int64_t>(idx) >= count) break;
+        const uint32_t p = order[idx];
+        double result;
+        const int ok = team_pair<TL>(t, A, p, tl, 0, result);
+        if (tl == 0) {
+            if (ok) A.area[p] = result;
+            else A.ovf_list[atomicAdd(A.ovf_count, 1ull)] = p;
+        }
+        __syncthreads();
+    }
+}
+
 // Appends every pair of a bucket to the overflow list (bucket too large for
 // the team's LDS).
 __global__ void k_route_bucket(BatchArgs A) {
@@ -561,12 +602,21 @@ __global__ void k_route_bucket(BatchArgs A) {
 template <int TL>
 int launch_bucket(const BatchArgs &A, int waves, size_t lds, hipStream_t s) {
     int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_clip_batched<TL>, waves * 64, lds) !=
-            hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    const unsigned grid = static_cast<unsigned>(device_cus() * per_cu);
-    PRT_LAUNCH(k_clip_batched<TL>, grid, waves * 64, lds, s, A);
+    if constexpr (TL > 64) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_clip_batched_team<TL>, TL, lds) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const unsigned grid = static_cast<unsigned>(device_cus() * per_cu);
+        PRT_LAUNCH(k_clip_batched_team<TL>, grid, TL, lds, s, A);
+    } else {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_clip_batched<TL>, waves * 64, lds) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const unsigned grid = static_cast<unsigned>(device_cus() * per_cu);
+        PRT_LAUNCH(k_clip_batched<TL>, grid, waves * 64, lds, s, A);
+    }
     PRT_LAUNCHED();
     return PRT_OK;
 }
@@ -652,9 +702,9 @@ extern "C" int prt_tune_default(prt_tune *t) {
     // the shapes the autotuner picks on the grid, the 2 x 4M-star and the
     // layer-pair workloads (MI355X)
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
-    const int lanes[NBUCKET] = {16, 64, 64, 64, 64, 64};
-    const int waves[NBUCKET] = {2, 1, 4, 1, 2, 1};
-    const int factor[NBUCKET] = {4, 4, 1, 2, 4, 4};
+    const int lanes[NBUCKET] = {16, 64, 128, 128, 128, 256};
+    const int waves[NBUCKET] = {2, 1, 1, 1, 1, 1};
+    const int factor[NBUCKET] = {4, 4, 1, 1, 1, 4};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];
         t->lanes_per_pair[b] = lanes[b];
@@ -668,9 +718,12 @@ namespace {
 int check_tune(const prt_tune *t) {
     for (int b = 0; b < NBUCKET; ++b) {
         const int L = t->lanes_per_pair[b], W = t->waves_per_block[b];
-        PRT_REQUIRE(L == 8 || L == 16 || L == 32 || L == 64, PRT_E_INVAL,
-                    "tune: lanes_per_pair[%d] = %d not in {8,16,32,64}", b, L);
+        PRT_REQUIRE(L == 8 || L == 16 || L == 32 || L == 64 || L == 128 || L == 256, PRT_E_INVAL,
+                    "tune: lanes_per_pair[%d] = %d not in {8,16,32,64,128,256}", b, L);
         PRT_REQUIRE(W >= 1 && W <= 8, PRT_E_INVAL, "tune: waves_per_block[%d] = %d", b, W);
+        PRT_REQUIRE(L <= 64 || W == 1, PRT_E_INVAL,
+                    "tune: lanes_per_pair[%d] = %d is a block of whole waves; waves_per_block must be 1",
+                    b, L);
         PRT_REQUIRE(t->list_factor[b] >= 1 && t->list_factor[b] <= 8, PRT_E_INVAL,
                     "tune: list_factor[%d] = %d not in [1, 8]", b, t->list_factor[b]);
         PRT_REQUIRE(t->bucket_max_verts[b] >= 3 && (b == 0 || t->bucket_max_verts[b] >
@@ -805,7 +858,9 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
             if (rc) return rc;
         }
         const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
-        const size_t lds = static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
+        // teams of whole waves (L > 64) are one per block
+        const size_t lds = L > 64 ? team_bytes(A.NV, A.SCAP)
+                                  : static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
         if (lds > static_cast<size_t>(max_lds)) {
             // too large for LDS: the whole bucket takes the plain path
             PRT_LAUNCH(k_route_bucket, 256, 256, 0, bs, A);
@@ -816,7 +871,9 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
         case 8: rc = launch_bucket<8>(A, W, lds, bs); break;
         case 16: rc = launch_bucket<16>(A, W, lds, bs); break;
         case 32: rc = launch_bucket<32>(A, W, lds, bs); break;
-        default: rc = launch_bucket<64>(A, W, lds, bs); break;
+        case 64: rc = launch_bucket<64>(A, W, lds, bs); break;
+        case 128: rc = launch_bucket<128>(A, W, lds, bs); break;
+        default: rc = launch_bucket<256>(A, W, lds, bs); break;
         }
         if (rc) return rc;
     }

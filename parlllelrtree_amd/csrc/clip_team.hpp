@@ -24,6 +24,14 @@ __device__ inline void team_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// A team of TL lanes is a slice of one wave (TL <= 64) or TL / 64 whole
+// waves that make up one block (TL = 128, 256): its barrier is the wave's
+// or the block's.
+template <int TL> __device__ inline void tsync() {
+    if constexpr (TL > 64) __syncthreads();
+    else team_sync();
+}
+
 struct TeamLds {
     double *bx, *by, *tx, *ty, *dx;  // [NV] edge slot = vertex index (P: i, Q: nP + j)
     uint16_t *ekb, *ekt;             // [NV] beams [ekb, ekt) spanned by the edge
@@ -41,6 +49,10 @@ struct TeamLds {
     double *giy;                     // [GC] crossing heights of the beam
     int32_t *gmisc;                  // [4] deferred-beam count, crossing count
     int GS, GC;
+    // teams of several waves: the waves' partial reductions (block scratch)
+    double *red;                     // [8 * waves]
+    int32_t *ired;                   // [waves + 1]
+    double *wgiy;                    // [64 * waves] crossing heights, 64 per wave
 };
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~size_t(15); }
@@ -117,6 +129,9 @@ __device__ inline TeamLds carve_team(char *p, int NV, int SCAP) {
     t.giy = reinterpret_cast<double *>(p);
     p += align16(8 * size_t(G));
     t.gmisc = reinterpret_cast<int32_t *>(p);
+    t.red = nullptr;
+    t.ired = nullptr;
+    t.wgiy = nullptr;
     return t;
 }
 
@@ -160,6 +175,59 @@ template <int TL> __device__ inline int team_bcast(int v, int src_lane_in_team, 
     return __shfl(v, team_base + src_lane_in_team, 64);
 }
 
+// The team's minima of v[0, 4) and maxima of v[4, 8), on every lane: by
+// butterflies inside the team's lanes of one wave, then, for teams of
+// several waves, over the waves' results in LDS.
+template <int TL> __device__ inline void team_minmax8(const TeamLds &t, double (&v)[8], int tl) {
+    constexpr int WL = TL < 64 ? TL : 64;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = team_min<WL>(v[i]);
+#pragma unroll
+    for (int i = 4; i < 8; ++i) v[i] = team_max<WL>(v[i]);
+    if constexpr (TL > 64) {
+        constexpr int W = TL / 64;
+        const int w = tl >> 6;
+        if ((tl & 63) == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) t.red[8 * w + i] = v[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            double r = t.red[i];
+            for (int u = 1; u < W; ++u) r = i < 4 ? fmin(r, t.red[8 * u + i]) : fmax(r, t.red[8 * u + i]);
+            v[i] = r;
+        }
+        __syncthreads();  // the scratch is free again
+    }
+}
+
+// Inclusive scan of v over the team's lanes (in lane order) and its total,
+// on every lane.
+template <int TL> __device__ inline int team_scan(const TeamLds &t, int v, int tl, int base,
+                                                 int &total) {
+    if constexpr (TL <= 64) {
+        const int incl = team_incl_scan<TL>(v, tl);
+        total = team_bcast<TL>(incl, TL - 1, base);
+        return incl;
+    } else {
+        constexpr int W = TL / 64;
+        const int w = tl >> 6, lane = tl & 63;
+        int incl = team_incl_scan<64>(v, lane);
+        if (lane == 63) t.ired[w] = incl;
+        __syncthreads();
+        int before = 0, all = 0;
+        for (int u = 0; u < W; ++u) {
+            const int c = t.ired[u];
+            if (u < w) before += c;
+            all += c;
+        }
+        __syncthreads();  // the scratch is free again
+        total = all;
+        return incl + before;
+    }
+}
+
 // Strict order (xa + xc, xa, id) inside a sub-beam.
 __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, double xa_b, int eb) {
     if (ka != kb) return ka < kb;
@@ -177,7 +245,7 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
 // crossing inside the beam), else 0.  Syncs the team before and after.
 template <int TL> __device__ void team_sort_window(const TeamLds &t, int k0, int k1, int tl) {
     const int wbase = t.boff[k0], S = t.boff[k1] - wbase;
-    team_sync();
+    tsync<TL>();
     for (int i = tl; i < S; i += TL) {
         const int k = t.beam_of[i];
         const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
@@ -201,20 +269,20 @@ template <int TL> __device__ void team_sort_window(const TeamLds &t, int k0, int
         t.tlist[lo + r] = static_cast<uint16_t>(e);
     }
     for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
-    team_sync();
+    tsync<TL>();
     for (int i = tl; i < S; i += TL) {
         const int k = t.beam_of[i], e = t.tlist[i];
         t.list[i] = static_cast<uint16_t>(e);
         t.lxb[i] = ex(t, e, t.ys[k]);
         t.lxt[i] = ex(t, e, t.ys[k + 1]);
     }
-    team_sync();
+    tsync<TL>();
     for (int i = tl; i < S; i += TL) {
         const int k = t.beam_of[i];
         if (i > t.boff[k] - wbase && (t.lxb[i] < t.lxb[i - 1] || t.lxt[i] < t.lxt[i - 1]))
             t.cnt[k] = 1;
     }
-    team_sync();
+    tsync<TL>();
 }
 
 // A double of lane `lane` (uniform) on every lane.
@@ -335,9 +403,9 @@ __device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int n
         const double x = Q[2 * i], y = Q[2 * i + 1];
         bq0 = fmin(bq0, x); bq2 = fmax(bq2, x); bq1 = fmin(bq1, y); bq3 = fmax(bq3, y);
     }
-    bp0 = team_min<TL>(bp0); bp1 = team_min<TL>(bp1); bp2 = team_max<TL>(bp2); bp3 = team_max<TL>(bp3);
-    bq0 = team_min<TL>(bq0); bq1 = team_min<TL>(bq1); bq2 = team_max<TL>(bq2); bq3 = team_max<TL>(bq3);
-    const double bpa[4] = {bp0, bp1, bp2, bp3}, bqa[4] = {bq0, bq1, bq2, bq3};
+    double mm[8] = {bp0, bp1, bq0, bq1, bp2, bp3, bq2, bq3};
+    team_minmax8<TL>(t, mm, tl);
+    const double bpa[4] = {mm[0], mm[1], mm[4], mm[5]}, bqa[4] = {mm[2], mm[3], mm[6], mm[7]};
     double ylo = 0.0, yhi = 0.0;
     if (!op_range(op, eP, eQ, bpa, bqa, ylo, yhi)) return 0;
 
@@ -366,7 +434,7 @@ __device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int n
         t.ty[s] = valid ? ty : -INF;  // -inf top: never active
         t.dx[s] = valid ? (tx - bx) / (ty - by) : 0.0;
     }
-    team_sync();
+    tsync<TL>();
     // bitonic sort of ys[0, P2)
     for (int k = 2; k <= P2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -380,7 +448,7 @@ __device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int n
                     t.ys[l] = x;
                 }
             }
-            team_sync();
+            tsync<TL>();
         }
     }
     // unique (finite values) -> compacted into bsum, then back into ys
@@ -391,19 +459,19 @@ __device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int n
         const double y = t.ys[s];
         if (y != INF && (s == 0 || y != t.ys[s - 1])) ++mine;
     }
-    const int incl = team_incl_scan<TL>(mine, tl);
-    const int NY = team_bcast<TL>(incl, TL - 1, base);
+    int NY = 0;
+    const int incl = team_scan<TL>(t, mine, tl, base, NY);
     int w = incl - mine;
     for (int s = c0; s < c1; ++s) {
         const double y = t.ys[s];
         if (y != INF && (s == 0 || y != t.ys[s - 1])) t.bsum[w++] = y;
     }
-    team_sync();
+    tsync<TL>();
     for (int s = tl; s < NY; s += TL) t.ys[s] = t.bsum[s];
     if (NY < 2) return 0;
     const int nb = NY - 1;
     for (int k = tl; k <= nb; k += TL) t.cnt[k] = 0;
-    team_sync();
+    tsync<TL>();
     // beam span of every edge; list entries per beam
     for (int e = tl; e < n; e += TL) {
         const double ty = t.ty[e];
@@ -417,21 +485,21 @@ __device__ int team_tables(const TeamLds &t, const double *__restrict__ P, int n
         t.ekt[e] = static_cast<uint16_t>(kt);
         for (int k = kb; k < kt; ++k) atomicAdd(&t.cnt[k], 1);
     }
-    team_sync();
+    tsync<TL>();
     // exclusive scan of cnt[0, nb) -> boff, boff[nb] = S
     const int pb = (nb + TL - 1) / TL;
     const int b0 = tl * pb, b1 = b0 + pb < nb ? b0 + pb : nb;
     int sum = 0;
     for (int k = b0; k < b1; ++k) sum += t.cnt[k];
-    const int sincl = team_incl_scan<TL>(sum, tl);
-    const int S = team_bcast<TL>(sincl, TL - 1, base);
+    int S = 0;
+    const int sincl = team_scan<TL>(t, sum, tl, base, S);
     int run = sincl - sum;
     for (int k = b0; k < b1; ++k) {
         t.boff[k] = run;
         run += t.cnt[k];
     }
     if (tl == 0) t.boff[nb] = S;
-    team_sync();
+    tsync<TL>();
     return NY;
 }
 
@@ -455,7 +523,7 @@ __device__ int team_window(const TeamLds &t, int k0, int nb, int n, int SCAP, in
     const int wbase = t.boff[k0];
     if (t.boff[k1] - wbase > SCAP) return -1;  // one beam alone exceeds the lists
     for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
-    team_sync();
+    tsync<TL>();
     for (int e = tl; e < n; e += TL) {
         const int kb = t.ekb[e] > k0 ? t.ekb[e] : k0;
         const int kt = t.ekt[e] < k1 ? t.ekt[e] : k1;

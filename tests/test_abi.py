@@ -125,3 +125,27 @@ def test_generator_thread_independence():
     big = gen.rects_uniform(gen.seed(4), 70000)
     small = gen.rects_uniform(gen.seed(4), 1000)
     np.testing.assert_array_equal(big[:1000], small)
+
+
+def test_tune_shapes_checked_before_any_launch():
+    # the default shapes are valid (teams of whole waves, 128 or 256 lanes,
+    # are one per block), and a call with an invalid shape is refused before
+    # anything is launched (the device pointers here are never touched)
+    lib = _lib.lib()
+    t = _lib.TuneStruct()
+    assert lib.prt_tune_default(ctypes.byref(t)) == 0
+    for b in range(_lib.PRT_TUNE_BUCKETS):
+        assert t.lanes_per_pair[b] in (8, 16, 32, 64, 128, 256)
+        assert t.lanes_per_pair[b] <= 64 or t.waves_per_block[b] == 1
+    fake = ctypes.c_void_p(256)
+
+    def call(tune):
+        return lib.prt_overlay_batched(fake, 1, fake, fake, fake, fake, 8, ctypes.byref(tune),
+                                       fake, None, 0, None)
+
+    for lanes, waves in [(128, 2), (256, 4), (48, 1), (512, 1)]:
+        bad = _lib.TuneStruct()
+        assert lib.prt_tune_default(ctypes.byref(bad)) == 0
+        bad.lanes_per_pair[3], bad.waves_per_block[3] = lanes, waves
+        assert call(bad) == -1, (lanes, waves)
+        assert "tune" in lib.prt_last_error().decode()

@@ -121,7 +121,7 @@ def run_batched(pairs, A, B, tune=None, max_overflow=0):
 
 
 @pytest.mark.parametrize("lanes,waves,factor", [(8, 4, 1), (16, 4, 2), (32, 2, 4), (64, 1, 1),
-                                                (64, 4, 2)])
+                                                (64, 4, 2), (128, 1, 1), (128, 1, 4), (256, 1, 2)])
 def test_batched_bit_exact_all_team_shapes(dev, lanes, waves, factor):
     pairs, S, T = _grid_pairs()
     tune = rtree.default_tune()
