@@ -553,6 +553,12 @@ __global__ __launch_bounds__(BK_THREADS_MAX) void k_clip_batched(BatchArgs A) {
     }
 }
 
+// static LDS of k_clip_batched_team for teams of w waves: its reduction
+// words, the waves' crossing heights and the ticket
+constexpr size_t team_static_lds(int w) {
+    return 8 * 8 * size_t(w) + 4 * size_t(w + 1) + 64 * 8 * size_t(w) + 8;
+}
+
 // Teams of TL / 64 whole waves (TL = 128 or 256), one team per block: the
 // pair's tables are shared by the waves, which split the lanes' loops and
 // take turns at the queued crossing beams, so a large pair is done by more
@@ -872,10 +878,12 @@ int batched_impl(const uint32_t *pairs, const int64_t *npairs_d, int64_t npairs,
             if (rc) return rc;
         }
         const int L = tune->lanes_per_pair[b], W = tune->waves_per_block[b];
-        // teams of whole waves (L > 64) are one per block
+        // teams of whole waves (L > 64) are one per block, with a few static
+        // LDS words per wave beside the team's tables
         const size_t lds = L > 64 ? team_bytes(A.NV, A.SCAP)
                                   : static_cast<size_t>(W) * (64 / L) * team_bytes(A.NV, A.SCAP);
-        if (lds > static_cast<size_t>(max_lds)) {
+        const size_t lds_static = L > 64 ? team_static_lds(L / 64) : 0;
+        if (lds + lds_static > static_cast<size_t>(max_lds)) {
             // too large for LDS: the whole bucket takes the plain path
             PRT_LAUNCH(k_route_bucket, 256, 256, 0, bs, A);
             PRT_LAUNCHED();
