@@ -554,9 +554,10 @@ __global__ __launch_bounds__(BK_THREADS_MAX) void k_clip_batched(BatchArgs A) {
 }
 
 // static LDS of k_clip_batched_team for teams of w waves: its reduction
-// words, the waves' crossing heights and the ticket
+// words, the waves' crossing heights and the ticket, with room for their
+// alignment (the compiler reports 1,184 and 2,336 bytes for 2 and 4 waves)
 constexpr size_t team_static_lds(int w) {
-    return 8 * 8 * size_t(w) + 4 * size_t(w + 1) + 64 * 8 * size_t(w) + 8;
+    return 8 * 8 * size_t(w) + 4 * size_t(w + 1) + 64 * 8 * size_t(w) + 8 + 64;
 }
 
 // Teams of TL / 64 whole waves (TL = 128 or 256), one team per block: the
