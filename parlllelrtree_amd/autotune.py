@@ -4,8 +4,9 @@ For every bucket of n1 + n2 (the bounds of prt_tune_default), the pairs of
 a workload that fall into it are timed under each feasible team shape
 (lanes per pair x waves per block) and the fastest is kept; then the
 beam-list capacity (list_factor x bucket bound) is swept for that shape.
-The result is a TuneStruct for prt_overlay_batched, and optionally a JSON
-cache:
+The buckets share the GPU in a call, so the search then continues on the
+whole call, one bucket at a time (autotune's docstring).  The result is a
+TuneStruct for prt_overlay_batched, and optionally a JSON cache:
 
     {"device": "...", "buckets": [{"max_verts": 32, "lanes": 8, "waves": 4,
                                    "list_factor": 1, "pairs": 1234,
@@ -13,6 +14,7 @@ cache:
 """
 from __future__ import annotations
 
+import ctypes
 import json
 from pathlib import Path
 
@@ -31,6 +33,20 @@ LDS_BYTES = 160 * 1024
 def _team_bytes(nv: int, factor: int) -> int:
     """LDS bytes of one team for pairs of up to nv vertices."""
     return int(lib().prt_overlay_team_lds(nv, factor))
+
+
+def _fits(nv: int, lanes: int, waves: int, factor: int) -> bool:
+    """Whether the teams of a block fit one CU's LDS (teams of whole waves,
+    lanes > 64, are one per block)."""
+    if lanes > 64:
+        return waves == 1 and _team_bytes(nv, factor) <= LDS_BYTES
+    return waves * (64 // lanes) * _team_bytes(nv, factor) <= LDS_BYTES
+
+
+def _copy(t: TuneStruct) -> TuneStruct:
+    c = TuneStruct()
+    ctypes.memmove(ctypes.addressof(c), ctypes.addressof(t), ctypes.sizeof(TuneStruct))
+    return c
 
 
 def _time(fn, reps: int = 3) -> float:
@@ -72,11 +88,15 @@ def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000
     """pairs: (np, 2) u32 host array; A, B: host PolyCSR; Ad, Bd: device
     PolyDev of the same polygons.  Returns (tune, per-bucket stats).
 
-    Buckets are timed alone on samples, but in a call they share the GPU
-    (the largest two run on helper streams beside the others), so with
-    whole_check the picked shapes are timed against the default shapes on
-    every pair, and the defaults are kept when they are faster (stats
-    ["whole"] records both times)."""
+    First every bucket is timed alone on a sample under each feasible
+    shape.  In a call, though, the buckets share the GPU (the largest two
+    run on helper streams beside the others), so with whole_check the
+    search then continues on the whole call: starting from the faster of the
+    sampled picks and the default shapes, each bucket in turn tries its
+    neighbouring shapes (half, same and twice the lanes; list factors 1, 2,
+    4), keeping a change that makes the whole call at least 1 % faster.
+    stats["whole"] records the default shapes' and the final shapes' times
+    on every pair and the number of whole calls timed."""
     tune = rtree.default_tune()
     n = np.diff(A.off)[pairs[:, 0]] + np.diff(B.off)[pairs[:, 1]]
     mv = Ad.max_verts + Bd.max_verts
@@ -96,10 +116,7 @@ def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000
         nv = min(hi, mv)
 
         def trial(lanes, waves, factor):
-            if lanes > 64:
-                if waves != 1 or _team_bytes(nv, factor) > LDS_BYTES:
-                    return None
-            elif waves * (64 // lanes) * _team_bytes(nv, factor) > LDS_BYTES:
+            if not _fits(nv, lanes, waves, factor):
                 return None
             cand = rtree.default_tune()
             for k in range(PRT_TUNE_BUCKETS):
@@ -129,12 +146,43 @@ def autotune(pairs: np.ndarray, A, B, Ad, Bd, max_pairs_per_bucket: int = 40_000
     if whole_check and len(pairs):
         allp = torch.from_numpy(np.ascontiguousarray(pairs).view(np.int32)).cuda()
         area = torch.empty(len(pairs), dtype=torch.float64, device="cuda")
+
+        def whole(t):
+            return _time(lambda: rtree.overlay_batched(allp, Ad, Bd, t, area), 2)
+
         default = rtree.default_tune()
-        t_tuned = _time(lambda: rtree.overlay_batched(allp, Ad, Bd, tune, area), 2)
-        t_def = _time(lambda: rtree.overlay_batched(allp, Ad, Bd, default, area), 2)
-        stats["whole"] = {"tuned_ms": round(t_tuned, 3), "default_ms": round(t_def, 3)}
-        if t_def < t_tuned:
-            tune = default
+        t_def = whole(default)
+        t_best = whole(tune)
+        calls = 2
+        if t_def < t_best:
+            tune, t_best = _copy(default), t_def
+        # buckets with pairs, the most pairs first
+        lo, order = 0, []
+        for b in range(PRT_TUNE_BUCKETS):
+            hi = tune.bucket_max_verts[b]
+            cnt = int(((n > lo) & (n <= hi)).sum())
+            lo = hi
+            if cnt:
+                order.append((cnt, b))
+        for _, b in sorted(order, reverse=True):
+            nv = min(tune.bucket_max_verts[b], mv)
+            l0 = tune.lanes_per_pair[b]
+            for lanes in sorted({max(8, l0 // 2), l0, min(256, l0 * 2)}):
+                waves = 1 if lanes >= 64 else 2
+                for factor in FACTORS:
+                    if (lanes, factor) == (l0, tune.list_factor[b]):
+                        continue
+                    if not _fits(nv, lanes, waves, factor):
+                        continue
+                    cand = _copy(tune)
+                    cand.lanes_per_pair[b], cand.waves_per_block[b] = lanes, waves
+                    cand.list_factor[b] = factor
+                    t = whole(cand)
+                    calls += 1
+                    if t < 0.99 * t_best:
+                        tune, t_best = cand, t
+        stats["whole"] = {"tuned_ms": round(t_best, 3), "default_ms": round(t_def, 3),
+                          "calls_timed": calls}
         del allp, area
     if cache is not None:
         Path(cache).write_text(json.dumps(tune_to_json(tune, stats), indent=1) + "\n")

@@ -136,6 +136,24 @@ def test_batched_bit_exact_all_team_shapes(dev, lanes, waves, factor):
     assert np.array_equal(got, ref), f"{(got != ref).sum()} areas differ"
 
 
+def test_autotune_picks_valid_shapes(dev):
+    # the autotuner's shapes (sampled per bucket, then refined on the whole
+    # call) are valid, give the oracle's areas, and time no slower than the
+    # defaults on the whole call (noise aside)
+    from parlllelrtree_amd.autotune import autotune, tune_to_json
+    pairs, S, T = _grid_pairs()
+    dS, dT = rtree.to_device_polys(S), rtree.to_device_polys(T)
+    tune, stats = autotune(pairs, S, T, dS, dT, max_pairs_per_bucket=2000, reps=1)
+    for e in tune_to_json(tune, stats)["buckets"]:
+        assert e["lanes"] in (8, 16, 32, 64, 128, 256)
+        assert e["lanes"] <= 64 or e["waves"] == 1
+    whole = stats["whole"]
+    assert whole["calls_timed"] >= 2
+    assert whole["tuned_ms"] <= 1.05 * whole["default_ms"]
+    got = run_batched(pairs, S, T, tune, max_overflow=None)
+    assert np.array_equal(got, O.overlay(pairs, S, T, nthreads=8))
+
+
 def test_batched_routes_large_buckets_to_plain(dev):
     pairs, S, T = _grid_pairs()
     tune = rtree.default_tune()
