@@ -13,9 +13,11 @@
 // Nothing is stored between forward and backward: the crossings of every
 // edge with the other polygon are recomputed here.  One wave per pair; both
 // polygons are staged in LDS in coordinates relative to the centre of A's
-// bounding box; lane e walks edge e's crossing parameters in increasing order
-// (next-larger selection) and decides each part by a midpoint point-in-polygon
-// test, exactly as the CPU oracle does, so the FP64 gradient matches it.
+// bounding box; lane e finds edge e's crossings with the other polygon in one
+// pass (up to 4 distinct ones kept sorted in registers; more fall back to a
+// pass per crossing), walks them in increasing order and decides each part
+// by a midpoint point-in-polygon test, exactly as the CPU oracle does, so
+// the FP64 gradient matches it.
 // Vertex gradients are accumulated with atomics (two contributions per
 // vertex and pair, which commute exactly when each polygon is in one pair).
 #include "prt_common.hpp"
@@ -40,60 +42,130 @@ template <class T> __device__ inline T wave_sum(T v) {
     return v;
 }
 
-// even-odd point in polygon, B in local coordinates (crossing number)
-template <class T>
-__device__ inline int point_in(const T *bx, const T *by, int nB, T px, T py) {
+// even-odd point in polygon, B in local coordinates (crossing number); B's
+// vertices are (x, y) pairs
+template <class T, class T2>
+__device__ inline int point_in(const T2 *b, int nB, T px, T py) {
     int in = 0;
+    T2 vi = b[0];
     for (int i = 0; i < nB; ++i) {
-        const int j = i + 1 == nB ? 0 : i + 1;
-        const T xi = bx[i], yi = by[i], xj = bx[j], yj = by[j];
-        if ((yi > py) != (yj > py)) {
-            const T xc = (xj - xi) * (py - yi) / (yj - yi) + xi;
+        const T2 vj = b[i + 1 == nB ? 0 : i + 1];
+        if ((vi.y > py) != (vj.y > py)) {
+            const T xc = (vj.x - vi.x) * (py - vi.y) / (vj.y - vi.y) + vi.x;
             if (px < xc) in ^= 1;
         }
+        vi = vj;
     }
     return in;
 }
 
-// Parts of edge (ax, ay) -> (bx_, by_) inside polygon B: J0 = int (1-t),
-// J1 = int t over the inside parameter set, green = int x dy over it.
+// Crossing parameter t of edge a + t r with edge i of B (c -> c + s), or -1
+// when the oracle's test rejects it: t in (0, 1), u in [-1e-9, 1 + 1e-9].
+// The quotients are formed only for the pairs that pass a division-free test
+// that is exact in the one direction needed: it rejects only pairs whose
+// rounded quotients the test would reject too (the sign and |t| < 1 tests on
+// the numerators; |u| bounds with a margin of 2e-9 (FP64) or 4e-7 (FP32,
+// > 1.5 ulp of 1) of |den|, skipped when |den| is so small that the margin
+// would underflow).
 template <class T>
-__device__ inline void edge_parts(T ax, T ay, T ex, T ey, const T *bx, const T *by, int nB,
-                                  T &j0, T &j1, T &green) {
-    const T rx = ex - ax, ry = ey - ay;
+__device__ inline T crossing_t(T ax, T ay, T rx, T ry, T cx, T cy, T sx, T sy) {
+    const T den = rx * sy - ry * sx;
+    if (den == T(0)) return T(-1);
+    const T qx = cx - ax, qy = cy - ay;
+    const T tn = qx * sy - qy * sx;
+    const T un = qx * ry - qy * rx;
+    const bool pos = den > T(0);
+    const T a = pos ? den : -den, tp = pos ? tn : -tn, up = pos ? un : -un;
+    if (!(tp > T(0) && tp < a)) return T(-1);
+    constexpr bool f64 = sizeof(T) == 8;
+    const T tiny = f64 ? T(1e-290) : T(1e-30);
+    if (a > tiny) {
+        const T m = (f64 ? T(2e-9) : T(4e-7)) * a;
+        if (up < -m || up > a + m) return T(-1);
+    }
+    const T t = tn / den, u = un / den;
     const T lo_u = T(-1e-9), hi_u = T(1) + T(1e-9);
-    T t0 = T(0);
+    return t > T(0) && t < T(1) && u >= lo_u && u <= hi_u ? t : T(-1);
+}
+
+// Parts of edge (ax, ay) -> (ex, ey) inside polygon B: J0 = int (1-t),
+// J1 = int t over the inside parameter set, green = int x dy over it.  The
+// parts lie between consecutive distinct crossing parameters; each one is
+// decided by a point test at its midpoint, in increasing t.
+template <class T, class T2>
+__device__ inline void edge_parts(T ax, T ay, T ex, T ey, const T2 *b, int nB, T &j0, T &j1,
+                                  T &green) {
+    const T rx = ex - ax, ry = ey - ay;
     j0 = T(0);
     j1 = T(0);
     green = T(0);
+    auto part = [&](T t0, T t1) {
+        const T tm = T(0.5) * (t0 + t1);
+        if (point_in<T, T2>(b, nB, ax + tm * rx, ay + tm * ry)) {
+            const T sq = T(0.5) * (t1 * t1 - t0 * t0);
+            j0 += (t1 - t0) - sq;
+            j1 += sq;
+            const T x0 = ax + t0 * rx, y0 = ay + t0 * ry;
+            const T x1 = ax + t1 * rx, y1 = ay + t1 * ry;
+            green += T(0.5) * (x0 + x1) * (y1 - y0);
+        }
+    };
+    // one pass over B: the distinct crossings, sorted, in registers (2 =
+    // empty slot, above every parameter)
+    constexpr int K = 4;
+    T cut[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) cut[k] = T(2);
+    bool more = false;  // a fifth distinct crossing
+    T2 vi = b[0];
+    for (int i = 0; i < nB; ++i) {
+        const T2 vj = b[i + 1 == nB ? 0 : i + 1];
+        const T t = crossing_t<T>(ax, ay, rx, ry, vi.x, vi.y, vj.x - vi.x, vj.y - vi.y);
+        vi = vj;
+        if (t < T(0)) continue;
+        bool dup = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) dup |= t == cut[k];
+        if (dup) continue;
+        more |= cut[K - 1] < T(2);
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) {
+            const T below = k ? cut[k - 1] : T(-1);
+            cut[k] = cut[k] < t ? cut[k] : (below < t ? t : below);
+        }
+    }
+    if (!more) {
+        T t0 = T(0);
+#pragma unroll
+        for (int k = 0; k <= K; ++k) {
+            const T t1 = k < K && cut[k] < T(1) ? cut[k] : T(1);
+            part(t0, t1);
+            if (t1 == T(1)) break;
+            t0 = t1;
+        }
+        return;
+    }
+    // more than K crossings on this edge: the next distinct crossing by a
+    // pass over B per part
+    T t0 = T(0);
     for (;;) {
         T t1 = T(1);
+        T2 wi = b[0];
         for (int i = 0; i < nB; ++i) {
-            const int j = i + 1 == nB ? 0 : i + 1;
-            const T cx = bx[i], cy = by[i];
-            const T sx = bx[j] - cx, sy = by[j] - cy;
-            const T den = rx * sy - ry * sx;
-            if (den == T(0)) continue;
-            const T qx = cx - ax, qy = cy - ay;
-            const T t = (qx * sy - qy * sx) / den;
-            const T u = (qx * ry - qy * rx) / den;
-            if (t > T(0) && t < T(1) && u >= lo_u && u <= hi_u && t > t0 && t < t1) t1 = t;
+            const T2 wj = b[i + 1 == nB ? 0 : i + 1];
+            const T t = crossing_t<T>(ax, ay, rx, ry, wi.x, wi.y, wj.x - wi.x, wj.y - wi.y);
+            wi = wj;
+            if (t > t0 && t < t1) t1 = t;
         }
-        if (t1 > t0) {
-            const T tm = T(0.5) * (t0 + t1);
-            if (point_in(bx, by, nB, ax + tm * rx, ay + tm * ry)) {
-                const T sq = T(0.5) * (t1 * t1 - t0 * t0);
-                j0 += (t1 - t0) - sq;
-                j1 += sq;
-                const T x0 = ax + t0 * rx, y0 = ay + t0 * ry;
-                const T x1 = ax + t1 * rx, y1 = ay + t1 * ry;
-                green += T(0.5) * (x0 + x1) * (y1 - y0);
-            }
-        }
+        part(t0, t1);
         if (t1 == T(1)) break;
         t0 = t1;
     }
 }
+
+template <class T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
 
 template <class T>
 __global__ __launch_bounds__(GR_THREADS) void k_area_grad(
@@ -104,8 +176,9 @@ __global__ __launch_bounds__(GR_THREADS) void k_area_grad(
     extern __shared__ __align__(16) char lds_raw[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = blockDim.x >> 6;  // waves (= pairs in flight) per block
-    T *base = reinterpret_cast<T *>(lds_raw) + static_cast<size_t>(wave) * 4 * max_verts;
-    T *px = base, *py = base + max_verts, *qx = base + 2 * max_verts, *qy = base + 3 * max_verts;
+    using T2 = typename Vec2<T>::type;
+    T2 *pv = reinterpret_cast<T2 *>(lds_raw) + static_cast<size_t>(wave) * 2 * max_verts;
+    T2 *qv = pv + max_verts;
     for (int64_t p = static_cast<int64_t>(blockIdx.x) * W + wave; p < np;
          p += static_cast<int64_t>(gridDim.x) * W) {
         const uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
@@ -126,8 +199,8 @@ __global__ __launch_bounds__(GR_THREADS) void k_area_grad(
         x0 = wave_min(x0); x1 = wave_max(x1); y0 = wave_min(y0); y1 = wave_max(y1);
         const T ox = T(0.5) * (x0 + x1), oy = T(0.5) * (y0 + y1);
         __builtin_amdgcn_wave_barrier();
-        for (int i = lane; i < nP; i += 64) { px[i] = P[2 * i] - ox; py[i] = P[2 * i + 1] - oy; }
-        for (int i = lane; i < nQ; i += 64) { qx[i] = Q[2 * i] - ox; qy[i] = Q[2 * i + 1] - oy; }
+        for (int i = lane; i < nP; i += 64) pv[i] = T2{P[2 * i] - ox, P[2 * i + 1] - oy};
+        for (int i = lane; i < nQ; i += 64) qv[i] = T2{Q[2 * i] - ox, Q[2 * i + 1] - oy};
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         // orientation of each polygon: sign of its shoelace area (relative to
@@ -154,12 +227,12 @@ __global__ __launch_bounds__(GR_THREADS) void k_area_grad(
             const int n = inP ? nP : nQ;
             const int i = inP ? e : e - nP;
             const int j = i + 1 == n ? 0 : i + 1;
-            const T *vx = inP ? px : qx, *vy = inP ? py : qy;
-            const T *ox_ = inP ? qx : px, *oy_ = inP ? qy : py;
+            const T2 *v = inP ? pv : qv, *o = inP ? qv : pv;
             const int nO = inP ? nQ : nP;
-            const T ax = vx[i], ay = vy[i], bx = vx[j], by = vy[j];
+            const T2 vi = v[i], vj = v[j];
+            const T ax = vi.x, ay = vi.y, bx = vj.x, by = vj.y;
             T j0, j1, gr;
-            edge_parts<T>(ax, ay, bx, by, ox_, oy_, nO, j0, j1, gr);
+            edge_parts<T, T2>(ax, ay, bx, by, o, nO, j0, j1, gr);
             const T s = inP ? sgnP : sgnQ;
             const T nx = s * (by - ay), ny = -s * (bx - ax);
             T *g = inP ? gA + 2 * oa : gB + 2 * ob;
@@ -190,7 +263,7 @@ int launch_grad(const uint32_t *pairs, int64_t np, const T *xyA, const int64_t *
     PRT_REQUIRE(max_verts >= 3 && max_verts <= PRT_MAX_POLY_VERTS, PRT_E_INVAL,
                 "area_grad: max_verts (largest polygon) %d not in [3, %d]", max_verts,
                 PRT_MAX_POLY_VERTS);
-    const size_t per_wave = 4 * static_cast<size_t>(max_verts) * sizeof(T);
+    const size_t per_wave = 4 * static_cast<size_t>(max_verts) * sizeof(T);  // P and Q as (x, y)
     int W = static_cast<int>((96 * 1024) / per_wave);
     W = W < 1 ? 1 : (W > GR_WAVES ? GR_WAVES : W);
     const size_t lds = W * per_wave;

@@ -138,3 +138,50 @@ def test_autograd_gradcheck_small(dev):
     pd = torch.from_numpy(pairs.view(np.int32)).cuda()
     f = lambda x: autograd.overlay_area(x, off, x, off, pd)  # noqa: E731
     assert torch.autograd.gradcheck(f, (xy,), eps=1e-6, atol=1e-6, rtol=1e-4)
+
+
+def _comb_pairs(n, seed=5):
+    """Each pair: a comb of 12 teeth and a quadrilateral whose lower edge
+    crosses the teeth (up to 24 crossings on one edge), at random heights and
+    slopes; the lower edge sometimes runs along the comb's tooth tops or
+    through its vertices."""
+    rng = np.random.default_rng(seed)
+    top = []
+    for k in range(12):
+        top += [(k, 1.0), (k + 0.5, 1.0), (k + 0.5, 0.2), (k + 1, 0.2)]
+    comb = [(0.0, 0.0), (12.0, 0.0)] + top[::-1]
+    A, B = [], []
+    for i in range(n):
+        y = [1.0, 0.2, 0.5][i % 3] if i % 4 == 0 else rng.uniform(0.1, 1.1)
+        x0, x1 = rng.uniform(-1, 3), rng.uniform(9, 13)
+        y1 = y if i % 4 == 0 else y + rng.uniform(-0.3, 0.3)
+        A.append([(x0, y), (x1, y1), (x1, 2.0), (x0, 2.0)])
+        B.append(comb)
+    return A, B
+
+
+def test_fp64_gradient_bit_exact_on_many_crossings_and_degenerate(dev):
+    """Edges that cross the other polygon more than four times (the kernel
+    keeps up to four crossings per edge in registers and takes a pass per
+    crossing beyond), and lattice polygons with shared vertices, collinear
+    and coincident edges: every gradient entry equals the oracle's."""
+    from test_gpu_degenerate import csr, lattice_polys
+    A, B = _comb_pairs(300)
+    A += lattice_polys(11, 400)
+    B += lattice_polys(12, 400)
+    L = lattice_polys(13, 100)
+    A += L
+    B += L                                    # coincident boundaries
+    CA, CB = csr(A), csr(B)
+    pairs = np.stack([np.arange(len(A)), np.arange(len(A))], 1).astype(np.uint32)
+    dA, dB = rtree.to_device_polys(CA), rtree.to_device_polys(CB)
+    pd = torch.from_numpy(pairs.view(np.int32)).cuda()
+    gA, gB, green = rtree.area_grad(pd, dA, dB, green=True)
+    gA, gB, green = gA.cpu().numpy(), gB.cpu().numpy(), green.cpu().numpy()
+    for k in range(len(A)):
+        P, Q = CA.poly(k), CB.poly(k)
+        rP, rQ = O.area_grad(P, Q)
+        assert np.array_equal(gA[CA.off[k]:CA.off[k + 1]], rP), k
+        assert np.array_equal(gB[CB.off[k]:CB.off[k + 1]], rQ), k
+        ga = O.green_area(P, Q)
+        assert abs(green[k] - ga) <= 1e-12 * max(abs(ga), 1.0), (k, green[k], ga)
