@@ -278,9 +278,11 @@ template <int J> __device__ inline void bitonic_xchg(int q, int K, double &a, do
     }
 }
 
-// Teams of one wave: the entries of lanes [0, M2) (M2 a power of two <= 64)
-// sorted ascending by the strict key (xa + xc, xa, edge id) across the
-// lanes, lane q ending with the q-th entry.  Every lane active.
+// Teams of one wave: the entries of every aligned group of M2 lanes (M2 a
+// power of two <= 64) sorted ascending by the strict key (xa + xc, xa, edge
+// id) across the group's lanes, the lane at position q of its group ending
+// with the group's q-th entry (the exchanges at distances below M2 stay
+// inside the group).  Every lane active.
 __device__ inline void wave_bitonic(int q, int M2, double &a, double &c, int &e) {
     for (int K = 2; K <= M2; K <<= 1) {
         for (int J = K >> 1; J > 0; J >>= 1) {
@@ -297,35 +299,23 @@ __device__ inline void wave_bitonic(int q, int M2, double &a, double &c, int &e)
 }
 
 // Teams of one wave: team_general with the beam's entries in registers
-// instead of LDS.  Lane q holds entry lo + q (m = hi - lo <= 64) with its
-// x at the beam ends and its edge's bottom, top and dx.  The crossing
-// heights come from the same pairs (i < j), tested with the same
-// expressions, one pair per lane per step; per sub-beam every lane
-// evaluates its entry at the sub-beam ends (the expressions of ex() and the
-// serial sweep) and a bitonic network across the lanes sorts the entries
-// by the strict key, so lane q holds the q-th entry of the sub-beam; its
-// parities are prefix counts of ballots, and its gap term is the serial
-// sweep's ((xa1 - xa0) + (xc1 - xc0)) * h.  The terms of the inside gaps
-// are added to the beam sum one at a time in gap order (read across
-// lanes), so the sum is the serial sweep's bit for bit.  Returns the beam
-// sum on every lane.
+// instead of LDS.  The crossing heights come from the same pairs (i < j),
+// tested with the same expressions, one pair per lane per step; they are
+// sorted across the lanes and their distinct values compacted.  The wave is
+// then split into aligned groups of M2 lanes (M2 = pow2 >= m entries) and
+// each group takes one sub-beam, 64 / M2 sub-beams per pass: lane qi of a
+// group holds entry lo + qi, evaluates it at its sub-beam's ends (the
+// expressions of ex() and the serial sweep), and a bitonic network inside
+// the group sorts the entries by the strict key, so lane qi holds the qi-th
+// entry of the sub-beam; its parities are prefix counts of the group's
+// ballots, and its gap term is the serial sweep's ((xa1 - xa0) + (xc1 -
+// xc0)) * h.  The terms of the inside gaps are added to the beam sum one at a
+// time in lane order -- sub-beam by sub-beam, gap by gap -- so the sum is the
+// serial sweep's bit for bit.  Returns the beam sum on every lane.
 __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
                                int op, int q) {
     const int m = hi - lo;
     const double dy = yt - yb;
-    const bool live = q < m;
-    int e = 0;
-    double xb = 0.0, xt = 0.0, ebx = 0.0, eby = 0.0, edx = 0.0, ety = 0.0, etx = 0.0;
-    if (live) {
-        e = t.list[lo + q];
-        xb = t.lxb[lo + q];
-        xt = t.lxt[lo + q];
-        ebx = t.bx[e];
-        eby = t.by[e];
-        edx = t.dx[e];
-        ety = t.ty[e];
-        etx = t.tx[e];
-    }
     // the pairs (i, j), i < j, spread over the lanes (pair index
     // j (j - 1) / 2 + i); the heights in (yb, yt) go to giy in lane order
     const int npairs = m * (m - 1) / 2;
@@ -370,9 +360,11 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
         team_sync();
         return v;
     }
-    // the heights ascending across the lanes: lane i holds the i-th (ranks
-    // read across the wave, equal heights in lane order)
-    double sh = 0.0;
+    // the distinct heights ascending across the lanes: lane i < nd holds the
+    // i-th (ranks read across the wave, equal heights in lane order; the last
+    // of each run of equal heights kept)
+    double ds = 0.0;
+    int nd = 0;
     if (nc > 0) {
         team_sync();
         const double h = q < nc ? t.giy[q] : 0.0;
@@ -385,38 +377,60 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
             }
             if (q < nc) rank = r;
         }
-        sh = permute_d(h, rank);
+        const double sh = permute_d(h, rank);
+        const bool keep = q < nc && (q + 1 == nc || next_lane_d(sh) != sh);
+        const uint64_t km = __ballot(keep);
+        nd = __popcll(km);
+        const int below = __popcll(km & ((1ull << q) - 1ull));
+        ds = permute_d(sh, keep ? below : nd + (q - below));
     }
     const int M2 = pow2_ge(m);
-    double bs = 0.0, ya = yb;
-    int ci = 0;
-    for (;;) {
-        while (ci < nc && !(readlane_d(sh, ci) > ya)) ++ci;
-        const double yc = ci < nc ? readlane_d(sh, ci) : yt;
+    const int G = 64 / M2, g = q / M2, qi = q & (M2 - 1);
+    // the entry of lane qi of every group
+    const bool live = qi < m;
+    int e = 0;
+    double xb = 0.0, xt = 0.0, ebx = 0.0, eby = 0.0, edx = 0.0, ety = 0.0, etx = 0.0;
+    if (live) {
+        e = t.list[lo + qi];
+        xb = t.lxb[lo + qi];
+        xt = t.lxt[lo + qi];
+        ebx = t.bx[e];
+        eby = t.by[e];
+        edx = t.dx[e];
+        ety = t.ty[e];
+        etx = t.tx[e];
+    }
+    const uint64_t gmask = M2 == 64 ? ~0ull : ((1ull << M2) - 1ull) << (g * M2);
+    const uint64_t upto = ((2ull << q) - 1ull) & gmask;  // lanes of the group up to q
+    double bs = 0.0;
+    for (int s0 = 0; s0 <= nd; s0 += G) {
+        const int s = s0 + g;  // this group's sub-beam
+        const bool act = s <= nd;
+        const double dlo = __shfl(ds, act && s > 0 ? s - 1 : 0, 64);
+        const double dhi = __shfl(ds, act && s < nd ? s : 0, 64);
+        const double ya = s == 0 ? yb : dlo, yc = s < nd ? dhi : yt;
         const double h = yc - ya;
-        // lanes past the entries hold +inf keys, which sort last
+        // lanes past the entries (and groups past the last sub-beam) hold
+        // +inf keys, which sort last
         double sa = __builtin_huge_val(), sc = __builtin_huge_val();
         int se = 0xFFFF;
-        if (live) {
+        if (act && live) {
             sa = ya == yb ? xb : (ya == yt ? xt : (ya == ety ? etx : ebx + edx * (ya - eby)));
             sc = yc == yb ? xb : (yc == yt ? xt : (yc == ety ? etx : ebx + edx * (yc - eby)));
             se = e;
         }
-        wave_bitonic(q, M2, sa, sc, se);
-        const bool in_p = q < m && se < nP, in_q = q < m && !(se < nP);
-        const uint64_t upto = (2ull << q) - 1ull;  // lanes 0..q
+        wave_bitonic(qi, M2, sa, sc, se);
+        const bool in_p = act && live && se < nP, in_q = act && live && !(se < nP);
         const int pp = __popcll(__ballot(in_p) & upto) & 1;
         const int pq = __popcll(__ballot(in_q) & upto) & 1;
         const double an = next_lane_d(sa), cn = next_lane_d(sc);
         const double term = ((an - sa) + (cn - sc)) * h;
-        uint64_t inside = __ballot(q + 1 < m && op_inside(op, pp, pq));
+        uint64_t inside = __ballot(act && qi + 1 < m && op_inside(op, pp, pq));
         while (inside) {
-            const int g = __builtin_ctzll(inside);
+            const int gl = __builtin_ctzll(inside);
             inside &= inside - 1ull;
-            bs += readlane_d(term, g);
+            bs += readlane_d(term, gl);
         }
-        if (yc == yt) break;
-        ya = yc;
     }
     return bs;
 }
