@@ -195,6 +195,40 @@ def test_batched_crossing_dense_beams(dev, k):
     assert np.array_equal(run_plain(pairs, A, B), ref)
 
 
+def padded_zigzag(k, lean, pad, y0=0.0, y1=1.0, x0=0.0):
+    """zigzag() with `pad` extra vertices on its bottom side (below the
+    band), so the pair lands in a larger bucket while the band's beam keeps
+    its 2k edges."""
+    v = zigzag(k, lean, y0, y1, x0)
+    bottom = [(x0 + k - (i + 1) * k / (pad + 1), y0 - 1.0 - 0.01 * ((i % 2) + 1)) for i in range(pad)]
+    return v[:-1] + bottom + v[-1:]
+
+
+@pytest.mark.parametrize("pad", [12, 40])
+@pytest.mark.parametrize("lanes", [64, 128, 256])
+def test_batched_sub_beams_in_several_passes(dev, pad, lanes):
+    # beams of 8..16 entries with more crossing heights than a wave has
+    # lane groups (64 / pow2(m) sub-beams per pass), in the 64- and
+    # 128-vertex buckets, under teams of one, two and four waves
+    polys_a, polys_b = [], []
+    for k in (2, 3, 4):
+        for lean in (0.7, 1.9):
+            polys_a.append(padded_zigzag(k, lean, pad, x0=0.05 * k))
+            polys_b.append(padded_zigzag(k, -lean, pad, y0=0.05, y1=1.1, x0=0.3))
+    A, B = csr(polys_a), csr(polys_b)
+    n = len(polys_a)
+    pairs = np.array([(i, j) for i in range(n) for j in range(n)], np.uint32)
+    tune = rtree.default_tune()
+    for b in range(6):
+        tune.lanes_per_pair[b] = lanes
+        tune.waves_per_block[b] = 1
+        tune.list_factor[b] = 1
+    got = run_batched(pairs, A, B, tune)
+    ref = O.overlay(pairs, A, B)
+    assert np.array_equal(got, ref), (got, ref)
+    assert (ref > 0).all()
+
+
 def test_batched_golden_and_edges(dev):
     cases = GOLD["clip"]
     A = csr([c["P"] if len(c["P"]) else [[0, 0]] for c in cases])
