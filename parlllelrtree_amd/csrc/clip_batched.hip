@@ -496,24 +496,26 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         tsync<TL>();
         const int ndef = t.gmisc[0];
         if constexpr (TL >= 64) {
-            // one wave per queued beam (teams of several waves take turns);
-            // a wave's queued beams' ranges and heights are loaded by its
-            // lanes at once, then read across the wave beam after beam
+            // one wave per queued beam: wave w of a team of W waves takes
+            // the queued beams w, w + W, w + 2W, ...; a wave's beams' ranges
+            // and heights are loaded by its lanes at once (64 at a time),
+            // then read across the wave beam after beam
             constexpr int W = TL / 64;
             const int w = tl >> 6, lane = tl & 63;
             TeamLds tw = t;
             if constexpr (W > 1) tw.giy = t.wgiy + 64 * w;
-            for (int q0 = 64 * w; q0 < ndef; q0 += 64 * W) {
+            const int nmine = ndef > w ? (ndef - w + W - 1) / W : 0;
+            for (int i0 = 0; i0 < nmine; i0 += 64) {
                 int kq = 0, loq = 0, hiq = 0;
                 double ybq = 0.0, ytq = 0.0;
-                if (q0 + lane < ndef) {
-                    kq = t.beam_of[q0 + lane];
+                if (i0 + lane < nmine) {
+                    kq = t.beam_of[w + W * (i0 + lane)];
                     loq = t.boff[kq] - wbase;
                     hiq = t.boff[kq + 1] - wbase;
                     ybq = t.ys[kq];
                     ytq = t.ys[kq + 1];
                 }
-                const int cnt = ndef - q0 < 64 ? ndef - q0 : 64;
+                const int cnt = nmine - i0 < 64 ? nmine - i0 : 64;
                 for (int j = 0; j < cnt; ++j) {
                     const int k = __builtin_amdgcn_readlane(kq, j);
                     const double bsk = wave_general(tw, __builtin_amdgcn_readlane(loq, j),
