@@ -659,7 +659,7 @@ def grid_batched_phase(dev, args, checks):
 
 def grad_phase(dev, args, checks, cpu_jobs):
     """262,144 independent star pairs (16..128 vertices): FP64 forward area
-    (batched), FP64 adjoint, FP32 adjoint (+ Green's area), errors against
+    (batched, with the default and the autotuned shapes), FP64 adjoint, FP32 adjoint (+ Green's area), errors against
     central finite differences (h = 1e-6) of the FP64 area, the FP64 CPU
     oracle gradient, and FP32 against FP64."""
     import numpy as np
@@ -667,6 +667,7 @@ def grad_phase(dev, args, checks, cpu_jobs):
 
     from parlllelrtree_amd import autograd, rtree, workloads
     from parlllelrtree_amd._lib import lib
+    from parlllelrtree_amd.autotune import autotune, tune_to_json
 
     pairs, S = workloads.w3()
     npairs = len(pairs)
@@ -677,7 +678,14 @@ def grad_phase(dev, args, checks, cpu_jobs):
     mv = 2 * S64.max_verts
     ws = torch.empty(lib().prt_overlay_batched_workspace_size(npairs, mv), dtype=torch.uint8,
                      device=dev)
-    ms_f = timed(lambda: rtree.overlay_batched(pd, S64, S64, None, area, ws), 3, 1)
+    # the forward with the default shapes, then with the autotuner's for
+    # these pairs (as the other overlay phases); the same areas either way
+    ms_fd = timed(lambda: rtree.overlay_batched(pd, S64, S64, None, area, ws), 3, 1)
+    area_d = area.clone()
+    tune, tstats = autotune(pairs, S, S, S64, S64)
+    ms_f = timed(lambda: rtree.overlay_batched(pd, S64, S64, tune, area, ws), 3, 1)
+    same_tuned = bool(torch.equal(area.view(torch.int64), area_d.view(torch.int64)))
+    del area_d
     gA, gB = torch.zeros_like(S64.xy), torch.zeros_like(S64.xy)
 
     def bwd64():
@@ -730,7 +738,11 @@ def grad_phase(dev, args, checks, cpu_jobs):
             G = g64[S.off[idx]:S.off[idx + 1]]
             sc = max(np.abs(G).max(), 1e-300)
             fd_err.append(np.abs(F - G).max() / sc)
-    out = {"pairs": npairs, "fp64_forward_ms": round(ms_f, 3), "fp64_adjoint_ms": round(ms_b, 3),
+    out = {"pairs": npairs, "fp64_forward_ms": round(ms_f, 3),
+           "fp64_forward_default_shapes_ms": round(ms_fd, 3),
+           "forward_tuned_equals_default": same_tuned,
+           "tune": tune_to_json(tune, tstats)["buckets"],
+           "fp64_adjoint_ms": round(ms_b, 3),
            "fp64_pairs_s": round(npairs / (ms_f + ms_b) * 1e3, 1),
            "fp64_fused_forward_adjoint_ms": round(ms_fb64, 3),
            "fp64_fused_pairs_s": round(npairs / ms_fb64 * 1e3, 1),
@@ -743,8 +755,9 @@ def grad_phase(dev, args, checks, cpu_jobs):
                                                 "median": float(np.median(fd_err)),
                                                 "pairs": int(len(fsel)),
                                                 "scale": "per polygon: max |g - fd| / max |g|"}}
-    log(f"grad: fwd {ms_f:.2f} ms, adjoint {ms_b:.2f} ms (fp64), fused fp64 {ms_fb64:.2f} ms "
-        f"(Green area rel err {green_rel:.1e}), fp32 {ms_32:.2f} ms; fp32 vs fp64 {rel32:.1e}; "
+    log(f"grad: fwd {ms_f:.2f} ms (default shapes {ms_fd:.2f} ms), adjoint {ms_b:.2f} ms (fp64), "
+        f"fused fp64 {ms_fb64:.2f} ms (Green area rel err {green_rel:.1e}), fp32 {ms_32:.2f} ms; "
+        f"fp32 vs fp64 {rel32:.1e}; "
         f"vs central FD max {np.max(fd_err):.1e} on {len(fsel)} pairs")
 
     csel = np.sort(np.random.default_rng(0).choice(npairs, 2000, replace=False))
