@@ -43,7 +43,8 @@ template <class T> __device__ inline T wave_sum(T v) {
 }
 
 // even-odd point in polygon, B in local coordinates (crossing number); B's
-// vertices are (x, y) pairs
+// vertices are (x, y) pairs (loading the next vertex a step ahead took 8 more
+// VGPRs, a wave less per SIMD, and was slower)
 template <class T, class T2>
 __device__ inline int point_in(const T2 *b, int nB, T px, T py) {
     int in = 0;
@@ -59,31 +60,36 @@ __device__ inline int point_in(const T2 *b, int nB, T px, T py) {
     return in;
 }
 
-// Crossing parameter t of edge a + t r with edge i of B (c -> c + s), or -1
-// when the oracle's test rejects it: t in (0, 1), u in [-1e-9, 1 + 1e-9].
-// The quotients are formed only for the pairs that pass a division-free test
-// that is exact in the one direction needed: it rejects only pairs whose
-// rounded quotients the test would reject too (the sign and |t| < 1 tests on
-// the numerators; |u| bounds with a margin of 2e-9 (FP64) or 4e-7 (FP32,
-// > 1.5 ulp of 1) of |den|, skipped when |den| is so small that the margin
-// would underflow).
+// The crossing of edge a + t r with edge c -> c + s of B, in two steps.
+// Candidate: the numerators and the denominator of the oracle's quotients
+// and a division-free test, without branches, that is exact in the one
+// direction needed -- it rejects only pairs whose rounded quotients the
+// oracle's test (t in (0, 1), u in [-1e-9, 1 + 1e-9]) would reject too: the
+// sign and |t| < 1 tests on the numerators, and |u| bounds with a margin of
+// 2e-9 (FP64) or 4e-7 (FP32, > 1.5 ulp of 1) of |den|, left out when |den|
+// is so small that the margin would underflow.
+template <class T> struct Cand {
+    T den, tn, un;
+    bool ok;
+};
 template <class T>
-__device__ inline T crossing_t(T ax, T ay, T rx, T ry, T cx, T cy, T sx, T sy) {
-    const T den = rx * sy - ry * sx;
-    if (den == T(0)) return T(-1);
+__device__ inline Cand<T> crossing_cand(T ax, T ay, T rx, T ry, T cx, T cy, T sx, T sy) {
+    Cand<T> c;
+    c.den = rx * sy - ry * sx;
     const T qx = cx - ax, qy = cy - ay;
-    const T tn = qx * sy - qy * sx;
-    const T un = qx * ry - qy * rx;
-    const bool pos = den > T(0);
-    const T a = pos ? den : -den, tp = pos ? tn : -tn, up = pos ? un : -un;
-    if (!(tp > T(0) && tp < a)) return T(-1);
+    c.tn = qx * sy - qy * sx;
+    c.un = qx * ry - qy * rx;
+    const bool pos = c.den > T(0);
+    const T a = pos ? c.den : -c.den, tp = pos ? c.tn : -c.tn, up = pos ? c.un : -c.un;
     constexpr bool f64 = sizeof(T) == 8;
-    const T tiny = f64 ? T(1e-290) : T(1e-30);
-    if (a > tiny) {
-        const T m = (f64 ? T(2e-9) : T(4e-7)) * a;
-        if (up < -m || up > a + m) return T(-1);
-    }
-    const T t = tn / den, u = un / den;
+    const T m = (f64 ? T(2e-9) : T(4e-7)) * a;
+    const bool small = !(a > (f64 ? T(1e-290) : T(1e-30)));
+    c.ok = (c.den != T(0)) & (tp > T(0)) & (tp < a) & (small | ((up >= -m) & (up <= a + m)));
+    return c;
+}
+// Exact step (a candidate only): the oracle's quotients and test; t or -1.
+template <class T> __device__ inline T crossing_exact(const Cand<T> &c) {
+    const T t = c.tn / c.den, u = c.un / c.den;
     const T lo_u = T(-1e-9), hi_u = T(1) + T(1e-9);
     return t > T(0) && t < T(1) && u >= lo_u && u <= hi_u ? t : T(-1);
 }
@@ -120,8 +126,10 @@ __device__ inline void edge_parts(T ax, T ay, T ex, T ey, const T2 *b, int nB, T
     T2 vi = b[0];
     for (int i = 0; i < nB; ++i) {
         const T2 vj = b[i + 1 == nB ? 0 : i + 1];
-        const T t = crossing_t<T>(ax, ay, rx, ry, vi.x, vi.y, vj.x - vi.x, vj.y - vi.y);
+        const Cand<T> c = crossing_cand<T>(ax, ay, rx, ry, vi.x, vi.y, vj.x - vi.x, vj.y - vi.y);
         vi = vj;
+        if (!c.ok) continue;
+        const T t = crossing_exact<T>(c);
         if (t < T(0)) continue;
         bool dup = false;
 #pragma unroll
@@ -153,8 +161,10 @@ __device__ inline void edge_parts(T ax, T ay, T ex, T ey, const T2 *b, int nB, T
         T2 wi = b[0];
         for (int i = 0; i < nB; ++i) {
             const T2 wj = b[i + 1 == nB ? 0 : i + 1];
-            const T t = crossing_t<T>(ax, ay, rx, ry, wi.x, wi.y, wj.x - wi.x, wj.y - wi.y);
+            const Cand<T> c = crossing_cand<T>(ax, ay, rx, ry, wi.x, wi.y, wj.x - wi.x, wj.y - wi.y);
             wi = wj;
+            if (!c.ok) continue;
+            const T t = crossing_exact<T>(c);
             if (t > t0 && t < t1) t1 = t;
         }
         part(t0, t1);
