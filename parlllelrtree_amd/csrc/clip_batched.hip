@@ -14,8 +14,11 @@
 //     beam finds its active edges without scanning the polygon;
 //   * a beam's entries are sorted by the sub-beam key; a beam whose sorted
 //     entries show no crossing is summed in one linear pass;
-//   * lane k sweeps beams k, k+TL, ...; beams are independent, their sums
-//     are combined with the fixed pairwise tree the oracle uses;
+//   * lane k sweeps the crossing-free beams k, k+TL, ...; a beam with
+//     crossings is swept by one wave, its sub-beams side by side in groups
+//     of lanes (one-wave and whole-wave teams) or by the team in LDS
+//     (smaller teams); beams are independent, their sums are combined with
+//     the fixed pairwise tree the oracle uses;
 //   * teams take the next pair from a per-bucket ticket (work stealing), so
 //     polygons of very uneven size do not leave lanes idle.
 // The lists of a pair are built for windows of consecutive beams that fit
@@ -578,8 +581,8 @@ constexpr size_t team_static_lds(int w) {
 
 // Teams of TL / 64 whole waves (TL = 128 or 256), one team per block: the
 // pair's tables are shared by the waves, which split the lanes' loops and
-// take turns at the queued crossing beams, so a large pair is done by more
-// lanes without more LDS.
+// share out the queued crossing beams (round-robin), so a pair is done by
+// more lanes without more LDS.
 template <int TL>
 __global__ __launch_bounds__(256) void k_clip_batched_team(BatchArgs A) {
     extern __shared__ __align__(16) char lds[];
