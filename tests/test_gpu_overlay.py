@@ -229,6 +229,27 @@ def test_batched_sub_beams_in_several_passes(dev, pad, lanes):
     assert (ref > 0).all()
 
 
+@pytest.mark.parametrize("K", [250, 300])
+@pytest.mark.parametrize("lanes", [64, 128, 256])
+def test_batched_many_crossing_beams_in_one_window(dev, K, lanes):
+    # two staircases whose zigzag sides cross once in every beam: K crossing
+    # beams of 4 entries in one window of beams, more than a wave's 64 when
+    # the beams are dealt over the waves of a team (two or four waves)
+    P = [(1 + 0.3 * (i % 2), float(i)) for i in range(K + 1)] + [(0.0, float(K)), (0.0, 0.0)]
+    Q = [(2.0, 0.0), (2.0, float(K))] + [(1.3 - 0.3 * (i % 2), float(i)) for i in reversed(range(K + 1))]
+    A, B = csr([P, P]), csr([Q, Q[::-1]])
+    pairs = np.array([(0, 0), (1, 1), (0, 1)], np.uint32)
+    tune = rtree.default_tune()
+    for b in range(6):
+        tune.lanes_per_pair[b] = lanes
+        tune.waves_per_block[b] = 1
+        tune.list_factor[b] = 4
+    got = run_batched(pairs, A, B, tune)
+    ref = O.overlay(pairs, A, B)
+    assert np.array_equal(got, ref), (got, ref)
+    assert (ref > 0).all()
+
+
 def test_batched_golden_and_edges(dev):
     cases = GOLD["clip"]
     A = csr([c["P"] if len(c["P"]) else [[0, 0]] for c in cases])
