@@ -26,7 +26,12 @@
 namespace prt {
 namespace {
 
-constexpr int GR_WAVES = 4;
+// One wave per block, one pair per wave: a block keeps its LDS until its
+// last wave ends, so blocks of one wave free it as each pair ends, and the
+// dispatcher hands out pairs of very uneven cost one at a time (262,144 star
+// pairs, FP64: blocks of 4 waves capped at 65,536 blocks 14.7 ms, 2 waves
+// 12.7 ms, one wave per block and pair 12.4 ms).
+constexpr int GR_WAVES = 1;
 constexpr int GR_THREADS = 64 * GR_WAVES;
 
 template <class T> __device__ inline T wave_min(T v) {
@@ -278,7 +283,7 @@ int launch_grad(const uint32_t *pairs, int64_t np, const T *xyA, const int64_t *
     W = W < 1 ? 1 : (W > GR_WAVES ? GR_WAVES : W);
     const size_t lds = W * per_wave;
     int64_t blocks = ceil_div(np, W);
-    if (blocks > 65536) blocks = 65536;
+    if (blocks > (int64_t(1) << 30)) blocks = int64_t(1) << 30;  // waves loop over the rest
     PRT_LAUNCH(k_area_grad<T>, dim3(static_cast<unsigned>(blocks)), W * 64, lds,
                as_stream(stream), pairs, np, xyA, offA, xyB, offB, w, gA, gB, area, max_verts);
     PRT_LAUNCHED();
