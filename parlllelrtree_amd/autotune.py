@@ -62,7 +62,8 @@ def _time(fn, reps: int = 3) -> float:
 
 
 def tune_to_json(t: TuneStruct, stats=None) -> dict:
-    out = {"device": torch.cuda.get_device_name(0), "buckets": []}
+    out = {"device": torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu",
+           "buckets": []}
     for b in range(PRT_TUNE_BUCKETS):
         e = {"max_verts": t.bucket_max_verts[b], "lanes": t.lanes_per_pair[b],
              "waves": t.waves_per_block[b], "list_factor": t.list_factor[b]}

@@ -149,3 +149,28 @@ def test_tune_shapes_checked_before_any_launch():
         bad.lanes_per_pair[3], bad.waves_per_block[3] = lanes, waves
         assert call(bad) == -1, (lanes, waves)
         assert "tune" in lib.prt_last_error().decode()
+
+
+def test_tune_json_round_trip_and_lds_fit():
+    """The autotuner's cache format round-trips a tune, and its LDS fit rule
+    follows the library's per-team LDS size: teams of whole waves (more than
+    64 lanes) are one per block; smaller teams share a block's LDS."""
+    from parlllelrtree_amd import autotune as at
+    from parlllelrtree_amd import rtree
+    from parlllelrtree_amd._lib import PRT_TUNE_BUCKETS
+    t = rtree.default_tune()
+    t.lanes_per_pair[1], t.waves_per_block[1], t.list_factor[1] = 32, 2, 4
+    d = at.tune_to_json(t, {1: {"pairs": 7, "ms": 0.5}})
+    assert d["buckets"][1] == {"max_verts": 64, "lanes": 32, "waves": 2, "list_factor": 4,
+                               "pairs": 7, "ms": 0.5}
+    u = at.tune_from_json(d)
+    for b in range(PRT_TUNE_BUCKETS):
+        assert (u.bucket_max_verts[b], u.lanes_per_pair[b], u.waves_per_block[b],
+                u.list_factor[b]) == (t.bucket_max_verts[b], t.lanes_per_pair[b],
+                                      t.waves_per_block[b], t.list_factor[b])
+    tb = at._team_bytes(128, 1)
+    assert tb > 0 and at._team_bytes(128, 2) > tb and at._team_bytes(256, 1) > tb
+    assert at._fits(128, 128, 1, 1) and not at._fits(128, 128, 2, 1)
+    per_block = 160 * 1024 // tb       # teams of the 128-vertex bucket a block's LDS holds
+    assert at._fits(128, 64, per_block, 1) and not at._fits(128, 64, per_block + 1, 1)
+    assert at._fits(128, 8, per_block // 8, 1) and not at._fits(128, 8, per_block // 8 + 1, 1)
