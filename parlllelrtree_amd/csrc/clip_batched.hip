@@ -264,40 +264,62 @@ __device__ inline double next_lane_d(double v) {
     return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// One compare-exchange of the bitonic network at distance J inside blocks
-// of K lanes (ascending where q & K == 0): the lane keeps the first of its
-// entry and its partner's in the strict key order (xa + xc, xa, edge id)
-// when it is the lower lane of an ascending pair or the upper lane of a
-// descending one, the other entry otherwise.
-template <int J> __device__ inline void bitonic_xchg(int q, int K, double &a, double &c, int &e) {
-    const double oa = xor_lane_d<J>(a), oc = xor_lane_d<J>(c);
-    const int oe = static_cast<int>(xor_lane<J>(static_cast<uint32_t>(e)));
-    const bool o_first = key_before(oa + oc, oa, oe, a + c, a, e);
-    const bool keep_first = ((q & J) == 0) == ((q & K) == 0);
-    if (o_first == keep_first) {
-        a = oa;
-        c = oc;
-        e = oe;
-    }
+// v of lane q - 1 (every lane active; lane 0 reads 0): a DPP shift of the
+// whole wave by one lane the other way; ints by the same two shifts
+__device__ inline double prev_lane_d(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), 0x138, 0xF, 0xF, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), 0x138, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
+__device__ inline int next_lane_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+__device__ inline int prev_lane_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
 
-// Teams of one wave: the entries of every aligned group of M2 lanes (M2 a
-// power of two <= 64) sorted ascending by the strict key (xa + xc, xa, edge
-// id) across the group's lanes, the lane at position q of its group ending
-// with the group's q-th entry (the exchanges at distances below M2 stay
-// inside the group).  Every lane active.
-__device__ inline void wave_bitonic(int q, int M2, double &a, double &c, int &e) {
-    for (int K = 2; K <= M2; K <<= 1) {
-        for (int J = K >> 1; J > 0; J >>= 1) {
-            switch (J) {
-            case 1: bitonic_xchg<1>(q, K, a, c, e); break;
-            case 2: bitonic_xchg<2>(q, K, a, c, e); break;
-            case 4: bitonic_xchg<4>(q, K, a, c, e); break;
-            case 8: bitonic_xchg<8>(q, K, a, c, e); break;
-            case 16: bitonic_xchg<16>(q, K, a, c, e); break;
-            default: bitonic_xchg<32>(q, K, a, c, e); break;
+// The entries of every aligned group of M2 lanes (the first m of each group
+// real, the rest +inf keys) sorted ascending by the strict key (xa + xc, xa,
+// edge id), from an order close to sorted (the beam's, which a sub-beam's
+// differs from by the swaps of its crossing pairs): rounds of odd-even
+// transposition -- compare-exchange of lanes (2k, 2k + 1), then of (2k + 1,
+// 2k + 2), inside the group -- until every real entry precedes the next
+// (at most M2 rounds, 2 M2 >= m phases, always enough).  The key is a strict
+// order, so the result is the one sorted order (a bitonic network over the
+// group, which this replaces, took 6-10 dependent exchange stages for the
+// usual 5-16 entries; these take 1-3 rounds).
+// Every lane active; act: the lane's group holds a sub-beam.
+__device__ inline void wave_oddeven(int qi, int M2, int m, bool act, double &a, double &c, int &e) {
+    for (int round = 0; round < M2; ++round) {
+        {  // pairs (2k, 2k + 1)
+            const double oa = xor_lane_d<1>(a), oc = xor_lane_d<1>(c);
+            const int oe = static_cast<int>(xor_lane<1>(static_cast<uint32_t>(e)));
+            const bool o_first = key_before(oa + oc, oa, oe, a + c, a, e);
+            if (o_first == ((qi & 1) == 0)) {
+                a = oa;
+                c = oc;
+                e = oe;
             }
         }
+        {  // pairs (2k + 1, 2k + 2) inside the group
+            const double na = next_lane_d(a), nc = next_lane_d(c);
+            const int ne = next_lane_i(e);
+            const double pa = prev_lane_d(a), pc = prev_lane_d(c);
+            const int pe = prev_lane_i(e);
+            const bool lower = (qi & 1) == 1;
+            const bool has = lower ? qi + 1 < M2 : qi > 0;
+            const double oa = lower ? na : pa, oc = lower ? nc : pc;
+            const int oe = lower ? ne : pe;
+            const bool o_first = key_before(oa + oc, oa, oe, a + c, a, e);
+            if (has && o_first == lower) {
+                a = oa;
+                c = oc;
+                e = oe;
+            }
+        }
+        const double na = next_lane_d(a), nc = next_lane_d(c);
+        const int ne = next_lane_i(e);
+        const bool out = act && qi + 1 < m && !key_before(a + c, a, e, na + nc, na, ne);
+        if (__ballot(out) == 0ull) break;
     }
 }
 
@@ -308,9 +330,9 @@ __device__ inline void wave_bitonic(int q, int M2, double &a, double &c, int &e)
 // then split into aligned groups of M2 lanes (M2 = pow2 >= m entries) and
 // each group takes one sub-beam, 64 / M2 sub-beams per pass: lane qi of a
 // group holds entry lo + qi, evaluates it at its sub-beam's ends (the
-// expressions of ex() and the serial sweep), and a bitonic network inside
-// the group sorts the entries by the strict key, so lane qi holds the qi-th
-// entry of the sub-beam; its parities are prefix counts of the group's
+// expressions of ex() and the serial sweep), and odd-even transposition
+// inside the group (wave_oddeven) sorts the entries by the strict key, so
+// lane qi holds the qi-th entry of the sub-beam; its parities are prefix counts of the group's
 // ballots, and its gap term is the serial sweep's ((xa1 - xa0) + (xc1 -
 // xc0)) * h.  The terms of the inside gaps are added to the beam sum one at a
 // time in lane order -- sub-beam by sub-beam, gap by gap -- so the sum is the
@@ -422,7 +444,7 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
             sc = yc == yb ? xb : (yc == yt ? xt : (yc == ety ? etx : ebx + edx * (yc - eby)));
             se = e;
         }
-        wave_bitonic(qi, M2, sa, sc, se);
+        wave_oddeven(qi, M2, m, act, sa, sc, se);
         const bool in_p = act && live && se < nP, in_q = act && live && !(se < nP);
         const int pp = __popcll(__ballot(in_p) & upto) & 1;
         const int pq = __popcll(__ballot(in_q) & upto) & 1;
@@ -732,7 +754,7 @@ extern "C" int prt_tune_default(prt_tune *t) {
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
     const int lanes[NBUCKET] = {16, 128, 128, 128, 256, 256};
     const int waves[NBUCKET] = {2, 1, 1, 1, 1, 1};
-    const int factor[NBUCKET] = {2, 2, 2, 1, 2, 4};
+    const int factor[NBUCKET] = {2, 1, 2, 1, 2, 4};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];
         t->lanes_per_pair[b] = lanes[b];
