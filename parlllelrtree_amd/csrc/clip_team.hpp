@@ -324,6 +324,53 @@ __device__ inline void wave_sort_sub_beam(const TeamLds &t, int lo, int m, doubl
     }
 }
 
+// Teams of one wave: the crossing heights in (yb, yt) of the beam whose
+// list entries are [lo, hi) -- the serial sweep's candidate set, from the
+// same pairs (i, j), i < j, with the same expressions -- written to
+// giy[0, min(nc, GC)) in no particular order; returns nc on every lane.  The
+// pairs are spread over the lanes (pair index j (j - 1) / 2 + i), and each
+// step's heights are compacted by a ballot prefix.  Every lane active.
+__device__ inline int wave_crossing_heights(const TeamLds &t, int lo, int hi, double yb, double yt,
+                                            int q) {
+    const int m = hi - lo;
+    const double dy = yt - yb;
+    const int npairs = m * (m - 1) / 2;
+    int nc = 0;
+    for (int t0 = 0; t0 < npairs; t0 += 64) {
+        const int tt = t0 + q;
+        bool has = false;
+        double iy = 0.0;
+        if (tt < npairs) {
+            int j = static_cast<int>((1.0f + __builtin_sqrtf(1.0f + 8.0f * static_cast<float>(tt))) * 0.5f);
+            while (j * (j - 1) / 2 > tt) --j;
+            while ((j + 1) * j / 2 <= tt) ++j;
+            const int i = tt - j * (j - 1) / 2;
+            const double xbi = t.lxb[lo + i], xti = t.lxt[lo + i];
+            const double xbj = t.lxb[lo + j], xtj = t.lxt[lo + j];
+            double xbL = 0.0, xtL = 0.0, xbR = 0.0, xtR = 0.0;
+            bool cr = true;
+            if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+            else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+            else cr = false;
+            if (cr) {
+                const double den = (xtL - xbL) - (xtR - xbR);
+                if (den > 0.0) {
+                    const double r = (xbR - xbL) / den;
+                    iy = r * dy + yb;
+                    has = iy > yb && iy < yt;
+                }
+            }
+        }
+        const uint64_t hm = __ballot(has);
+        if (has) {
+            const int pos = nc + __popcll(hm & ((1ull << q) - 1ull));
+            if (pos < t.GC) t.giy[pos] = iy;
+        }
+        nc += __popcll(hm);
+    }
+    return nc;
+}
+
 // Teams of one wave: giy[0, n) ascending (n <= 64), by ranks as above
 // (equal heights keep their order).  The caller syncs the team afterwards.
 __device__ inline void wave_sort_heights(const TeamLds &t, int n, int q) {

@@ -1068,29 +1068,38 @@ __device__ int team_contour_pair(const TeamLds &t, const ContourWs &cw, const Te
             int mode = (m < 2 || t.cnt[k] == 0) ? 0 : (m <= t.GS ? 1 : 2);
             int nc = 0;
             if (mode == 1) {  // the beam's crossing heights
-                const double dy = yt - yb;
-                if (tl == 0) t.gmisc[1] = 0;
-                team_sync();
-                for (int i = lo + tl; i < hi; i += TL) {
-                    const double xbi = t.lxb[i], xti = t.lxt[i];
-                    for (int j = i + 1; j < hi; ++j) {
-                        const double xbj = t.lxb[j], xtj = t.lxt[j];
-                        double xbL, xtL, xbR, xtR;
-                        if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
-                        else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
-                        else continue;
-                        const double den = (xtL - xbL) - (xtR - xbR);
-                        if (!(den > 0.0)) continue;
-                        const double r = (xbR - xbL) / den;
-                        const double iy = r * dy + yb;
-                        if (iy > yb && iy < yt) {
-                            const int pos = atomicAdd(&t.gmisc[1], 1);
-                            if (pos < t.GC) t.giy[pos] = iy;
+                if constexpr (TL == 64) {
+                    // the pairs spread over the wave, heights compacted by
+                    // ballots (lane i looping over the pairs (i, j > i) left
+                    // the first lanes with the longest loops)
+                    team_sync();
+                    nc = wave_crossing_heights(t, lo, hi, yb, yt, tl);
+                    team_sync();
+                } else {
+                    const double dy = yt - yb;
+                    if (tl == 0) t.gmisc[1] = 0;
+                    team_sync();
+                    for (int i = lo + tl; i < hi; i += TL) {
+                        const double xbi = t.lxb[i], xti = t.lxt[i];
+                        for (int j = i + 1; j < hi; ++j) {
+                            const double xbj = t.lxb[j], xtj = t.lxt[j];
+                            double xbL, xtL, xbR, xtR;
+                            if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+                            else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+                            else continue;
+                            const double den = (xtL - xbL) - (xtR - xbR);
+                            if (!(den > 0.0)) continue;
+                            const double r = (xbR - xbL) / den;
+                            const double iy = r * dy + yb;
+                            if (iy > yb && iy < yt) {
+                                const int pos = atomicAdd(&t.gmisc[1], 1);
+                                if (pos < t.GC) t.giy[pos] = iy;
+                            }
                         }
                     }
+                    team_sync();
+                    nc = t.gmisc[1];
                 }
-                team_sync();
-                nc = t.gmisc[1];
                 if (nc > t.GC) {
                     mode = 2;  // more heights than the scratch holds
                 } else if (nc > 1) {
