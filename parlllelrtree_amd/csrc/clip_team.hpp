@@ -239,21 +239,21 @@ __device__ inline bool key_before(double ka, double xa_a, int ea, double kb, dou
 // sub-beam key (xb + xt, xb, edge id), by the whole team at once: every
 // entry counts the entries of its beam that come before it (the key is a
 // strict order, so the counts are a permutation) and its edge id goes to
-// that place; the x's at the beam ends are then computed again for the new
-// order (the expressions of the list fill, so the same values).  Then
+// that place with its x's at the beam ends -- moved from registers in
+// windows of up to 2 TL entries, else computed again for the new order (the
+// expressions of the list fill, so the same values either way).  Then
 // cnt[k] = 1 for a beam whose sorted xb or xt decreases somewhere (edges
 // crossing inside the beam), else 0.  Syncs the team before and after.
 template <int TL> __device__ void team_sort_window(const TeamLds &t, int k0, int k1, int tl) {
     const int wbase = t.boff[k0], S = t.boff[k1] - wbase;
     tsync<TL>();
-    for (int i = tl; i < S; i += TL) {
+    // entries before entry i of its beam: every operand loaded unconditionally
+    // and compared without branches, four entries per step, so that the
+    // loads of a step are in flight together
+    auto rank_of = [&](int i, double xb, double xt, int e) -> int {
         const int k = t.beam_of[i];
         const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
-        const double xb = t.lxb[i], xt = t.lxt[i], key = xb + xt;
-        const int e = t.list[i];
-        // entries before this one: every operand loaded unconditionally and
-        // compared without branches, four entries per step, so that the
-        // loads of a step are in flight together
+        const double key = xb + xt;
         auto before = [&](double xbj, double xtj, int ej) -> int {
             const double kj = xbj + xtj;
             return static_cast<int>((kj < key) | ((kj == key) & ((xbj < xb) | ((xbj == xb) & (ej < e)))));
@@ -266,17 +266,48 @@ template <int TL> __device__ void team_sort_window(const TeamLds &t, int k0, int
             r += before(b0, c0, e0) + before(b1, c1, e1) + before(b2, c2, e2) + before(b3, c3, e3);
         }
         for (; j < hi; ++j) r += before(t.lxb[j], t.lxt[j], t.list[j]);
-        t.tlist[lo + r] = static_cast<uint16_t>(e);
+        return lo + r;
+    };
+    if (S <= 2 * TL) {
+        // at most two entries per lane: ranked, then (once every lane has
+        // read the lists) moved to their places with their x's -- the
+        // values the list fill computed, which ex() would compute again
+        int pos[2] = {-1, -1}, ev[2] = {0, 0};
+        double bv[2] = {0.0, 0.0}, tv[2] = {0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int i = tl + r * TL;
+            if (i < S) {
+                ev[r] = t.list[i];
+                bv[r] = t.lxb[i];
+                tv[r] = t.lxt[i];
+                pos[r] = rank_of(i, bv[r], tv[r], ev[r]);
+            }
+        }
+        for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
+        tsync<TL>();
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (pos[r] >= 0) {
+                t.list[pos[r]] = static_cast<uint16_t>(ev[r]);
+                t.lxb[pos[r]] = bv[r];
+                t.lxt[pos[r]] = tv[r];
+            }
+        }
+        tsync<TL>();
+    } else {
+        for (int i = tl; i < S; i += TL)
+            t.tlist[rank_of(i, t.lxb[i], t.lxt[i], t.list[i])] = t.list[i];
+        for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
+        tsync<TL>();
+        for (int i = tl; i < S; i += TL) {
+            const int k = t.beam_of[i], e = t.tlist[i];
+            t.list[i] = static_cast<uint16_t>(e);
+            t.lxb[i] = ex(t, e, t.ys[k]);
+            t.lxt[i] = ex(t, e, t.ys[k + 1]);
+        }
+        tsync<TL>();
     }
-    for (int k = k0 + tl; k < k1; k += TL) t.cnt[k] = 0;
-    tsync<TL>();
-    for (int i = tl; i < S; i += TL) {
-        const int k = t.beam_of[i], e = t.tlist[i];
-        t.list[i] = static_cast<uint16_t>(e);
-        t.lxb[i] = ex(t, e, t.ys[k]);
-        t.lxt[i] = ex(t, e, t.ys[k + 1]);
-    }
-    tsync<TL>();
     for (int i = tl; i < S; i += TL) {
         const int k = t.beam_of[i];
         if (i > t.boff[k] - wbase && (t.lxb[i] < t.lxb[i - 1] || t.lxt[i] < t.lxt[i - 1]))
