@@ -713,12 +713,12 @@ extern "C" int prt_tune_default(prt_tune *t) {
     PRT_REQUIRE(t != nullptr, PRT_E_INVAL, "tune_default: null");
     // the fastest shapes on the grid workload (8.4M pairs of lognormal
     // stars) and on the 2 x 4M-star overlay, searched bucket by bucket over
-    // the whole call on MI355X; every bucket but the smallest takes teams of
+    // the whole call on MI355X; the buckets above 64 vertices take teams of
     // two or four whole waves
     const int bounds[NBUCKET] = {32, 64, 128, 256, 512, 2 * PRT_MAX_POLY_VERTS};
-    const int lanes[NBUCKET] = {16, 128, 128, 128, 256, 256};
-    const int waves[NBUCKET] = {2, 1, 1, 1, 1, 1};
-    const int factor[NBUCKET] = {2, 1, 2, 1, 2, 4};
+    const int lanes[NBUCKET] = {16, 32, 128, 128, 256, 256};
+    const int waves[NBUCKET] = {1, 2, 1, 1, 1, 1};
+    const int factor[NBUCKET] = {2, 1, 2, 1, 2, 2};
     for (int b = 0; b < NBUCKET; ++b) {
         t->bucket_max_verts[b] = bounds[b];
         t->lanes_per_pair[b] = lanes[b];
