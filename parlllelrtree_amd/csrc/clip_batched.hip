@@ -224,24 +224,6 @@ __device__ double team_general(const TeamLds &t, int lo, int hi, double yb, doub
     return __shfl(bs, base, 64);
 }
 
-// v of lane (this lane ^ J), every lane active: within a quad and across
-// half a row a DPP move, within 32 lanes a swizzle, across the wave's
-// halves a permute
-template <int J> __device__ inline uint32_t xor_lane(uint32_t v) {
-    const int x = static_cast<int>(v);
-    if constexpr (J == 1) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
-    else if constexpr (J == 2) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
-    else if constexpr (J == 4) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x101F));
-    else if constexpr (J == 8) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
-    else if constexpr (J == 16) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x401F));
-    else return static_cast<uint32_t>(__shfl_xor(x, 32, 64));
-}
-template <int J> __device__ inline double xor_lane_d(double v) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = xor_lane<J>(static_cast<uint32_t>(u)), hi = xor_lane<J>(static_cast<uint32_t>(u >> 32));
-    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-}
-
 // A double pushed to lane dst (every lane active, dst a permutation of the
 // lanes): lane dst receives this lane's v.
 __device__ inline double permute_d(double v, int dst) {

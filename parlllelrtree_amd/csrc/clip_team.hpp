@@ -151,15 +151,45 @@ __device__ inline int lower_bound(const double *ys, int n, double v) {
     return lo;
 }
 
-// team-wide min/max/sum over TL lanes (butterfly inside the team's lanes)
+// v of lane (this lane ^ J), the partner lane active: within a quad and
+// across half a row a DPP move, within 32 lanes a swizzle, across the wave's
+// halves a permute
+template <int J> __device__ inline uint32_t xor_lane(uint32_t v) {
+    const int x = static_cast<int>(v);
+    if constexpr (J == 1) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    else if constexpr (J == 2) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+    else if constexpr (J == 4) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x101F));
+    else if constexpr (J == 8) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
+    else if constexpr (J == 16) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x401F));
+    else return static_cast<uint32_t>(__shfl_xor(x, 32, 64));
+}
+template <int J> __device__ inline double xor_lane_d(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = xor_lane<J>(static_cast<uint32_t>(u)), hi = xor_lane<J>(static_cast<uint32_t>(u >> 32));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// team-wide min/max over TL lanes (butterfly inside the team's lanes, by
+// the lane exchanges above: DPP moves and swizzles, a permute only across
+// the wave's halves)
+template <int J> __device__ inline double bfly_min(double v) { return fmin(v, xor_lane_d<J>(v)); }
+template <int J> __device__ inline double bfly_max(double v) { return fmax(v, xor_lane_d<J>(v)); }
 template <int TL> __device__ inline double team_min(double v) {
-#pragma unroll
-    for (int o = 1; o < TL; o <<= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    if constexpr (TL > 1) v = bfly_min<1>(v);
+    if constexpr (TL > 2) v = bfly_min<2>(v);
+    if constexpr (TL > 4) v = bfly_min<4>(v);
+    if constexpr (TL > 8) v = bfly_min<8>(v);
+    if constexpr (TL > 16) v = bfly_min<16>(v);
+    if constexpr (TL > 32) v = bfly_min<32>(v);
     return v;
 }
 template <int TL> __device__ inline double team_max(double v) {
-#pragma unroll
-    for (int o = 1; o < TL; o <<= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if constexpr (TL > 1) v = bfly_max<1>(v);
+    if constexpr (TL > 2) v = bfly_max<2>(v);
+    if constexpr (TL > 4) v = bfly_max<4>(v);
+    if constexpr (TL > 8) v = bfly_max<8>(v);
+    if constexpr (TL > 16) v = bfly_max<16>(v);
+    if constexpr (TL > 32) v = bfly_max<32>(v);
     return v;
 }
 // inclusive scan inside the team
