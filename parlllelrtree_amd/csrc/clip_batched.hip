@@ -406,6 +406,142 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
     return bs;
 }
 
+// Teams of 32 lanes (half a wave, lanes base .. base + 31; the other half
+// may be another team anywhere in its own pair): wave_general's sweep of a
+// beam of m <= 32 entries inside the team's lanes, the same heights, orders,
+// terms and summation order.  What wave_general reads across the wave with
+// readlane (an index uniform over the wave) goes through the team's LDS
+// scratch here: the heights' ranks are counted from giy, and the inside
+// gaps' terms are added by the team's lane 0 from gxa, gap after gap.
+// Ballots are shifted to the team's half.  Returns the beam sum on the
+// team's lane 0.
+__device__ double half_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                               int op, int q, int base) {
+    const int m = hi - lo;
+    const double dy = yt - yb;
+    auto half = [&](bool v) { return static_cast<uint32_t>(__ballot(v) >> base); };
+    // the pairs (i, j), i < j, spread over the team's lanes
+    const int npairs = m * (m - 1) / 2;
+    int nc = 0;
+    team_sync();
+    for (int t0 = 0; t0 < npairs; t0 += 32) {
+        const int tt = t0 + q;
+        bool has = false;
+        double iy = 0.0;
+        if (tt < npairs) {
+            int j = static_cast<int>((1.0f + __builtin_sqrtf(1.0f + 8.0f * static_cast<float>(tt))) * 0.5f);
+            while (j * (j - 1) / 2 > tt) --j;
+            while ((j + 1) * j / 2 <= tt) ++j;
+            const int i = tt - j * (j - 1) / 2;
+            const double xbi = t.lxb[lo + i], xti = t.lxt[lo + i];
+            const double xbj = t.lxb[lo + j], xtj = t.lxt[lo + j];
+            double xbL = 0.0, xtL = 0.0, xbR = 0.0, xtR = 0.0;
+            bool cr = true;
+            if (xbi < xbj && xti > xtj) { xbL = xbi; xtL = xti; xbR = xbj; xtR = xtj; }
+            else if (xbj < xbi && xtj > xti) { xbL = xbj; xtL = xtj; xbR = xbi; xtR = xti; }
+            else cr = false;
+            if (cr) {
+                const double den = (xtL - xbL) - (xtR - xbR);
+                if (den > 0.0) {
+                    const double r = (xbR - xbL) / den;
+                    iy = r * dy + yb;
+                    has = iy > yb && iy < yt;
+                }
+            }
+        }
+        const uint32_t hm = half(has);
+        if (has) {
+            const int pos = nc + __popc(hm & ((1u << q) - 1u));
+            if (pos < t.GC) t.giy[pos] = iy;
+        }
+        nc += __popc(hm);
+    }
+    if (nc > t.GC || nc > 32) {
+        // more crossing heights than the scratch (or the team) holds: the
+        // serial sweep
+        team_sync();
+        double r = 0.0;
+        if (q == 0) r = beam_sum_serial(t, lo, hi, yb, yt, nP, op);
+        team_sync();
+        return r;
+    }
+    // the heights ascending (ranks counted over giy, equal heights in their
+    // order there), then the distinct ones compacted into giy[0, nd)
+    int nd = 0;
+    if (nc > 0) {
+        team_sync();
+        const double h = q < nc ? t.giy[q] : 0.0;
+        int r = 0;
+        for (int p = 0; p < nc; ++p) {
+            const double hp = t.giy[p];
+            r += (hp < h || (hp == h && p < q)) ? 1 : 0;
+        }
+        team_sync();
+        if (q < nc) t.giy[r] = h;
+        team_sync();
+        const double sh = q < nc ? t.giy[q] : 0.0;
+        const double nx = q + 1 < nc ? t.giy[q + 1] : 0.0;
+        const bool keep = q < nc && (q + 1 == nc || nx != sh);
+        const uint32_t km = half(keep);
+        nd = __popc(km);
+        team_sync();
+        if (keep) t.giy[__popc(km & ((1u << q) - 1u))] = sh;
+        team_sync();
+    }
+    const int M2 = pow2_ge(m);
+    const int G = 32 / M2, g = q / M2, qi = q & (M2 - 1);
+    const bool live = qi < m;
+    int e = 0;
+    double xb = 0.0, xt = 0.0, ebx = 0.0, eby = 0.0, edx = 0.0, ety = 0.0, etx = 0.0;
+    if (live) {
+        e = t.list[lo + qi];
+        xb = t.lxb[lo + qi];
+        xt = t.lxt[lo + qi];
+        ebx = t.bx[e];
+        eby = t.by[e];
+        edx = t.dx[e];
+        ety = t.ty[e];
+        etx = t.tx[e];
+    }
+    const uint32_t gmask = M2 == 32 ? ~0u : ((1u << M2) - 1u) << (g * M2);
+    const uint32_t upto = (q == 31 ? ~0u : (2u << q) - 1u) & gmask;  // group lanes up to q
+    double bs = 0.0;
+    for (int s0 = 0; s0 <= nd; s0 += G) {
+        const int s = s0 + g;  // this group's sub-beam
+        const bool act = s <= nd;
+        const double ya = act && s > 0 ? t.giy[s - 1] : yb;
+        const double yc = act && s < nd ? t.giy[s] : yt;
+        const double h = yc - ya;
+        double sa = __builtin_huge_val(), sc = __builtin_huge_val();
+        int se = 0xFFFF;
+        if (act && live) {
+            sa = ya == yb ? xb : (ya == yt ? xt : (ya == ety ? etx : ebx + edx * (ya - eby)));
+            sc = yc == yb ? xb : (yc == yt ? xt : (yc == ety ? etx : ebx + edx * (yc - eby)));
+            se = e;
+        }
+        wave_oddeven(qi, M2, m, act, sa, sc, se);
+        const bool in_p = act && live && se < nP, in_q = act && live && !(se < nP);
+        const int pp = __popc(half(in_p) & upto) & 1;
+        const int pq = __popc(half(in_q) & upto) & 1;
+        const double an = next_lane_d(sa), cn = next_lane_d(sc);
+        const double term = ((an - sa) + (cn - sc)) * h;
+        const bool ins = act && qi + 1 < m && op_inside(op, pp, pq);
+        uint32_t inside = half(ins);
+        team_sync();  // the scratch's earlier readers are done
+        if (ins) t.gxa[q] = term;
+        team_sync();
+        if (q == 0) {
+            while (inside) {
+                const int gl = __builtin_ctz(inside);
+                inside &= inside - 1u;
+                bs += t.gxa[gl];
+            }
+        }
+    }
+    team_sync();
+    return bs;
+}
+
 struct BatchArgs {
     const uint32_t *pairs;
     const uint32_t *order;   // pair indices grouped by bucket
@@ -498,8 +634,12 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
         } else {
             for (int q = 0; q < ndef; ++q) {
                 const int k = t.beam_of[q];
-                const double bsk = team_general<TL>(t, t.boff[k] - wbase, t.boff[k + 1] - wbase,
-                                                    t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
+                const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
+                double bsk;
+                if (TL == 32 && hi - lo <= 32)
+                    bsk = half_general(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
+                else
+                    bsk = team_general<TL>(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
                 if (tl == 0) t.bsum[k] = bsk;
             }
         }
