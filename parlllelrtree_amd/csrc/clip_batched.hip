@@ -406,25 +406,27 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
     return bs;
 }
 
-// Teams of 32 lanes (half a wave, lanes base .. base + 31; the other half
-// may be another team anywhere in its own pair): wave_general's sweep of a
-// beam of m <= 32 entries inside the team's lanes, the same heights, orders,
-// terms and summation order.  What wave_general reads across the wave with
+// Teams of TS = 8, 16 or 32 lanes (lanes base .. base + TS - 1 of a wave
+// whose other lanes may be other teams anywhere in their own pairs):
+// wave_general's sweep of a beam of m <= TS entries inside the team's lanes,
+// the same heights, orders, terms and summation order.  What wave_general reads across the wave with
 // readlane (an index uniform over the wave) goes through the team's LDS
 // scratch here: the heights' ranks are counted from giy, and the inside
 // gaps' terms are added by the team's lane 0 from gxa, gap after gap.
-// Ballots are shifted to the team's half.  Returns the beam sum on the
+// Ballots are shifted to the team's lanes.  Returns the beam sum on the
 // team's lane 0.
-__device__ double half_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
-                               int op, int q, int base) {
+template <int TS>
+__device__ double small_general(const TeamLds &t, int lo, int hi, double yb, double yt, int nP,
+                                int op, int q, int base) {
+    constexpr uint32_t TMASK = TS == 32 ? ~0u : (1u << TS) - 1u;
     const int m = hi - lo;
     const double dy = yt - yb;
-    auto half = [&](bool v) { return static_cast<uint32_t>(__ballot(v) >> base); };
+    auto half = [&](bool v) { return static_cast<uint32_t>(__ballot(v) >> base) & TMASK; };
     // the pairs (i, j), i < j, spread over the team's lanes
     const int npairs = m * (m - 1) / 2;
     int nc = 0;
     team_sync();
-    for (int t0 = 0; t0 < npairs; t0 += 32) {
+    for (int t0 = 0; t0 < npairs; t0 += TS) {
         const int tt = t0 + q;
         bool has = false;
         double iy = 0.0;
@@ -456,7 +458,7 @@ __device__ double half_general(const TeamLds &t, int lo, int hi, double yb, doub
         }
         nc += __popc(hm);
     }
-    if (nc > t.GC || nc > 32) {
+    if (nc > t.GC || nc > TS) {
         // more crossing heights than the scratch (or the team) holds: the
         // serial sweep
         team_sync();
@@ -489,7 +491,7 @@ __device__ double half_general(const TeamLds &t, int lo, int hi, double yb, doub
         team_sync();
     }
     const int M2 = pow2_ge(m);
-    const int G = 32 / M2, g = q / M2, qi = q & (M2 - 1);
+    const int G = TS / M2, g = q / M2, qi = q & (M2 - 1);
     const bool live = qi < m;
     int e = 0;
     double xb = 0.0, xt = 0.0, ebx = 0.0, eby = 0.0, edx = 0.0, ety = 0.0, etx = 0.0;
@@ -636,8 +638,8 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
                 const int k = t.beam_of[q];
                 const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
                 double bsk;
-                if (TL == 32 && hi - lo <= 32)
-                    bsk = half_general(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
+                if (TL >= 8 && hi - lo <= TL)
+                    bsk = small_general<TL>(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
                 else
                     bsk = team_general<TL>(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
                 if (tl == 0) t.bsum[k] = bsk;
