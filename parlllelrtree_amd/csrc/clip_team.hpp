@@ -64,14 +64,16 @@ __host__ __device__ inline int pow2_ge(int v) {
 }
 
 // entries (and crossing heights) of one beam the team sweeps cooperatively;
-// none for the smallest pairs, whose serial sweep is short and whose teams
-// are limited by LDS.  At most 64: a beam holds few active edges (jittered
+// 16 for the smallest pairs (up to 32 vertices, whose teams are limited by
+// LDS: their beams rarely hold more, and the register sweep of small teams
+// beat the serial one there once it existed, grid overlay 97.6 -> 94.8 ms).
+// At most 64: a beam holds few active edges (jittered
 // grid of lognormal stars, 82,241 beams of 3,000 pairs: 7.1 on average, 24 at
 // the 99th percentile, 42 at most), and a beam with more entries or crossing
 // heights takes the serial sweep (same sums), so the LDS a larger scratch
 // would cost every team goes to more teams per CU instead.
 __host__ __device__ inline int gen_cap(int NV) {
-    if (NV <= 32) return 0;
+    if (NV <= 32) return 16;
     const int p = pow2_ge(NV);
     return p < 64 ? p : 64;
 }
