@@ -407,9 +407,10 @@ __device__ double wave_general(const TeamLds &t, int lo, int hi, double yb, doub
 }
 
 // Teams of TS = 8, 16 or 32 lanes (lanes base .. base + TS - 1 of a wave
-// whose other lanes may be other teams anywhere in their own pairs):
-// wave_general's sweep of a beam of m <= TS entries inside the team's lanes,
-// the same heights, orders, terms and summation order.  What wave_general reads across the wave with
+// whose other lanes may be other teams anywhere in their own pairs), TS at
+// most the team's scratch (GS, GC): wave_general's sweep of a beam of m <= TS
+// entries inside the team's lanes, the same heights, orders, terms and
+// summation order.  What wave_general reads across the wave with
 // readlane (an index uniform over the wave) goes through the team's LDS
 // scratch here: the heights' ranks are counted from giy, and the inside
 // gaps' terms are added by the team's lane 0 from gxa, gap after gap.
@@ -638,7 +639,9 @@ __device__ int team_pair(const TeamLds &t, const BatchArgs &A, uint32_t p, int t
                 const int k = t.beam_of[q];
                 const int lo = t.boff[k] - wbase, hi = t.boff[k + 1] - wbase;
                 double bsk;
-                if (TL >= 8 && hi - lo <= TL)
+                // the register sweep writes a term per lane of the team into
+                // the team's scratch (gxa), so the team must not outnumber it
+                if (TL >= 8 && hi - lo <= TL && TL <= t.GS)
                     bsk = small_general<TL>(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
                 else
                     bsk = team_general<TL>(t, lo, hi, t.ys[k], t.ys[k + 1], nP, A.op, tl, base);
